@@ -1,0 +1,170 @@
+"""ctypes binding of libfoilbench_amd.so (the C ABI declared in include/foilbench_amd.h).
+
+The library is loaded from this package directory. There is no CPU fallback: if the
+library is missing, or no GPU is visible, the solvers raise instead of computing anything.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libfoilbench_amd.so"
+
+FB_OK = 0
+FB_F64, FB_F32, FB_F32_MIXED, FB_F16_MIXED = 0, 1, 2, 3
+PRECISION_NAMES = {FB_F64: "f64", FB_F32: "f32", FB_F32_MIXED: "f32-mixed", FB_F16_MIXED: "f16-mixed"}
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str) -> None:
+        super().__init__(message)
+        self.status = status
+
+
+class fb_domain(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("periodic_x", C.c_int32), ("periodic_y", C.c_int32),
+        ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+    ]
+
+
+class fb_foil(C.Structure):
+    _fields_ = [
+        ("chord", C.c_double), ("pivot_x", C.c_double), ("pivot_y", C.c_double),
+        ("half_scale", C.c_double), ("camber_fwd", C.c_double), ("camber_aft", C.c_double),
+        ("two_p", C.c_double), ("one_minus_two_p", C.c_double), ("p", C.c_double),
+        ("cambered", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class fb_pose(C.Structure):
+    _fields_ = [("angle_degrees", C.c_double), ("cos_a", C.c_double), ("sin_a", C.c_double)]
+
+
+class fb_lbm_params(C.Structure):
+    _fields_ = [
+        ("omega_plus", C.c_double), ("omega_minus", C.c_double), ("u_lattice", C.c_double),
+        ("target_ux", C.c_double), ("target_uy", C.c_double), ("wall_omega_rad", C.c_double),
+        ("wall_scale", C.c_double), ("velocity_scale", C.c_double),
+        ("feq_boundary", C.c_double * 9), ("feq_uncovered", C.c_double * 9),
+    ]
+
+
+class fb_lbm_stats(C.Structure):
+    _fields_ = [
+        ("min_population", C.c_double), ("max_speed", C.c_double), ("density_excursion", C.c_double),
+        ("nonfinite_populations", C.c_int64), ("bad_fluid_density", C.c_int64),
+        ("nonfinite_velocity", C.c_int64),
+    ]
+
+
+_P = C.c_void_p
+_SIGNATURES: dict[str, tuple] = {
+    "fb_last_error": (C.c_char_p, []),
+    "fb_version": (C.c_int, []),
+    "fb_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "fb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "fb_ctx_destroy": (C.c_int, [_P]),
+    "fb_ctx_synchronize": (C.c_int, [_P]),
+    "fb_naca_cell_sdf": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.POINTER(fb_pose), _P, _P]),
+    "fb_naca_points": (C.c_int, [_P, C.POINTER(fb_foil), C.POINTER(fb_pose), C.c_int64, _P, _P, _P]),
+    "fb_lbm_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.c_int, _P, _P,
+                                C.POINTER(_P)]),
+    "fb_lbm_destroy": (C.c_int, [_P]),
+    "fb_lbm_set_state": (C.c_int, [_P, _P, _P]),
+    "fb_lbm_get_state": (C.c_int, [_P, _P, _P]),
+    "fb_lbm_set_equilibrium": (C.c_int, [_P, _P, _P]),
+    "fb_lbm_set_pose": (C.c_int, [_P, C.POINTER(fb_pose)]),
+    "fb_lbm_update_pose": (C.c_int, [_P, C.POINTER(fb_pose), C.POINTER(fb_lbm_params)]),
+    "fb_lbm_get_geometry": (C.c_int, [_P, _P, _P]),
+    "fb_lbm_checkpoint": (C.c_int, [_P]),
+    "fb_lbm_rollback": (C.c_int, [_P]),
+    "fb_lbm_rescale": (C.c_int, [_P, C.c_double]),
+    "fb_lbm_advance": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(fb_pose),
+                                 C.POINTER(fb_lbm_stats)]),
+    "fb_lbm_stats_now": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.POINTER(fb_lbm_stats)]),
+    "fb_lbm_macroscopic": (C.c_int, [_P, C.c_double, _P, _P]),
+    "fb_lbm_run_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int]),
+}
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def load(require_gpu: bool = True) -> C.CDLL:
+    """Load the library once; raise if it is missing (there is no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -m foilbench_amd.build` "
+                    "(the HIP library is required; there is no CPU fallback)"
+                )
+            lib = C.CDLL(str(LIB_PATH))
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+        lib = _lib
+    if require_gpu:
+        n = C.c_int(0)
+        rc = lib.fb_device_count(C.byref(n))
+        if rc != FB_OK or n.value < 1:
+            raise RuntimeError("no HIP device is visible; the foilbench_amd solvers run only on the GPU")
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != FB_OK:
+        msg = _lib.fb_last_error().decode() if _lib is not None else "unknown error"
+        raise NativeError(rc, msg or f"status {rc}")
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    if not a.flags.c_contiguous:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP device and stream (fb_ctx)."""
+
+    _default: "Context | None" = None
+
+    def __init__(self, device: int = 0) -> None:
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.fb_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls) -> "Context":
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def synchronize(self) -> None:
+        check(self.lib.fb_ctx_synchronize(self.handle))
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            if self.handle:
+                self.lib.fb_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

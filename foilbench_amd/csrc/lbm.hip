@@ -1,0 +1,1268 @@
+// D2Q9 two-relaxation-time LBM on gfx950.
+//
+// State layout (SoA, x fastest): the persistent field is the POST-collision population
+// f*[d][y][x] -- exactly the array the reference materialises between its collision and
+// streaming kernels (kern/lbm.py:12-76 -> 80-148). One fused kernel per lattice step
+//   pull-stream f* (periodic index wrap, kern/lbm.py:270-274)
+//   -> interpolated moving-wall bounce-back at cut links (kern/lbm.py:276-308)
+//   -> Zou-He / channel rows, inlet, convective outlet, corners (fb/solvers/lbm.py:612-645)
+//   -> sponge (kern/lbm.py:152-168)
+//   -> [uncovered-cell reset for the next pose (fb/solvers/lbm.py:463-484)]
+//   -> TRT collision (kern/lbm.py:12-76) -> write f*,
+// so each lattice step reads 9 and writes 9 populations per cell. The pre-collision field
+// (the reference's solver state) is written only on the last substep of an advance, where
+// the postcondition reductions are fused in.
+#include "geometry.h"
+
+namespace fb {
+
+enum : uint8_t { FL_SOLID = 1, FL_NEAR = 2 };
+
+struct D2Q9 {
+  static constexpr int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static constexpr int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr double w[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0, 1.0 / 9.0,
+                                  1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+};
+
+template <int P>
+struct Prec;
+// FP64 storage, FP64 arithmetic.
+template <>
+struct Prec<FB_F64> {
+  using S = double;
+  using C = double;
+  using B = double;
+  static constexpr bool shift = false;
+};
+// FP32 storage, FP64 kernel arithmetic, FP32 boundary arithmetic (the reference's float32).
+template <>
+struct Prec<FB_F32> {
+  using S = float;
+  using C = double;
+  using B = float;
+  static constexpr bool shift = false;
+};
+// Shifted FP32 storage (f - w), FP32 arithmetic.
+template <>
+struct Prec<FB_F32_MIXED> {
+  using S = float;
+  using C = float;
+  using B = float;
+  static constexpr bool shift = true;
+};
+// Shifted FP16 storage (f - w), FP32 arithmetic.
+template <>
+struct Prec<FB_F16_MIXED> {
+  using S = __half;
+  using C = float;
+  using B = float;
+  static constexpr bool shift = true;
+};
+
+template <class T>
+__device__ __forceinline__ T ldv(const double* p) { return (T)*p; }
+
+__device__ __forceinline__ double to_c(double v) { return v; }
+__device__ __forceinline__ float to_cf(float v) { return v; }
+
+template <class S, class C>
+__device__ __forceinline__ C ld(const S* p) { return (C)(*p); }
+template <>
+__device__ __forceinline__ float ld<__half, float>(const __half* p) { return __half2float(*p); }
+
+template <class S, class C>
+__device__ __forceinline__ void st(S* p, C v) { *p = (S)v; }
+template <>
+__device__ __forceinline__ void st<__half, float>(__half* p, float v) { *p = __float2half_rn(v); }
+
+// Round a compute value to the storage grid (only for the reference-faithful forms).
+template <class P>
+__device__ __forceinline__ typename P::C rnd(typename P::C v) {
+  if constexpr (P::shift) return v;
+  else return (typename P::C)(typename P::S)v;
+}
+
+struct LbmFeq {
+  double v[9];
+};
+
+template <class S>
+struct LbmArgs {
+  int nx, ny;
+  const S* __restrict__ fin;      // post-collision f* of the previous step (or pre-collision f for priming)
+  S* __restrict__ fout;           // post-collision f* of this step
+  S* __restrict__ fpre;           // pre-collision f of this step (written when write_pre)
+  const S* __restrict__ rho_in;   // collision density at near-wall cells of the streaming geometry
+  S* __restrict__ rho_out;        // collision density at near-wall cells of the next geometry
+  const S* __restrict__ outlet_in;
+  S* __restrict__ outlet_out;
+  const uint8_t* __restrict__ flags;       // streaming geometry
+  const uint8_t* __restrict__ flags_next;  // geometry of the next collision
+  const double* __restrict__ sdf;
+  const double* __restrict__ sponge_y;
+  const double* __restrict__ sponge_x;
+  double x0, y0, dx, dy, pivot_x, pivot_y;
+  double wall_omega, wall_scale;
+  double even, odd;
+  double u_lattice, tux, tuy;
+  double vscale;
+  double feq_b[9];
+  double feq_u[9];
+  int periodic_x, periodic_y, channel;
+  int write_pre, do_reset, prime;
+  unsigned long long* red;  // [min f key, max speed key, max excursion key, nonfinite f, bad rho, nonfinite u]
+};
+
+// ------------------------------------------------------------------ streaming pieces
+
+#pragma clang fp contract(off)
+
+// Bouzidi reflection at fluid cell (x,y) for the link e pointing into solid cell n;
+// the result lands in direction opp(e) at (x,y) (kern/lbm.py:276-308).
+template <class P>
+__device__ __forceinline__ typename P::C reflect(const LbmArgs<typename P::S>& a, int x, int y, int e, int nxs,
+                                                 int nys) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const size_t N = (size_t)a.nx * a.ny;
+  const size_t i = (size_t)y * a.nx + x;
+  double sx = a.sdf[i];
+  double sn = a.sdf[(size_t)nys * a.nx + nxs];
+  double den = fmax(sx - sn, 1.0e-12);
+  double q = fmin(fmax(sx / den, 0.05), 1.0);
+  C src = ld<S, C>(a.fin + (size_t)e * N + i);
+  C r;
+  if (q < 0.5) {
+    int ux = wrapi(x - D2Q9::cx[e], a.nx), uy = wrapi(y - D2Q9::cy[e], a.ny);
+    C up = ld<S, C>(a.fin + (size_t)e * N + (size_t)uy * a.nx + ux);
+    C q2 = (C)(2.0 * q);
+    C q12 = (C)(1.0 - 2.0 * q);
+    r = q2 * src + q12 * up;
+  } else {
+    C q2 = (C)(2.0 * q);
+    C q21 = (C)(2.0 * q - 1.0);
+    C fo = ld<S, C>(a.fin + (size_t)D2Q9::opp[e] * N + i);
+    r = src / q2 + q21 * fo / q2;
+  }
+  if (a.wall_omega != 0.0) {
+    double wx = a.x0 + ((double)x + 0.5) * a.dx + (q * (double)D2Q9::cx[e]) * a.dx;
+    double wy = a.y0 + ((double)y + 0.5) * a.dy + (q * (double)D2Q9::cy[e]) * a.dy;
+    double wu = (-a.wall_omega) * (wy - a.pivot_y);
+    double wv = a.wall_omega * (wx - a.pivot_x);
+    double wp = (double)D2Q9::cx[e] * wu + (double)D2Q9::cy[e] * wv;
+    C rho = ld<S, C>(a.rho_in + i);
+    if constexpr (P::shift) rho = rho + (C)1;
+    r = r - (C)6.0 * (C)D2Q9::w[e] * rho * (C)wp * (C)a.wall_scale;
+  }
+  return rnd<P>(r);
+}
+
+// Pull-stream of cell (x,y): the value each direction receives (kern/lbm.py:262-308).
+template <class P>
+__device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const size_t N = (size_t)a.nx * a.ny;
+  const size_t i = (size_t)y * a.nx + x;
+  g[0] = ld<S, C>(a.fin + i);
+  if (a.flags[i] & FL_SOLID) {
+#pragma unroll
+    for (int d = 1; d < 9; ++d) g[d] = (C)0;
+    if constexpr (P::shift) {
+#pragma unroll
+      for (int d = 1; d < 9; ++d) g[d] = -(C)D2Q9::w[d];
+    }
+    return;
+  }
+  const bool near = a.flags[i] & FL_NEAR;
+#pragma unroll
+  for (int d = 1; d < 9; ++d) {
+    int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
+    size_t s = (size_t)sy * a.nx + sx;
+    if (near && (a.flags[s] & FL_SOLID))
+      g[d] = reflect<P>(a, x, y, D2Q9::opp[d], sx, sy);
+    else
+      g[d] = ld<S, C>(a.fin + (size_t)d * N + s);
+  }
+}
+
+// Zou-He velocity rows and inlet in storage arithmetic (fb/solvers/lbm.py:486-565).
+template <class P>
+__device__ __forceinline__ void unshift(typename P::C g[9]) {
+  if constexpr (P::shift) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = g[d] + (typename P::C)D2Q9::w[d];
+  }
+}
+template <class P>
+__device__ __forceinline__ void reshift(typename P::C g[9]) {
+  if constexpr (P::shift) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = g[d] - (typename P::C)D2Q9::w[d];
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void bottom_bc(const LbmArgs<typename P::S>& a, typename P::C g[9]) {
+  using B = typename P::B;
+  unshift<P>(g);
+  B b[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) b[d] = (B)g[d];
+  B ux = (B)a.tux, uy = (B)a.tuy;
+  B rho = (b[0] + b[1] + b[3] + (B)2.0 * (b[4] + b[7] + b[8])) / (B)(1.0 - a.tuy);
+  b[2] = b[4] + (B)(2.0 / 3.0) * rho * uy;
+  b[5] = b[7] + (B)0.5 * (b[3] - b[1]) + (B)(1.0 / 6.0) * rho * uy + (B)0.5 * rho * ux;
+  b[6] = b[8] + (B)0.5 * (b[1] - b[3]) + (B)(1.0 / 6.0) * rho * uy - (B)0.5 * rho * ux;
+  g[2] = b[2];
+  g[5] = b[5];
+  g[6] = b[6];
+  reshift<P>(g);
+}
+
+template <class P>
+__device__ __forceinline__ void top_bc(const LbmArgs<typename P::S>& a, typename P::C g[9]) {
+  using B = typename P::B;
+  unshift<P>(g);
+  B b[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) b[d] = (B)g[d];
+  B ux = (B)a.tux, uy = (B)a.tuy;
+  B rho = (b[0] + b[1] + b[3] + (B)2.0 * (b[2] + b[5] + b[6])) / (B)(1.0 + a.tuy);
+  b[4] = b[2] - (B)(2.0 / 3.0) * rho * uy;
+  b[7] = b[5] + (B)0.5 * (b[1] - b[3]) - (B)(1.0 / 6.0) * rho * uy - (B)0.5 * rho * ux;
+  b[8] = b[6] + (B)0.5 * (b[3] - b[1]) - (B)(1.0 / 6.0) * rho * uy + (B)0.5 * rho * ux;
+  g[4] = b[4];
+  g[7] = b[7];
+  g[8] = b[8];
+  reshift<P>(g);
+}
+
+template <class P>
+__device__ __forceinline__ void left_bc(const LbmArgs<typename P::S>& a, typename P::C g[9]) {
+  using B = typename P::B;
+  unshift<P>(g);
+  B b[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) b[d] = (B)g[d];
+  B ux = (B)a.tux, uy = (B)a.tuy;
+  B rho = (b[0] + b[2] + b[4] + (B)2.0 * (b[3] + b[6] + b[7])) / (B)(1.0 - a.tux);
+  b[1] = b[3] + (B)(2.0 / 3.0) * rho * ux;
+  b[5] = b[7] + (B)0.5 * (b[4] - b[2]) + (B)(1.0 / 6.0) * rho * ux + (B)0.5 * rho * uy;
+  b[8] = b[6] + (B)0.5 * (b[2] - b[4]) + (B)(1.0 / 6.0) * rho * ux - (B)0.5 * rho * uy;
+  g[1] = b[1];
+  g[5] = b[5];
+  g[8] = b[8];
+  reshift<P>(g);
+}
+
+// Streamed values of (x,y) after the y-boundary treatment (fb/solvers/lbm.py:612-627).
+template <class P>
+__device__ __forceinline__ void streamed_row(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+  using C = typename P::C;
+  const bool edge = !a.periodic_y && (y == 0 || y == a.ny - 1);
+  if (edge && a.channel) {
+    C h[9];
+    pull<P>(a, x, y == 0 ? 1 : a.ny - 2, h);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = h[D2Q9::opp[d]];
+    return;
+  }
+  pull<P>(a, x, y, g);
+  if (edge) {
+    if (y == 0) bottom_bc<P>(a, g);
+    else top_bc<P>(a, g);
+  }
+}
+
+// Final pre-collision value of cell (x,y) for this step, before the sponge
+// (fb/solvers/lbm.py:612-645).
+template <class P>
+__device__ __forceinline__ void boundary_cell(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using B = typename P::B;
+  if (!a.periodic_x && x == a.nx - 1) {
+    C g2[9];
+    streamed_row<P>(a, a.nx - 2, y, g2);
+    const B u = (B)a.u_lattice;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      B prev = (B)ld<S, C>(a.outlet_in + (size_t)y * 9 + d);
+      B v = prev + u * ((B)g2[d] - prev);
+      g[d] = (C)v;
+      st<S, C>(a.outlet_out + (size_t)y * 9 + d, (C)v);
+    }
+  } else {
+    streamed_row<P>(a, x, y, g);
+    if (!a.periodic_x && x == 0) left_bc<P>(a, g);
+  }
+  if (!a.periodic_x && !a.periodic_y && (x == 0 || x == a.nx - 1) && (y == 0 || y == a.ny - 1)) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_b[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void apply_sponge(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+  using C = typename P::C;
+  double s = fmax(fmax(0.0, a.sponge_y[y]), a.sponge_x[x]);
+  if (s == 0.0) return;
+  if constexpr (P::shift) {
+    float sf = (float)s, rf = (float)(1.0 - s);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = rf * g[d] + sf * (float)(a.feq_b[d] - D2Q9::w[d]);
+  } else {
+    double r = 1.0 - s;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = rnd<P>((C)(r * (double)g[d] + s * a.feq_b[d]));
+  }
+}
+
+// TRT collision (kern/lbm.py:189-236). Returns the collision density.
+template <class P>
+__device__ __forceinline__ typename P::C collide(const LbmArgs<typename P::S>& a, const typename P::C g[9],
+                                                 typename P::C post[9]) {
+  using C = typename P::C;
+  if constexpr (!P::shift) {
+    C rho = g[0];
+#pragma unroll
+    for (int d = 1; d < 9; ++d) rho = rho + g[d];
+    C safe = fmax(rho, (C)1.0e-12);
+    C ux = ((((g[1] - g[3]) + g[5]) - g[6]) - g[7]) + g[8];
+    ux = ux / safe;
+    C uy = ((((g[2] - g[4]) + g[5]) + g[6]) - g[7]) - g[8];
+    uy = uy / safe;
+    C s2 = ux * ux + uy * uy;
+    C feq[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      C p = (C)D2Q9::cx[d] * ux + (C)D2Q9::cy[d] * uy;
+      feq[d] = rho * (C)D2Q9::w[d] * (((C)1.0 + (C)3.0 * p + (C)4.5 * p * p) - (C)1.5 * s2);
+    }
+    const C even = (C)a.even, odd = (C)a.odd;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const int o = D2Q9::opp[d];
+      post[d] = g[d] - even * (g[d] - feq[d]) - odd * (g[o] - feq[o]);
+    }
+    return rho;
+  } else {
+    return C(0);
+  }
+}
+
+#pragma clang fp contract(fast)
+
+// Shifted-population TRT collision in FP32: h = f - w, rho = 1 + sum(h).
+template <class P>
+__device__ __forceinline__ float collide_shifted(const LbmArgs<typename P::S>& a, const float h[9], float post[9]) {
+  float drho = h[0];
+#pragma unroll
+  for (int d = 1; d < 9; ++d) drho += h[d];
+  float rho = 1.0f + drho;
+  float inv = 1.0f / fmaxf(rho, 1.0e-12f);
+  float ux = ((((h[1] - h[3]) + h[5]) - h[6]) - h[7]) + h[8];
+  float uy = ((((h[2] - h[4]) + h[5]) + h[6]) - h[7]) - h[8];
+  ux *= inv;
+  uy *= inv;
+  float s2 = ux * ux + uy * uy;
+  float heq[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    float p = (float)D2Q9::cx[d] * ux + (float)D2Q9::cy[d] * uy;
+    heq[d] = (float)D2Q9::w[d] * (drho + rho * ((3.0f * p + 4.5f * p * p) - 1.5f * s2));
+  }
+  const float even = (float)a.even, odd = (float)a.odd;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const int o = D2Q9::opp[d];
+    post[d] = h[d] - even * (h[d] - heq[d]) - odd * (h[o] - heq[o]);
+  }
+  return drho;
+}
+
+#pragma clang fp contract(off)
+
+// Postcondition reductions of the pre-collision state g (fb/solvers/lbm.py:730-793):
+// min population, finiteness, fluid density excursion, max physical speed. Density uses
+// NumPy's pairwise order over the 9 directions and the momentum a sequential sum, in the
+// storage dtype.
+template <class P>
+__device__ __forceinline__ void cell_stats(const LbmArgs<typename P::S>& a, const typename P::C gin[9], bool fluid,
+                                           double& mn, double& vmax, double& exc, long long& nf, long long& bad,
+                                           long long& nfu) {
+  using B = typename P::B;
+  using C = typename P::C;
+  B f[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    C v = gin[d];
+    if constexpr (P::shift) v = v + (C)D2Q9::w[d];
+    f[d] = (B)v;
+    double dv = (double)f[d];
+    if (!isfinite(dv)) nf += 1;
+    mn = fmin(mn, dv);
+  }
+  B rho = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+  B mx = (B)0, my = (B)0;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    mx = mx + f[d] * (B)D2Q9::cx[d];
+    my = my + f[d] * (B)D2Q9::cy[d];
+  }
+  B den = rho > (B)1.0e-12 ? rho : (B)1.0e-12;
+  if (!(rho == rho)) den = rho;
+  B ux = (mx / den) * (B)a.vscale;
+  B uy = (my / den) * (B)a.vscale;
+  B sp = sqrt(ux * ux + uy * uy);
+  if (!isfinite((double)ux) || !isfinite((double)uy)) nfu += 1;
+  vmax = fmax(vmax, (double)sp);
+  if (fluid) {
+    double r = (double)rho;
+    if (!isfinite(r) || r <= 0.0) bad += 1;
+    else exc = fmax(exc, (double)fabs(rho - (B)1.0));
+  }
+}
+
+// ------------------------------------------------------------------ the step kernel
+
+template <class P>
+__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const unsigned gx = gridDim.x;
+  const unsigned b = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+  const int x = (int)((b % gx) * blockDim.x + threadIdx.x);
+  const int y = (int)(b / gx);
+  const bool active = x < a.nx && y < a.ny;
+  double mn = INFINITY, vmax = 0.0, exc = 0.0;
+  long long nf = 0, bad = 0, nfu = 0;
+  if (active) {
+    const size_t N = (size_t)a.nx * a.ny;
+    const size_t i = (size_t)y * a.nx + x;
+    C g[9];
+    if (a.prime) {
+#pragma unroll
+      for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(a.fin + (size_t)d * N + i);
+    } else {
+      const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && a.flags[i] == 0;
+      if (interior) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+          g[d] = ld<S, C>(a.fin + (size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d]));
+      } else {
+        boundary_cell<P>(a, x, y, g);
+      }
+      apply_sponge<P>(a, x, y, g);
+    }
+    const uint8_t fnext = a.flags_next[i];
+    if (a.do_reset && (a.flags[i] & FL_SOLID) && !(fnext & FL_SOLID)) {
+#pragma unroll
+      for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_u[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
+    }
+    if (a.write_pre) {
+#pragma unroll
+      for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
+      cell_stats<P>(a, g, !(fnext & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
+    }
+    C post[9];
+    C rho;
+    if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
+    else rho = collide<P>(a, g, post);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
+    if (fnext & FL_NEAR) st<S, C>(a.rho_out + i, rho);
+  }
+  if (a.write_pre) {
+    // Block reduction: wave shuffles, then one atomic per wave.
+    mn = wave_min(mn);
+    vmax = wave_max(vmax);
+    exc = wave_max(exc);
+    nf = wave_sum_ll(nf);
+    bad = wave_sum_ll(bad);
+    nfu = wave_sum_ll(nfu);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&a.red[0], dkey(mn));
+      atomicMax(&a.red[1], dkey(vmax));
+      atomicMax(&a.red[2], dkey(exc));
+      if (nf) atomicAdd(&a.red[3], (unsigned long long)nf);
+      if (bad) atomicAdd(&a.red[4], (unsigned long long)bad);
+      if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
+    }
+  }
+}
+
+// Near-wall flags: fluid cells with a solid cell among their 8 periodic neighbours.
+__global__ void __launch_bounds__(256) lbm_flags_kernel(int nx, int ny, const uint8_t* __restrict__ solid,
+                                                        uint8_t* __restrict__ flags) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  size_t i = (size_t)y * nx + x;
+  uint8_t f = solid[i] ? FL_SOLID : 0;
+  if (!f) {
+    for (int d = 1; d < 9; ++d) {
+      int sx = wrapi(x + D2Q9::cx[d], nx), sy = wrapi(y + D2Q9::cy[d], ny);
+      if (solid[(size_t)sy * nx + sx]) f |= FL_NEAR;
+    }
+  }
+  flags[i] = f;
+}
+
+// Rescale (fb/solvers/lbm.py:357-368) on the pre-collision field, storage arithmetic.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_rescale_kernel(int nx, int ny, const typename P::S* __restrict__ fin,
+                                                          typename P::S* __restrict__ fout, double ratio,
+                                                          typename P::S* __restrict__ outlet) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using B = typename P::B;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t N = (size_t)nx * ny;
+  const size_t i = (size_t)y * nx + x;
+  B f[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    C v = ld<S, C>(fin + (size_t)d * N + i);
+    if constexpr (P::shift) v = v + (C)D2Q9::w[d];
+    f[d] = (B)v;
+  }
+  B rho = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+  B mx = (B)0, my = (B)0;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    mx = mx + f[d] * (B)D2Q9::cx[d];
+    my = my + f[d] * (B)D2Q9::cy[d];
+  }
+  B den = rho > (B)1.0e-12 ? rho : (B)1.0e-12;
+  B ux = mx / den, uy = my / den;
+  B r = (B)ratio;
+  B nux = ux * r, nuy = uy * r;
+  B s2o = ux * ux + uy * uy;
+  B s2n = nux * nux + nuy * nuy;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    B cx = (B)D2Q9::cx[d], cy = (B)D2Q9::cy[d], w = (B)D2Q9::w[d];
+    B cuo = cx * ux + cy * uy;
+    B cun = cx * nux + cy * nuy;
+    B eo = rho * w * ((B)1.0 + (B)3.0 * cuo + (B)4.5 * cuo * cuo - (B)1.5 * s2o);
+    B en = rho * w * ((B)1.0 + (B)3.0 * cun + (B)4.5 * cun * cun - (B)1.5 * s2n);
+    B v = en + r * (f[d] - eo);
+    C out = (C)v;
+    if constexpr (P::shift) out = out - (C)D2Q9::w[d];
+    st<S, C>(fout + (size_t)d * N + i, out);
+    if (x == nx - 1) st<S, C>(outlet + (size_t)y * 9 + d, out);
+  }
+}
+
+// Density and physical velocity for export/sampling (fb/solvers/lbm.py:447-461).
+template <class P>
+__global__ void __launch_bounds__(256) lbm_macro_kernel(int nx, int ny, const typename P::S* __restrict__ fin,
+                                                        double vscale, typename P::B* __restrict__ rho_out,
+                                                        typename P::B* __restrict__ vel_out) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using B = typename P::B;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t N = (size_t)nx * ny;
+  const size_t i = (size_t)y * nx + x;
+  B f[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    C v = ld<S, C>(fin + (size_t)d * N + i);
+    if constexpr (P::shift) v = v + (C)D2Q9::w[d];
+    f[d] = (B)v;
+  }
+  B rho = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+  B mx = (B)0, my = (B)0;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    mx = mx + f[d] * (B)D2Q9::cx[d];
+    my = my + f[d] * (B)D2Q9::cy[d];
+  }
+  B den = rho > (B)1.0e-12 ? rho : (B)1.0e-12;
+  rho_out[i] = rho;
+  vel_out[2 * i] = (mx / den) * (B)vscale;
+  vel_out[2 * i + 1] = (my / den) * (B)vscale;
+}
+
+// Pose change outside an advance: uncovered cells (solid before, fluid after) take the
+// equilibrium feq_u (fb/solvers/lbm.py:463-484); other cells are copied.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_uncover_kernel(int nx, int ny, const uint8_t* __restrict__ old_flags,
+                                                          const uint8_t* __restrict__ new_flags,
+                                                          const typename P::S* __restrict__ fin,
+                                                          typename P::S* __restrict__ fout, LbmFeq feq_u) {
+  using S = typename P::S;
+  using C = typename P::C;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t N = (size_t)nx * ny;
+  const size_t i = (size_t)y * nx + x;
+  const bool unc = (old_flags[i] & FL_SOLID) && !(new_flags[i] & FL_SOLID);
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    if (unc) {
+      C v = (C)feq_u.v[d];
+      if constexpr (P::shift) v = v - (C)D2Q9::w[d];
+      st<S, C>(fout + (size_t)d * N + i, v);
+    } else {
+      fout[(size_t)d * N + i] = fin[(size_t)d * N + i];
+    }
+  }
+}
+
+// f <- f_eq(rho, u) in storage arithmetic (fb/solvers/lbm.py:428-445); outlet <- f[:, -1].
+template <class P>
+__global__ void __launch_bounds__(256) lbm_equilibrium_kernel(int nx, int ny, const typename P::B* __restrict__ rho_in,
+                                                              const typename P::B* __restrict__ vel_in,
+                                                              typename P::S* __restrict__ fout,
+                                                              typename P::S* __restrict__ outlet) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using B = typename P::B;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t N = (size_t)nx * ny;
+  const size_t i = (size_t)y * nx + x;
+  B rho = rho_in[i];
+  B ux = vel_in[2 * i], uy = vel_in[2 * i + 1];
+  B s2 = ux * ux + uy * uy;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    B cu = (B)D2Q9::cx[d] * ux + (B)D2Q9::cy[d] * uy;
+    B e = rho * (B)D2Q9::w[d] * ((B)1.0 + (B)3.0 * cu + (B)4.5 * cu * cu - (B)1.5 * s2);
+    C out = (C)e;
+    if constexpr (P::shift) out = out - (C)D2Q9::w[d];
+    st<S, C>(fout + (size_t)d * N + i, out);
+    if (x == nx - 1) st<S, C>(outlet + (size_t)y * 9 + d, out);
+  }
+}
+
+// Host AoS (ny, nx, 9) <-> device SoA planes, with optional shift.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_aos_to_soa(int nx, int ny, const typename P::B* __restrict__ aos,
+                                                      typename P::S* __restrict__ soa) {
+  using C = typename P::C;
+  size_t N = (size_t)nx * ny;
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N * 9) return;
+  size_t i = k / 9;
+  int d = (int)(k % 9);
+  C v = (C)aos[k];
+  if constexpr (P::shift) v = v - (C)D2Q9::w[d];
+  st<typename P::S, C>(soa + (size_t)d * N + i, v);
+}
+template <class P>
+__global__ void __launch_bounds__(256) lbm_soa_to_aos(int nx, int ny, const typename P::S* __restrict__ soa,
+                                                      typename P::B* __restrict__ aos) {
+  using C = typename P::C;
+  size_t N = (size_t)nx * ny;
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N * 9) return;
+  size_t i = k / 9;
+  int d = (int)(k % 9);
+  C v = ld<typename P::S, C>(soa + (size_t)d * N + i);
+  if constexpr (P::shift) v = v + (C)D2Q9::w[d];
+  aos[k] = (typename P::B)v;
+}
+template <class P>
+__global__ void lbm_outlet_convert(int ny, const typename P::B* __restrict__ in_b, typename P::S* __restrict__ out_s,
+                                   const typename P::S* __restrict__ in_s, typename P::B* __restrict__ out_b) {
+  using C = typename P::C;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ny * 9) return;
+  int d = k % 9;
+  if (in_b) {
+    C v = (C)in_b[k];
+    if constexpr (P::shift) v = v - (C)D2Q9::w[d];
+    st<typename P::S, C>(out_s + k, v);
+  } else {
+    C v = ld<typename P::S, C>(in_s + k);
+    if constexpr (P::shift) v = v + (C)D2Q9::w[d];
+    out_b[k] = (typename P::B)v;
+  }
+}
+
+// ------------------------------------------------------------------ host handle
+
+struct Geometry {
+  double* sdf = nullptr;
+  uint8_t* solid = nullptr;
+  uint8_t* flags = nullptr;
+  double angle = 0.0;
+  bool valid = false;
+};
+
+struct Lattice {
+  Ctx* ctx = nullptr;
+  fb_domain dom{};
+  fb_foil foil{};
+  int prec = FB_F64;
+  int channel = 0;
+  size_t N = 0;
+  size_t ssize = 8;     // bytes per stored value
+  size_t bsize = 8;     // bytes per host value
+  void* F[3] = {};      // post-collision planes
+  void* R[3] = {};      // collision density (near-wall cells)
+  void* O[3] = {};      // outlet history (ny*9)
+  void* PRE[3] = {};    // pre-collision planes
+  Geometry G[3];
+  double* sponge_y = nullptr;
+  double* sponge_x = nullptr;
+  unsigned long long* red = nullptr;
+  unsigned long long* red_host = nullptr;
+  void* host_stage = nullptr;
+  // current indices
+  int f = 0, o = 0, pre = 0, g = 0;
+  bool fstar_valid = false;  // F[f] == collide(PRE[pre]) for geometry G[g]
+  // checkpoint
+  int cf = 0, co = 0, cpre = 0, cg = 0;
+  bool cvalid = false, cfstar = false;
+};
+
+static int other3(int a, int b) {
+  for (int k = 0; k < 3; ++k)
+    if (k != a && k != b) return k;
+  return 0;
+}
+
+template <class P>
+static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool prime, int fout_idx, int o_in, int o_out,
+                       int g_cur, int g_next, bool write_pre, int pre_out, bool do_reset) {
+  using S = typename P::S;
+  LbmArgs<S> a{};
+  a.nx = L.dom.nx;
+  a.ny = L.dom.ny;
+  a.fin = prime ? (const S*)L.PRE[L.pre] : (const S*)L.F[fin_idx];
+  a.fout = (S*)L.F[fout_idx];
+  a.fpre = (S*)L.PRE[pre_out];
+  a.rho_in = (const S*)L.R[fin_idx];
+  a.rho_out = (S*)L.R[fout_idx];
+  a.outlet_in = (const S*)L.O[o_in];
+  a.outlet_out = (S*)L.O[o_out];
+  a.flags = L.G[g_cur].flags;
+  a.flags_next = L.G[g_next].flags;
+  a.sdf = L.G[g_cur].sdf;
+  a.sponge_y = L.sponge_y;
+  a.sponge_x = L.sponge_x;
+  a.x0 = L.dom.x0;
+  a.y0 = L.dom.y0;
+  a.dx = (L.dom.x1 - L.dom.x0) / L.dom.nx;
+  a.dy = (L.dom.y1 - L.dom.y0) / L.dom.ny;
+  a.pivot_x = L.foil.pivot_x;
+  a.pivot_y = L.foil.pivot_y;
+  a.wall_omega = p.wall_omega_rad;
+  a.wall_scale = p.wall_scale;
+  a.even = 0.5 * (p.omega_plus + p.omega_minus);
+  a.odd = 0.5 * (p.omega_plus - p.omega_minus);
+  a.u_lattice = p.u_lattice;
+  a.tux = p.target_ux;
+  a.tuy = p.target_uy;
+  a.vscale = p.velocity_scale;
+  for (int d = 0; d < 9; ++d) {
+    a.feq_b[d] = p.feq_boundary[d];
+    a.feq_u[d] = p.feq_uncovered[d];
+  }
+  a.periodic_x = L.dom.periodic_x;
+  a.periodic_y = L.dom.periodic_y;
+  a.channel = L.channel;
+  a.write_pre = write_pre ? 1 : 0;
+  a.do_reset = do_reset ? 1 : 0;
+  a.prime = prime ? 1 : 0;
+  a.red = L.red;
+  dim3 block(256);
+  dim3 grid((L.dom.nx + 255) / 256, L.dom.ny);
+  hipLaunchKernelGGL(lbm_step_kernel<P>, grid, block, 0, L.ctx->stream, a);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+static int compute_geometry(Lattice& L, int gi, const fb_pose& pose) {
+  Geometry& G = L.G[gi];
+  int rc = launch_cell_sdf(L.ctx->stream, make_cell_grid(L.dom), make_naca(L.foil, pose), G.sdf, G.solid);
+  if (rc) return rc;
+  dim3 grid((L.dom.nx + 255) / 256, L.dom.ny);
+  hipLaunchKernelGGL(lbm_flags_kernel, grid, dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.solid, G.flags);
+  FB_HIP(hipGetLastError());
+  G.angle = pose.angle_degrees;
+  G.valid = true;
+  return FB_OK;
+}
+
+static int reset_red(Lattice& L) {
+  unsigned long long init[6] = {~0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+  // Encoded: min starts at +inf key (all ones is above every finite key), max at key of -inf.
+  init[1] = 0ull;
+  init[2] = 0ull;
+  FB_HIP(hipMemcpyAsync(L.red, init, sizeof(init), hipMemcpyHostToDevice, L.ctx->stream));
+  return FB_OK;
+}
+
+static int read_red(Lattice& L, fb_lbm_stats* s) {
+  FB_HIP(hipMemcpyAsync(L.red_host, L.red, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.ctx->stream));
+  FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  if (s) {
+    s->min_population = dkey_inv(L.red_host[0]);
+    s->max_speed = L.red_host[1] ? dkey_inv(L.red_host[1]) : 0.0;
+    s->density_excursion = L.red_host[2] ? dkey_inv(L.red_host[2]) : 0.0;
+    s->nonfinite_populations = (int64_t)L.red_host[3];
+    s->bad_fluid_density = (int64_t)L.red_host[4];
+    s->nonfinite_velocity = (int64_t)L.red_host[5];
+  }
+  return FB_OK;
+}
+
+template <class P>
+static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const fb_pose* poses, fb_lbm_stats* stats,
+                        bool sync_stats) {
+  int rc;
+  // Prime f* from the pre-collision state when it is stale or the first pose differs.
+  int g_first = L.g;
+  bool moved0 = poses && poses[0].angle_degrees != L.G[L.g].angle;
+  if (moved0) {
+    g_first = other3(L.cvalid ? L.cg : -1, L.g);
+    if ((rc = compute_geometry(L, g_first, poses[0]))) return rc;
+  }
+  int fcur = L.f;
+  if (!L.fstar_valid || moved0) {
+    int fo = other3(L.cvalid ? L.cf : -1, L.f);
+    if ((rc = launch_step<P>(L, p, L.f, true, fo, L.o, L.o, L.g, g_first, false, 0, moved0))) return rc;
+    fcur = fo;
+    L.f = fo;
+    L.g = g_first;
+    L.fstar_valid = true;
+  }
+  if ((rc = reset_red(L))) return rc;
+  int pre_out = other3(L.cvalid ? L.cpre : -1, L.pre);
+  for (int k = 0; k < substeps; ++k) {
+    bool last = (k == substeps - 1);
+    int g_next = L.g;
+    bool moving = poses && !last && poses[k + 1].angle_degrees != poses[k].angle_degrees;
+    if (moving) {
+      g_next = other3(L.cvalid ? L.cg : -1, L.g);
+      if ((rc = compute_geometry(L, g_next, poses[k + 1]))) return rc;
+    }
+    int fo = other3(L.cvalid ? L.cf : -1, fcur);
+    int oo = other3(L.cvalid ? L.co : -1, L.o);
+    if ((rc = launch_step<P>(L, p, fcur, false, fo, L.o, oo, L.g, g_next, last, pre_out, moving)))
+      return rc;
+    fcur = fo;
+    L.f = fo;
+    L.o = oo;
+    L.g = g_next;
+  }
+  L.pre = pre_out;
+  if (sync_stats) return read_red(L, stats);
+  return FB_OK;
+}
+
+template <class P>
+static int stats_impl(Lattice& L, const fb_lbm_params& p, fb_lbm_stats* stats);
+
+template <class P>
+__global__ void __launch_bounds__(256) lbm_stats_kernel(LbmArgs<typename P::S> a) {
+  using S = typename P::S;
+  using C = typename P::C;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  double mn = INFINITY, vmax = 0.0, exc = 0.0;
+  long long nf = 0, bad = 0, nfu = 0;
+  if (x < a.nx) {
+    const size_t N = (size_t)a.nx * a.ny;
+    const size_t i = (size_t)y * a.nx + x;
+    C g[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(a.fin + (size_t)d * N + i);
+    cell_stats<P>(a, g, !(a.flags[i] & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
+  }
+  mn = wave_min(mn);
+  vmax = wave_max(vmax);
+  exc = wave_max(exc);
+  nf = wave_sum_ll(nf);
+  bad = wave_sum_ll(bad);
+  nfu = wave_sum_ll(nfu);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&a.red[0], dkey(mn));
+    atomicMax(&a.red[1], dkey(vmax));
+    atomicMax(&a.red[2], dkey(exc));
+    if (nf) atomicAdd(&a.red[3], (unsigned long long)nf);
+    if (bad) atomicAdd(&a.red[4], (unsigned long long)bad);
+    if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
+  }
+}
+
+template <class P>
+static int stats_impl(Lattice& L, const fb_lbm_params& p, fb_lbm_stats* stats) {
+  using S = typename P::S;
+  int rc;
+  if ((rc = reset_red(L))) return rc;
+  LbmArgs<S> a{};
+  a.nx = L.dom.nx;
+  a.ny = L.dom.ny;
+  a.fin = (const S*)L.PRE[L.pre];
+  a.flags = L.G[L.g].flags;
+  a.vscale = p.velocity_scale;
+  a.red = L.red;
+  hipLaunchKernelGGL(lbm_stats_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream, a);
+  FB_HIP(hipGetLastError());
+  return read_red(L, stats);
+}
+
+template <class P>
+static int rescale_impl(Lattice& L, double ratio) {
+  using S = typename P::S;
+  int po = other3(L.cvalid ? L.cpre : -1, L.pre);
+  int oo = other3(L.cvalid ? L.co : -1, L.o);
+  hipLaunchKernelGGL(lbm_rescale_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
+                     L.dom.nx, L.dom.ny, (const S*)L.PRE[L.pre], (S*)L.PRE[po], ratio, (S*)L.O[oo]);
+  FB_HIP(hipGetLastError());
+  L.pre = po;
+  L.o = oo;
+  L.fstar_valid = false;
+  return FB_OK;
+}
+
+template <class P>
+static int macro_impl(Lattice& L, double vscale, void* rho_host, void* vel_host) {
+  using S = typename P::S;
+  using B = typename P::B;
+  B* d_rho = nullptr;
+  B* d_vel = nullptr;
+  FB_HIP(hipMalloc((void**)&d_rho, L.N * sizeof(B)));
+  FB_HIP(hipMalloc((void**)&d_vel, 2 * L.N * sizeof(B)));
+  hipLaunchKernelGGL(lbm_macro_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
+                     L.dom.nx, L.dom.ny, (const S*)L.PRE[L.pre], vscale, d_rho, d_vel);
+  FB_HIP(hipGetLastError());
+  if (rho_host) FB_HIP(hipMemcpyAsync(rho_host, d_rho, L.N * sizeof(B), hipMemcpyDeviceToHost, L.ctx->stream));
+  if (vel_host) FB_HIP(hipMemcpyAsync(vel_host, d_vel, 2 * L.N * sizeof(B), hipMemcpyDeviceToHost, L.ctx->stream));
+  FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  FB_HIP(hipFree(d_rho));
+  FB_HIP(hipFree(d_vel));
+  return FB_OK;
+}
+
+template <class P>
+static int set_state_impl(Lattice& L, const void* f_host, const void* outlet_host) {
+  using S = typename P::S;
+  using B = typename P::B;
+  const size_t n9 = L.N * 9;
+  B* stage = nullptr;
+  FB_HIP(hipMalloc((void**)&stage, n9 * sizeof(B)));
+  FB_HIP(hipMemcpyAsync(stage, f_host, n9 * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
+  hipLaunchKernelGGL(lbm_aos_to_soa<P>, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
+                     L.dom.ny, stage, (S*)L.PRE[L.pre]);
+  FB_HIP(hipGetLastError());
+  FB_HIP(hipMemcpyAsync(stage, outlet_host, (size_t)L.dom.ny * 9 * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
+  hipLaunchKernelGGL(lbm_outlet_convert<P>, dim3((L.dom.ny * 9 + 255) / 256), dim3(256), 0, L.ctx->stream, L.dom.ny,
+                     stage, (S*)L.O[L.o], (const S*)nullptr, (B*)nullptr);
+  FB_HIP(hipGetLastError());
+  FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  FB_HIP(hipFree(stage));
+  L.fstar_valid = false;
+  return FB_OK;
+}
+
+template <class P>
+static int get_state_impl(Lattice& L, void* f_host, void* outlet_host) {
+  using S = typename P::S;
+  using B = typename P::B;
+  const size_t n9 = L.N * 9;
+  B* stage = nullptr;
+  FB_HIP(hipMalloc((void**)&stage, n9 * sizeof(B)));
+  if (f_host) {
+    hipLaunchKernelGGL(lbm_soa_to_aos<P>, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
+                       L.dom.ny, (const S*)L.PRE[L.pre], stage);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipMemcpyAsync(f_host, stage, n9 * sizeof(B), hipMemcpyDeviceToHost, L.ctx->stream));
+    FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  }
+  if (outlet_host) {
+    hipLaunchKernelGGL(lbm_outlet_convert<P>, dim3((L.dom.ny * 9 + 255) / 256), dim3(256), 0, L.ctx->stream,
+                       L.dom.ny, (const B*)nullptr, (S*)nullptr, (const S*)L.O[L.o], stage);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipMemcpyAsync(outlet_host, stage, (size_t)L.dom.ny * 9 * sizeof(B), hipMemcpyDeviceToHost,
+                          L.ctx->stream));
+    FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  }
+  FB_HIP(hipFree(stage));
+  return FB_OK;
+}
+
+template <class P>
+static int equilibrium_impl(Lattice& L, const void* rho_host, const void* vel_host) {
+  using S = typename P::S;
+  using B = typename P::B;
+  B* d_rho = nullptr;
+  B* d_vel = nullptr;
+  FB_HIP(hipMalloc((void**)&d_rho, L.N * sizeof(B)));
+  FB_HIP(hipMalloc((void**)&d_vel, 2 * L.N * sizeof(B)));
+  FB_HIP(hipMemcpyAsync(d_rho, rho_host, L.N * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
+  FB_HIP(hipMemcpyAsync(d_vel, vel_host, 2 * L.N * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
+  int po = other3(L.cvalid ? L.cpre : -1, L.pre);
+  int oo = other3(L.cvalid ? L.co : -1, L.o);
+  hipLaunchKernelGGL(lbm_equilibrium_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
+                     L.dom.nx, L.dom.ny, d_rho, d_vel, (S*)L.PRE[po], (S*)L.O[oo]);
+  FB_HIP(hipGetLastError());
+  FB_HIP(hipStreamSynchronize(L.ctx->stream));
+  FB_HIP(hipFree(d_rho));
+  FB_HIP(hipFree(d_vel));
+  L.pre = po;
+  L.o = oo;
+  L.fstar_valid = false;
+  return FB_OK;
+}
+
+template <class P>
+static int uncover_impl(Lattice& L, const fb_pose& pose, const double* feq_u) {
+  using S = typename P::S;
+  int gi = other3(L.cvalid ? L.cg : -1, L.g);
+  int rc = compute_geometry(L, gi, pose);
+  if (rc) return rc;
+  int po = other3(L.cvalid ? L.cpre : -1, L.pre);
+  LbmFeq fq;
+  for (int d = 0; d < 9; ++d) fq.v[d] = feq_u[d];
+  hipLaunchKernelGGL(lbm_uncover_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
+                     L.dom.nx, L.dom.ny, L.G[L.g].flags, L.G[gi].flags, (const S*)L.PRE[L.pre], (S*)L.PRE[po], fq);
+  FB_HIP(hipGetLastError());
+  L.pre = po;
+  L.g = gi;
+  L.fstar_valid = false;
+  return FB_OK;
+}
+
+#define FB_DISPATCH(prec, fn, ...)                        \
+  switch (prec) {                                         \
+    case FB_F64: return fn<Prec<FB_F64>>(__VA_ARGS__);     \
+    case FB_F32: return fn<Prec<FB_F32>>(__VA_ARGS__);     \
+    case FB_F32_MIXED: return fn<Prec<FB_F32_MIXED>>(__VA_ARGS__); \
+    case FB_F16_MIXED: return fn<Prec<FB_F16_MIXED>>(__VA_ARGS__); \
+    default: set_error("unknown precision %d", prec); return FB_ERR_ARG; \
+  }
+
+static int advance_dispatch(Lattice& L, const fb_lbm_params& p, int n, const fb_pose* poses, fb_lbm_stats* s,
+                            bool red) {
+  FB_DISPATCH(L.prec, advance_impl, L, p, n, poses, s, red);
+}
+static int stats_dispatch(Lattice& L, const fb_lbm_params& p, fb_lbm_stats* s) {
+  FB_DISPATCH(L.prec, stats_impl, L, p, s);
+}
+static int rescale_dispatch(Lattice& L, double r) { FB_DISPATCH(L.prec, rescale_impl, L, r); }
+static int uncover_dispatch(Lattice& L, const fb_pose& p, const double* f) {
+  FB_DISPATCH(L.prec, uncover_impl, L, p, f);
+}
+static int equilibrium_dispatch(Lattice& L, const void* r, const void* v) {
+  FB_DISPATCH(L.prec, equilibrium_impl, L, r, v);
+}
+static int macro_dispatch(Lattice& L, double v, void* a, void* b) { FB_DISPATCH(L.prec, macro_impl, L, v, a, b); }
+static int set_state_dispatch(Lattice& L, const void* a, const void* b) { FB_DISPATCH(L.prec, set_state_impl, L, a, b); }
+static int get_state_dispatch(Lattice& L, void* a, void* b) { FB_DISPATCH(L.prec, get_state_impl, L, a, b); }
+
+static void free_lattice(Lattice* L) {
+  for (int k = 0; k < 3; ++k) {
+    if (L->F[k]) (void)hipFree(L->F[k]);
+    if (L->R[k]) (void)hipFree(L->R[k]);
+    if (L->O[k]) (void)hipFree(L->O[k]);
+    if (L->G[k].sdf) (void)hipFree(L->G[k].sdf);
+    if (L->G[k].solid) (void)hipFree(L->G[k].solid);
+    if (L->G[k].flags) (void)hipFree(L->G[k].flags);
+  }
+  for (int k = 0; k < 3; ++k)
+    if (L->PRE[k]) (void)hipFree(L->PRE[k]);
+  if (L->sponge_x) (void)hipFree(L->sponge_x);
+  if (L->sponge_y) (void)hipFree(L->sponge_y);
+  if (L->red) (void)hipFree(L->red);
+  if (L->red_host) (void)hipHostFree(L->red_host);
+  delete L;
+}
+
+}  // namespace fb
+
+using fb::Lattice;
+
+extern "C" {
+
+int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, int channel_walls,
+                  const double* sponge_y, const double* sponge_x, fb_lbm** out) {
+  FB_REQUIRE(ctx && dom && foil && sponge_y && sponge_x && out, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(dom->nx >= 4 && dom->ny >= 4, FB_ERR_ARG, "each resolution axis must contain at least four cells");
+  FB_REQUIRE(precision >= FB_F64 && precision <= FB_F16_MIXED, FB_ERR_ARG, "unknown precision %d", precision);
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  auto* L = new Lattice();
+  L->ctx = c;
+  L->dom = *dom;
+  L->foil = *foil;
+  L->prec = precision;
+  L->channel = channel_walls;
+  L->N = (size_t)dom->nx * dom->ny;
+  L->ssize = precision == FB_F64 ? 8 : (precision == FB_F16_MIXED ? 2 : 4);
+  L->bsize = precision == FB_F64 ? 8 : 4;
+  const size_t N = L->N;
+  auto fail = [&](hipError_t e) {
+    fb::set_error("device allocation failed: %s", hipGetErrorString(e));
+    fb::free_lattice(L);
+    return FB_ERR_NOMEM;
+  };
+  hipError_t e;
+  for (int k = 0; k < 3; ++k) {
+    if ((e = hipMalloc(&L->F[k], 9 * N * L->ssize))) return fail(e);
+    if ((e = hipMalloc(&L->R[k], N * L->ssize))) return fail(e);
+    if ((e = hipMalloc(&L->O[k], (size_t)dom->ny * 9 * L->ssize))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].sdf, N * sizeof(double)))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].solid, N))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].flags, N))) return fail(e);
+    if ((e = hipMemsetAsync(L->R[k], 0, N * L->ssize, c->stream))) return fail(e);
+  }
+  for (int k = 0; k < 3; ++k)
+    if ((e = hipMalloc(&L->PRE[k], 9 * N * L->ssize))) return fail(e);
+  if ((e = hipMalloc((void**)&L->sponge_y, dom->ny * sizeof(double)))) return fail(e);
+  if ((e = hipMalloc((void**)&L->sponge_x, dom->nx * sizeof(double)))) return fail(e);
+  if ((e = hipMalloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
+  if ((e = hipHostMalloc((void**)&L->red_host, 8 * sizeof(unsigned long long), 0))) return fail(e);
+  if ((e = hipMemcpy(L->sponge_y, sponge_y, dom->ny * sizeof(double), hipMemcpyHostToDevice))) return fail(e);
+  if ((e = hipMemcpy(L->sponge_x, sponge_x, dom->nx * sizeof(double), hipMemcpyHostToDevice))) return fail(e);
+  *out = reinterpret_cast<fb_lbm*>(L);
+  return FB_OK;
+}
+
+int fb_lbm_destroy(fb_lbm* lbm) {
+  if (!lbm) return FB_OK;
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  FB_HIP(hipStreamSynchronize(L->ctx->stream));
+  fb::free_lattice(L);
+  return FB_OK;
+}
+
+int fb_lbm_set_state(fb_lbm* lbm, const void* f_host, const void* outlet_host) {
+  FB_REQUIRE(lbm && f_host && outlet_host, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::set_state_dispatch(*L, f_host, outlet_host);
+}
+
+int fb_lbm_get_state(fb_lbm* lbm, void* f_host, void* outlet_host) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::get_state_dispatch(*L, f_host, outlet_host);
+}
+
+int fb_lbm_set_pose(fb_lbm* lbm, const fb_pose* pose) {
+  FB_REQUIRE(lbm && pose, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  int gi = fb::other3(L->cvalid ? L->cg : -1, L->g);
+  int rc = fb::compute_geometry(*L, gi, *pose);
+  if (rc) return rc;
+  L->g = gi;
+  L->fstar_valid = false;
+  return FB_OK;
+}
+
+int fb_lbm_update_pose(fb_lbm* lbm, const fb_pose* pose, const fb_lbm_params* params) {
+  FB_REQUIRE(lbm && pose && params, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  if (pose->angle_degrees == L->G[L->g].angle) return FB_OK;
+  return fb::uncover_dispatch(*L, *pose, params->feq_uncovered);
+}
+
+int fb_lbm_get_geometry(fb_lbm* lbm, double* sdf_host, uint8_t* solid_host) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  if (sdf_host)
+    FB_HIP(hipMemcpyAsync(sdf_host, L->G[L->g].sdf, L->N * sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream));
+  if (solid_host)
+    FB_HIP(hipMemcpyAsync(solid_host, L->G[L->g].solid, L->N, hipMemcpyDeviceToHost, L->ctx->stream));
+  FB_HIP(hipStreamSynchronize(L->ctx->stream));
+  return FB_OK;
+}
+
+int fb_lbm_checkpoint(fb_lbm* lbm) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  L->cf = L->f;
+  L->co = L->o;
+  L->cpre = L->pre;
+  L->cg = L->g;
+  L->cfstar = L->fstar_valid;
+  L->cvalid = true;
+  return FB_OK;
+}
+
+int fb_lbm_rollback(fb_lbm* lbm) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->cvalid, FB_ERR_STATE, "no checkpoint to roll back to");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  FB_HIP(hipStreamSynchronize(L->ctx->stream));
+  L->f = L->cf;
+  L->o = L->co;
+  L->pre = L->cpre;
+  L->g = L->cg;
+  L->fstar_valid = L->cfstar;
+  return FB_OK;
+}
+
+int fb_lbm_rescale(fb_lbm* lbm, double ratio) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::rescale_dispatch(*L, ratio);
+}
+
+int fb_lbm_set_equilibrium(fb_lbm* lbm, const void* rho_host, const void* vel_host) {
+  FB_REQUIRE(lbm && rho_host && vel_host, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::equilibrium_dispatch(*L, rho_host, vel_host);
+}
+
+int fb_lbm_advance(fb_lbm* lbm, const fb_lbm_params* params, int substeps, const fb_pose* poses,
+                   fb_lbm_stats* stats) {
+  FB_REQUIRE(lbm && params && poses, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(substeps >= 1, FB_ERR_ARG, "substeps must be positive");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::advance_dispatch(*L, *params, substeps, poses, stats, true);
+}
+
+int fb_lbm_stats_now(fb_lbm* lbm, const fb_lbm_params* params, fb_lbm_stats* stats) {
+  FB_REQUIRE(lbm && params && stats, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::stats_dispatch(*L, *params, stats);
+}
+
+int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void* vel_host) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::macro_dispatch(*L, velocity_scale, rho_host, vel_host);
+}
+
+int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps) {
+  FB_REQUIRE(lbm && params, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(steps >= 1, FB_ERR_ARG, "steps must be positive");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::advance_dispatch(*L, *params, steps, nullptr, nullptr, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,543 @@
+"""D2Q9 TRT lattice Boltzmann solver on the GPU, behind the reference's FlowSolver lifecycle.
+
+Host orchestration mirrors ``LBMSolver`` (fb/solvers/lbm.py:95-1146): time mapping and
+relaxation (scalars), transactional ``advance`` with deterministic Mach retries, and
+postconditions. Every per-cell operation (equilibrium set-up, geometry, collide-stream,
+boundaries, sponge, rescale, reductions, macroscopic export) runs in the HIP kernels of
+csrc/lbm.hip through the C ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _native as N
+from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
+from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
+from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
+                           NumericalFailure, RestartState, ReynoldsOutcome, Scenario, SolverInfo, StepReport)
+from ..core.validation import validate_advance_request, validate_canonical_import, validate_restart_state
+
+_C = np.asarray([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]], dtype=np.int64)
+_W = np.asarray([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
+_SOUND = float(1.0 / np.sqrt(3.0))
+_MAX_MACH = 0.08
+_MAX_LATTICE_SPEED = _MAX_MACH * _SOUND
+_MAX_SUBSTEPS = 512
+_MIN_POPULATION = -0.05
+
+FORMS = {"plain": None, "f64": N.FB_F64, "f32": N.FB_F32, "f32-mixed": N.FB_F32_MIXED, "f16-mixed": N.FB_F16_MIXED}
+
+
+def sponge_profiles(nx: int, ny: int, periodic_x: bool, periodic_y: bool, channel: bool):
+    """Separable sponge strength (fb/solvers/lbm.py:50-74, 270-294): cell = max(sy[y], sx[x])."""
+    width = max(3, min(nx, ny) // 16)
+    sy = np.zeros(ny, dtype=np.float64)
+    sx = np.zeros(nx, dtype=np.float64)
+    if not periodic_y and not channel:
+        y = np.arange(ny)
+        dist = np.minimum(y, ny - 1 - y)
+        sy = 0.12 * np.clip((width - dist) / width, 0.0, 1.0) ** 2
+    if not periodic_x:
+        ow = 2 * width
+        dist = nx - 1 - np.arange(nx)
+        sx = 0.08 * np.clip((ow - dist) / ow, 0.0, 1.0) ** 2
+    return np.ascontiguousarray(sy, dtype=np.float64), np.ascontiguousarray(sx, dtype=np.float64)
+
+
+def lbm_sponge_strength(nx, ny, x, y, *, periodic_x, periodic_y, channel_walls=False) -> float:
+    """Scalar sponge strength at (x, y) (fb/solvers/lbm.py:50-74)."""
+    sy, sx = sponge_profiles(nx, ny, periodic_x, periodic_y, channel_walls)
+    return float(max(0.0, sy[y], sx[x]))
+
+
+def feq9(ux, uy, dtype) -> np.ndarray:
+    """Nine equilibrium values at unit density in ``dtype`` arithmetic (fb/solvers/lbm.py:428-445)."""
+    t = np.dtype(dtype).type
+    ux = t(ux)
+    uy = t(uy)
+    s2 = ux * ux + uy * uy
+    out = np.empty(9, dtype=dtype)
+    for d in range(9):
+        cu = t(_C[d, 0]) * ux + t(_C[d, 1]) * uy
+        out[d] = t(1.0) * t(_W[d]) * (t(1.0) + t(3.0) * cu + t(4.5) * cu * cu - t(1.5) * s2)
+    return out
+
+
+class LBMSolver:
+    info = SolverInfo(
+        id="lbm-d2q9",
+        display_name="D2Q9 TRT LBM",
+        dimensions=(2,),
+        supports_moving_boundary=True,
+        supported_precisions=("float32", "float64"),
+        acceleration="gfx950 HIP fused collide-stream (foilbench_amd)",
+    )
+
+    def __init__(self, form: str = "plain", ctx: N.Context | None = None) -> None:
+        if form not in FORMS:
+            raise ValueError(f"unknown LBM form {form!r}; choose from {', '.join(FORMS)}")
+        self._form = form
+        self._ctx = ctx
+        self._handle = None
+        self._scenario: Scenario | None = None
+        self._geometry: NacaFoil | None = None
+        self._control = ControlState(0.0, 0.0, 0.0)
+        self._time = 0.0
+        self._reynolds = 1.0
+        self._revision = 0
+        self._lattice_speed = 0.08
+        self._lattice_dt = 1.0
+        self._reference_speed = 1.0
+        self._effective_reynolds = 0.0
+        self._viscosity_clamped = False
+        self._omega_plus = 1.0
+        self._omega_minus = 1.0
+        self._density_initial = 1.0
+        self._solid_angle: float | None = None
+
+    # ---------------------------------------------------------------- properties
+    @property
+    def reynolds(self) -> float:
+        return self._reynolds
+
+    @property
+    def state_revision(self) -> int:
+        return self._revision
+
+    @property
+    def precision_code(self) -> int:
+        if self._scenario is None:
+            raise RuntimeError("solver has not been initialized")
+        code = FORMS[self._form]
+        if code is None:
+            code = N.FB_F32 if self._scenario.precision == "float32" else N.FB_F64
+        return code
+
+    @property
+    def _host_dtype(self):
+        return np.float64 if self.precision_code == N.FB_F64 else np.float32
+
+    def set_reynolds(self, reynolds: float) -> ReynoldsOutcome:
+        if not np.isfinite(reynolds) or reynolds <= 0.0:
+            raise ValueError("Reynolds number must be finite and positive")
+        prev = (self._reynolds, self._effective_reynolds, self._viscosity_clamped, self._omega_plus,
+                self._omega_minus, self._revision)
+        requested = float(reynolds)
+        try:
+            self._reynolds = requested
+            if self._scenario is not None:
+                self._configure_relaxation()
+        except Exception:
+            (self._reynolds, self._effective_reynolds, self._viscosity_clamped, self._omega_plus,
+             self._omega_minus, self._revision) = prev
+            raise
+        if requested != prev[0]:
+            self._revision += 1
+        warnings = ((f"effective Reynolds clamped to {self._effective_reynolds:.1f}",)
+                    if self._viscosity_clamped else ())
+        return ReynoldsOutcome(requested, self._effective_reynolds, warnings)
+
+    def interactive_tuning(self):
+        return None
+
+    def adjust_interactive_tuning(self, direction: int):
+        del direction
+        return None
+
+    def apply_interactive_tuning(self, value):
+        del value
+        return None
+
+    # ---------------------------------------------------------------- lifecycle
+    def _require(self):
+        if self._scenario is None or self._geometry is None or self._handle is None:
+            raise RuntimeError("solver has not been initialized")
+        return self._scenario, self._geometry
+
+    def _lib(self):
+        return self._ctx.lib
+
+    def initialize(self, scenario: Scenario, geometry: NacaFoil, seed: int) -> None:
+        # fb/solvers/lbm.py:205-294
+        del seed
+        if scenario.domain.dimension != 2:
+            raise NotImplementedError("D2Q9 supports only 2D")
+        if self._ctx is None:
+            self._ctx = N.Context.default()
+        self._scenario = scenario
+        self._geometry = geometry if isinstance(geometry, NacaFoil) else NacaFoil(geometry.spec)
+        self._control = scenario.control_at(0.0)
+        self._time = 0.0
+        self._reynolds = float(scenario.reynolds)
+        self._revision = 0
+        self._reference_speed = scenario.reference_speed
+        dom = scenario.domain
+        subs = max(1, int(np.ceil(scenario.output_dt * self._reference_speed / (_MAX_LATTICE_SPEED * dom.dx) - 1.0e-12)))
+        self._lattice_dt = scenario.output_dt / subs
+        self._lattice_speed = self._reference_speed * self._lattice_dt / dom.dx
+        self._configure_relaxation()
+        dt = scenario.dtype
+        vel = np.empty((dom.ny, dom.nx, 2), dtype=dt)
+        vel[...] = np.asarray(scenario.freestream[:2], dtype=dt) * (self._lattice_speed / self._reference_speed)
+        init = str(scenario.solver_options.get("initial_condition", "freestream"))
+        pos = cell_centers(dom)
+        if init == "taylor-green":
+            vel[:, :, 0] = self._lattice_speed * np.sin(pos[:, :, 0]) * np.cos(pos[:, :, 1])
+            vel[:, :, 1] = -self._lattice_speed * np.cos(pos[:, :, 0]) * np.sin(pos[:, :, 1])
+        elif init == "poiseuille":
+            y0, y1 = dom.bounds[1]
+            r = 0.5 * (y1 - y0)
+            c = 0.5 * (y0 + y1)
+            vel[:, :, 0] = self._lattice_speed * 1.5 * (1.0 - ((pos[:, :, 1] - c) / r) ** 2)
+            vel[:, :, 1] = 0.0
+        self._channel = init == "poiseuille"
+        self._create_handle()
+        rho = np.ones((dom.ny, dom.nx), dtype=self._host_dtype)
+        N.check(self._lib().fb_lbm_set_pose(self._handle, C.byref(pose_struct(self._control.angle_degrees))))
+        self._solid_angle = self._control.angle_degrees
+        self._set_equilibrium(rho, vel)
+
+    def _create_handle(self) -> None:
+        if self._handle is not None:
+            self._lib().fb_lbm_destroy(self._handle)
+            self._handle = None
+        sc = self._scenario
+        dom = sc.domain
+        sy, sx = sponge_profiles(dom.nx, dom.ny, "x" in dom.periodic_axes, "y" in dom.periodic_axes, self._channel)
+        h = C.c_void_p()
+        N.check(self._lib().fb_lbm_create(self._ctx.handle, C.byref(domain_struct(dom)),
+                                          C.byref(foil_struct(sc.foil)), self.precision_code,
+                                          1 if self._channel else 0, N.ptr(sy), N.ptr(sx), C.byref(h)))
+        self._handle = h
+
+    def _set_equilibrium(self, rho: np.ndarray, vel: np.ndarray) -> None:
+        hd = self._host_dtype
+        r = np.ascontiguousarray(rho, dtype=hd)
+        v = np.ascontiguousarray(vel, dtype=hd)
+        N.check(self._lib().fb_lbm_set_equilibrium(self._handle, N.ptr(r), N.ptr(v)))
+
+    def restart(self, scenario: Scenario, geometry: NacaFoil, seed: int, start: RestartState) -> None:
+        # fb/solvers/lbm.py:296-309
+        validate_restart_state(start)
+        self.initialize(scenario, geometry, seed)
+        self.set_reynolds(start.reynolds)
+        self._time = start.time
+        self._control = ControlState(start.time, start.angle_degrees, 0.0)
+        self._update_pose_host(start.angle_degrees)
+        self._revision = 0
+
+    def _update_pose_host(self, angle: float) -> None:
+        """_update_solid outside an advance (fb/solvers/lbm.py:463-484)."""
+        if self._solid_angle == angle:
+            return
+        N.check(self._lib().fb_lbm_update_pose(self._handle, C.byref(pose_struct(angle)),
+                                               C.byref(self._params(0.0))))
+        self._solid_angle = angle
+
+    def _configure_relaxation(self) -> None:
+        # fb/solvers/lbm.py:311-338
+        sc = self._scenario
+        chord_cells = sc.foil.chord / sc.domain.dx
+        requested = self._lattice_speed * chord_cells / self._reynolds
+        minimum = (0.52 - 0.5) / 3.0
+        nu = max(requested, minimum)
+        self._viscosity_clamped = nu > requested
+        self._effective_reynolds = self._lattice_speed * chord_cells / nu
+        tp = 0.5 + 3.0 * nu
+        tm = 0.5 + (3.0 / 16.0) / max(tp - 0.5, 1.0e-6)
+        self._omega_plus = 1.0 / tp
+        self._omega_minus = 1.0 / tm
+        if not (0.0 < self._omega_plus < 2.0 and 0.0 < self._omega_minus < 2.0):
+            raise NumericalFailure("invalid_relaxation", "TRT relaxation frequency left the interval (0, 2)",
+                                   "collision", {"omega_plus": self._omega_plus, "omega_minus": self._omega_minus})
+
+    def _params(self, angular_velocity_degrees: float) -> N.fb_lbm_params:
+        sc = self._scenario
+        dt = np.float64 if self._host_dtype == np.float64 else np.float32
+        ratio = self._lattice_speed / self._reference_speed
+        target = np.asarray(sc.freestream[:2], dtype=dt) * ratio
+        uncovered = (np.asarray(sc.freestream[:2]) * ratio).astype(dt)
+        p = N.fb_lbm_params()
+        p.omega_plus = self._omega_plus
+        p.omega_minus = self._omega_minus
+        p.u_lattice = self._lattice_speed
+        p.target_ux = float(target[0])
+        p.target_uy = float(target[1])
+        p.wall_omega_rad = float(np.deg2rad(angular_velocity_degrees))
+        p.wall_scale = self._lattice_speed / self._reference_speed
+        p.velocity_scale = self._reference_speed / self._lattice_speed
+        fb_ = feq9(target[0], target[1], dt)
+        fu = feq9(uncovered[0], uncovered[1], dt)
+        for d in range(9):
+            p.feq_boundary[d] = float(fb_[d])
+            p.feq_uncovered[d] = float(fu[d])
+        return p
+
+    def _stats_now(self) -> N.fb_lbm_stats:
+        s = N.fb_lbm_stats()
+        N.check(self._lib().fb_lbm_stats_now(self._handle, C.byref(self._params(0.0)), C.byref(s)))
+        return s
+
+    def _configure_temporal_scaling(self, target_dt: float, max_speed: float, min_substeps: int = 1) -> int:
+        # fb/solvers/lbm.py:370-410
+        dom = self._scenario.domain
+        sel = max(max_speed, self._reference_speed)
+        subs = max(min_substeps, int(np.ceil(target_dt * sel / (_MAX_LATTICE_SPEED * dom.dx) - 1.0e-12)))
+        if subs > _MAX_SUBSTEPS:
+            raise NumericalFailure("excessive_velocity", "LBM motion requires too many lattice substeps",
+                                   "time-mapping", {"required_substeps": subs, "maximum_substeps": _MAX_SUBSTEPS,
+                                                    "maximum_physical_speed": max_speed})
+        speed = self._reference_speed * target_dt / (subs * dom.dx)
+        if abs(speed - self._lattice_speed) > 1.0e-12:
+            N.check(self._lib().fb_lbm_rescale(self._handle, speed / self._lattice_speed))
+        self._lattice_dt = target_dt / subs
+        self._lattice_speed = speed
+        self._configure_relaxation()
+        return subs
+
+    def _host_checkpoint(self):
+        return (self._solid_angle, self._control, self._time, self._lattice_speed, self._lattice_dt,
+                self._effective_reynolds, self._viscosity_clamped, self._omega_plus, self._omega_minus,
+                self._revision)
+
+    def _host_restore(self, c) -> None:
+        (self._solid_angle, self._control, self._time, self._lattice_speed, self._lattice_dt,
+         self._effective_reynolds, self._viscosity_clamped, self._omega_plus, self._omega_minus,
+         self._revision) = c
+
+    def _advance_once(self, control: ControlState, target_dt: float, min_substeps: int, retries: int) -> StepReport:
+        # fb/solvers/lbm.py:651-850
+        sc, geometry = self._require()
+        validate_advance_request(self._time, control, target_dt, sc.precision)
+        cur = self._stats_now()
+        if cur.nonfinite_populations:
+            raise NumericalFailure("nonfinite_state", "LBM populations are non-finite", "postcondition")
+        N.check(self._lib().fb_lbm_checkpoint(self._handle))
+        saved = self._host_checkpoint()
+        start_time = self._time
+        start_angle = self._control.angle_degrees
+        wall = abs(np.deg2rad(control.angular_velocity_degrees)) * geometry.maximum_radius
+        sweep = abs(np.deg2rad(control.angle_degrees - start_angle)) * geometry.maximum_radius / target_dt
+        mps = max(cur.max_speed, self._reference_speed, wall, sweep)
+        try:
+            subs = self._configure_temporal_scaling(target_dt, 1.25 * mps, min_substeps)
+            poses = (N.fb_pose * subs)()
+            for k in range(subs):
+                frac = (k + 1) / subs
+                poses[k] = pose_struct(start_angle + frac * (control.angle_degrees - start_angle))
+            stats = N.fb_lbm_stats()
+            N.check(self._lib().fb_lbm_advance(self._handle, C.byref(self._params(control.angular_velocity_degrees)),
+                                               subs, poses, C.byref(stats)))
+            self._solid_angle = poses[subs - 1].angle_degrees
+            if stats.nonfinite_populations:
+                raise NumericalFailure("invalid_population", "LBM produced non-finite populations", "postcondition")
+            min_pop = stats.min_population
+            if min_pop < _MIN_POPULATION:
+                raise NumericalFailure("invalid_population",
+                                       "LBM population left the admissible nonnegative envelope", "postcondition",
+                                       {"minimum_population": min_pop,
+                                        "minimum_allowed_population": _MIN_POPULATION})
+            if stats.bad_fluid_density:
+                raise NumericalFailure("invalid_density", "LBM density became non-positive or non-finite",
+                                       "postcondition")
+            excursion = stats.density_excursion
+            if excursion > 0.75:
+                raise NumericalFailure("invalid_density", "LBM density excursion exceeded the interactive envelope",
+                                       "postcondition", {"density_excursion": excursion,
+                                                         "maximum_density_excursion": 0.75})
+            if stats.nonfinite_velocity:
+                raise NumericalFailure("nonfinite_state", "LBM produced non-finite macroscopic velocity",
+                                       "postcondition")
+            max_speed = stats.max_speed
+            mach = (max(max_speed, self._reference_speed, wall, sweep) * self._lattice_speed
+                    / (self._reference_speed * _SOUND))
+            if mach > _MAX_MACH * (1.0 + 1.0e-6):
+                raise NumericalFailure("excessive_velocity", "LBM lattice Mach exceeded the admissible limit",
+                                       "postcondition", {"maximum_lattice_mach": mach,
+                                                         "maximum_lattice_mach_limit": _MAX_MACH,
+                                                         "attempted_substeps": subs})
+        except Exception:
+            N.check(self._lib().fb_lbm_rollback(self._handle))
+            self._host_restore(saved)
+            raise
+        self._time = start_time + target_dt
+        self._control = ControlState(self._time, control.angle_degrees, control.angular_velocity_degrees)
+        self._revision += 1
+        warnings = ((f"LBM relaxation clamp active: effective Re={self._effective_reynolds:.1f}",)
+                    if self._viscosity_clamped else ())
+        tp = 1.0 / self._omega_plus
+        tm = 1.0 / self._omega_minus
+        return StepReport(target_dt, target_dt, subs, max_speed, warnings, self._revision, {
+            "maximum_fluid_speed": max_speed,
+            "stability_retries": retries,
+            "maximum_wall_speed": wall,
+            "maximum_geometry_sweep_speed": sweep,
+            "maximum_lattice_mach": mach,
+            "density_excursion": excursion,
+            "minimum_population": min_pop,
+            "omega_plus": self._omega_plus,
+            "omega_minus": self._omega_minus,
+            "trt_magic": (tp - 0.5) * (tm - 0.5),
+            "requested_reynolds": self._reynolds,
+            "effective_reynolds": self._effective_reynolds,
+            "degraded_motion": wall == 0.0 and abs(control.angle_degrees - start_angle) > 1.0e-9,
+        })
+
+    def advance(self, control: ControlState, target_dt: float) -> StepReport:
+        # fb/solvers/lbm.py:852-886
+        min_subs = 1
+        for retries in range(4):
+            try:
+                return self._advance_once(control, target_dt, min_subs, retries)
+            except NumericalFailure as err:
+                if err.reason != "excessive_velocity" or err.stage != "postcondition" or retries == 3:
+                    raise
+                att = int(err.evidence.get("attempted_substeps", 0))
+                obs = float(err.evidence.get("maximum_lattice_mach", 0.0))
+                if att < 1 or not np.isfinite(obs) or obs <= 0.0:
+                    raise
+                min_subs = max(att + 1, int(np.ceil(att * obs / _MAX_MACH * 1.05)))
+                if min_subs > _MAX_SUBSTEPS:
+                    raise
+        raise RuntimeError("unreachable LBM stability retry state")
+
+    # ---------------------------------------------------------------- state access
+    def populations(self) -> tuple[np.ndarray, np.ndarray]:
+        """Pre-collision populations (ny, nx, 9) and outlet history (ny, 1, 9)."""
+        sc, _ = self._require()
+        hd = self._host_dtype
+        f = np.empty((sc.domain.ny, sc.domain.nx, 9), dtype=hd)
+        o = np.empty((sc.domain.ny, 1, 9), dtype=hd)
+        N.check(self._lib().fb_lbm_get_state(self._handle, N.ptr(f), N.ptr(o)))
+        return f, o
+
+    def set_populations(self, f: np.ndarray, outlet: np.ndarray) -> None:
+        hd = self._host_dtype
+        N.check(self._lib().fb_lbm_set_state(self._handle, N.ptr(np.ascontiguousarray(f, dtype=hd)),
+                                             N.ptr(np.ascontiguousarray(outlet, dtype=hd))))
+
+    def geometry_fields(self) -> tuple[np.ndarray, np.ndarray]:
+        sc, _ = self._require()
+        sdf = np.empty((sc.domain.ny, sc.domain.nx), dtype=np.float64)
+        solid = np.empty((sc.domain.ny, sc.domain.nx), dtype=np.uint8)
+        N.check(self._lib().fb_lbm_get_geometry(self._handle, N.ptr(sdf), N.ptr(solid)))
+        return sdf, solid.astype(bool)
+
+    def _macroscopic(self):
+        sc, _ = self._require()
+        hd = self._host_dtype
+        rho = np.empty((sc.domain.ny, sc.domain.nx), dtype=hd)
+        vel = np.empty((sc.domain.ny, sc.domain.nx, 2), dtype=hd)
+        N.check(self._lib().fb_lbm_macroscopic(self._handle, self._reference_speed / self._lattice_speed,
+                                               N.ptr(rho), N.ptr(vel)))
+        return rho.astype(sc.dtype, copy=False), vel.astype(sc.dtype, copy=False)
+
+    def sample_velocity(self, points: np.ndarray) -> np.ndarray:
+        sc, _ = self._require()
+        _, vel = self._macroscopic()
+        return sample_vector(vel, points, sc.domain)
+
+    def export_state(self) -> CanonicalFlowState:
+        # fb/solvers/lbm.py:892-915
+        sc, _ = self._require()
+        rho, vel = self._macroscopic()
+        _, solid = self.geometry_fields()
+        vel = vel.copy()
+        vel[solid] = 0.0
+        rho = np.asarray(rho, dtype=sc.dtype).copy()
+        rho[solid] = 1.0
+        return CanonicalFlowState(
+            schema_version=2, dimension=2, bounds=sc.domain.bounds, resolution=sc.domain.resolution,
+            periodic_axes=sc.domain.periodic_axes, time=self._time, precision=sc.precision,
+            angle_degrees=self._control.angle_degrees, angular_velocity_degrees=self._control.angular_velocity_degrees,
+            source_language="python", source_solver=self.info.id, velocity=vel[None, ...], density=rho[None, ...],
+            geometry=sc.foil, producer_execution_target="native",
+        )
+
+    def import_state(self, state: CanonicalFlowState, control: ControlState) -> ImportOutcome:
+        # fb/solvers/lbm.py:917-1052: equilibrium reconstruction at the import pose.
+        sc, geometry = self._require()
+        saved = self._host_checkpoint()
+        N.check(self._lib().fb_lbm_checkpoint(self._handle))
+        try:
+            solid = geometry.mask(sc.domain, control.angle_degrees, self._ctx)
+            validate_canonical_import(state, sc, control, solid)
+            physical = np.asarray(state.velocity[0], dtype=sc.dtype).copy()
+            maximum = float(np.max(np.linalg.norm(physical, axis=2)))
+            wall = abs(np.deg2rad(control.angular_velocity_degrees)) * geometry.maximum_radius
+            sel = max(max(maximum, wall), self._reference_speed)
+            self._lattice_speed = _MAX_LATTICE_SPEED * self._reference_speed / sel
+            self._lattice_dt = self._lattice_speed * sc.domain.dx / self._reference_speed
+            self._configure_relaxation()
+            density = (np.ones((sc.domain.ny, sc.domain.nx), dtype=sc.dtype) if state.density is None
+                       else np.asarray(state.density[0], dtype=sc.dtype).copy())
+            fluid_density = density[~solid]
+            if (not np.isfinite(density).all() or np.any(fluid_density <= 0.0)
+                    or (fluid_density.size > 0 and float(np.max(np.abs(fluid_density - 1.0))) > 0.75)):
+                raise NumericalFailure("invalid_density", "LBM warm import density is outside the admissible envelope",
+                                       "canonical-import")
+            mach = (max(maximum, wall, self._reference_speed) * self._lattice_speed
+                    / (self._reference_speed * _SOUND))
+            if mach > _MAX_MACH * (1.0 + 1.0e-6):
+                raise NumericalFailure("excessive_velocity", "LBM warm import exceeds its Mach limit",
+                                       "canonical-import", {"maximum_lattice_mach": mach,
+                                                            "maximum_lattice_mach_limit": _MAX_MACH})
+            self._time = state.time
+            self._control = control
+            centers = cell_centers(sc.domain)
+            wallv = geometry.wall_velocity(centers.reshape(-1, 2), control).reshape(sc.domain.ny, sc.domain.nx, 2)
+            physical[solid] = wallv[solid]
+            density[solid] = 1.0
+            lattice = physical * (self._lattice_speed / self._reference_speed)
+            N.check(self._lib().fb_lbm_set_pose(self._handle, C.byref(pose_struct(control.angle_degrees))))
+            self._solid_angle = control.angle_degrees
+            self._set_equilibrium(density, lattice)
+        except NumericalFailure as failure:
+            N.check(self._lib().fb_lbm_rollback(self._handle))
+            self._host_restore(saved)
+            return ImportOutcome("rejected", failure.reason, warnings=(str(failure),), stage=failure.stage,
+                                 evidence=failure.evidence)
+        except Exception:
+            N.check(self._lib().fb_lbm_rollback(self._handle))
+            self._host_restore(saved)
+            raise
+        self._revision += 1
+        report = ImportReport(state.source_solver, self.info.id,
+                              ("non-equilibrium lattice populations", "TRT kinetic modes"),
+                              ("LBM resumes from local equilibrium; an initialization transient is expected.",))
+        return ImportOutcome("accepted", "none", report, report.warnings)
+
+    def diagnostics(self) -> Diagnostics:
+        # fb/solvers/lbm.py:1103-1146 (run-level metrics on the exported fields)
+        sc, _ = self._require()
+        rho, vel = self._macroscopic()
+        _, solid = self.geometry_fields()
+        values = {
+            "time": self._time,
+            "requested_reynolds": self._reynolds,
+            "kinetic_energy": kinetic_energy(vel),
+            "enstrophy": enstrophy(vel, sc.domain),
+            "divergence_l2": divergence_l2(vel, sc.domain),
+            "solid_leakage": 0.0,
+            "density_mean": float(np.mean(rho)),
+            "density_drift": float(np.mean(rho) - self._density_initial),
+            "effective_reynolds": self._effective_reynolds,
+            "wake_width": wake_width(vel, sc.domain, sc.foil.pivot[0], sc.foil.chord, sc.freestream[0], solid),
+            "recirculation_area": recirculation_area(vel, sc.domain, sc.foil.pivot[0], solid),
+        }
+        if not all(np.isfinite(v) for v in values.values()):
+            raise NumericalFailure("nonfinite_state", "LBM produced non-finite diagnostics")
+        warnings = ((f"LBM relaxation clamp active: effective Re={self._effective_reynolds:.1f}",)
+                    if self._viscosity_clamped else ())
+        return Diagnostics(values, warnings, self._revision)
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            if self._handle is not None and self._ctx is not None:
+                self._ctx.lib.fb_lbm_destroy(self._handle)
+                self._handle = None
+        except Exception:
+            pass

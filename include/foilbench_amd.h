@@ -1,0 +1,158 @@
+/*
+ * foilbench_amd: C ABI of the MI355X (gfx950) per-step grid update of FoilBench's three
+ * 2D solvers. Plain pointers, sizes and POD structs only; every entry point returns an
+ * int status (FB_OK = 0) and fb_last_error() describes the last failure of the calling
+ * thread.
+ *
+ * The reference (Gravifer/FoilBench, Python) has no FFI: its hot path is Python calling
+ * numba kernels and NumPy/SciPy stages in-process. Each entry point below names the
+ * reference function(s) it replaces (paths relative to implementations/python/src/;
+ * fb/ = foilbench_py/, kern/ = foilbench_kernels/). The Python binding a maintainer adds on
+ * the reference side is the ctypes layer in foilbench_amd/_native.py.
+ *
+ * State lives on the device for the lifetime of a handle; host arrays passed in are
+ * copied and never retained. Host arrays use the reference's C-order layouts:
+ * populations (ny, nx, 9), faces u (ny, nx+1) and v (ny+1, nx), fields (ny, nx[, 2]).
+ */
+#ifndef FOILBENCH_AMD_H
+#define FOILBENCH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fb_status {
+  FB_OK = 0,
+  FB_ERR_ARG = 1,      /* invalid argument (ValueError/TypeError in the reference) */
+  FB_ERR_HIP = 2,      /* HIP runtime failure */
+  FB_ERR_STATE = 3,    /* handle used out of order (RuntimeError) */
+  FB_ERR_NUMERIC = 4,  /* classified numerical failure; see the report struct */
+  FB_ERR_NOMEM = 5
+};
+
+/* Storage/arithmetic forms.
+ * F64: FP64 storage and arithmetic (the plain drop-in form).
+ * F32: FP32 storage with FP64 kernel arithmetic, like the reference's float32 scenarios.
+ * F32_MIXED / F16_MIXED: populations stored shifted by their lattice weights (f - w) in
+ *   FP32 / FP16 with FP32 arithmetic (the precision form). */
+enum fb_precision { FB_F64 = 0, FB_F32 = 1, FB_F32_MIXED = 2, FB_F16_MIXED = 3 };
+
+typedef struct fb_ctx fb_ctx;
+typedef struct fb_lbm fb_lbm;
+
+/* Uniform cell grid: cells span [x0,x1] x [y0,y1] (fb/core/models.py:50-93). */
+typedef struct fb_domain {
+  int32_t nx, ny;
+  int32_t periodic_x, periodic_y;
+  double x0, x1, y0, y1;
+} fb_domain;
+
+/* NACA four-digit foil with the scalar coefficients of fb/core/geometry.py:60-90
+ * evaluated by the host exactly as the reference does in Python floats. */
+typedef struct fb_foil {
+  double chord, pivot_x, pivot_y;
+  double half_scale;      /* 5.0 * thickness * chord */
+  double camber_fwd;      /* m / p**2 */
+  double camber_aft;      /* m / (1.0 - p)**2 */
+  double two_p;           /* 2.0 * p */
+  double one_minus_two_p; /* 1.0 - 2.0 * p */
+  double p;               /* camber position */
+  int32_t cambered;       /* m > 0 and p > 0 */
+  int32_t pad_;
+} fb_foil;
+
+/* Rigid pose: cos and sin of np.deg2rad(angle_degrees), evaluated by the host with NumPy
+ * (fb/core/geometry.py:35-44). */
+typedef struct fb_pose {
+  double angle_degrees, cos_a, sin_a;
+} fb_pose;
+
+const char* fb_last_error(void);
+int fb_version(void);
+int fb_device_count(int* count);
+
+int fb_ctx_create(int device, fb_ctx** out);
+int fb_ctx_destroy(fb_ctx* ctx);
+int fb_ctx_synchronize(fb_ctx* ctx);
+
+/* Signed distance and mask of cell centres (fb/core/geometry.py:92-106, 157-168) into
+ * host arrays sdf (ny, nx) float64 and solid (ny, nx) uint8. */
+int fb_naca_cell_sdf(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, const fb_pose* pose,
+                     double* sdf_host, uint8_t* solid_host);
+
+/* Signed distance and normals of arbitrary points (fb/core/geometry.py:92-126); points are
+ * (n, 2) float64; normals may be NULL. */
+int fb_naca_points(fb_ctx* ctx, const fb_foil* foil, const fb_pose* pose, int64_t n,
+                   const double* points_host, double* sdf_host, double* normals_host);
+
+/* ------------------------------------------------------------------ D2Q9 TRT LBM ---- */
+/* Replaces kern/lbm.py:12-168 (trt_collision_kernel, moving_wall_stream_kernel,
+ * sponge_kernel) and the boundary/step logic of fb/solvers/lbm.py:486-649, 463-484,
+ * 357-368. One handle = one lattice with its geometry and transactional checkpoint. */
+
+typedef struct fb_lbm_params {
+  double omega_plus, omega_minus; /* TRT rates (fb/solvers/lbm.py:311-338) */
+  double u_lattice;               /* convective outlet speed (lbm.py:637-639) */
+  double target_ux, target_uy;    /* Zou-He target, storage-rounded (lbm.py:599-600) */
+  double wall_omega_rad;          /* foil angular velocity in rad/s (lbm.py:587) */
+  double wall_scale;              /* u_lattice / reference speed (lbm.py:606) */
+  double velocity_scale;          /* reference speed / u_lattice (lbm.py:457-461) */
+  double feq_boundary[9];         /* f_eq(1, target) in storage dtype (lbm.py:340-355) */
+  double feq_uncovered[9];        /* f_eq of newly uncovered cells (lbm.py:476-481) */
+} fb_lbm_params;
+
+typedef struct fb_lbm_stats {
+  double min_population;     /* min over all populations (lbm.py:736) */
+  double max_speed;          /* max physical |u| over all cells (lbm.py:777) */
+  double density_excursion;  /* max |rho-1| over fluid cells (lbm.py:760) */
+  int64_t nonfinite_populations;
+  int64_t bad_fluid_density; /* non-finite or <= 0 fluid densities (lbm.py:754) */
+  int64_t nonfinite_velocity;
+} fb_lbm_stats;
+
+/* sponge_y (ny) and sponge_x (nx): float64 strength profiles; the cell strength is
+ * max(sponge_y[y], sponge_x[x]) (fb/solvers/lbm.py:270-294). */
+int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision,
+                  int channel_walls, const double* sponge_y, const double* sponge_x, fb_lbm** out);
+int fb_lbm_destroy(fb_lbm* lbm);
+/* Pre-collision populations (ny, nx, 9) and outlet history (ny, 9), host dtype double for
+ * FB_F64 and float otherwise (mixed forms take unshifted values). */
+int fb_lbm_set_state(fb_lbm* lbm, const void* f_host, const void* outlet_host);
+int fb_lbm_get_state(fb_lbm* lbm, void* f_host, void* outlet_host);
+/* Pre-collision state from density (ny, nx) and lattice velocity (ny, nx, 2), host dtype:
+ * f = f_eq(rho, u) evaluated on the device (fb/solvers/lbm.py:252-254, 428-445, 994-995);
+ * the outlet history becomes f[:, -1] (lbm.py:254). */
+int fb_lbm_set_equilibrium(fb_lbm* lbm, const void* rho_host, const void* vel_host);
+/* Geometry for a pose without resetting uncovered cells (initialize/import). */
+int fb_lbm_set_pose(fb_lbm* lbm, const fb_pose* pose);
+/* Pose change outside an advance (fb/solvers/lbm.py:463-484, used by restart): new geometry,
+ * and cells uncovered by the move take params->feq_uncovered. No-op for the same angle. */
+int fb_lbm_update_pose(fb_lbm* lbm, const fb_pose* pose, const fb_lbm_params* params);
+int fb_lbm_get_geometry(fb_lbm* lbm, double* sdf_host, uint8_t* solid_host);
+/* Transaction: the state at checkpoint is kept intact until the next checkpoint. */
+int fb_lbm_checkpoint(fb_lbm* lbm);
+int fb_lbm_rollback(fb_lbm* lbm);
+/* f <- f_eq(rho, u*ratio) + ratio*(f - f_eq(rho, u)); outlet <- f[:, -1]
+ * (fb/solvers/lbm.py:357-368). */
+int fb_lbm_rescale(fb_lbm* lbm, double ratio);
+/* Run `substeps` lattice steps; poses[k] is the foil pose of substep k
+ * (fb/solvers/lbm.py:714-722). The stats of the final state are written to *stats. */
+int fb_lbm_advance(fb_lbm* lbm, const fb_lbm_params* params, int substeps, const fb_pose* poses,
+                   fb_lbm_stats* stats);
+/* Postcondition reductions of the current state (fb/solvers/lbm.py:687-688, 736-793). */
+int fb_lbm_stats_now(fb_lbm* lbm, const fb_lbm_params* params, fb_lbm_stats* stats);
+/* Density (ny, nx) and physical velocity (ny, nx, 2) in the host dtype
+ * (fb/solvers/lbm.py:447-461). */
+int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void* vel_host);
+/* Timing harness: run `steps` lattice steps with fixed parameters and no pose change,
+ * without postcondition reductions; returns once queued (use fb_ctx_synchronize). */
+int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOILBENCH_AMD_H */
