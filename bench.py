@@ -1,0 +1,176 @@
+"""Headline benchmark: cell-steps/s of the D2Q9 TRT LBM lattice step on one MI355X.
+
+Workload (one "step" = one lattice step of the whole lattice, inputs resident in HBM):
+NACA 2412 at AoA 12 degrees (angle_degrees = -12), 1024 x 512 lattice, chord 160 cells,
+equilibrium inflow u = 0.05 lattice units, Re = 1000, TRT magic parameter 3/16,
+interpolated bounce-back on the foil, FP32 population storage.
+
+Prints one JSON line. Besides the headline FP32 value it reports the FP64 plain form and the
+shifted FP32/FP16 precision forms (throughput and relative velocity error against FP64),
+the roofline of the step kernel from HIP events, and the NumPy oracle timed on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _gpu_forms(steps: int, warmup: int):
+    import numpy as np
+
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import BYTES_PER_CELL, Lattice, LatticeCase
+
+    case = LatticeCase()
+    ctx = N.Context.default()
+    out = {}
+    velocities = {}
+    solid = None
+    for name, prec in (("f32", N.FB_F32), ("f64", N.FB_F64), ("f32-mixed", N.FB_F32_MIXED),
+                       ("f16-mixed", N.FB_F16_MIXED)):
+        lat = Lattice(case, prec, ctx)
+        lat.run(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        lat.run(steps)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        ev_steps = min(steps, 2000)
+        ev_wall, ev_kern = lat.time(ev_steps)
+        avg_ms = ev_kern / ev_steps
+        bytes_launch = BYTES_PER_CELL[prec] * lat.cells
+        out[name] = {
+            "cell_steps_per_s": lat.cells * steps / wall,
+            "ms_per_step": 1e3 * wall / steps,
+            "kernel_ms": avg_ms,
+            "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
+            "bytes_per_cell": BYTES_PER_CELL[prec],
+        }
+        velocities[name] = lat.velocity().astype(np.float64)
+        if solid is None:
+            solid = lat.solid()
+        lat.close()
+        print(f"[bench] {name}: {out[name]['cell_steps_per_s']:.4g} cell-steps/s, "
+              f"kernel {avg_ms * 1e3:.2f} us ({out[name]['kernel_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+    fl = ~solid
+    ref = velocities["f64"][fl]
+    for name in out:
+        d = velocities[name][fl] - ref
+        out[name]["velocity_rel_err_vs_f64"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+    return case, out
+
+
+def _cpu_baseline(sample_steps: int):
+    import numpy as np
+
+    from oracle import lbm as L
+    from oracle.geometry import NacaFoil, cell_centers
+    from foilbench_amd.benchmark.lbm_harness import LatticeCase
+
+    case = LatticeCase()
+    nx, ny = case.nx, case.ny
+    dx = (case.bounds[0][1] - case.bounds[0][0]) / nx
+    dy = (case.bounds[1][1] - case.bounds[1][0]) / ny
+    foil = NacaFoil(case.naca, case.chord, case.pivot)
+    sdf = foil.signed_distance(cell_centers(case.bounds, nx, ny).reshape(-1, 2), case.angle_degrees).reshape(ny, nx)
+    solid = sdf <= 0.0
+    u = case.u_lattice
+    vel = np.zeros((ny, nx, 2), np.float32)
+    vel[..., 0] = u
+    f = L.equilibrium(np.ones((ny, nx), np.float32), vel)
+    feqb = f[0, 0].copy()
+    outlet = f[:, -1:].copy()
+    sp = L.sponge_strength_profile(nx, ny, False, False, False)
+    _, op, om = case.relaxation()
+    args = (solid, sdf, op, om, u, (np.float32(u), np.float32(0.0)), feqb, sp)
+    L.kernel_steps(f, *args, outlet, case.bounds, dx, dy, case.pivot, 0.0, u, False, False, False, 1)
+    t0 = time.perf_counter()
+    L.kernel_steps(f, *args, outlet, case.bounds, dx, dy, case.pivot, 0.0, u, False, False, False, sample_steps)
+    el = time.perf_counter() - t0
+    return {
+        "value": nx * ny * sample_steps / el,
+        "unit": "cell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy oracle (oracle/lbm.py kernel_steps), same 1024x512 FP32 lattice, {sample_steps} "
+                  f"lattice steps after 1 warm-up step, {el:.1f} s, single-threaded NumPy",
+    }
+
+
+def _traffic():
+    """HBM bytes per launch of the step kernel, from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / "lbm_step_kernel_1024x512_f32_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return float(json.loads(p.read_text())["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path does not shard "
+              "across GPUs in this build)", file=sys.stderr)
+        return 2
+    case, forms = _gpu_forms(args.steps, args.warmup)
+    main_form = forms["f32"]
+    achieved = main_form["kernel_gbs"]
+    traffic = _traffic()
+    cpu = _cpu_baseline(args.cpu_steps) if args.cpu_steps > 0 else None
+    line = {
+        "metric": "cell-steps/s/GPU",
+        "value": main_form["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_form["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": case.name + " fp32 storage, fp64 kernel arithmetic, interpolated bounce-back",
+            "lattice": [case.nx, case.ny],
+            "cells": case.nx * case.ny,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "lbm_step_kernel<Prec<FB_F32>>",
+            "bytes_per_cell": main_form["bytes_per_cell"],
+        },
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -64,6 +64,45 @@ class fb_lbm_stats(C.Structure):
     ]
 
 
+class fb_sf_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("viscosity", C.c_double), ("tolerance", C.c_double),
+        ("max_iterations", C.c_int32), ("maccormack", C.c_int32),
+        ("freestream_x", C.c_double), ("freestream_y", C.c_double), ("wall_omega_rad", C.c_double),
+        ("cfl_limit", C.c_double), ("channel_walls", C.c_int32), ("mixed_poisson", C.c_int32),
+    ]
+
+
+class fb_sf_report(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("viscosity_iterations", C.c_int32), ("viscosity_residual", C.c_double),
+        ("viscosity_converged", C.c_int32), ("pressure_iterations", C.c_int32), ("pressure_residual", C.c_double),
+        ("pressure_info", C.c_int32), ("pressure_converged", C.c_int32), ("pressure_refinements", C.c_int32),
+        ("pad_", C.c_int32), ("projection_cfl", C.c_double),
+    ]
+
+
+class fb_mac_stats(C.Structure):
+    _fields_ = [
+        ("max_face_speed", C.c_double), ("max_wall_speed", C.c_double), ("max_cell_speed", C.c_double),
+        ("divergence_linf", C.c_double), ("solid_leakage", C.c_double), ("nonfinite_faces", C.c_int64),
+        ("nonfinite_wall", C.c_int64),
+    ]
+
+
+SF_STATUS = {
+    0: None,
+    1: ("convergence_failure", "viscosity", "implicit viscosity did not converge"),
+    2: ("nonfinite_state", "projection", "projection received non-finite velocity"),
+    3: ("excessive_velocity", "projection", "projection CFL exceeds its limit"),
+    4: ("nonfinite_state", "projection", "pressure RHS contains non-finite values"),
+    5: ("projection_failure", "projection", "pressure RHS exceeds the safe CG dot-product range"),
+    6: ("nonfinite_state", "projection", "pressure CG produced a non-finite iterate"),
+    7: ("projection_failure", "projection", "pressure CG failed: floating-point breakdown"),
+    8: ("projection_failure", "projection", "pressure CG did not converge"),
+    9: ("nonfinite_state", "projection", "projection produced non-finite velocity"),
+}
+
 _P = C.c_void_p
 _SIGNATURES: dict[str, tuple] = {
     "fb_last_error": (C.c_char_p, []),
@@ -72,6 +111,7 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "fb_ctx_destroy": (C.c_int, [_P]),
     "fb_ctx_synchronize": (C.c_int, [_P]),
+    "fb_copy_benchmark": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "fb_naca_cell_sdf": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.POINTER(fb_pose), _P, _P]),
     "fb_naca_points": (C.c_int, [_P, C.POINTER(fb_foil), C.POINTER(fb_pose), C.c_int64, _P, _P, _P]),
     "fb_lbm_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.c_int, _P, _P,
@@ -91,6 +131,20 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_stats_now": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.POINTER(fb_lbm_stats)]),
     "fb_lbm_macroscopic": (C.c_int, [_P, C.c_double, _P, _P]),
     "fb_lbm_run_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int]),
+    "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+    "fb_mac_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.POINTER(_P)]),
+    "fb_mac_destroy": (C.c_int, [_P]),
+    "fb_mac_set_faces": (C.c_int, [_P, _P, _P]),
+    "fb_mac_get_faces": (C.c_int, [_P, _P, _P]),
+    "fb_mac_set_pose": (C.c_int, [_P, C.POINTER(fb_pose)]),
+    "fb_mac_get_geometry": (C.c_int, [_P, _P, _P]),
+    "fb_mac_get_pressure": (C.c_int, [_P, _P]),
+    "fb_mac_checkpoint": (C.c_int, [_P]),
+    "fb_mac_rollback": (C.c_int, [_P]),
+    "fb_sf_substep": (C.c_int, [_P, C.POINTER(fb_sf_params), C.POINTER(fb_pose), C.POINTER(fb_sf_report)]),
+    "fb_mac_project": (C.c_int, [_P, C.POINTER(fb_sf_params), C.POINTER(fb_sf_report)]),
+    "fb_mac_get_stats": (C.c_int, [_P, C.c_double, C.POINTER(fb_mac_stats)]),
 }
 
 _lock = threading.Lock()

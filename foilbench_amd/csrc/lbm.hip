@@ -12,6 +12,8 @@
 // so each lattice step reads 9 and writes 9 populations per cell. The pre-collision field
 // (the reference's solver state) is written only on the last substep of an advance, where
 // the postcondition reductions are fused in.
+#include <vector>
+
 #include "geometry.h"
 
 namespace fb {
@@ -824,7 +826,7 @@ static int read_red(Lattice& L, fb_lbm_stats* s) {
 
 template <class P>
 static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const fb_pose* poses, fb_lbm_stats* stats,
-                        bool sync_stats) {
+                        bool sync_stats, hipEvent_t* ev = nullptr) {
   int rc;
   // Prime f* from the pre-collision state when it is stale or the first pose differs.
   int g_first = L.g;
@@ -853,9 +855,12 @@ static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const 
       if ((rc = compute_geometry(L, g_next, poses[k + 1]))) return rc;
     }
     int fo = other3(L.cvalid ? L.cf : -1, fcur);
-    int oo = other3(L.cvalid ? L.co : -1, L.o);
+    // the outlet history only evolves with a non-periodic x axis (fb/solvers/lbm.py:628-640)
+    int oo = L.dom.periodic_x ? L.o : other3(L.cvalid ? L.co : -1, L.o);
+    if (ev) FB_HIP(hipEventRecord(ev[2 * k], L.ctx->stream));
     if ((rc = launch_step<P>(L, p, fcur, false, fo, L.o, oo, L.g, g_next, last, pre_out, moving)))
       return rc;
+    if (ev) FB_HIP(hipEventRecord(ev[2 * k + 1], L.ctx->stream));
     fcur = fo;
     L.f = fo;
     L.o = oo;
@@ -1050,8 +1055,8 @@ static int uncover_impl(Lattice& L, const fb_pose& pose, const double* feq_u) {
   }
 
 static int advance_dispatch(Lattice& L, const fb_lbm_params& p, int n, const fb_pose* poses, fb_lbm_stats* s,
-                            bool red) {
-  FB_DISPATCH(L.prec, advance_impl, L, p, n, poses, s, red);
+                            bool red, hipEvent_t* ev = nullptr) {
+  FB_DISPATCH(L.prec, advance_impl, L, p, n, poses, s, red, ev);
 }
 static int stats_dispatch(Lattice& L, const fb_lbm_params& p, fb_lbm_stats* s) {
   FB_DISPATCH(L.prec, stats_impl, L, p, s);
@@ -1254,6 +1259,38 @@ int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void*
   auto* L = reinterpret_cast<Lattice*>(lbm);
   FB_HIP(hipSetDevice(L->ctx->device));
   return fb::macro_dispatch(*L, velocity_scale, rho_host, vel_host);
+}
+
+int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* wall_ms, double* kernel_ms) {
+  FB_REQUIRE(lbm && params && wall_ms && kernel_ms, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(steps >= 1 && steps <= 1000000, FB_ERR_ARG, "steps out of range");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  // Prime outside the timed region so every timed launch is a lattice step.
+  if (!L->fstar_valid) {
+    int rc = fb::advance_dispatch(*L, *params, 1, nullptr, nullptr, false);
+    if (rc) return rc;
+  }
+  std::vector<hipEvent_t> ev(2 * (size_t)steps);
+  for (auto& e : ev) FB_HIP(hipEventCreate(&e));
+  FB_HIP(hipStreamSynchronize(L->ctx->stream));
+  int rc = fb::advance_dispatch(*L, *params, steps, nullptr, nullptr, false, ev.data());
+  if (rc == FB_OK) {
+    FB_HIP(hipEventSynchronize(ev.back()));
+    double tot = 0.0;
+    for (int k = 0; k < steps; ++k) {
+      float ms = 0.f;
+      FB_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      tot += ms;
+    }
+    float wall = 0.f;
+    FB_HIP(hipEventElapsedTime(&wall, ev.front(), ev.back()));
+    *wall_ms = wall;
+    *kernel_ms = tot;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps) {

@@ -1,0 +1,929 @@
+// Staggered MAC grid on gfx950: the Stable Fluids substep (fb/solvers/stable_fluids.py:424-494)
+// and the grid pieces PIC/FLIP shares (fb/core/grid.py).
+//
+// Layout: faces u (ny, nx+1) and v (ny+1, nx) row-major in the storage dtype; cell fields as
+// separate x/y planes (ny, nx). The pressure solve is FP64 (or FP32 sweeps with FP64
+// refinement in the mixed form) on (ny, nx). Arithmetic follows the reference's NumPy dtype
+// rules: Python-float scalars are rounded to the storage dtype, bilinear weights are FP64
+// (fb/core/interpolation.py:9-52), and the pressure gradient is applied in FP64 then rounded.
+#include <algorithm>
+#include <vector>
+
+#include "geometry.h"
+#include "poisson.h"
+
+namespace fb {
+
+#pragma clang fp contract(off)
+
+struct MacGeom {
+  int nx, ny, px, py;
+  double x0, y0, dx, dy;
+  // cell-centre linspace coefficients
+  double sx, stx, ex, sy, sty, ey;
+};
+
+inline MacGeom make_geom(const fb_domain& d) {
+  MacGeom g;
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.px = d.periodic_x;
+  g.py = d.periodic_y;
+  g.x0 = d.x0;
+  g.y0 = d.y0;
+  g.dx = (d.x1 - d.x0) / d.nx;
+  g.dy = (d.y1 - d.y0) / d.ny;
+  CellGrid c = make_cell_grid(d);
+  g.sx = c.sx;
+  g.stx = c.stx;
+  g.ex = c.ex;
+  g.sy = c.sy;
+  g.sty = c.sty;
+  g.ey = c.ey;
+  return g;
+}
+
+__device__ __forceinline__ double cx_at(const MacGeom& g, int i) { return linspace_at(i, g.nx, g.sx, g.stx, g.ex); }
+__device__ __forceinline__ double cy_at(const MacGeom& g, int j) { return linspace_at(j, g.ny, g.sy, g.sty, g.ey); }
+
+// np.mod for floats (Python sign convention).
+__device__ __forceinline__ double pymod(double a, double b) {
+  double m = fmod(a, b);
+  if (m != 0.0) {
+    if ((b < 0.0) != (m < 0.0)) m += b;
+  } else {
+    m = copysign(0.0, b);
+  }
+  return m;
+}
+
+// sample_staggered_scalar with offset (0.5, 0.5) on a (ny, nx) cell field
+// (fb/core/interpolation.py:9-52): FP64 weights, result rounded to T.
+template <class T>
+__device__ __forceinline__ T sample_cell(const MacGeom& g, const T* __restrict__ f, double px, double py) {
+  double gx = (px - g.x0) / g.dx - 0.5;
+  double gy = (py - g.y0) / g.dy - 0.5;
+  gx = g.px ? pymod(gx, (double)g.nx) : fmin(fmax(gx, 0.0), (double)g.nx - 1.0);
+  gy = g.py ? pymod(gy, (double)g.ny) : fmin(fmax(gy, 0.0), (double)g.ny - 1.0);
+  double fx = floor(gx), fy = floor(gy);
+  int ix0 = (int)fx, iy0 = (int)fy;
+  double tx = gx - fx, ty = gy - fy;
+  // in-bounds guard: np.mod can round up to the period for tiny negative arguments
+  ix0 = min(max(ix0, 0), g.nx - 1);
+  iy0 = min(max(iy0, 0), g.ny - 1);
+  int ix1 = g.px ? (ix0 + 1) % g.nx : min(ix0 + 1, g.nx - 1);
+  int iy1 = g.py ? (iy0 + 1) % g.ny : min(iy0 + 1, g.ny - 1);
+  double a = (double)f[(size_t)iy0 * g.nx + ix0], b = (double)f[(size_t)iy0 * g.nx + ix1];
+  double c = (double)f[(size_t)iy1 * g.nx + ix0], d = (double)f[(size_t)iy1 * g.nx + ix1];
+  double s = (1.0 - tx) * (1.0 - ty) * a + tx * (1.0 - ty) * b + (1.0 - tx) * ty * c + tx * ty * d;
+  return (T)s;
+}
+
+// rk2_backtrace (fb/core/grid.py:527-537) from point (px, py) through the field (fu, fv).
+template <class T>
+__device__ __forceinline__ void backtrace(const MacGeom& g, const T* fu, const T* fv, double px, double py, double dt,
+                                          double& ox, double& oy) {
+  T u0 = sample_cell(g, fu, px, py), v0 = sample_cell(g, fv, px, py);
+  T h = (T)(0.5 * dt);
+  double mx = px - (double)(h * u0);
+  double my = py - (double)(h * v0);
+  T um = sample_cell(g, fu, mx, my), vm = sample_cell(g, fv, mx, my);
+  T d = (T)dt;
+  ox = px - (double)(d * um);
+  oy = py - (double)(d * vm);
+}
+
+// faces_to_cell (fb/core/grid.py:25-29)
+template <class T>
+__global__ void __launch_bounds__(256) faces_to_cell_kernel(int nx, int ny, const T* __restrict__ u,
+                                                            const T* __restrict__ v, T* __restrict__ cu,
+                                                            T* __restrict__ cv) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= nx) return;
+  size_t i = (size_t)y * nx + x;
+  cu[i] = (T)0.5 * (u[(size_t)y * (nx + 1) + x] + u[(size_t)y * (nx + 1) + x + 1]);
+  cv[i] = (T)0.5 * (v[(size_t)y * nx + x] + v[(size_t)(y + 1) * nx + x]);
+}
+
+// cell_to_faces (fb/core/grid.py:32-44)
+template <class T>
+__global__ void __launch_bounds__(256) cell_to_faces_kernel(int nx, int ny, const T* __restrict__ cu,
+                                                            const T* __restrict__ cv, T* __restrict__ u,
+                                                            T* __restrict__ v) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (y < ny && x <= nx) {
+    T val;
+    if (x == 0) val = cu[(size_t)y * nx];
+    else if (x == nx) val = cu[(size_t)y * nx + nx - 1];
+    else val = (T)0.5 * (cu[(size_t)y * nx + x - 1] + cu[(size_t)y * nx + x]);
+    u[(size_t)y * (nx + 1) + x] = val;
+  }
+  if (y <= ny && x < nx) {
+    T val;
+    if (y == 0) val = cv[x];
+    else if (y == ny) val = cv[(size_t)(ny - 1) * nx + x];
+    else val = (T)0.5 * (cv[(size_t)(y - 1) * nx + x] + cv[(size_t)y * nx + x]);
+    v[(size_t)y * nx + x] = val;
+  }
+}
+
+// First semi-Lagrangian pass (fb/core/grid.py:512-515).
+template <class T>
+__global__ void __launch_bounds__(256) advect_first_kernel(MacGeom g, const T* __restrict__ cu,
+                                                           const T* __restrict__ cv, double dt, T* __restrict__ fu,
+                                                           T* __restrict__ fv) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= g.nx) return;
+  double px = cx_at(g, x), py = cy_at(g, y), dxp, dyp;
+  backtrace(g, cu, cv, px, py, dt, dxp, dyp);
+  size_t i = (size_t)y * g.nx + x;
+  fu[i] = sample_cell(g, cu, dxp, dyp);
+  fv[i] = sample_cell(g, cv, dxp, dyp);
+}
+
+// Limited MacCormack correction (fb/core/grid.py:518-524, 540-569).
+template <class T>
+__global__ void __launch_bounds__(256) advect_correct_kernel(MacGeom g, const T* __restrict__ cu,
+                                                             const T* __restrict__ cv, const T* __restrict__ fu,
+                                                             const T* __restrict__ fv, double dt, T* __restrict__ ou,
+                                                             T* __restrict__ ov) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= g.nx) return;
+  double px = cx_at(g, x), py = cy_at(g, y), bx, by;
+  backtrace(g, fu, fv, px, py, -dt, bx, by);
+  T wu = sample_cell(g, fu, bx, by), wv = sample_cell(g, fv, bx, by);
+  size_t i = (size_t)y * g.nx + x;
+  T fu0 = fu[i], fv0 = fv[i];
+  T cu0 = cu[i], cv0 = cv[i];
+  T corr_u = fu0 + (T)0.5 * (cu0 - wu);
+  T corr_v = fv0 + (T)0.5 * (cv0 - wv);
+  T lo_u = cu0, hi_u = cu0, lo_v = cv0, hi_v = cv0;
+  for (int oy = -1; oy <= 1; ++oy) {
+    int r = g.py ? (y + oy + g.ny) % g.ny : min(max(y + oy, 0), g.ny - 1);
+    for (int ox = -1; ox <= 1; ++ox) {
+      int c = g.px ? (x + ox + g.nx) % g.nx : min(max(x + ox, 0), g.nx - 1);
+      size_t k = (size_t)r * g.nx + c;
+      T a = cu[k], b = cv[k];
+      lo_u = a < lo_u ? a : lo_u;
+      hi_u = a > hi_u ? a : hi_u;
+      lo_v = b < lo_v ? b : lo_v;
+      hi_v = b > hi_v ? b : hi_v;
+    }
+  }
+  lo_u = fu0 < lo_u ? fu0 : lo_u;
+  hi_u = fu0 > hi_u ? fu0 : hi_u;
+  lo_v = fv0 < lo_v ? fv0 : lo_v;
+  hi_v = fv0 > hi_v ? fv0 : hi_v;
+  T ru = corr_u < lo_u ? lo_u : corr_u;
+  ru = ru > hi_u ? hi_u : ru;
+  T rv = corr_v < lo_v ? lo_v : corr_v;
+  rv = rv > hi_v ? hi_v : rv;
+  ou[i] = ru;
+  ov[i] = rv;
+}
+
+// Max |a| over a field into keys[slot] (order-free, exact).
+template <class T>
+__global__ void __launch_bounds__(256) absmax_kernel(size_t n, const T* __restrict__ a, unsigned long long* keys,
+                                                     int slot) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double m = 0.0;
+  if (i < n) m = (double)fabs(a[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(&keys[slot], dkey(m));
+}
+
+// One Jacobi sweep of (I - nu dt Lap) u = u0 (fb/core/grid.py:245-267) on a logical W x H
+// field (row pitch P), edge replication or periodic wrap; blockIdx.z selects the component.
+// Sweep k (1-based) is skipped once the component's sweep k-1 met the tolerance.
+template <class T>
+struct JacobiArgs {
+  const T* in[2];
+  const T* orig[2];
+  T* out[2];
+  int W[2], H[2], P[2];
+  int px, py;
+  double ax, ay, den;
+  double scale[2];
+  double tol;
+  int k;
+  unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
+  int stride;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
+  const int c = blockIdx.z;
+  if (a.k > 1) {
+    double prev = dkey_inv(a.resid[c * a.stride + a.k - 1]);
+    if (prev / a.scale[c] <= a.tol) return;
+  }
+  const int W = a.W[c], H = a.H[c], P = a.P[c];
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  double m = 0.0;
+  if (x < W && y < H) {
+    const T* r = a.in[c];
+    auto at = [&](int xx, int yy) { return r[(size_t)yy * P + xx]; };
+    int xl = a.px ? (x - 1 + W) % W : max(x - 1, 0);
+    int xr = a.px ? (x + 1) % W : min(x + 1, W - 1);
+    int yd = a.py ? (y - 1 + H) % H : max(y - 1, 0);
+    int yu = a.py ? (y + 1) % H : min(y + 1, H - 1);
+    const T ax = (T)a.ax, ay = (T)a.ay, den = (T)a.den;
+    T o = a.orig[c][(size_t)y * P + x];
+    T upd = (o + ax * (at(xl, y) + at(xr, y)) + ay * (at(x, yd) + at(x, yu))) / den;
+    a.out[c][(size_t)y * P + x] = upd;
+    m = (double)fabs(upd - at(x, y));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(&a.resid[c * a.stride + a.k], dkey(m));
+}
+
+// Periodic duplicate endpoint copy after face diffusion (fb/core/grid.py:271-274).
+template <class T>
+__global__ void dup_copy_kernel(T* __restrict__ f, int rows, int cols, int pitch, int dup_x, int dup_y) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dup_x && t < rows) f[(size_t)t * pitch + cols - 1] = f[(size_t)t * pitch];
+  if (dup_y && t < cols) f[(size_t)(rows - 1) * pitch + t] = f[t];
+}
+
+// Wall velocity of a cell centre (fb/core/geometry.py:131-139), FP64.
+__device__ __forceinline__ void wall_at(const MacGeom& g, int x, int y, double w, double pvx, double pvy, double& wu,
+                                        double& wv) {
+  double rx = cx_at(g, x) - pvx, ry = cy_at(g, y) - pvy;
+  wu = (-w) * ry;
+  wv = w * rx;
+}
+
+struct BcArgs {
+  MacGeom g;
+  double ux, uy;  // freestream (Python floats)
+  int channel;
+  double w, pvx, pvy;  // angular velocity (rad/s) and pivot
+  const uint8_t* solid;
+};
+
+// apply_domain_boundaries then enforce_solid_faces on u faces, out of place
+// (fb/core/grid.py:47-97).
+template <class T>
+__global__ void __launch_bounds__(256) bc_u_kernel(BcArgs b, const T* __restrict__ uin, T* __restrict__ uout) {
+  const MacGeom& g = b.g;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nx
+  int j = blockIdx.y;
+  if (i > g.nx) return;
+  const size_t P = g.nx + 1;
+  auto raw = [&](int jj, int ii) { return uin[(size_t)jj * P + ii]; };
+  T val = raw(j, i);
+  if (g.px) {
+    if (i == 0 || i == g.nx) val = (T)0.5 * (raw(j, 0) + raw(j, g.nx));
+  } else {
+    if (i == 0) val = (T)b.ux;
+    else if (i == g.nx) val = raw(j, g.nx - 1);
+  }
+  if (!g.py && (j == 0 || j == g.ny - 1)) val = b.channel ? (T)0 : (T)b.ux;
+  // solid faces: a face touching a solid cell takes the face-averaged wall velocity
+  bool left = i > 0 && b.solid[(size_t)j * g.nx + i - 1];
+  bool right = i < g.nx && b.solid[(size_t)j * g.nx + i];
+  if (left || right) {
+    double wl, wr, tmp;
+    if (i == 0) {
+      wall_at(g, 0, j, b.w, b.pvx, b.pvy, wl, tmp);
+      val = (T)wl;
+    } else if (i == g.nx) {
+      wall_at(g, g.nx - 1, j, b.w, b.pvx, b.pvy, wl, tmp);
+      val = (T)wl;
+    } else {
+      wall_at(g, i - 1, j, b.w, b.pvx, b.pvy, wl, tmp);
+      wall_at(g, i, j, b.w, b.pvx, b.pvy, wr, tmp);
+      val = (T)(0.5 * (wl + wr));
+    }
+  }
+  uout[(size_t)j * P + i] = val;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) bc_v_kernel(BcArgs b, const T* __restrict__ vin, T* __restrict__ vout) {
+  const MacGeom& g = b.g;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nx-1
+  int j = blockIdx.y;                             // 0..ny
+  if (i >= g.nx) return;
+  const size_t P = g.nx;
+  auto raw = [&](int jj, int ii) { return vin[(size_t)jj * P + ii]; };
+  // x-part (fb/core/grid.py:60-64)
+  auto X = [&](int jj, int ii) -> T {
+    if (!g.px) {
+      if (ii == 0) return (T)b.uy;
+      if (ii == g.nx - 1) return raw(jj, g.nx - 2);
+    }
+    return raw(jj, ii);
+  };
+  T val = X(j, i);
+  if (j == 0 || j == g.ny) {
+    if (g.py) val = (T)0.5 * (X(0, i) + X(g.ny, i));
+    else if (b.channel) val = (T)0;
+    else val = (T)b.uy;
+  }
+  bool below = j > 0 && b.solid[(size_t)(j - 1) * g.nx + i];
+  bool above = j < g.ny && b.solid[(size_t)j * g.nx + i];
+  if (below || above) {
+    double tmp, wb, wa;
+    if (j == 0) {
+      wall_at(g, i, 0, b.w, b.pvx, b.pvy, tmp, wb);
+      val = (T)wb;
+    } else if (j == g.ny) {
+      wall_at(g, i, g.ny - 1, b.w, b.pvx, b.pvy, tmp, wb);
+      val = (T)wb;
+    } else {
+      wall_at(g, i, j - 1, b.w, b.pvx, b.pvy, tmp, wb);
+      wall_at(g, i, j, b.w, b.pvx, b.pvy, tmp, wa);
+      val = (T)(0.5 * (wb + wa));
+    }
+  }
+  vout[(size_t)j * P + i] = val;
+}
+
+// rhs = fluid ? -div / max(dt, 1e-12) : 0 (fb/core/grid.py:148-150), dtype arithmetic, then FP64.
+template <class T>
+__global__ void __launch_bounds__(256) rhs_kernel(MacGeom g, const T* __restrict__ u, const T* __restrict__ v,
+                                                  const uint8_t* __restrict__ solid, double dt,
+                                                  double* __restrict__ rhs) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x >= g.nx) return;
+  size_t i = (size_t)y * g.nx + x;
+  T du = u[(size_t)y * (g.nx + 1) + x + 1] - u[(size_t)y * (g.nx + 1) + x];
+  T dv = v[(size_t)(y + 1) * g.nx + x] - v[(size_t)y * g.nx + x];
+  T div = du / (T)g.dx + dv / (T)g.dy;
+  T r = solid[i] ? (T)0 : (-div) / (T)fmax(dt, 1.0e-12);
+  rhs[i] = (double)r;
+}
+
+// Pressure gradient (fb/core/grid.py:164-171), FP64 then rounded; out of place.
+template <class T>
+__global__ void __launch_bounds__(256) grad_u_kernel(MacGeom g, const T* __restrict__ uin, const double* __restrict__ p,
+                                                     double dt, T* __restrict__ uout) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i > g.nx) return;
+  const size_t P = g.nx + 1;
+  const double* pr = p + (size_t)j * g.nx;
+  double val = (double)uin[(size_t)j * P + i];
+  if (i > 0 && i < g.nx) {
+    val = val - dt * (pr[i] - pr[i - 1]) / g.dx;
+  } else if (g.px) {
+    // u[:, 0] -= dt (p[:, 0] - p[:, -1]) / dx; u[:, -1] = u[:, 0]
+    val = (double)uin[(size_t)j * P] - dt * (pr[0] - pr[g.nx - 1]) / g.dx;
+  }
+  uout[(size_t)j * P + i] = (T)val;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) grad_v_kernel(MacGeom g, const T* __restrict__ vin, const double* __restrict__ p,
+                                                     double dt, T* __restrict__ vout) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= g.nx) return;
+  const size_t P = g.nx;
+  double val = (double)vin[(size_t)j * P + i];
+  if (j > 0 && j < g.ny) {
+    val = val - dt * (p[(size_t)j * g.nx + i] - p[(size_t)(j - 1) * g.nx + i]) / g.dy;
+  } else if (g.py) {
+    val = (double)vin[i] - dt * (p[i] - p[(size_t)(g.ny - 1) * g.nx + i]) / g.dy;
+  }
+  vout[(size_t)j * P + i] = (T)val;
+}
+
+// Reductions over the MAC state (keys: order-free max; counts):
+// [0] max|u|,|v| faces  [1] max wall speed on solid cells  [2] max cell speed (dtype norm)
+// [3] fluid divergence Linf  [4] solid-face leakage  [5] non-finite faces  [6] non-finite wall
+struct StatArgs {
+  MacGeom g;
+  double w, pvx, pvy;
+  const uint8_t* solid;
+  unsigned long long* keys;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __restrict__ u, const T* __restrict__ v) {
+  const MacGeom& g = s.g;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  double face = 0.0, wsp = 0.0, csp = 0.0, dv = 0.0, leak = 0.0;
+  long long nf = 0, nfw = 0;
+  const size_t PU = g.nx + 1;
+  if (x <= g.nx && y <= g.ny) {
+    if (y < g.ny) {
+      T a = u[(size_t)y * PU + x];
+      if (!isfinite((double)a)) nf++;
+      else face = fmax(face, (double)fabs(a));
+    }
+    if (x < g.nx) {
+      T b = v[(size_t)y * g.nx + x];
+      if (!isfinite((double)b)) nf++;
+      else face = fmax(face, (double)fabs(b));
+    }
+    if (x < g.nx && y < g.ny) {
+      size_t i = (size_t)y * g.nx + x;
+      bool sol = s.solid[i];
+      double wu, wv;
+      wall_at(g, x, y, s.w, s.pvx, s.pvy, wu, wv);
+      if (!isfinite(wu) || !isfinite(wv)) nfw++;
+      if (sol) wsp = sqrt(wu * wu + wv * wv);
+      T ul = u[(size_t)y * PU + x], ur = u[(size_t)y * PU + x + 1];
+      T vb = v[i], va = v[(size_t)(y + 1) * g.nx + x];
+      T cu = (T)0.5 * (ul + ur), cv = (T)0.5 * (vb + va);
+      csp = (double)sqrt(cu * cu + cv * cv);
+      if (!sol) {
+        T div = (ur - ul) / (T)g.dx + (va - vb) / (T)g.dy;
+        dv = (double)fabs(div);
+      }
+      // leakage on the right u face and the upper v face of this cell (interior interfaces)
+      if (x + 1 < g.nx && (sol != (bool)s.solid[i + 1])) {
+        double w1u, w1v, w2u, w2v;
+        wall_at(g, x, y, s.w, s.pvx, s.pvy, w1u, w1v);
+        wall_at(g, x + 1, y, s.w, s.pvx, s.pvy, w2u, w2v);
+        leak = fmax(leak, fabs((double)ur - 0.5 * (w1u + w2u)));
+      }
+      if (y + 1 < g.ny && (sol != (bool)s.solid[i + g.nx])) {
+        double w1u, w1v, w2u, w2v;
+        wall_at(g, x, y, s.w, s.pvx, s.pvy, w1u, w1v);
+        wall_at(g, x, y + 1, s.w, s.pvx, s.pvy, w2u, w2v);
+        leak = fmax(leak, fabs((double)va - 0.5 * (w1v + w2v)));
+      }
+    }
+  }
+  face = wave_max(face);
+  wsp = wave_max(wsp);
+  csp = wave_max(csp);
+  dv = wave_max(dv);
+  leak = wave_max(leak);
+  nf = wave_sum_ll(nf);
+  nfw = wave_sum_ll(nfw);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&s.keys[0], dkey(face));
+    atomicMax(&s.keys[1], dkey(wsp));
+    atomicMax(&s.keys[2], dkey(csp));
+    atomicMax(&s.keys[3], dkey(dv));
+    atomicMax(&s.keys[4], dkey(leak));
+    if (nf) atomicAdd(&s.keys[5], (unsigned long long)nf);
+    if (nfw) atomicAdd(&s.keys[6], (unsigned long long)nfw);
+  }
+}
+
+// ---------------------------------------------------------------- host handle
+
+struct MacGrid {
+  Ctx* ctx = nullptr;
+  fb_domain dom{};
+  fb_foil foil{};
+  MacGeom g{};
+  int prec = FB_F64;
+  size_t sz = 8;
+  size_t N = 0, NU = 0, NV = 0;
+  void *U[2] = {}, *V[2] = {};  // current faces and scratch
+  int cur = 0;
+  void *CU[4] = {}, *CV[4] = {};  // cell planes: velocity, first, corrected, spare
+  void *DU[3] = {}, *DV[3] = {};  // diffusion: orig, ping, pong (face-sized to serve PIC)
+  void *CKU = nullptr, *CKV = nullptr;  // checkpoint faces
+  double* sdf = nullptr;
+  uint8_t* solid = nullptr;
+  uint8_t* ck_solid = nullptr;
+  double angle = 0.0, ck_angle = 0.0;
+  bool geom_valid = false, ck_valid = false;
+  double* rhs = nullptr;
+  double* pres = nullptr;
+  unsigned long long* keys = nullptr;
+  unsigned long long* host_keys = nullptr;
+  unsigned long long* resid = nullptr;  // 2 x (maxit + 2)
+  unsigned long long* host_resid = nullptr;
+  int resid_cap = 0;
+  Poisson poisson;
+  bool poisson_ready = false;
+};
+
+static int mac_geometry(MacGrid& M, const fb_pose& pose) {
+  int rc = launch_cell_sdf(M.ctx->stream, make_cell_grid(M.dom), make_naca(M.foil, pose), M.sdf, M.solid);
+  if (rc) return rc;
+  M.angle = pose.angle_degrees;
+  M.geom_valid = true;
+  if ((rc = M.poisson.setup(M.ctx->stream, M.dom.nx, M.dom.ny, M.g.dx, M.g.dy, M.dom.periodic_x, M.dom.periodic_y,
+                            M.solid)))
+    return rc;
+  M.poisson_ready = true;
+  return FB_OK;
+}
+
+static int read_keys(MacGrid& M, int n) {
+  FB_HIP(hipMemcpyAsync(M.host_keys, M.keys, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, M.ctx->stream));
+  FB_HIP(hipStreamSynchronize(M.ctx->stream));
+  return FB_OK;
+}
+
+template <class T>
+static int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
+  FB_HIP(hipMemsetAsync(M.keys, 0, 8 * sizeof(unsigned long long), M.ctx->stream));
+  StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.keys};
+  dim3 grid((M.dom.nx + 1 + 255) / 256, M.dom.ny + 1);
+  hipLaunchKernelGGL(mac_stats_kernel<T>, grid, dim3(256), 0, M.ctx->stream, s, (const T*)M.U[M.cur],
+                     (const T*)M.V[M.cur]);
+  FB_HIP(hipGetLastError());
+  int rc = read_keys(M, 7);
+  if (rc) return rc;
+  auto val = [&](int k) { return M.host_keys[k] ? dkey_inv(M.host_keys[k]) : 0.0; };
+  st->max_face_speed = val(0);
+  st->max_wall_speed = val(1);
+  st->max_cell_speed = val(2);
+  st->divergence_linf = val(3);
+  st->solid_leakage = val(4);
+  st->nonfinite_faces = (int64_t)M.host_keys[5];
+  st->nonfinite_wall = (int64_t)M.host_keys[6];
+  return FB_OK;
+}
+
+// Jacobi implicit viscosity on two fields (cell planes, or u/v faces for PIC).
+template <class T>
+static int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int W1, int H1, int P1,
+                   double nu, double dt, double tol, int maxit, T** result, fb_sf_report* rep) {
+  hipStream_t s = M.ctx->stream;
+  const int px = M.dom.periodic_x, py = M.dom.periodic_y;
+  if (nu <= 0.0) {
+    result[0] = (T*)in0;
+    result[1] = (T*)in1;
+    rep->viscosity_iterations = 0;
+    rep->viscosity_residual = 0.0;
+    rep->viscosity_converged = 1;
+    return FB_OK;
+  }
+  if (maxit + 2 > M.resid_cap) {
+    if (M.resid) (void)hipFree(M.resid);
+    if (M.host_resid) (void)hipHostFree(M.host_resid);
+    M.resid_cap = maxit + 2;
+    FB_HIP(hipMalloc((void**)&M.resid, 2 * M.resid_cap * sizeof(unsigned long long)));
+    FB_HIP(hipHostMalloc((void**)&M.host_resid, 2 * M.resid_cap * sizeof(unsigned long long), 0));
+  }
+  FB_HIP(hipMemsetAsync(M.resid, 0, 2 * M.resid_cap * sizeof(unsigned long long), s));
+  // scale = max(max |original|, 1)
+  FB_HIP(hipMemsetAsync(M.keys, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H0 * P0 + 255) / 256), dim3(256), 0, s,
+                     (size_t)H0 * P0, in0, M.keys, 0);
+  hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H1 * P1 + 255) / 256), dim3(256), 0, s,
+                     (size_t)H1 * P1, in1, M.keys, 1);
+  FB_HIP(hipGetLastError());
+  int rc = read_keys(M, 2);
+  if (rc) return rc;
+  JacobiArgs<T> a{};
+  a.orig[0] = in0;
+  a.orig[1] = in1;
+  a.W[0] = W0;
+  a.H[0] = H0;
+  a.P[0] = P0;
+  a.W[1] = W1;
+  a.H[1] = H1;
+  a.P[1] = P1;
+  a.px = px;
+  a.py = py;
+  a.ax = nu * dt / (M.g.dx * M.g.dx);
+  a.ay = nu * dt / (M.g.dy * M.g.dy);
+  a.den = 1.0 + 2.0 * a.ax + 2.0 * a.ay;
+  a.scale[0] = std::max(dkey_inv(M.host_keys[0]), 1.0);
+  a.scale[1] = std::max(dkey_inv(M.host_keys[1]), 1.0);
+  a.tol = tol;
+  a.resid = M.resid;
+  a.stride = M.resid_cap;
+  T* buf[2][2] = {{(T*)M.DU[1], (T*)M.DU[2]}, {(T*)M.DV[1], (T*)M.DV[2]}};
+  const int Wm = std::max(W0, W1), Hm = std::max(H0, H1);
+  dim3 grid((Wm + 255) / 256, Hm, 2);
+  int done[2] = {0, 0};
+  double fres[2] = {INFINITY, INFINITY};
+  int k = 1;
+  int chunk = 8;
+  while (k <= maxit && !(done[0] && done[1])) {
+    int kend = std::min(maxit, k + chunk - 1);
+    for (int kk = k; kk <= kend; ++kk) {
+      for (int c = 0; c < 2; ++c) {
+        a.in[c] = (kk == 1) ? a.orig[c] : buf[c][(kk - 1) % 2];
+        a.out[c] = buf[c][kk % 2];
+      }
+      a.k = kk;
+      hipLaunchKernelGGL(jacobi_sweep_kernel<T>, grid, dim3(256), 0, s, a);
+      FB_HIP(hipGetLastError());
+    }
+    FB_HIP(hipMemcpyAsync(M.host_resid, M.resid, 2 * M.resid_cap * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    FB_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < 2; ++c) {
+      if (done[c]) continue;
+      for (int kk = k; kk <= kend; ++kk) {
+        double r = dkey_inv(M.host_resid[c * M.resid_cap + kk]) / a.scale[c];
+        fres[c] = r;
+        if (r <= tol) {
+          done[c] = kk;
+          break;
+        }
+      }
+    }
+    k = kend + 1;
+    chunk = std::min(chunk * 2, 64);
+  }
+  int its[2];
+  for (int c = 0; c < 2; ++c) {
+    its[c] = done[c] ? done[c] : maxit;
+    result[c] = buf[c][its[c] % 2];
+  }
+  rep->viscosity_iterations = std::max(its[0], its[1]);
+  rep->viscosity_residual = std::max(fres[0], fres[1]);
+  rep->viscosity_converged = (fres[0] <= tol && fres[1] <= tol) ? 1 : 0;
+  return FB_OK;
+}
+
+template <class T>
+static int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
+  hipStream_t s = M.ctx->stream;
+  int rc;
+  // projection CFL guard (fb/solvers/stable_fluids.py:256-298)
+  fb_mac_stats st{};
+  if ((rc = mac_stats<T>(M, p.wall_omega_rad, &st))) return rc;
+  if (st.nonfinite_faces || st.nonfinite_wall) {
+    rep->status = FB_SF_NONFINITE_PROJECTION_INPUT;
+    return FB_OK;
+  }
+  double pc = std::max(st.max_face_speed, st.max_wall_speed) * dt / std::min(M.g.dx, M.g.dy);
+  rep->projection_cfl = pc;
+  if (pc > p.cfl_limit) {
+    rep->status = FB_SF_PROJECTION_CFL;
+    return FB_OK;
+  }
+  BcArgs b{M.g, p.freestream_x, p.freestream_y, p.channel_walls, p.wall_omega_rad, M.foil.pivot_x, M.foil.pivot_y,
+           M.solid};
+  const int a = M.cur, o = 1 - M.cur;
+  dim3 gu((M.dom.nx + 1 + 255) / 256, M.dom.ny), gv((M.dom.nx + 255) / 256, M.dom.ny + 1),
+      gc((M.dom.nx + 255) / 256, M.dom.ny);
+  hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
+  hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)M.V[a], (T*)M.V[o]);
+  hipLaunchKernelGGL(rhs_kernel<T>, gc, dim3(256), 0, s, M.g, (const T*)M.U[o], (const T*)M.V[o], M.solid, dt,
+                     M.rhs);
+  FB_HIP(hipGetLastError());
+  PoissonReport pr;
+  if ((rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr))) return rc;
+  rep->pressure_iterations = pr.iterations;
+  rep->pressure_residual = pr.relative_residual;
+  rep->pressure_info = pr.info;
+  rep->pressure_refinements = pr.refinements;
+  if (pr.failure) {
+    rep->status = pr.failure == 1 ? FB_SF_NONFINITE_RHS
+                  : pr.failure == 2 ? FB_SF_RHS_RANGE
+                  : pr.failure == 3 ? FB_SF_NONFINITE_ITERATE
+                                    : FB_SF_CG_BREAKDOWN;
+    return FB_OK;
+  }
+  hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
+  hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
+  hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
+  hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)M.V[a], (T*)M.V[o]);
+  FB_HIP(hipGetLastError());
+  M.cur = o;
+  rep->pressure_converged =
+      (pr.info == 0 && pr.relative_residual <= p.tolerance * (1.0 + 1.0e-6)) ? 1 : 0;
+  if (!rep->pressure_converged) {
+    rep->status = FB_SF_PROJECTION_NOT_CONVERGED;
+    return FB_OK;
+  }
+  // non-finite velocity after projection
+  if ((rc = mac_stats<T>(M, p.wall_omega_rad, &st))) return rc;
+  if (st.nonfinite_faces) rep->status = FB_SF_NONFINITE_PROJECTION_OUTPUT;
+  return FB_OK;
+}
+
+template <class T>
+static int substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_report* rep) {
+  hipStream_t s = M.ctx->stream;
+  int rc;
+  const int nx = M.dom.nx, ny = M.dom.ny;
+  dim3 gc((nx + 255) / 256, ny);
+  T* cu = (T*)M.CU[0];
+  T* cv = (T*)M.CV[0];
+  hipLaunchKernelGGL(faces_to_cell_kernel<T>, gc, dim3(256), 0, s, nx, ny, (const T*)M.U[M.cur],
+                     (const T*)M.V[M.cur], cu, cv);
+  hipLaunchKernelGGL(advect_first_kernel<T>, gc, dim3(256), 0, s, M.g, (const T*)cu, (const T*)cv, p.dt,
+                     (T*)M.CU[1], (T*)M.CV[1]);
+  FB_HIP(hipGetLastError());
+  const T* au = (const T*)M.CU[1];
+  const T* av = (const T*)M.CV[1];
+  if (p.maccormack) {
+    hipLaunchKernelGGL(advect_correct_kernel<T>, gc, dim3(256), 0, s, M.g, (const T*)cu, (const T*)cv,
+                       (const T*)M.CU[1], (const T*)M.CV[1], p.dt, (T*)M.CU[2], (T*)M.CV[2]);
+    FB_HIP(hipGetLastError());
+    au = (const T*)M.CU[2];
+    av = (const T*)M.CV[2];
+  }
+  T* res[2];
+  if ((rc = diffuse<T>(M, au, av, nx, ny, nx, nx, ny, nx, p.viscosity, p.dt, p.tolerance, p.max_iterations, res, rep)))
+    return rc;
+  dim3 gf((nx + 1 + 255) / 256, ny + 1);
+  hipLaunchKernelGGL(cell_to_faces_kernel<T>, gf, dim3(256), 0, s, nx, ny, (const T*)res[0], (const T*)res[1],
+                     (T*)M.U[M.cur], (T*)M.V[M.cur]);
+  FB_HIP(hipGetLastError());
+  if (!rep->viscosity_converged) {
+    rep->status = FB_SF_VISCOSITY_NOT_CONVERGED;
+    return FB_OK;
+  }
+  if (!M.geom_valid || pose.angle_degrees != M.angle) {
+    if ((rc = mac_geometry(M, pose))) return rc;
+  }
+  return project<T>(M, p, p.dt, rep);
+}
+
+static void free_mac(MacGrid* M) {
+  for (void* ptr : {M->U[0], M->U[1], M->V[0], M->V[1], M->CU[0], M->CU[1], M->CU[2], M->CU[3], M->CV[0], M->CV[1],
+                    M->CV[2], M->CV[3], M->DU[0], M->DU[1], M->DU[2], M->DV[0], M->DV[1], M->DV[2], M->CKU, M->CKV,
+                    (void*)M->sdf, (void*)M->solid, (void*)M->ck_solid, (void*)M->rhs, (void*)M->pres,
+                    (void*)M->keys, (void*)M->resid})
+    if (ptr) (void)hipFree(ptr);
+  if (M->host_keys) (void)hipHostFree(M->host_keys);
+  if (M->host_resid) (void)hipHostFree(M->host_resid);
+  delete M;
+}
+
+}  // namespace fb
+
+using fb::MacGrid;
+
+#define MAC_DISPATCH(M, fn, ...) \
+  ((M)->prec == FB_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+int fb_mac_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, fb_mac** out) {
+  FB_REQUIRE(ctx && dom && foil && out, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(dom->nx >= 4 && dom->ny >= 4, FB_ERR_ARG, "each resolution axis must contain at least four cells");
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "MAC storage must be FB_F64 or FB_F32");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  auto* M = new MacGrid();
+  M->ctx = c;
+  M->dom = *dom;
+  M->foil = *foil;
+  M->g = fb::make_geom(*dom);
+  M->prec = precision;
+  M->sz = precision == FB_F64 ? 8 : 4;
+  M->N = (size_t)dom->nx * dom->ny;
+  M->NU = (size_t)dom->ny * (dom->nx + 1);
+  M->NV = (size_t)(dom->ny + 1) * dom->nx;
+  const size_t NF = std::max(M->NU, M->NV);
+  hipError_t e = hipSuccess;
+  auto A = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  for (int k = 0; k < 2; ++k) {
+    A(&M->U[k], M->NU * M->sz);
+    A(&M->V[k], M->NV * M->sz);
+  }
+  for (int k = 0; k < 4; ++k) {
+    A(&M->CU[k], M->N * M->sz);
+    A(&M->CV[k], M->N * M->sz);
+  }
+  for (int k = 0; k < 3; ++k) {
+    A(&M->DU[k], NF * M->sz);
+    A(&M->DV[k], NF * M->sz);
+  }
+  A(&M->CKU, M->NU * M->sz);
+  A(&M->CKV, M->NV * M->sz);
+  A((void**)&M->sdf, M->N * sizeof(double));
+  A((void**)&M->solid, M->N);
+  A((void**)&M->ck_solid, M->N);
+  A((void**)&M->rhs, M->N * sizeof(double));
+  A((void**)&M->pres, M->N * sizeof(double));
+  A((void**)&M->keys, 16 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&M->host_keys, 16 * sizeof(unsigned long long), 0);
+  if (e != hipSuccess) {
+    fb::set_error("device allocation failed: %s", hipGetErrorString(e));
+    fb::free_mac(M);
+    return FB_ERR_NOMEM;
+  }
+  *out = reinterpret_cast<fb_mac*>(M);
+  return FB_OK;
+}
+
+int fb_mac_destroy(fb_mac* mac) {
+  if (!mac) return FB_OK;
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  fb::free_mac(M);
+  return FB_OK;
+}
+
+int fb_mac_set_faces(fb_mac* mac, const void* u_host, const void* v_host) {
+  FB_REQUIRE(mac && u_host && v_host, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  FB_HIP(hipMemcpyAsync(M->U[M->cur], u_host, M->NU * M->sz, hipMemcpyHostToDevice, M->ctx->stream));
+  FB_HIP(hipMemcpyAsync(M->V[M->cur], v_host, M->NV * M->sz, hipMemcpyHostToDevice, M->ctx->stream));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_get_faces(fb_mac* mac, void* u_host, void* v_host) {
+  FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  if (u_host) FB_HIP(hipMemcpyAsync(u_host, M->U[M->cur], M->NU * M->sz, hipMemcpyDeviceToHost, M->ctx->stream));
+  if (v_host) FB_HIP(hipMemcpyAsync(v_host, M->V[M->cur], M->NV * M->sz, hipMemcpyDeviceToHost, M->ctx->stream));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_set_pose(fb_mac* mac, const fb_pose* pose) {
+  FB_REQUIRE(mac && pose, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  return fb::mac_geometry(*M, *pose);
+}
+
+int fb_mac_get_geometry(fb_mac* mac, double* sdf_host, uint8_t* solid_host) {
+  FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  if (sdf_host) FB_HIP(hipMemcpyAsync(sdf_host, M->sdf, M->N * sizeof(double), hipMemcpyDeviceToHost, M->ctx->stream));
+  if (solid_host) FB_HIP(hipMemcpyAsync(solid_host, M->solid, M->N, hipMemcpyDeviceToHost, M->ctx->stream));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_get_pressure(fb_mac* mac, double* p_host) {
+  FB_REQUIRE(mac && p_host, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  FB_HIP(hipMemcpyAsync(p_host, M->pres, M->N * sizeof(double), hipMemcpyDeviceToHost, M->ctx->stream));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_checkpoint(fb_mac* mac) {
+  FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  hipStream_t s = M->ctx->stream;
+  FB_HIP(hipMemcpyAsync(M->CKU, M->U[M->cur], M->NU * M->sz, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(M->CKV, M->V[M->cur], M->NV * M->sz, hipMemcpyDeviceToDevice, s));
+  if (M->geom_valid) FB_HIP(hipMemcpyAsync(M->ck_solid, M->solid, M->N, hipMemcpyDeviceToDevice, s));
+  M->ck_angle = M->angle;
+  M->ck_valid = true;
+  return FB_OK;
+}
+
+int fb_mac_rollback(fb_mac* mac) {
+  FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->ck_valid, FB_ERR_STATE, "no checkpoint to roll back to");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  hipStream_t s = M->ctx->stream;
+  FB_HIP(hipMemcpyAsync(M->U[M->cur], M->CKU, M->NU * M->sz, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(M->V[M->cur], M->CKV, M->NV * M->sz, hipMemcpyDeviceToDevice, s));
+  if (M->geom_valid && M->angle != M->ck_angle) {
+    fb_pose pose{};
+    pose.angle_degrees = M->ck_angle;
+    // restore the checkpointed mask and rebuild the operator from it
+    FB_HIP(hipMemcpyAsync(M->solid, M->ck_solid, M->N, hipMemcpyDeviceToDevice, s));
+    M->angle = M->ck_angle;
+    int rc = M->poisson.setup(s, M->dom.nx, M->dom.ny, M->g.dx, M->g.dy, M->dom.periodic_x, M->dom.periodic_y,
+                              M->solid);
+    if (rc) return rc;
+    // the signed distance is recomputed lazily at the next pose change
+  }
+  FB_HIP(hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, fb_sf_report* report) {
+  FB_REQUIRE(mac && params && pose && report, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  *report = fb_sf_report{};
+  return MAC_DISPATCH(M, fb::substep, *M, *params, *pose, report);
+}
+
+int fb_mac_project(fb_mac* mac, const fb_sf_params* params, fb_sf_report* report) {
+  FB_REQUIRE(mac && params && report, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  *report = fb_sf_report{};
+  return MAC_DISPATCH(M, fb::project, *M, *params, params->dt, report);
+}
+
+int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats) {
+  FB_REQUIRE(mac && stats, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  return MAC_DISPATCH(M, fb::mac_stats, *M, wall_omega_rad, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Masked pressure Poisson solve on gfx950: SciPy-sequence preconditioned CG with the
+// reference's DST-I fast-Poisson preconditioner (fb/core/_scipy_adapter.py:16-161).
+#pragma once
+
+#include <vector>
+
+#include "fb_common.h"
+
+namespace fb {
+
+// Result of one solve (mirrors the reference's (info, iterations, relative residual)).
+struct PoissonReport {
+  int info = 0;                    // 0 converged; maxiter when exhausted (SciPy convention)
+  int iterations = 0;              // completed CG iterations (the reference's callback count)
+  double relative_residual = 0.0;  // ||b - A x|| / max(||b||, eps), recomputed in FP64
+  int failure = 0;                 // 0 ok, 1 non-finite RHS, 2 RHS above the safe dot range,
+                                   // 3 non-finite iterate, 4 breakdown (overflow / division)
+  int refinements = 0;             // outer FP64 refinement passes (mixed form)
+};
+
+struct DstPlan;
+
+template <class T>
+struct CgVecs {
+  T* r = nullptr;
+  T* z = nullptr;
+  T* p = nullptr;  // 2N (ping-pong)
+  T* q = nullptr;
+  T* x = nullptr;
+  T* b = nullptr;
+  T* work = nullptr;
+  T* scratch = nullptr;
+  DstPlan* plan = nullptr;
+};
+
+class Poisson {
+ public:
+  ~Poisson();
+  // Geometry from the solid mask (ny*nx bytes on the device). Periodic axes select Jacobi.
+  int setup(hipStream_t s, int nx, int ny, double dx, double dy, int periodic_x, int periodic_y,
+            const uint8_t* d_solid);
+  // Solve A x = b (b FP64 on device, ny*nx). `mixed`: FP32 PCG sweeps inside an FP64
+  // residual-refinement loop to the same relative tolerance.
+  int solve(const double* d_rhs, double* d_x, double tolerance, int max_iterations, bool mixed,
+            PoissonReport* rep);
+
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0, periodic_x = 0, periodic_y = 0;
+  double dx = 0, dy = 0;
+  size_t N = 0;
+  int nblocks = 0;
+  uint8_t* flags = nullptr;
+  double* diag = nullptr;
+  double* ex = nullptr;
+  double* partial = nullptr;
+  double* scal = nullptr;
+  double* host_scal = nullptr;
+  double* b64 = nullptr;  // compatible right-hand side
+  double* r64 = nullptr;  // FP64 residual
+  CgVecs<double> v64;
+  CgVecs<float> v32;
+
+ private:
+  template <class T>
+  int alloc(CgVecs<T>& v);
+  template <class T>
+  int precond(CgVecs<T>& v, const T* rin, T* zout);
+  template <class T>
+  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* iterations, int* info, int* failure);
+  int reduce(const double* partial_in, int n, int mode);
+  int read_scalars();
+};
+
+}  // namespace fb

@@ -1,0 +1,829 @@
+// Pressure Poisson solve: preconditioned CG in SciPy's iteration order with the
+// reference's DST-I fast-Poisson preconditioner, on gfx950.
+//
+// Preconditioner (fb/core/_scipy_adapter.py:76-101): z = S (Lambda^-1 o (S r_fluid)) with the
+// orthonormal 2D DST-I S, then z = r on solid cells; Jacobi when any axis is periodic.
+// Here the y direction is applied as the exact tridiagonal solve it diagonalises,
+// (T_y/dy^2 + lambda_x I) z = r_hat (one Thomas sweep per x-mode), and the x direction as a
+// DST-I per row computed with a Bluestein chirp-z transform: power-of-two radix-2 FFTs in
+// LDS, since the DST-I lengths n+1 (129, 257, 1025, 2049, ...) have large prime factors.
+// Both are exact up to rounding, so the CG iterates follow the reference's sequence.
+//
+// CG (scipy/sparse/linalg/_isolve/iterative.py cg): x0 = 0, r = b; per iteration: stop when
+// ||r|| < max(0, rtol ||b||) (strict, tested first); z = M r; rho = r.z;
+// p = z (first) or beta p + z; q = A p; alpha = rho / p.q; x += alpha p; r -= alpha q.
+// Scalars live on the device; dot products use a fixed-order two-pass reduction so every
+// run (and every batch composition) gives the same bits.
+#include <cmath>
+#include <complex>
+
+#include "poisson.h"
+
+namespace fb {
+
+enum : uint8_t { PF_SOLID = 1, PF_RIGHT = 2, PF_UP = 4, PF_WRAPX = 8, PF_WRAPY = 16 };
+// Device scalar slots.
+enum { S_RN2 = 0, S_RHO, S_RHOPREV, S_PQ, S_ALPHA, S_BETA, S_ATOL, S_DONE, S_ITER, S_FAIL, S_INFO, S_MAXIT, S_DOT,
+       S_COUNT = 16 };
+
+template <class T>
+struct cplx {
+  T x, y;
+};
+template <class T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+struct DstPlan {
+  int n = 0, M = 0, logM = 0;
+  double scale = 0.0;
+  void* chirp = nullptr;  // cplx<T>[n]: exp(-i pi m^2 / (2(n+1))), m = 1..n
+  void* fb = nullptr;     // cplx<T>[M]: FFT of the cyclic exp(+i pi m^2 / (2(n+1)))
+  void* tw = nullptr;     // cplx<T>[M/2]: exp(-2 pi i k / M)
+};
+
+// ---------------------------------------------------------------- host table set-up
+
+static void host_fft(std::vector<std::complex<long double>>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len) {
+      for (size_t k = 0; k < len / 2; ++k) {
+        long double ang = -2.0L * pi * (long double)k / (long double)len;
+        std::complex<long double> w(cosl(ang), sinl(ang));
+        auto u = a[i + k];
+        auto v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+template <class T>
+static int make_plan(DstPlan*& plan, int n) {
+  plan = new DstPlan();
+  plan->n = n;
+  int M = 1, logM = 0;
+  while (M < 2 * n - 1) {
+    M <<= 1;
+    ++logM;
+  }
+  if (M < 2) {
+    M = 2;
+    logM = 1;
+  }
+  plan->M = M;
+  plan->logM = logM;
+  plan->scale = std::sqrt(2.0 / (double)(n + 1));
+  const long double pi = 3.141592653589793238462643383279502884L;
+  const long long fourN = 4LL * (n + 1);
+  auto phase = [&](long long m) {  // pi m^2 / (2(n+1)) reduced exactly modulo 2 pi
+    long long r = (m * m) % fourN;
+    return pi * (long double)r / (long double)(2 * (n + 1));
+  };
+  std::vector<cplx<T>> chirp(n), fbv(M), tw(M / 2);
+  for (int m = 1; m <= n; ++m) {
+    long double a = phase(m);
+    chirp[m - 1] = {(T)cosl(a), (T)(-sinl(a))};
+  }
+  std::vector<std::complex<long double>> b(M, {0.0L, 0.0L});
+  for (int m = -(n - 1); m <= n - 1; ++m) {
+    long double a = phase(m < 0 ? -m : m);
+    b[(size_t)((m + M) % M)] = {cosl(a), sinl(a)};
+  }
+  host_fft(b);
+  for (int k = 0; k < M; ++k) fbv[k] = {(T)b[k].real(), (T)b[k].imag()};
+  for (int k = 0; k < M / 2; ++k) {
+    long double a = -2.0L * pi * (long double)k / (long double)M;
+    tw[k] = {(T)cosl(a), (T)sinl(a)};
+  }
+  FB_HIP(hipMalloc(&plan->chirp, n * sizeof(cplx<T>)));
+  FB_HIP(hipMalloc(&plan->fb, M * sizeof(cplx<T>)));
+  FB_HIP(hipMalloc(&plan->tw, (M / 2) * sizeof(cplx<T>)));
+  FB_HIP(hipMemcpy(plan->chirp, chirp.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
+  FB_HIP(hipMemcpy(plan->fb, fbv.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice));
+  FB_HIP(hipMemcpy(plan->tw, tw.data(), (M / 2) * sizeof(cplx<T>), hipMemcpyHostToDevice));
+  return FB_OK;
+}
+
+static void free_plan(DstPlan* p) {
+  if (!p) return;
+  if (p->chirp) (void)hipFree(p->chirp);
+  if (p->fb) (void)hipFree(p->fb);
+  if (p->tw) (void)hipFree(p->tw);
+  delete p;
+}
+
+// ---------------------------------------------------------------- device pieces
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  // fixed-order tree: butterfly within each wave, then over waves
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += lds[w];
+  }
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+#define CG_DONE(scal) ((scal)[S_DONE] != 0.0)
+
+// Sum `n` partials in fixed order into scal[slot]; optional post-processing by `mode`.
+// mode 0: plain dot -> scal[S_DOT]
+// mode 1: rho = sum; beta = rho / rho_prev (iteration > 0)
+// mode 2: pq = sum; alpha = rho / pq
+// mode 3: rn2 = sum; rho_prev = rho; iteration += 1; then the top-of-loop check
+// mode 4: rn2 = sum (initial) and the top-of-loop check
+__global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restrict__ partial, int n, double* scal,
+                                                          int mode) {
+  __shared__ double lds[4];
+  if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  s = block_sum<256>(s, lds);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    scal[S_DOT] = s;
+    return;
+  }
+  if (mode == 1) {
+    scal[S_RHO] = s;
+    if (scal[S_ITER] > 0.0) scal[S_BETA] = s / scal[S_RHOPREV];
+    if (!isfinite(s) || !isfinite(scal[S_BETA])) {
+      scal[S_FAIL] = 4.0;
+      scal[S_DONE] = 1.0;
+    }
+    return;
+  }
+  if (mode == 2) {
+    scal[S_PQ] = s;
+    double a = scal[S_RHO] / s;
+    scal[S_ALPHA] = a;
+    if (!isfinite(s) || s == 0.0 || !isfinite(a)) {
+      scal[S_FAIL] = 4.0;
+      scal[S_DONE] = 1.0;
+    }
+    return;
+  }
+  if (mode == 3) {
+    scal[S_RHOPREV] = scal[S_RHO];
+    scal[S_ITER] += 1.0;
+  }
+  if (mode == 4) {
+    // scipy: atol = max(atol, rtol*||b||); ||b|| == 0 returns b with info 0 before the loop
+    scal[S_ATOL] = fmax(0.0, scal[S_ATOL] * sqrt(s));
+    if (s == 0.0) {
+      scal[S_RN2] = 0.0;
+      scal[S_DONE] = 1.0;
+      scal[S_INFO] = 0.0;
+      return;
+    }
+  }
+  scal[S_RN2] = s;
+  if (!isfinite(s)) {
+    scal[S_FAIL] = 4.0;
+    scal[S_DONE] = 1.0;
+    return;
+  }
+  // top of the next iteration: maxiter exhausted, then the strict residual test
+  if (scal[S_ITER] >= scal[S_MAXIT]) {
+    scal[S_DONE] = 1.0;
+    scal[S_INFO] = scal[S_MAXIT];
+  } else if (sqrt(s) < scal[S_ATOL]) {
+    scal[S_DONE] = 1.0;
+    scal[S_INFO] = 0.0;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, size_t n,
+                                                          double* __restrict__ partial) {
+  __shared__ double lds[4];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = i < n ? (double)a[i] * (double)b[i] : 0.0;
+  if constexpr (sizeof(T) == 4) v = i < n ? (double)(a[i] * b[i]) : 0.0;
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// Masked 5-point operator row at cell (x, y) applied to a field given by `val` (fb/core/
+// _scipy_adapter.py:56-70, same operation order).
+template <class T, class F>
+__device__ __forceinline__ T apply_row(int x, int y, int nx, int ny, const uint8_t* __restrict__ fl,
+                                       const double* __restrict__ diag, T ix2, T iy2, F val) {
+  const size_t i = (size_t)y * nx + x;
+  const uint8_t f = fl[i];
+  T pc = val(x, y);
+  if (f & PF_SOLID) return pc;
+  T r = (T)diag[i] * pc;
+  if (f & PF_RIGHT) r = r - val(x + 1, y) * ix2;
+  if (x > 0 && (fl[i - 1] & PF_RIGHT)) r = r - val(x - 1, y) * ix2;
+  if (f & PF_UP) r = r - val(x, y + 1) * iy2;
+  if (y > 0 && (fl[i - nx] & PF_UP)) r = r - val(x, y - 1) * iy2;
+  if (f & PF_WRAPX) {  // fluid pair across the periodic x seam
+    if (x == nx - 1) r = r - val(0, y) * ix2;
+    if (x == 0) r = r - val(nx - 1, y) * ix2;
+  }
+  if (f & PF_WRAPY) {
+    if (y == ny - 1) r = r - val(x, 0) * iy2;
+    if (y == 0) r = r - val(x, ny - 1) * iy2;
+  }
+  return r;
+}
+
+// p_new = first ? z : p_old*beta + z (SciPy: p *= beta; p += z); q = A p_new; partial p.q.
+template <class T>
+__global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_t* __restrict__ fl,
+                                                    const double* __restrict__ diag, const T* __restrict__ z,
+                                                    const T* __restrict__ pold, T* __restrict__ pnew,
+                                                    T* __restrict__ q, double* __restrict__ partial,
+                                                    const double* __restrict__ scal, T ix2, T iy2) {
+  __shared__ double lds[4];
+  if (CG_DONE(scal)) return;
+  const bool first = scal[S_ITER] == 0.0;
+  const T beta = (T)scal[S_BETA];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t N = (size_t)nx * ny;
+  double v = 0.0;
+  if (i < N) {
+    int x = (int)(i % nx), y = (int)(i / nx);
+    auto pv = [&](int xx, int yy) -> T {
+      size_t k = (size_t)yy * nx + xx;
+      return first ? z[k] : pold[k] * beta + z[k];
+    };
+    T pc = pv(x, y);
+    T qv = apply_row<T>(x, y, nx, ny, fl, diag, ix2, iy2, pv);
+    pnew[i] = pc;
+    q[i] = qv;
+    v = (sizeof(T) == 4) ? (double)(pc * qv) : (double)pc * (double)qv;
+  }
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2; non-finite iterate flag (the reference's callback).
+template <class T>
+__global__ void __launch_bounds__(256) cg_update_kernel(size_t N, T* __restrict__ x, T* __restrict__ r,
+                                                        const T* __restrict__ p, const T* __restrict__ q,
+                                                        double* __restrict__ partial, double* scal) {
+  __shared__ double lds[4];
+  if (CG_DONE(scal)) return;
+  const T alpha = (T)scal[S_ALPHA];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  bool bad = false;
+  if (i < N) {
+    T xv = x[i] + alpha * p[i];
+    T rv = r[i] - alpha * q[i];
+    x[i] = xv;
+    r[i] = rv;
+    bad = !isfinite((double)xv);
+    v = (sizeof(T) == 4) ? (double)(rv * rv) : (double)rv * (double)rv;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) scal[S_FAIL] = 3.0;
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// Jacobi preconditioner (periodic domains): z = (1/max(diag, eps)) r; partial r.z.
+template <class T>
+__global__ void __launch_bounds__(256) jacobi_kernel(size_t N, const double* __restrict__ diag,
+                                                     const T* __restrict__ r, T* __restrict__ z,
+                                                     double* __restrict__ partial, const double* scal, double eps) {
+  __shared__ double lds[4];
+  if (scal && CG_DONE(scal)) return;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (i < N) {
+    T inv = (T)(1.0 / fmax(diag[i], eps));
+    T zv = inv * r[i];
+    z[i] = zv;
+    v = (sizeof(T) == 4) ? (double)(r[i] * zv) : (double)r[i] * (double)zv;
+  }
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0 && partial) partial[blockIdx.x] = v;
+}
+
+__device__ __forceinline__ int bitrev(int j, int logM) { return (int)(__brev((unsigned)j) >> (32 - logM)); }
+
+template <class T>
+__device__ __forceinline__ void fft_dit(cplx<T>* buf, const cplx<T>* __restrict__ tw, int M, int logM) {
+  for (int s = 1; s <= logM; ++s) {
+    const int h = 1 << (s - 1);
+    const int ts = M >> s;
+    for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
+      const int k = b & (h - 1);
+      const int i = ((b >> (s - 1)) << s) + k;
+      const int j = i + h;
+      cplx<T> t = cmul(tw[k * ts], buf[j]);
+      cplx<T> a = buf[i];
+      buf[i] = {a.x + t.x, a.y + t.y};
+      buf[j] = {a.x - t.x, a.y - t.y};
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void fft_dif(cplx<T>* buf, const cplx<T>* __restrict__ tw, int M, int logM) {
+  for (int s = logM; s >= 1; --s) {
+    const int h = 1 << (s - 1);
+    const int ts = M >> s;
+    for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
+      const int k = b & (h - 1);
+      const int i = ((b >> (s - 1)) << s) + k;
+      const int j = i + h;
+      cplx<T> a = buf[i], c = buf[j];
+      buf[i] = {a.x + c.x, a.y + c.y};
+      buf[j] = cmul(cplx<T>{a.x - c.x, a.y - c.y}, tw[k * ts]);
+    }
+    __syncthreads();
+  }
+}
+
+// Orthonormal DST-I along x of every row (one row per block), Bluestein chirp-z.
+// mode 0: input r masked to fluid cells, output to `out`.
+// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row block.
+template <class T>
+__global__ void __launch_bounds__(256) dst_rows_kernel(int nx, int ny, const T* __restrict__ in,
+                                                       const uint8_t* __restrict__ fl, const T* __restrict__ rsrc,
+                                                       T* __restrict__ out, const cplx<T>* __restrict__ chirp,
+                                                       const cplx<T>* __restrict__ fbt, const cplx<T>* __restrict__ tw,
+                                                       int M, int logM, T scale, int mode, double* __restrict__ partial,
+                                                       const double* scal) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
+  __shared__ double lds[4];
+  if (scal && CG_DONE(scal)) return;
+  const int row = blockIdx.x;
+  const size_t base = (size_t)row * nx;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    cplx<T> a{(T)0, (T)0};
+    if (j < nx) {
+      T v = in[base + j];
+      if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
+      cplx<T> c = chirp[j];
+      a = {v * c.x, v * c.y};
+    }
+    buf[bitrev(j, logM)] = a;
+  }
+  __syncthreads();
+  fft_dit(buf, tw, M, logM);
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    cplx<T> v = cmul(buf[k], fbt[k]);
+    buf[k] = {v.x, -v.y};
+  }
+  __syncthreads();
+  fft_dif(buf, tw, M, logM);
+  const T invM = (T)1 / (T)M;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+    cplx<T> w = buf[bitrev(k, logM)];
+    cplx<T> conv{w.x * invM, -w.y * invM};
+    cplx<T> y = cmul(chirp[k], conv);
+    T d = -y.y * scale;
+    if (mode == 0) {
+      out[base + k] = d;
+    } else {
+      T rv = rsrc[base + k];
+      T zv = (fl[base + k] & PF_SOLID) ? rv : d;
+      out[base + k] = zv;
+      acc += (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+    }
+  }
+  if (mode == 1) {
+    __syncthreads();
+    // deterministic per-row partial: fixed strided assignment and tree
+    double s = block_sum<256>(acc, lds);
+    if (threadIdx.x == 0) partial[row] = s;
+  }
+}
+
+// Thomas solve of (T_y/dy^2 + ex[i] I) z = rhs for every x-mode column i.
+template <class T>
+__global__ void __launch_bounds__(64) thomas_kernel(int nx, int ny, T* __restrict__ w, T* __restrict__ cpb,
+                                                    const double* __restrict__ ex, T iy2, const double* scal) {
+  if (scal && CG_DONE(scal)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  const T a = -iy2;
+  const T b = (T)2 * iy2 + (T)ex[i];
+  T cp = a / b;
+  T dp = w[i] / b;
+  cpb[i] = cp;
+  w[i] = dp;
+  for (int j = 1; j < ny; ++j) {
+    size_t k = (size_t)j * nx + i;
+    T m = b - a * cp;
+    T d = w[k];
+    cp = a / m;
+    dp = (d - a * dp) / m;
+    cpb[k] = cp;
+    w[k] = dp;
+  }
+  T zz = dp;
+  for (int j = ny - 2; j >= 0; --j) {
+    size_t k = (size_t)j * nx + i;
+    zz = w[k] - cpb[k] * zz;
+    w[k] = zz;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) copy_kernel(size_t N, const T* __restrict__ a, T* __restrict__ b,
+                                                   const double* scal) {
+  if (scal && CG_DONE(scal)) return;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) b[i] = a[i];
+}
+
+__global__ void __launch_bounds__(256) setup_kernel(int nx, int ny, const uint8_t* __restrict__ solid, int px, int py,
+                                                    double ix2, double iy2, uint8_t* __restrict__ fl,
+                                                    double* __restrict__ diag) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  int x = (int)(i % nx), y = (int)(i / nx);
+  auto fluid = [&](int xx, int yy) { return solid[(size_t)yy * nx + xx] == 0; };
+  bool f = fluid(x, y);
+  uint8_t b = f ? 0 : PF_SOLID;
+  if (f && x + 1 < nx && fluid(x + 1, y)) b |= PF_RIGHT;
+  if (f && y + 1 < ny && fluid(x, y + 1)) b |= PF_UP;
+  if (px && f && ((x == nx - 1 && fluid(0, y)) || (x == 0 && fluid(nx - 1, y)))) b |= PF_WRAPX;
+  if (py && f && ((y == ny - 1 && fluid(x, 0)) || (y == 0 && fluid(x, ny - 1)))) b |= PF_WRAPY;
+  fl[i] = b;
+  // diagonal in the reference's accumulation order (fb/core/_scipy_adapter.py:31-54)
+  double d = 0.0;
+  if (x + 1 < nx && f && fluid(x + 1, y)) d += ix2;
+  if (x > 0 && f && fluid(x - 1, y)) d += ix2;
+  if (y + 1 < ny && f && fluid(x, y + 1)) d += iy2;
+  if (y > 0 && f && fluid(x, y - 1)) d += iy2;
+  if (px) {
+    if (x == nx - 1 && f && fluid(0, y)) d += ix2;
+    if (x == 0 && f && fluid(nx - 1, y)) d += ix2;
+  } else {
+    if (x == 0 && f) d += ix2;
+    if (x == nx - 1 && f) d += ix2;
+  }
+  if (py) {
+    if (y == ny - 1 && f && fluid(x, 0)) d += iy2;
+    if (y == 0 && f && fluid(x, ny - 1)) d += iy2;
+  } else {
+    if (y == 0 && f) d += iy2;
+    if (y == ny - 1 && f) d += iy2;
+  }
+  diag[i] = f ? d : 1.0;
+}
+
+// q = A p in FP64 (true residual); r = b - q with partial ||r||^2 into partial.
+__global__ void __launch_bounds__(256) residual_kernel(int nx, int ny, const uint8_t* __restrict__ fl,
+                                                       const double* __restrict__ diag, const double* __restrict__ b,
+                                                       const double* __restrict__ x, double* __restrict__ r,
+                                                       double* __restrict__ partial, double ix2, double iy2) {
+  __shared__ double lds[4];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (i < (size_t)nx * ny) {
+    int xx = (int)(i % nx), yy = (int)(i / nx);
+    auto xv = [&](int a, int c) -> double { return x[(size_t)c * nx + a]; };
+    double ax = apply_row<double>(xx, yy, nx, ny, fl, diag, ix2, iy2, xv);
+    double rv = b[i] - ax;
+    r[i] = rv;
+    v = rv * rv;
+  }
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// RHS preparation (fb/core/_scipy_adapter.py:108-120): b = rhs on fluid, 0 on solid;
+// block partials of the fluid sum and count; non-finite count and max |b| via order-free atomics.
+__global__ void __launch_bounds__(256) rhs_prep_kernel(size_t N, const double* __restrict__ rhs,
+                                                       const uint8_t* __restrict__ fl, double* __restrict__ b,
+                                                       double* __restrict__ partial, double* __restrict__ partial_n,
+                                                       unsigned long long* __restrict__ stat) {
+  __shared__ double lds[4];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0, c = 0.0, m = 0.0;
+  bool nf = false;
+  if (i < N) {
+    bool fluid = !(fl[i] & PF_SOLID);
+    double x = fluid ? rhs[i] : 0.0;
+    b[i] = x;
+    if (fluid) {
+      v = x;
+      c = 1.0;
+    }
+    nf = !isfinite(x);
+    m = nf ? 0.0 : fabs(x);
+  }
+  if (__any(nf) && (threadIdx.x & 63) == 0) atomicAdd(&stat[0], 1ull);
+  double mw = m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&stat[1], (unsigned long long)__double_as_longlong(mw));
+  double sv = block_sum<256>(v, lds);
+  double sc = block_sum<256>(c, lds);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = sv;
+    partial_n[blockIdx.x] = sc;
+  }
+}
+
+// b[fluid] -= mean (doubly periodic compatibility, fb/core/_scipy_adapter.py:109-111), or the
+// same on the solution (:153-154).
+__global__ void __launch_bounds__(256) sub_mean_kernel(size_t N, const uint8_t* __restrict__ fl, double* __restrict__ a,
+                                                       double mean) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N && !(fl[i] & PF_SOLID)) a[i] -= mean;
+}
+
+__global__ void __launch_bounds__(256) to_f32_scaled(size_t N, const double* __restrict__ a, float* __restrict__ b,
+                                                     double s) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) b[i] = (float)(a[i] * s);
+}
+
+__global__ void __launch_bounds__(256) add_f32_scaled(size_t N, double* __restrict__ x, const float* __restrict__ e,
+                                                      double s) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) x[i] += (double)e[i] * s;
+}
+
+__global__ void __launch_bounds__(256) fill_zero(size_t N, double* __restrict__ a) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- host driver
+
+Poisson::~Poisson() {
+  for (void* ptr : {(void*)flags, (void*)diag, (void*)ex, (void*)partial, (void*)scal, (void*)b64, (void*)r64})
+    if (ptr) (void)hipFree(ptr);
+  if (host_scal) (void)hipHostFree(host_scal);
+  auto freev = [](auto& v) {
+    for (void* ptr : {(void*)v.r, (void*)v.z, (void*)v.p, (void*)v.q, (void*)v.x, (void*)v.b, (void*)v.work,
+                      (void*)v.scratch})
+      if (ptr) (void)hipFree(ptr);
+    free_plan(v.plan);
+  };
+  freev(v64);
+  freev(v32);
+}
+
+template <class T>
+int Poisson::alloc(CgVecs<T>& v) {
+  if (v.r) return FB_OK;
+  for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(hipMalloc((void**)a, N * sizeof(T)));
+  FB_HIP(hipMalloc((void**)&v.p, 2 * N * sizeof(T)));
+  if (!periodic_x && !periodic_y) return make_plan<T>(v.plan, nx);
+  return FB_OK;
+}
+
+int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int px, int py, const uint8_t* d_solid) {
+  stream = s;
+  if (flags == nullptr || nx_ != nx || ny_ != ny || px != periodic_x || py != periodic_y) {
+    if (flags) {
+      set_error("Poisson solver cannot change shape after set-up");
+      return FB_ERR_STATE;
+    }
+    nx = nx_;
+    ny = ny_;
+    dx = dx_;
+    dy = dy_;
+    periodic_x = px;
+    periodic_y = py;
+    N = (size_t)nx * ny;
+    nblocks = (int)((N + 255) / 256);
+    FB_HIP(hipMalloc((void**)&flags, N));
+    FB_HIP(hipMalloc((void**)&diag, N * sizeof(double)));
+    FB_HIP(hipMalloc((void**)&partial, 2 * (size_t)(nblocks + ny + 64) * sizeof(double)));
+    FB_HIP(hipMalloc((void**)&scal, (S_COUNT + 8) * sizeof(double)));
+    FB_HIP(hipHostMalloc((void**)&host_scal, (S_COUNT + 8) * sizeof(double), 0));
+    FB_HIP(hipMalloc((void**)&b64, N * sizeof(double)));
+    FB_HIP(hipMalloc((void**)&r64, N * sizeof(double)));
+    FB_HIP(hipMalloc((void**)&ex, nx * sizeof(double)));
+    std::vector<double> exh(nx);
+    for (int i = 0; i < nx; ++i)
+      exh[i] = 2.0 * (1.0 - std::cos(M_PI * (double)(i + 1) / (double)(nx + 1))) * (1.0 / (dx * dx));
+    FB_HIP(hipMemcpy(ex, exh.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    int rc = alloc(v64);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(setup_kernel, dim3(nblocks), dim3(256), 0, stream, nx, ny, d_solid, px, py, 1.0 / (dx * dx),
+                     1.0 / (dy * dy), flags, diag);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+int Poisson::reduce(const double* pin, int n, int mode) {
+  hipLaunchKernelGGL(cg_finalize_kernel, dim3(1), dim3(256), 0, stream, pin, n, scal, mode);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+int Poisson::read_scalars() {
+  FB_HIP(hipMemcpyAsync(host_scal, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FB_HIP(hipStreamSynchronize(stream));
+  return FB_OK;
+}
+
+template <class T>
+int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
+  if (periodic_x || periodic_y) {
+    hipLaunchKernelGGL(jacobi_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, diag, rin, zout, partial, scal,
+                       (double)std::numeric_limits<double>::epsilon());
+    FB_HIP(hipGetLastError());
+    return reduce(partial, nblocks, 1);
+  }
+  const DstPlan& P = *v.plan;
+  const int threads = std::min(256, std::max(64, P.M / 2));
+  const size_t lds = (size_t)P.M * sizeof(cplx<T>);
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(ny), dim3(threads), lds, stream, nx, ny, rin, flags, rin, v.work,
+                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 0,
+                     partial, scal);
+  FB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(thomas_kernel<T>, dim3((nx + 63) / 64), dim3(64), 0, stream, nx, ny, v.work, v.scratch, ex,
+                     (T)(1.0 / (dy * dy)), scal);
+  FB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(ny), dim3(threads), lds, stream, nx, ny, v.work, flags, rin, zout,
+                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 1,
+                     partial, scal);
+  FB_HIP(hipGetLastError());
+  return reduce(partial, ny, 1);
+}
+
+template <class T>
+int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* iterations, int* info, int* failure) {
+  FB_HIP(hipMemsetAsync(x, 0, N * sizeof(T), stream));
+  FB_HIP(hipMemcpyAsync(v.r, b, N * sizeof(T), hipMemcpyDeviceToDevice, stream));
+  double init[S_COUNT] = {};
+  init[S_ATOL] = rtol;
+  init[S_MAXIT] = (double)maxit;
+  FB_HIP(hipMemcpyAsync(scal, init, sizeof(init), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks), dim3(256), 0, stream, v.r, v.r, N, partial);
+  FB_HIP(hipGetLastError());
+  int rc = reduce(partial, nblocks, 4);
+  if (rc) return rc;
+  const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
+  T* pa = v.p;
+  T* pb = v.p + N;
+  int launched = 0;
+  int chunk = 4;
+  for (;;) {
+    for (int c = 0; c < chunk && launched < maxit; ++c, ++launched) {
+      if ((rc = precond(v, v.r, v.z))) return rc;
+      hipLaunchKernelGGL(cg_pq_kernel<T>, dim3(nblocks), dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q,
+                         partial, scal, ix2, iy2);
+      FB_HIP(hipGetLastError());
+      if ((rc = reduce(partial, nblocks, 2))) return rc;
+      hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal);
+      FB_HIP(hipGetLastError());
+      if ((rc = reduce(partial, nblocks, 3))) return rc;
+      std::swap(pa, pb);
+    }
+    if ((rc = read_scalars())) return rc;
+    if (host_scal[S_DONE] != 0.0 || launched >= maxit) break;
+    chunk = std::min(2 * chunk, 32);
+  }
+  *iterations = (int)host_scal[S_ITER];
+  *info = (host_scal[S_DONE] != 0.0) ? (int)host_scal[S_INFO] : maxit;
+  *failure = (int)host_scal[S_FAIL];
+  return FB_OK;
+}
+
+int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* rep) {
+  *rep = PoissonReport{};
+  const int nb = nblocks;
+  unsigned long long* stat = reinterpret_cast<unsigned long long*>(scal + S_COUNT);
+  FB_HIP(hipMemsetAsync(stat, 0, 2 * sizeof(unsigned long long), stream));
+  hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb), dim3(256), 0, stream, N, d_rhs, flags, b64, partial, partial + nb,
+                     stat);
+  FB_HIP(hipGetLastError());
+  const bool singular = periodic_x && periodic_y;
+  int rc;
+  if (singular) {
+    if ((rc = reduce(partial, nb, 0))) return rc;
+    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if ((rc = reduce(partial + nb, nb, 0))) return rc;
+    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT + 1, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+  unsigned long long st[2];
+  FB_HIP(hipMemcpyAsync(st, stat, sizeof(st), hipMemcpyDeviceToHost, stream));
+  FB_HIP(hipStreamSynchronize(stream));
+  if (singular && host_scal[S_COUNT + 1] > 0.0) {
+    double mean = host_scal[S_COUNT] / host_scal[S_COUNT + 1];
+    hipLaunchKernelGGL(sub_mean_kernel, dim3(nb), dim3(256), 0, stream, N, flags, b64, mean);
+    FB_HIP(hipGetLastError());
+  }
+  if (st[0] != 0) {
+    rep->failure = 1;
+    return FB_OK;
+  }
+  double bmax;
+  {
+    long long bits = (long long)st[1];
+    memcpy(&bmax, &bits, sizeof(bmax));
+  }
+  const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
+  if (bmax > safe) {
+    rep->failure = 2;
+    return FB_OK;
+  }
+  const double rtol = 0.9 * tolerance;
+  const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
+  if (!mixed) {
+    int its = 0, info = 0, fail = 0;
+    if ((rc = pcg<double>(v64, b64, d_x, rtol, maxit, &its, &info, &fail))) return rc;
+    rep->iterations = its;
+    rep->info = info;
+    rep->failure = fail;
+  } else {
+    // FP32 PCG sweeps on the scaled FP64 residual, FP64 correction and residual.
+    if ((rc = alloc(v32))) return rc;
+    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, N, d_x);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipMemcpyAsync(r64, b64, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb), dim3(256), 0, stream, b64, b64, N, partial);
+    if ((rc = reduce(partial, nb, 0))) return rc;
+    if ((rc = read_scalars())) return rc;
+    const double bn = std::sqrt(host_scal[S_DOT]);
+    double rn = bn;
+    int total = 0;
+    rep->info = maxit;
+    for (int outer = 0; outer < 12 && total < maxit; ++outer) {
+      if (bn == 0.0 || rn < rtol * bn) {
+        rep->info = 0;
+        break;
+      }
+      hipLaunchKernelGGL(to_f32_scaled, dim3(nb), dim3(256), 0, stream, N, r64, v32.b, 1.0 / rn);
+      FB_HIP(hipGetLastError());
+      int its = 0, info = 0, fail = 0;
+      // inner tolerance: a relative reduction FP32 reaches reliably
+      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, maxit - total, &its, &info, &fail))) return rc;
+      total += its;
+      rep->refinements = outer + 1;
+      if (fail) {
+        rep->failure = fail;
+        break;
+      }
+      hipLaunchKernelGGL(add_f32_scaled, dim3(nb), dim3(256), 0, stream, N, d_x, v32.x, rn);
+      FB_HIP(hipGetLastError());
+      hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64, partial,
+                         ix2, iy2);
+      FB_HIP(hipGetLastError());
+      if ((rc = reduce(partial, nb, 0))) return rc;
+      if ((rc = read_scalars())) return rc;
+      rn = std::sqrt(host_scal[S_DOT]);
+      if (!std::isfinite(rn)) {
+        rep->failure = 4;
+        break;
+      }
+    }
+    if (rep->info != 0 && rn < rtol * bn) rep->info = 0;
+    rep->iterations = total;
+  }
+  if (singular) {
+    hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb), dim3(256), 0, stream, N, d_x, flags, r64, partial, partial + nb,
+                       stat);
+    if ((rc = reduce(partial, nb, 0))) return rc;
+    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if ((rc = reduce(partial + nb, nb, 0))) return rc;
+    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT + 1, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
+    FB_HIP(hipStreamSynchronize(stream));
+    if (host_scal[S_COUNT + 1] > 0.0) {
+      hipLaunchKernelGGL(sub_mean_kernel, dim3(nb), dim3(256), 0, stream, N, flags, d_x,
+                         host_scal[S_COUNT] / host_scal[S_COUNT + 1]);
+      FB_HIP(hipGetLastError());
+    }
+  }
+  // true relative residual ||b - A x|| / max(||b||, eps) in FP64
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64, partial, ix2,
+                     iy2);
+  FB_HIP(hipGetLastError());
+  if ((rc = reduce(partial, nb, 0))) return rc;
+  if ((rc = read_scalars())) return rc;
+  const double rnorm = std::sqrt(host_scal[S_DOT]);
+  hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb), dim3(256), 0, stream, b64, b64, N, partial);
+  if ((rc = reduce(partial, nb, 0))) return rc;
+  if ((rc = read_scalars())) return rc;
+  const double bnorm = std::sqrt(host_scal[S_DOT]);
+  rep->relative_residual = rnorm / std::max(bnorm, std::numeric_limits<double>::epsilon());
+  return FB_OK;
+}
+
+}  // namespace fb

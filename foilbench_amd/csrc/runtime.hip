@@ -1,4 +1,5 @@
 // Context, error state and device selection of libfoilbench_amd.
+#include <algorithm>
 #include <cstdarg>
 
 #include "fb_common.h"
@@ -18,9 +19,46 @@ void set_error(const char* fmt, ...) {
 
 void clear_error() { g_error.clear(); }
 
+// Streaming copy used to calibrate HBM counters and the achievable bandwidth.
+__global__ void __launch_bounds__(256) copy_f4_kernel(size_t n4, const float4* __restrict__ a, float4* __restrict__ b) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  for (; i < n4; i += stride) b[i] = a[i];
+}
+
 }  // namespace fb
 
 extern "C" {
+
+int fb_copy_benchmark(fb_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy) {
+  FB_REQUIRE(ctx && ms_per_copy, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(bytes >= 16 && bytes % 16 == 0 && reps >= 1, FB_ERR_ARG, "bytes must be a positive multiple of 16");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  FB_HIP(hipMalloc(&a, (size_t)bytes));
+  FB_HIP(hipMalloc(&b, (size_t)bytes));
+  FB_HIP(hipMemsetAsync(a, 0, (size_t)bytes, c->stream));
+  size_t n4 = (size_t)bytes / 16;
+  unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, (size_t)c->cu_count * 32);
+  hipEvent_t e0, e1;
+  FB_HIP(hipEventCreate(&e0));
+  FB_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(fb::copy_f4_kernel, dim3(blocks), dim3(256), 0, c->stream, n4, (const float4*)a, (float4*)b);
+  FB_HIP(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(fb::copy_f4_kernel, dim3(blocks), dim3(256), 0, c->stream, n4, (const float4*)a, (float4*)b);
+  FB_HIP(hipEventRecord(e1, c->stream));
+  FB_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / reps;
+  FB_HIP(hipEventDestroy(e0));
+  FB_HIP(hipEventDestroy(e1));
+  FB_HIP(hipFree(a));
+  FB_HIP(hipFree(b));
+  return FB_OK;
+}
 
 const char* fb_last_error(void) { return fb::g_error.c_str(); }
 

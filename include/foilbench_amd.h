@@ -77,6 +77,9 @@ int fb_device_count(int* count);
 int fb_ctx_create(int device, fb_ctx** out);
 int fb_ctx_destroy(fb_ctx* ctx);
 int fb_ctx_synchronize(fb_ctx* ctx);
+/* Device-to-device streaming copy of `bytes` (reps times), timed with HIP events; used to
+ * calibrate the HBM byte counters and the achievable bandwidth. No reference counterpart. */
+int fb_copy_benchmark(fb_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy);
 
 /* Signed distance and mask of cell centres (fb/core/geometry.py:92-106, 157-168) into
  * host arrays sdf (ny, nx) float64 and solid (ny, nx) uint8. */
@@ -150,6 +153,86 @@ int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void*
 /* Timing harness: run `steps` lattice steps with fixed parameters and no pose change,
  * without postcondition reductions; returns once queued (use fb_ctx_synchronize). */
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps);
+/* Same, timed with a HIP event pair around every step launch on the handle's stream:
+ * wall_ms spans first to last launch, kernel_ms sums the per-launch durations. */
+int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* wall_ms, double* kernel_ms);
+
+/* ------------------------------------------------------------ Stable Fluids (MAC) ---- */
+/* Replaces the MAC-grid stages of fb/solvers/stable_fluids.py:424-494 and fb/core/grid.py
+ * (faces_to_cell, advect_velocity/rk2_backtrace/local_velocity_bounds, implicit_diffuse,
+ * cell_to_faces, apply_domain_boundaries, enforce_solid_faces, project_faces) with the
+ * pressure solve of fb/core/_scipy_adapter.py:16-161. Faces: u (ny, nx+1), v (ny+1, nx)
+ * in the storage dtype (FB_F64 or FB_F32); the pressure solve is FP64. */
+
+typedef struct fb_mac fb_mac;
+
+/* Substep outcome (the reference's classified failures of one substep). */
+enum fb_sf_status {
+  FB_SF_OK = 0,
+  FB_SF_VISCOSITY_NOT_CONVERGED = 1,     /* convergence_failure, stage viscosity */
+  FB_SF_NONFINITE_PROJECTION_INPUT = 2,  /* nonfinite_state, stage projection */
+  FB_SF_PROJECTION_CFL = 3,              /* excessive_velocity, stage projection */
+  FB_SF_NONFINITE_RHS = 4,               /* nonfinite_state (pressure RHS) */
+  FB_SF_RHS_RANGE = 5,                   /* projection_failure (RHS above safe range) */
+  FB_SF_NONFINITE_ITERATE = 6,           /* nonfinite_state (CG iterate) */
+  FB_SF_CG_BREAKDOWN = 7,                /* projection_failure (CG floating-point failure) */
+  FB_SF_PROJECTION_NOT_CONVERGED = 8,    /* projection_failure (not converged) */
+  FB_SF_NONFINITE_PROJECTION_OUTPUT = 9  /* nonfinite_state after projection */
+};
+
+typedef struct fb_sf_params {
+  double dt;               /* substep (or projection pseudo-step) */
+  double viscosity;        /* U_ref * chord / Re */
+  double tolerance;        /* pressure_tolerance (also the Jacobi tolerance) */
+  int32_t max_iterations;  /* pressure_max_iterations (also the Jacobi cap) */
+  int32_t maccormack;      /* 1: limited MacCormack, 0: semi-Lagrangian */
+  double freestream_x, freestream_y;
+  double wall_omega_rad;   /* foil angular velocity in rad/s */
+  double cfl_limit;        /* projection CFL limit max(1, 2*stable_cfl) */
+  int32_t channel_walls;   /* Poiseuille channel walls */
+  int32_t mixed_poisson;   /* 1: FP32 CG sweeps inside FP64 residual refinement */
+} fb_sf_params;
+
+typedef struct fb_sf_report {
+  int32_t status;  /* enum fb_sf_status */
+  int32_t viscosity_iterations;
+  double viscosity_residual;
+  int32_t viscosity_converged;
+  int32_t pressure_iterations;
+  double pressure_residual;
+  int32_t pressure_info;
+  int32_t pressure_converged;
+  int32_t pressure_refinements;
+  int32_t pad_;
+  double projection_cfl;
+} fb_sf_report;
+
+typedef struct fb_mac_stats {
+  double max_face_speed;   /* max(|u|, |v|) (stable_fluids.py:276) */
+  double max_wall_speed;   /* max wall speed over solid cells (:277-279) */
+  double max_cell_speed;   /* max |faces_to_cell| (:337, :502) */
+  double divergence_linf;  /* fluid MAC divergence (fb/core/grid.py:100-111) */
+  double solid_leakage;    /* fluid-solid face leakage (fb/core/grid.py:114-131) */
+  int64_t nonfinite_faces;
+  int64_t nonfinite_wall;
+} fb_mac_stats;
+
+int fb_mac_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, fb_mac** out);
+int fb_mac_destroy(fb_mac* mac);
+int fb_mac_set_faces(fb_mac* mac, const void* u_host, const void* v_host);
+int fb_mac_get_faces(fb_mac* mac, void* u_host, void* v_host);
+/* Mask for a pose (fb/core/geometry.py:157-159); rebuilds the masked Poisson operator. */
+int fb_mac_set_pose(fb_mac* mac, const fb_pose* pose);
+int fb_mac_get_geometry(fb_mac* mac, double* sdf_host, uint8_t* solid_host);
+int fb_mac_get_pressure(fb_mac* mac, double* pressure_host);
+int fb_mac_checkpoint(fb_mac* mac);
+int fb_mac_rollback(fb_mac* mac);
+/* One Stable Fluids substep (fb/solvers/stable_fluids.py:468-494): advect, implicit viscosity,
+ * cell_to_faces, mask at `pose`, projection with the CFL guard (:256-330). */
+int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, fb_sf_report* report);
+/* Projection only (fb/solvers/stable_fluids.py:256-330; initialize and import). */
+int fb_mac_project(fb_mac* mac, const fb_sf_params* params, fb_sf_report* report);
+int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats);
 
 #ifdef __cplusplus
 }
