@@ -47,9 +47,9 @@ def _gpu_forms(steps: int, warmup: int):
         lat.run(steps)
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        ev_steps = min(steps, 2000)
-        ev_wall, ev_kern = lat.time(ev_steps)
-        avg_ms = ev_kern / ev_steps
+        # average launch duration: one HIP event pair around back-to-back launches on the kernel's stream
+        ev_steps = max(1, min(steps, 5000))
+        avg_ms = lat.time(ev_steps) / ev_steps
         bytes_launch = BYTES_PER_CELL[prec] * lat.cells
         out[name] = {
             "cell_steps_per_s": lat.cells * steps / wall,

@@ -131,8 +131,7 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_stats_now": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.POINTER(fb_lbm_stats)]),
     "fb_lbm_macroscopic": (C.c_int, [_P, C.c_double, _P, _P]),
     "fb_lbm_run_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int]),
-    "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double),
-                                    C.POINTER(C.c_double)]),
+    "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double)]),
     "fb_mac_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.POINTER(_P)]),
     "fb_mac_destroy": (C.c_int, [_P]),
     "fb_mac_set_faces": (C.c_int, [_P, _P, _P]),

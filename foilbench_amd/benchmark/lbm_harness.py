@@ -100,13 +100,12 @@ class Lattice:
     def run(self, steps: int) -> None:
         N.check(self.ctx.lib.fb_lbm_run_fixed(self.handle, C.byref(self.params), steps))
 
-    def time(self, steps: int) -> tuple[float, float]:
-        """(wall ms, summed kernel ms) of `steps` lattice steps, from HIP events."""
-        wall = C.c_double()
-        kern = C.c_double()
-        N.check(self.ctx.lib.fb_lbm_time_fixed(self.handle, C.byref(self.params), steps, C.byref(wall),
-                                               C.byref(kern)))
-        return wall.value, kern.value
+    def time(self, steps: int) -> float:
+        """Milliseconds of `steps` back-to-back lattice steps, from one HIP event pair on the stream
+        the step kernel runs on."""
+        ms = C.c_double()
+        N.check(self.ctx.lib.fb_lbm_time_fixed(self.handle, C.byref(self.params), steps, C.byref(ms)))
+        return ms.value
 
     def velocity(self) -> np.ndarray:
         hd = np.float64 if self.precision == N.FB_F64 else np.float32

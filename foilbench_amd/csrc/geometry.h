@@ -93,6 +93,71 @@ __device__ __forceinline__ double linspace_at(int i, int n, double start, double
 
 #pragma clang fp contract(fast)
 
+
+#pragma clang fp contract(off)
+
+// Signed distance of particle positions in the particle dtype T, with the reference's NumPy
+// promotions (fb/core/geometry.py:35-106 on a (P, 2) array of dtype T): the rotation uses
+// FP64 cos/sin scalars and rounds to T, everything after it is T arithmetic with Python-float
+// coefficients rounded to T. For T = double this is naca_sdf.
+template <class T>
+__device__ __forceinline__ T naca_sdf_t(const NacaDev& g, T px, T py) {
+  if constexpr (sizeof(T) == 8) {
+    return (T)naca_sdf(g, (double)px, (double)py);
+  } else {
+    T tx = px - (T)g.pivot_x;
+    T ty = py - (T)g.pivot_y;
+    T lx = (T)(g.cos_a * (double)tx + g.sin_a * (double)ty);
+    T ly = (T)((-g.sin_a) * (double)tx + g.cos_a * (double)ty);
+    lx = lx + (T)(0.25 * g.chord);
+    const T chord = (T)g.chord;
+    T x = fminf(fmaxf(lx / chord, (T)0), (T)1);
+    T x2 = x * x;
+    T x3 = (T)((double)x * (double)x * (double)x);
+    T x4 = (T)((double)x2 * (double)x2);
+    T poly = (T)0.2969 * sqrtf(fmaxf(x, (T)0)) - (T)0.1260 * x - (T)0.3516 * x2 + (T)0.2843 * x3 - (T)0.1036 * x4;
+    T half = (T)g.half_scale * poly;
+    T camber = (T)0;
+    if (g.cambered) {
+      if (x < (T)g.p)
+        camber = (T)g.camber_fwd * ((T)g.two_p * x - x2);
+      else
+        camber = (T)g.camber_aft * (((T)g.one_minus_two_p + (T)g.two_p * x) - x2);
+      camber = camber * chord;
+    }
+    T upper = camber + half;
+    T lower = camber - half;
+    T y = ly;
+    bool inside_x = (lx >= (T)0) && (lx <= chord);
+    T outside_v = fmaxf(y - upper, lower - y);
+    T inside_v = -fminf(upper - y, y - lower);
+    T vertical = ((y <= upper) && (y >= lower)) ? inside_v : outside_v;
+    T ddx = fmaxf(-lx, lx - chord);
+    T outside_x = fmaxf(ddx, (T)0);
+    return inside_x ? vertical : hypotf(outside_x, fmaxf(vertical, (T)0));
+  }
+}
+
+// normals (fb/core/geometry.py:108-126) in the particle dtype.
+template <class T>
+__device__ __forceinline__ void naca_normal_t(const NacaDev& g, T px, T py, T& nx_out, T& ny_out) {
+  const double epsd = fmax(g.chord * 1.0e-4, 1.0e-6);
+  const T eps = (T)epsd;
+  T gx = naca_sdf_t<T>(g, px + eps, py + (T)0) - naca_sdf_t<T>(g, px - eps, py - (T)0);
+  T gy = naca_sdf_t<T>(g, px + (T)0, py + eps) - naca_sdf_t<T>(g, px - (T)0, py - eps);
+  T len = sqrt(gx * gx + gy * gy);
+  if (len < eps) {
+    gx = (T)(-g.sin_a);
+    gy = (T)g.cos_a;
+    len = (T)1;
+  }
+  T den = len > eps ? len : eps;
+  nx_out = gx / den;
+  ny_out = gy / den;
+}
+
+#pragma clang fp contract(fast)
+
 struct CellGrid {
   int nx, ny;
   double sx, stx, ex;  // x centres: start, step, stop

@@ -857,10 +857,10 @@ static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const 
     int fo = other3(L.cvalid ? L.cf : -1, fcur);
     // the outlet history only evolves with a non-periodic x axis (fb/solvers/lbm.py:628-640)
     int oo = L.dom.periodic_x ? L.o : other3(L.cvalid ? L.co : -1, L.o);
-    if (ev) FB_HIP(hipEventRecord(ev[2 * k], L.ctx->stream));
+    if (ev && k == 0) FB_HIP(hipEventRecord(ev[0], L.ctx->stream));
     if ((rc = launch_step<P>(L, p, fcur, false, fo, L.o, oo, L.g, g_next, last, pre_out, moving)))
       return rc;
-    if (ev) FB_HIP(hipEventRecord(ev[2 * k + 1], L.ctx->stream));
+    if (ev && last) FB_HIP(hipEventRecord(ev[1], L.ctx->stream));
     fcur = fo;
     L.f = fo;
     L.o = oo;
@@ -1261,9 +1261,9 @@ int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void*
   return fb::macro_dispatch(*L, velocity_scale, rho_host, vel_host);
 }
 
-int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* wall_ms, double* kernel_ms) {
-  FB_REQUIRE(lbm && params && wall_ms && kernel_ms, FB_ERR_ARG, "NULL argument");
-  FB_REQUIRE(steps >= 1 && steps <= 1000000, FB_ERR_ARG, "steps out of range");
+int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* elapsed_ms) {
+  FB_REQUIRE(lbm && params && elapsed_ms, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(steps >= 1 && steps <= 100000000, FB_ERR_ARG, "steps out of range");
   auto* L = reinterpret_cast<Lattice*>(lbm);
   FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
   FB_HIP(hipSetDevice(L->ctx->device));
@@ -1272,24 +1272,20 @@ int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, doubl
     int rc = fb::advance_dispatch(*L, *params, 1, nullptr, nullptr, false);
     if (rc) return rc;
   }
-  std::vector<hipEvent_t> ev(2 * (size_t)steps);
-  for (auto& e : ev) FB_HIP(hipEventCreate(&e));
+  hipEvent_t ev[2];
+  FB_HIP(hipEventCreate(&ev[0]));
+  FB_HIP(hipEventCreate(&ev[1]));
   FB_HIP(hipStreamSynchronize(L->ctx->stream));
-  int rc = fb::advance_dispatch(*L, *params, steps, nullptr, nullptr, false, ev.data());
+  // One event pair around the back-to-back launches: events between launches would add gaps.
+  int rc = fb::advance_dispatch(*L, *params, steps, nullptr, nullptr, false, ev);
   if (rc == FB_OK) {
-    FB_HIP(hipEventSynchronize(ev.back()));
-    double tot = 0.0;
-    for (int k = 0; k < steps; ++k) {
-      float ms = 0.f;
-      FB_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-      tot += ms;
-    }
-    float wall = 0.f;
-    FB_HIP(hipEventElapsedTime(&wall, ev.front(), ev.back()));
-    *wall_ms = wall;
-    *kernel_ms = tot;
+    FB_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    *elapsed_ms = ms;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
   return rc;
 }
 

@@ -1,0 +1,80 @@
+// MAC-grid handle shared by the Stable Fluids substep and the PIC/FLIP grid stages.
+#pragma once
+
+#include "geometry.h"
+#include "poisson.h"
+
+namespace fb {
+
+#pragma clang fp contract(off)
+
+struct MacGeom {
+  int nx, ny, px, py;
+  double x0, y0, dx, dy;
+  // cell-centre linspace coefficients
+  double sx, stx, ex, sy, sty, ey;
+};
+
+inline MacGeom make_geom(const fb_domain& d) {
+  MacGeom g;
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.px = d.periodic_x;
+  g.py = d.periodic_y;
+  g.x0 = d.x0;
+  g.y0 = d.y0;
+  g.dx = (d.x1 - d.x0) / d.nx;
+  g.dy = (d.y1 - d.y0) / d.ny;
+  CellGrid c = make_cell_grid(d);
+  g.sx = c.sx;
+  g.stx = c.stx;
+  g.ex = c.ex;
+  g.sy = c.sy;
+  g.sty = c.sty;
+  g.ey = c.ey;
+  return g;
+}
+
+#pragma clang fp contract(fast)
+
+struct MacGrid {
+  Ctx* ctx = nullptr;
+  fb_domain dom{};
+  fb_foil foil{};
+  MacGeom g{};
+  int prec = FB_F64;
+  size_t sz = 8;
+  size_t N = 0, NU = 0, NV = 0;
+  void *U[2] = {}, *V[2] = {};  // current faces and scratch
+  int cur = 0;
+  void *CU[4] = {}, *CV[4] = {};  // cell planes: velocity, first, corrected, spare
+  void *DU[3] = {}, *DV[3] = {};  // diffusion: orig, ping, pong (face-sized to serve PIC)
+  void *CKU = nullptr, *CKV = nullptr;  // checkpoint faces
+  double* sdf = nullptr;
+  uint8_t* solid = nullptr;
+  uint8_t* ck_solid = nullptr;
+  double angle = 0.0, ck_angle = 0.0;
+  bool geom_valid = false, ck_valid = false;
+  double* rhs = nullptr;
+  double* pres = nullptr;
+  unsigned long long* keys = nullptr;
+  unsigned long long* host_keys = nullptr;
+  unsigned long long* resid = nullptr;  // 2 x (maxit + 2)
+  unsigned long long* host_resid = nullptr;
+  int resid_cap = 0;
+  Poisson poisson;
+  bool poisson_ready = false;
+};
+
+int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision, MacGrid** out);
+void free_mac(MacGrid* M);
+int mac_geometry(MacGrid& M, const fb_pose& pose);
+template <class T>
+int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int W1, int H1, int P1, double nu,
+            double dt, double tol, int maxit, T** result, fb_sf_report* rep);
+template <class T>
+int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep);
+template <class T>
+int mac_stats(MacGrid& M, double w, fb_mac_stats* st);
+
+}  // namespace fb

@@ -9,39 +9,11 @@
 #include <algorithm>
 #include <vector>
 
-#include "geometry.h"
-#include "poisson.h"
+#include "mac.h"
 
 namespace fb {
 
 #pragma clang fp contract(off)
-
-struct MacGeom {
-  int nx, ny, px, py;
-  double x0, y0, dx, dy;
-  // cell-centre linspace coefficients
-  double sx, stx, ex, sy, sty, ey;
-};
-
-inline MacGeom make_geom(const fb_domain& d) {
-  MacGeom g;
-  g.nx = d.nx;
-  g.ny = d.ny;
-  g.px = d.periodic_x;
-  g.py = d.periodic_y;
-  g.x0 = d.x0;
-  g.y0 = d.y0;
-  g.dx = (d.x1 - d.x0) / d.nx;
-  g.dy = (d.y1 - d.y0) / d.ny;
-  CellGrid c = make_cell_grid(d);
-  g.sx = c.sx;
-  g.stx = c.stx;
-  g.ex = c.ex;
-  g.sy = c.sy;
-  g.sty = c.sty;
-  g.ey = c.ey;
-  return g;
-}
 
 __device__ __forceinline__ double cx_at(const MacGeom& g, int i) { return linspace_at(i, g.nx, g.sx, g.stx, g.ex); }
 __device__ __forceinline__ double cy_at(const MacGeom& g, int j) { return linspace_at(j, g.ny, g.sy, g.sty, g.ey); }
@@ -476,36 +448,7 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
 
 // ---------------------------------------------------------------- host handle
 
-struct MacGrid {
-  Ctx* ctx = nullptr;
-  fb_domain dom{};
-  fb_foil foil{};
-  MacGeom g{};
-  int prec = FB_F64;
-  size_t sz = 8;
-  size_t N = 0, NU = 0, NV = 0;
-  void *U[2] = {}, *V[2] = {};  // current faces and scratch
-  int cur = 0;
-  void *CU[4] = {}, *CV[4] = {};  // cell planes: velocity, first, corrected, spare
-  void *DU[3] = {}, *DV[3] = {};  // diffusion: orig, ping, pong (face-sized to serve PIC)
-  void *CKU = nullptr, *CKV = nullptr;  // checkpoint faces
-  double* sdf = nullptr;
-  uint8_t* solid = nullptr;
-  uint8_t* ck_solid = nullptr;
-  double angle = 0.0, ck_angle = 0.0;
-  bool geom_valid = false, ck_valid = false;
-  double* rhs = nullptr;
-  double* pres = nullptr;
-  unsigned long long* keys = nullptr;
-  unsigned long long* host_keys = nullptr;
-  unsigned long long* resid = nullptr;  // 2 x (maxit + 2)
-  unsigned long long* host_resid = nullptr;
-  int resid_cap = 0;
-  Poisson poisson;
-  bool poisson_ready = false;
-};
-
-static int mac_geometry(MacGrid& M, const fb_pose& pose) {
+int mac_geometry(MacGrid& M, const fb_pose& pose) {
   int rc = launch_cell_sdf(M.ctx->stream, make_cell_grid(M.dom), make_naca(M.foil, pose), M.sdf, M.solid);
   if (rc) return rc;
   M.angle = pose.angle_degrees;
@@ -524,7 +467,7 @@ static int read_keys(MacGrid& M, int n) {
 }
 
 template <class T>
-static int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
+int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
   FB_HIP(hipMemsetAsync(M.keys, 0, 8 * sizeof(unsigned long long), M.ctx->stream));
   StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.keys};
   dim3 grid((M.dom.nx + 1 + 255) / 256, M.dom.ny + 1);
@@ -546,7 +489,7 @@ static int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
 
 // Jacobi implicit viscosity on two fields (cell planes, or u/v faces for PIC).
 template <class T>
-static int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int W1, int H1, int P1,
+int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int W1, int H1, int P1,
                    double nu, double dt, double tol, int maxit, T** result, fb_sf_report* rep) {
   hipStream_t s = M.ctx->stream;
   const int px = M.dom.periodic_x, py = M.dom.periodic_y;
@@ -641,7 +584,7 @@ static int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P
 }
 
 template <class T>
-static int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
+int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   hipStream_t s = M.ctx->stream;
   int rc;
   // projection CFL guard (fb/solvers/stable_fluids.py:256-298)
@@ -737,7 +680,16 @@ static int substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf
   return project<T>(M, p, p.dt, rep);
 }
 
-static void free_mac(MacGrid* M) {
+template int diffuse<double>(MacGrid&, const double*, const double*, int, int, int, int, int, int, double, double,
+                             double, int, double**, fb_sf_report*);
+template int diffuse<float>(MacGrid&, const float*, const float*, int, int, int, int, int, int, double, double, double,
+                            int, float**, fb_sf_report*);
+template int project<double>(MacGrid&, const fb_sf_params&, double, fb_sf_report*);
+template int project<float>(MacGrid&, const fb_sf_params&, double, fb_sf_report*);
+template int mac_stats<double>(MacGrid&, double, fb_mac_stats*);
+template int mac_stats<float>(MacGrid&, double, fb_mac_stats*);
+
+void free_mac(MacGrid* M) {
   for (void* ptr : {M->U[0], M->U[1], M->V[0], M->V[1], M->CU[0], M->CU[1], M->CU[2], M->CU[3], M->CV[0], M->CV[1],
                     M->CV[2], M->CV[3], M->DU[0], M->DU[1], M->DU[2], M->DV[0], M->DV[1], M->DV[2], M->CKU, M->CKV,
                     (void*)M->sdf, (void*)M->solid, (void*)M->ck_solid, (void*)M->rhs, (void*)M->pres,
@@ -759,15 +711,26 @@ extern "C" {
 
 int fb_mac_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, fb_mac** out) {
   FB_REQUIRE(ctx && dom && foil && out, FB_ERR_ARG, "NULL argument");
+  MacGrid* M = nullptr;
+  int rc = fb::create_mac(reinterpret_cast<fb::Ctx*>(ctx), dom, foil, precision, &M);
+  if (rc) return rc;
+  *out = reinterpret_cast<fb_mac*>(M);
+  return FB_OK;
+}
+
+}  // extern "C"
+
+namespace fb {
+
+int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision, MacGrid** out) {
   FB_REQUIRE(dom->nx >= 4 && dom->ny >= 4, FB_ERR_ARG, "each resolution axis must contain at least four cells");
   FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "MAC storage must be FB_F64 or FB_F32");
-  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
   FB_HIP(hipSetDevice(c->device));
   auto* M = new MacGrid();
   M->ctx = c;
   M->dom = *dom;
   M->foil = *foil;
-  M->g = fb::make_geom(*dom);
+  M->g = make_geom(*dom);
   M->prec = precision;
   M->sz = precision == FB_F64 ? 8 : 4;
   M->N = (size_t)dom->nx * dom->ny;
@@ -800,13 +763,17 @@ int fb_mac_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   A((void**)&M->keys, 16 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipHostMalloc((void**)&M->host_keys, 16 * sizeof(unsigned long long), 0);
   if (e != hipSuccess) {
-    fb::set_error("device allocation failed: %s", hipGetErrorString(e));
-    fb::free_mac(M);
+    set_error("device allocation failed: %s", hipGetErrorString(e));
+    free_mac(M);
     return FB_ERR_NOMEM;
   }
-  *out = reinterpret_cast<fb_mac*>(M);
+  *out = M;
   return FB_OK;
 }
+
+}  // namespace fb
+
+extern "C" {
 
 int fb_mac_destroy(fb_mac* mac) {
   if (!mac) return FB_OK;

@@ -153,9 +153,9 @@ int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void*
 /* Timing harness: run `steps` lattice steps with fixed parameters and no pose change,
  * without postcondition reductions; returns once queued (use fb_ctx_synchronize). */
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps);
-/* Same, timed with a HIP event pair around every step launch on the handle's stream:
- * wall_ms spans first to last launch, kernel_ms sums the per-launch durations. */
-int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* wall_ms, double* kernel_ms);
+/* Same, timed with one HIP event pair on the handle's stream around the `steps` back-to-back
+ * step launches; elapsed_ms / steps is the average launch duration (gaps included). */
+int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, double* elapsed_ms);
 
 /* ------------------------------------------------------------ Stable Fluids (MAC) ---- */
 /* Replaces the MAC-grid stages of fb/solvers/stable_fluids.py:424-494 and fb/core/grid.py
