@@ -103,6 +103,29 @@ SF_STATUS = {
     9: ("nonfinite_state", "projection", "projection produced non-finite velocity"),
 }
 
+class fb_pic_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("viscosity", C.c_double), ("tolerance", C.c_double),
+        ("max_iterations", C.c_int32), ("channel_walls", C.c_int32),
+        ("freestream_x", C.c_double), ("freestream_y", C.c_double), ("wall_omega_rad", C.c_double),
+        ("blend", C.c_double), ("wall_travel", C.c_double), ("maximum_radius", C.c_double),
+        ("mixed_poisson", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class fb_pic_report(C.Structure):
+    _fields_ = [("grid", fb_sf_report), ("unsupported_faces", C.c_int64), ("swept_hits", C.c_int64),
+                ("escaped", C.c_int64)]
+
+
+class fb_pic_stats(C.Structure):
+    _fields_ = [
+        ("max_particle_speed", C.c_double), ("max_grid_speed", C.c_double), ("particles_inside", C.c_int64),
+        ("nonfinite", C.c_int64), ("fluid_cells", C.c_int64), ("empty_cells", C.c_int64),
+        ("underfilled_cells", C.c_int64), ("particle_count", C.c_int64),
+    ]
+
+
 _P = C.c_void_p
 _SIGNATURES: dict[str, tuple] = {
     "fb_last_error": (C.c_char_p, []),
@@ -144,6 +167,30 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_sf_substep": (C.c_int, [_P, C.POINTER(fb_sf_params), C.POINTER(fb_pose), C.POINTER(fb_sf_report)]),
     "fb_mac_project": (C.c_int, [_P, C.POINTER(fb_sf_params), C.POINTER(fb_sf_report)]),
     "fb_mac_get_stats": (C.c_int, [_P, C.c_double, C.POINTER(fb_mac_stats)]),
+    "fb_pic_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.POINTER(_P)]),
+    "fb_pic_destroy": (C.c_int, [_P]),
+    "fb_pic_mac": (C.c_int, [_P, C.POINTER(_P)]),
+    "fb_pic_set_pose": (C.c_int, [_P, C.POINTER(fb_pose)]),
+    "fb_pic_set_grid": (C.c_int, [_P, _P, _P]),
+    "fb_pic_get_grid": (C.c_int, [_P, _P, _P]),
+    "fb_pic_set_rng": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
+    "fb_pic_get_rng": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fb_pic_seed": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    "fb_pic_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "fb_pic_set_particles": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "fb_pic_get_particles": (C.c_int, [_P, _P, _P]),
+    "fb_pic_particle_to_grid": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    "fb_pic_regrid": (C.c_int, [_P, _P, C.c_int64]),
+    "fb_pic_resolve_collisions": (C.c_int, [_P, C.POINTER(fb_pose), C.c_double]),
+    "fb_pic_substep": (C.c_int, [_P, C.POINTER(fb_pic_params), C.POINTER(fb_pose), _P, C.POINTER(fb_pic_report)]),
+    "fb_pic_get_stats": (C.c_int, [_P, C.POINTER(fb_pose), C.POINTER(fb_pic_stats)]),
+    "fb_pic_checkpoint": (C.c_int, [_P]),
+    "fb_pic_rollback": (C.c_int, [_P]),
+    "fb_pic_particle_to_faces": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, C.c_int, C.c_int, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P, _P,
+                                           C.POINTER(C.c_double)]),
+    "fb_pic_faces_to_particle": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, C.c_int, C.c_int, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P]),
 }
 
 _lock = threading.Lock()

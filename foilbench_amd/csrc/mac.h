@@ -76,5 +76,11 @@ template <class T>
 int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep);
 template <class T>
 int mac_stats(MacGrid& M, double w, fb_mac_stats* st);
+template <class T>
+int launch_faces_to_cell(MacGrid& M, const T* u, const T* v, T* cu, T* cv);
+template <class T>
+int launch_cell_to_faces(MacGrid& M, const T* cu, const T* cv, T* u, T* v);
+template <class T>
+int launch_dup_copy(MacGrid& M, T* f, int rows, int cols, int pitch, int dup_x, int dup_y);
 
 }  // namespace fb

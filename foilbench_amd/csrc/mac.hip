@@ -680,6 +680,39 @@ static int substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf
   return project<T>(M, p, p.dt, rep);
 }
 
+// Launch helpers of the grid kernels shared with the PIC/FLIP path.
+template <class T>
+int launch_faces_to_cell(MacGrid& M, const T* u, const T* v, T* cu, T* cv) {
+  dim3 gc((M.dom.nx + 255) / 256, M.dom.ny);
+  hipLaunchKernelGGL(faces_to_cell_kernel<T>, gc, dim3(256), 0, M.ctx->stream, M.dom.nx, M.dom.ny, u, v, cu, cv);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+template <class T>
+int launch_cell_to_faces(MacGrid& M, const T* cu, const T* cv, T* u, T* v) {
+  dim3 gf((M.dom.nx + 1 + 255) / 256, M.dom.ny + 1);
+  hipLaunchKernelGGL(cell_to_faces_kernel<T>, gf, dim3(256), 0, M.ctx->stream, M.dom.nx, M.dom.ny, cu, cv, u, v);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+template <class T>
+int launch_dup_copy(MacGrid& M, T* f, int rows, int cols, int pitch, int dup_x, int dup_y) {
+  if (!dup_x && !dup_y) return FB_OK;
+  int n = std::max(rows, cols);
+  hipLaunchKernelGGL(dup_copy_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, M.ctx->stream, f, rows, cols, pitch,
+                     dup_x, dup_y);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+template int launch_faces_to_cell<double>(MacGrid&, const double*, const double*, double*, double*);
+template int launch_faces_to_cell<float>(MacGrid&, const float*, const float*, float*, float*);
+template int launch_cell_to_faces<double>(MacGrid&, const double*, const double*, double*, double*);
+template int launch_cell_to_faces<float>(MacGrid&, const float*, const float*, float*, float*);
+template int launch_dup_copy<double>(MacGrid&, double*, int, int, int, int, int);
+template int launch_dup_copy<float>(MacGrid&, float*, int, int, int, int, int);
 template int diffuse<double>(MacGrid&, const double*, const double*, int, int, int, int, int, int, double, double,
                              double, int, double**, fb_sf_report*);
 template int diffuse<float>(MacGrid&, const float*, const float*, int, int, int, int, int, int, double, double, double,

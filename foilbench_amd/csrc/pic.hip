@@ -1,0 +1,1589 @@
+// Blended PIC/FLIP on gfx950: the particle half of fb/solvers/pic_flip.py (seeding, collisions,
+// swept collisions, midpoint advection, periodic wrap, escape respawn) and the quadratic
+// B-spline transfers of kern/pic.py:105-205 behind fb/solvers/_numba_adapter.py:108-160.
+// The grid half (implicit viscosity on faces, projection) is the MAC grid of mac.hip.
+//
+// Layout: positions and velocities (P, 2) interleaved in the storage dtype, like the
+// reference arrays; the grid velocity as two (ny, nx) cell planes.
+//
+// P2G is a per-face gather over particles sorted by cell (counting sort, then each cell's
+// list ordered by particle index), so every face sums its contributions in a fixed order
+// and no floating-point atomics are used. Each addition follows the numba kernel: the FP64
+// contribution is added to the dtype accumulator in FP64 and rounded back (kern/pic.py:144-145).
+// G2P is a per-particle gather in the reference's fixed 3 x 3 order with FP64 accumulation.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mac.h"
+
+namespace fb {
+
+#pragma clang fp contract(off)
+
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+template <class T>
+__device__ __forceinline__ void ld2(const T* __restrict__ a, size_t i, T& x, T& y) {
+  auto v = reinterpret_cast<const typename Vec2<T>::type*>(a)[i];
+  x = v.x;
+  y = v.y;
+}
+template <class T>
+__device__ __forceinline__ void st2(T* __restrict__ a, size_t i, T x, T y) {
+  typename Vec2<T>::type v;
+  v.x = x;
+  v.y = y;
+  reinterpret_cast<typename Vec2<T>::type*>(a)[i] = v;
+}
+
+struct PicGeom {
+  int nx, ny, px, py;
+  double x0, y0, x1, y1, dx, dy;
+};
+
+inline PicGeom make_pic_geom(const fb_domain& d) {
+  PicGeom g;
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.px = d.periodic_x;
+  g.py = d.periodic_y;
+  g.x0 = d.x0;
+  g.y0 = d.y0;
+  g.x1 = d.x1;
+  g.y1 = d.y1;
+  g.dx = (d.x1 - d.x0) / d.nx;
+  g.dy = (d.y1 - d.y0) / d.ny;
+  return g;
+}
+
+// One face component of the transfers (fb/solvers/_numba_adapter.py:126-133, 150-157).
+struct Comp {
+  int W, H;    // stored width and height
+  int uw, uh;  // unique width and height (a periodic axis drops its duplicate endpoint)
+  int dupx, dupy;
+  double gxo, gyo;
+};
+
+__host__ __device__ inline Comp comp_u(const PicGeom& g) {
+  Comp c;
+  c.W = g.nx + 1;
+  c.H = g.ny;
+  c.dupx = g.px;
+  c.dupy = 0;
+  c.uw = c.W - c.dupx;
+  c.uh = c.H;
+  c.gxo = 0.0;
+  c.gyo = -0.5;
+  return c;
+}
+
+__host__ __device__ inline Comp comp_v(const PicGeom& g) {
+  Comp c;
+  c.W = g.nx;
+  c.H = g.ny + 1;
+  c.dupx = 0;
+  c.dupy = g.py;
+  c.uw = c.W;
+  c.uh = c.H - c.dupy;
+  c.gxo = -0.5;
+  c.gyo = 0.0;
+  return c;
+}
+
+// Quadratic B-spline (kern/pic.py:11-19).
+__device__ __forceinline__ double bspline(double d) {
+  double a = fabs(d);
+  if (a < 0.5) return 0.75 - a * a;
+  if (a < 1.5) {
+    double e = 1.5 - a;
+    return 0.5 * e * e;
+  }
+  return 0.0;
+}
+
+// Python's integer modulo.
+__device__ __forceinline__ int pmod(int a, int n) {
+  int m = a % n;
+  return m < 0 ? m + n : m;
+}
+
+// int(np.floor(v)); values beyond +-1e9 are clamped (they lie outside every face range).
+__device__ __forceinline__ int ifloor(double v) {
+  double f = floor(v);
+  f = fmin(fmax(f, -1.0e9), 1.0e9);
+  return (int)f;
+}
+
+// Gather of one component at (x, y) (kern/pic.py:164-205): FP64 weights and sums in the
+// fixed 3 x 3 order. With `q` and `p` given, also gathers the dtype difference q - p with the
+// same weights (the FLIP increment of fb/solvers/pic_flip.py:822-826).
+template <class T, bool DIFF>
+__device__ __forceinline__ void gather(const PicGeom& g, const Comp& c, const T* __restrict__ q,
+                                       const T* __restrict__ p, double x, double y, double& out_q,
+                                       double& out_d) {
+  double gx = (x - g.x0) / g.dx + c.gxo;
+  double gy = (y - g.y0) / g.dy + c.gyo;
+  int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+  double vq = 0.0, vd = 0.0, ws = 0.0;
+  for (int oy = 0; oy < 3; ++oy) {
+    int sy = by + oy;
+    int ty = g.py ? pmod(sy, c.uh) : sy;
+    if (ty < 0 || ty >= c.H) continue;
+    double wy = bspline(gy - (double)sy);
+    for (int ox = 0; ox < 3; ++ox) {
+      int sx = bx + ox;
+      int tx = g.px ? pmod(sx, c.uw) : sx;
+      if (tx < 0 || tx >= c.W) continue;
+      double w = wy * bspline(gx - (double)sx);
+      size_t k = (size_t)ty * c.W + tx;
+      T qv = q[k];
+      vq += w * (double)qv;
+      if constexpr (DIFF) vd += w * (double)(T)(qv - p[k]);
+      ws += w;
+    }
+  }
+  double den = fmax(ws, 1.0e-12);
+  out_q = vq / den;
+  if constexpr (DIFF) out_d = vd / den;
+}
+
+// Home cell used to bin particles for the P2G gather: the cell containing the particle,
+// wrapped on periodic axes and clamped otherwise.
+__device__ __forceinline__ int bin_cell(const PicGeom& g, double x, double y) {
+  int cx = ifloor((x - g.x0) / g.dx);
+  int cy = ifloor((y - g.y0) / g.dy);
+  cx = g.px ? pmod(cx, g.nx) : min(max(cx, 0), g.nx - 1);
+  cy = g.py ? pmod(cy, g.ny) : min(max(cy, 0), g.ny - 1);
+  return cy * g.nx + cx;
+}
+
+__device__ __forceinline__ long long wave_sum_i(long long v) { return wave_sum_ll(v); }
+
+// ------------------------------------------------------------------ PCG32 (fb/core/rng.py)
+
+struct Pcg {
+  unsigned long long state, inc;
+};
+
+constexpr unsigned long long kPcgMult = 6364136223846793005ull;
+
+__host__ __device__ inline unsigned pcg_output(unsigned long long old) {
+  unsigned xs = (unsigned)(((old >> 18) ^ old) >> 27);
+  unsigned rot = (unsigned)(old >> 59);
+  return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+// State after `delta` steps (the logarithmic jump-ahead of O'Neill's PCG paper, section 4.3.1).
+__host__ __device__ inline unsigned long long pcg_advance(unsigned long long state, unsigned long long inc,
+                                                          unsigned long long delta) {
+  unsigned long long cur_mult = kPcgMult, cur_plus = inc, acc_mult = 1ull, acc_plus = 0ull;
+  while (delta > 0) {
+    if (delta & 1ull) {
+      acc_mult *= cur_mult;
+      acc_plus = acc_plus * cur_mult + cur_plus;
+    }
+    cur_plus = (cur_mult + 1ull) * cur_plus;
+    cur_mult *= cur_mult;
+    delta >>= 1;
+  }
+  return acc_mult * state + acc_plus;
+}
+
+// float32(u32) * float32(2^-32) (fb/core/rng.py:46-52).
+__device__ __forceinline__ float pcg_unit(unsigned u) { return (float)u * (1.0f / 4294967296.0f); }
+
+// Draw number `k` (0-based) after the state `s`.
+__device__ __forceinline__ float pcg_draw(const Pcg& s, unsigned long long k) {
+  return pcg_unit(pcg_output(pcg_advance(s.state, s.inc, k)));
+}
+
+// uniform(low, high) of fb/core/rng.py:54-55: float32 arithmetic with Python floats rounded.
+__device__ __forceinline__ float pcg_uniform(float r, double low, double high) {
+  return (float)low + (float)(high - low) * r;
+}
+
+// ------------------------------------------------------------------ scan
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of 1024 ints per block; block totals into bsum.
+__global__ void __launch_bounds__(256) scan_block_kernel(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                         int* __restrict__ bsum) {
+  __shared__ int wsum[4];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int v[4];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int incl = wave_incl_scan(s);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int off = 0;
+  for (int j = 0; j < w; ++j) off += wsum[j];
+  int run = off + incl - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = off + incl;
+}
+
+// Exclusive scan of the block totals in place (one block); the grand total into *total.
+__global__ void __launch_bounds__(256) scan_sums_kernel(int* __restrict__ bsum, int nb, int* __restrict__ total) {
+  __shared__ int wsum[4];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < nb; c0 += 1024) {
+    const int base = c0 + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = (base + k < nb) ? bsum[base + k] : 0;
+      s += v[k];
+    }
+    int incl = wave_incl_scan(s);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int off = carry;
+    for (int j = 0; j < w; ++j) off += wsum[j];
+    int run = off + incl - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (base + k < nb) bsum[base + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 255) carry = off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) scan_add_kernel(int* __restrict__ out, int n, const int* __restrict__ bsum,
+                                                       const int* __restrict__ total) {
+  const int base = blockIdx.x * 1024;
+  const int add = bsum[blockIdx.x];
+  for (int k = threadIdx.x; k < 1024; k += 256)
+    if (base + k < n) out[base + k] += add;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+}
+
+__global__ void __launch_bounds__(256) fill_int_kernel(int* __restrict__ a, int n, int v) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ------------------------------------------------------------------ sort by cell
+
+template <class T>
+__global__ void __launch_bounds__(256) bin_count_kernel(PicGeom g, int n, const T* __restrict__ pos,
+                                                        int* __restrict__ cellof, int* __restrict__ count) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  T x, y;
+  ld2(pos, i, x, y);
+  int c = bin_cell(g, (double)x, (double)y);
+  cellof[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+__global__ void __launch_bounds__(256) bin_scatter_kernel(int n, const int* __restrict__ cellof,
+                                                          const int* __restrict__ start, int* __restrict__ cursor,
+                                                          int* __restrict__ sidx) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int c = cellof[i];
+  sidx[start[c] + atomicAdd(&cursor[c], 1)] = i;
+}
+
+// Order each cell's particles by index (insertion sort; cells hold a handful of particles).
+__global__ void __launch_bounds__(256) cell_sort_kernel(int ncell, const int* __restrict__ start,
+                                                        int* __restrict__ sidx) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  int a = start[c], b = start[c + 1];
+  for (int i = a + 1; i < b; ++i) {
+    int key = sidx[i];
+    int j = i - 1;
+    while (j >= a && sidx[j] > key) {
+      sidx[j + 1] = sidx[j];
+      --j;
+    }
+    sidx[j + 1] = key;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) sorted_copy_kernel(int n, const int* __restrict__ sidx,
+                                                          const T* __restrict__ pos, const T* __restrict__ vel,
+                                                          T* __restrict__ spos, T* __restrict__ sval) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int p = sidx[k];
+  T x, y;
+  ld2(pos, p, x, y);
+  st2(spos, k, x, y);
+  ld2(vel, p, x, y);
+  st2(sval, k, x, y);
+}
+
+// ------------------------------------------------------------------ P2G
+
+template <class T>
+struct P2GArgs {
+  PicGeom g;
+  Comp c;
+  int comp;  // 0: u faces from velocity x, 1: v faces from velocity y
+  const int* start;
+  const T* spos;
+  const T* sval;
+  const T* fallback;
+  T* out;
+  unsigned long long* unsupported;
+};
+
+// scatter_face_component_kernel (kern/pic.py:105-161) as a gather: face (fx, fy) visits the
+// 4 x 4 cells whose particles can reach it and keeps those whose 3 x 3 support contains it.
+template <class T>
+__global__ void __launch_bounds__(256) p2g_kernel(P2GArgs<T> a) {
+  const Comp& c = a.c;
+  const PicGeom& g = a.g;
+  const int tx = blockIdx.x * 256 + threadIdx.x;
+  const int ty = blockIdx.y;
+  long long uns = 0;
+  if (tx < c.W && ty < c.H) {
+    const bool dupc = c.dupx && tx == c.W - 1;
+    const bool dupr = c.dupy && ty == c.H - 1;
+    const int fx = dupc ? 0 : tx;
+    const int fy = dupr ? 0 : ty;
+    T wsum = (T)0, msum = (T)0;
+    for (int wy = -2; wy <= 1; ++wy) {
+      int cy = fy + wy;
+      if (g.py) cy = pmod(cy, g.ny);
+      else if (cy < 0 || cy >= g.ny) continue;
+      for (int wx = -2; wx <= 1; ++wx) {
+        int cx = fx + wx;
+        if (g.px) cx = pmod(cx, g.nx);
+        else if (cx < 0 || cx >= g.nx) continue;
+        const int cell = cy * g.nx + cx;
+        const int k1 = a.start[cell + 1];
+        for (int k = a.start[cell]; k < k1; ++k) {
+          T x, y;
+          ld2(a.spos, k, x, y);
+          double gx = ((double)x - g.x0) / g.dx + c.gxo;
+          double gy = ((double)y - g.y0) / g.dy + c.gyo;
+          int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+          int sx, sy;
+          if (g.px) {
+            int o = pmod(fx - bx, c.uw);
+            if (o > 2) continue;
+            sx = bx + o;
+          } else {
+            if (fx < bx || fx > bx + 2) continue;
+            sx = fx;
+          }
+          if (g.py) {
+            int o = pmod(fy - by, c.uh);
+            if (o > 2) continue;
+            sy = by + o;
+          } else {
+            if (fy < by || fy > by + 2) continue;
+            sy = fy;
+          }
+          double w = bspline(gy - (double)sy) * bspline(gx - (double)sx);
+          T vx, vy;
+          ld2(a.sval, k, vx, vy);
+          double val = (double)(a.comp ? vy : vx);
+          wsum = (T)((double)wsum + w);
+          msum = (T)((double)msum + w * val);
+        }
+      }
+    }
+    T outv;
+    bool supported = (double)wsum > 1.0e-12;
+    if (supported) outv = msum / wsum;
+    else outv = a.fallback[(size_t)fy * c.W + fx];
+    // the duplicate endpoint itself never receives weight (kern/pic.py:136-151)
+    uns = (!supported || dupc || dupr) ? 1 : 0;
+    a.out[(size_t)ty * c.W + tx] = outv;
+  }
+  uns = wave_sum_i(uns);
+  if ((threadIdx.x & 63) == 0 && uns) atomicAdd(a.unsupported, (unsigned long long)uns);
+}
+
+// ------------------------------------------------------------------ G2P and the FLIP update
+
+template <class T>
+__global__ void __launch_bounds__(256) g2p_kernel(PicGeom g, int n, const int* __restrict__ idx,
+                                                  const T* __restrict__ pos, const T* __restrict__ u,
+                                                  const T* __restrict__ v, T* __restrict__ vel) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  int i = idx ? idx[k] : k;
+  T x, y;
+  ld2(pos, i, x, y);
+  double a, b, dummy;
+  gather<T, false>(g, comp_u(g), u, nullptr, (double)x, (double)y, a, dummy);
+  gather<T, false>(g, comp_v(g), v, nullptr, (double)x, (double)y, b, dummy);
+  st2(vel, i, (T)a, (T)b);
+}
+
+
+// ------------------------------------------------------------------ collisions
+
+// ω x r wall velocity (fb/core/geometry.py:131-138): the FP64 angular velocity times the
+// dtype offset from the pivot, rounded to the dtype.
+template <class T>
+__device__ __forceinline__ void wall_t(const NacaDev& g, double omega, T x, T y, T& wx, T& wy) {
+  T rx = x - (T)g.pivot_x, ry = y - (T)g.pivot_y;
+  wx = (T)((-omega) * (double)ry);
+  wy = (T)(omega * (double)rx);
+}
+
+// Remove the inward wall-relative normal velocity (fb/solvers/pic_flip.py:589-596, 612-617).
+template <class T>
+__device__ __forceinline__ void reflect_t(T& vx, T& vy, T wx, T wy, T nx, T ny) {
+  T rx = vx - wx, ry = vy - wy;
+  T inward = rx * nx + ry * ny;
+  if (inward < (T)0) {
+    rx = rx - inward * nx;
+    ry = ry - inward * ny;
+  }
+  vx = wx + rx;
+  vy = wy + ry;
+}
+
+// _resolve_particle_collisions (fb/solvers/pic_flip.py:604-618).
+template <class T>
+__global__ void __launch_bounds__(256) collide_kernel(int n, T* __restrict__ pos, T* __restrict__ vel, NacaDev g,
+                                                      double omega) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  T x, y;
+  ld2(pos, i, x, y);
+  T d = naca_sdf_t<T>(g, x, y);
+  if (!(d <= (T)0)) return;
+  T nx, ny;
+  naca_normal_t<T>(g, x, y, nx, ny);
+  T s = d - (T)1.0e-4;
+  x = x - s * nx;
+  y = y - s * ny;
+  st2(pos, i, x, y);
+  T wx, wy, vx, vy;
+  wall_t(g, omega, x, y, wx, wy);
+  ld2(vel, i, vx, vy);
+  reflect_t(vx, vy, wx, wy, nx, ny);
+  st2(vel, i, vx, vy);
+}
+
+// ------------------------------------------------------------------ FLIP update and advection
+
+struct FlipArgs {
+  PicGeom g;
+  double blend, dt;
+  double pivot_x, pivot_y;
+  double reach;  // maximum_radius + collision margin
+  unsigned long long* travel_key;
+};
+
+// Blend (fb/solvers/pic_flip.py:817-831), then the midpoint step of _advect_particles
+// (:489-494) with the PIC velocity, and the swept-collision candidate test (:533-550).
+template <class T>
+__global__ void __launch_bounds__(256) flip_advect_kernel(FlipArgs a, int n, T* __restrict__ pos,
+                                                          T* __restrict__ vel, T* __restrict__ start,
+                                                          uint8_t* __restrict__ cand, const T* __restrict__ qu,
+                                                          const T* __restrict__ qv, const T* __restrict__ pu,
+                                                          const T* __restrict__ pv) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double travel = 0.0;
+  if (i < n) {
+    const PicGeom& g = a.g;
+    const Comp cu = comp_u(g), cv = comp_v(g);
+    T x, y, vx, vy;
+    ld2(pos, i, x, y);
+    ld2(vel, i, vx, vy);
+    double qa, da, qb, db;
+    gather<T, true>(g, cu, qu, pu, (double)x, (double)y, qa, da);
+    gather<T, true>(g, cv, qv, pv, (double)x, (double)y, qb, db);
+    const T picx = (T)qa, picy = (T)qb, dlx = (T)da, dly = (T)db;
+    const T omb = (T)(1.0 - a.blend), b = (T)a.blend;
+    vx = omb * picx + b * (vx + dlx);
+    vy = omb * picy + b * (vy + dly);
+    st2(vel, i, vx, vy);
+    // midpoint = positions + 0.5 * dt * velocity_0; positions += dt * velocity_mid
+    const T h = (T)(0.5 * a.dt), dt = (T)a.dt;
+    T mx = x + h * picx, my = y + h * picy;
+    double ma, mb, dummy;
+    gather<T, false>(g, cu, qu, nullptr, (double)mx, (double)my, ma, dummy);
+    gather<T, false>(g, cv, qv, nullptr, (double)mx, (double)my, mb, dummy);
+    T nx = x + dt * (T)ma, ny = y + dt * (T)mb;
+    st2(start, i, x, y);
+    st2(pos, i, nx, ny);
+    // candidate: the path passes within maximum_radius + margin of the pivot
+    T px = nx - x, py = ny - y;
+    T pvx = (T)a.pivot_x, pvy = (T)a.pivot_y;
+    T l2 = px * px + py * py;
+    T l2c = l2 > (T)1.0e-12 ? l2 : (T)1.0e-12;
+    T proj = ((pvx - x) * px + (pvy - y) * py) / l2c;
+    proj = proj < (T)0 ? (T)0 : (proj > (T)1 ? (T)1 : proj);
+    T cx = x + proj * px - pvx, cy = y + proj * py - pvy;
+    T dist = sqrt(cx * cx + cy * cy);
+    bool c = dist <= (T)a.reach;
+    cand[i] = c ? 1 : 0;
+    if (c) travel = (double)sqrt(px * px + py * py);
+  }
+  travel = wave_max(travel);
+  if ((threadIdx.x & 63) == 0 && travel > 0.0) atomicMax(a.travel_key, dkey(travel));
+  // a key of 0 (no candidate moved) reads as zero travel
+}
+
+struct SweptArgs {
+  NacaDev g;           // foil with the pivot; cos/sin per sample from the table
+  const double* table;  // (cos, sin) per sample: counts 2..16, samples 1..count
+  double omega;
+  double wall_travel;  // |deg2rad(angle change)| * maximum_radius
+  double spacing;      // 0.1 * min(dx, dy)
+  double margin;       // 0.05 * min(dx, dy)
+  const unsigned long long* travel_key;
+  unsigned long long* hits;
+};
+
+// _resolve_swept_particle_collisions (fb/solvers/pic_flip.py:527-602) for the candidates.
+template <class T>
+__global__ void __launch_bounds__(256) swept_kernel(SweptArgs a, int n, T* __restrict__ pos, T* __restrict__ vel,
+                                                    const T* __restrict__ start, const uint8_t* __restrict__ cand) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  long long hit = 0;
+  if (i < n && cand[i]) {
+    const unsigned long long key = *a.travel_key;
+    double travel = key ? dkey_inv(key) : 0.0;
+    double q = ceil((travel + a.wall_travel) / fmax(a.spacing, 1.0e-12));
+    int count = (int)fmin(fmax(q, 2.0), 16.0);
+    const int off = (count - 1) * count / 2 - 1;
+    T sx, sy, ex, ey;
+    ld2(start, i, sx, sy);
+    ld2(pos, i, ex, ey);
+    const T px = ex - sx, py = ey - sy;
+    const T margin = (T)a.margin;
+    for (int s = 1; s <= count; ++s) {
+      const double frac = (double)s / (double)count;
+      NacaDev gs = a.g;
+      gs.cos_a = a.table[2 * (off + s - 1)];
+      gs.sin_a = a.table[2 * (off + s - 1) + 1];
+      const T f = (T)frac;
+      T qx = sx + f * px, qy = sy + f * py;
+      T d = naca_sdf_t<T>(gs, qx, qy);
+      if (!(d <= margin)) continue;
+      T nx, ny;
+      naca_normal_t<T>(gs, qx, qy, nx, ny);
+      T e = d - margin;
+      T rx = qx - e * nx, ry = qy - e * ny;
+      st2(pos, i, rx, ry);
+      T wx, wy, vx, vy;
+      wall_t(a.g, a.omega, rx, ry, wx, wy);
+      ld2(vel, i, vx, vy);
+      reflect_t(vx, vy, wx, wy, nx, ny);
+      st2(vel, i, vx, vy);
+      hit = 1;
+      break;
+    }
+  }
+  hit = wave_sum_i(hit);
+  if ((threadIdx.x & 63) == 0 && hit) atomicAdd(a.hits, (unsigned long long)hit);
+}
+
+// np.mod for floats of the dtype (Python sign convention).
+template <class T>
+__device__ __forceinline__ T pymod_t(T a, T b) {
+  T m = fmod(a, b);
+  if (m != (T)0) {
+    if ((b < (T)0) != (m < (T)0)) m += b;
+  } else {
+    m = copysign((T)0, b);
+  }
+  return m;
+}
+
+// Periodic wrap and escape flags (fb/solvers/pic_flip.py:499-511).
+template <class T>
+__global__ void __launch_bounds__(256) wrap_escape_kernel(PicGeom g, int n, T* __restrict__ pos,
+                                                          int* __restrict__ flag) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  T x, y;
+  ld2(pos, i, x, y);
+  const T x0 = (T)g.x0, x1 = (T)g.x1, y0 = (T)g.y0, y1 = (T)g.y1;
+  if (g.px) x = x0 + pymod_t<T>(x - x0, (T)(g.x1 - g.x0));
+  if (g.py) y = y0 + pymod_t<T>(y - y0, (T)(g.y1 - g.y0));
+  if (g.px || g.py) st2(pos, i, x, y);
+  bool esc = false;
+  if (!g.px) esc = esc || (x < x0) || (x > x1);
+  if (!g.py) esc = esc || (y < y0) || (y > y1);
+  flag[i] = esc ? 1 : 0;
+}
+
+// Escape respawn (fb/solvers/pic_flip.py:512-524): escaped particle r (in index order) takes
+// draw r for x and draw count + r for y.
+template <class T>
+__global__ void __launch_bounds__(256) respawn_kernel(PicGeom g, int n, const int* __restrict__ flag,
+                                                      const int* __restrict__ rank, const int* __restrict__ total,
+                                                      const Pcg* __restrict__ rng, double fsx, double fsy,
+                                                      T* __restrict__ pos, T* __restrict__ vel) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const Pcg s = *rng;
+  const unsigned long long r = (unsigned long long)rank[i];
+  const unsigned long long c = (unsigned long long)*total;
+  float xs;
+  if (g.px) {
+    xs = pcg_uniform(pcg_draw(s, r), g.x0, g.x1);
+  } else {
+    xs = (float)g.x0 + pcg_uniform(pcg_draw(s, r), 0.0, 0.5 * g.dx);
+  }
+  float ys = pcg_uniform(pcg_draw(s, c + r), g.y0, g.y1);
+  st2(pos, i, (T)xs, (T)ys);
+  st2(vel, i, (T)fsx, (T)fsy);
+}
+
+// Advance the stream state by 2 * total (or by `fixed` draws when total is null).
+__global__ void rng_advance_kernel(Pcg* rng, const int* total, unsigned long long fixed) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned long long d = total ? 2ull * (unsigned long long)*total : fixed;
+  rng->state = pcg_advance(rng->state, rng->inc, d);
+}
+
+// ------------------------------------------------------------------ seeding
+
+__global__ void __launch_bounds__(256) fluid_flag_kernel(int ncell, const uint8_t* __restrict__ solid,
+                                                         int* __restrict__ flag) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncell) flag[c] = solid[c] ? 0 : 1;
+}
+
+__global__ void __launch_bounds__(256) fluid_list_kernel(int ncell, const uint8_t* __restrict__ solid,
+                                                         const int* __restrict__ rank, int* __restrict__ list) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncell && !solid[c]) list[rank[c]] = c;
+}
+
+// _seed_particles (fb/solvers/pic_flip.py:317-333): `per_cell` particles per fluid cell in
+// row-major order, jitter draws 2k and 2k+1 for particle k.
+template <class T>
+__global__ void __launch_bounds__(256) seed_kernel(PicGeom g, int n, int per_cell, const int* __restrict__ list,
+                                                   const Pcg* __restrict__ rng, T* __restrict__ pos) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const Pcg s = *rng;
+  const int c = list[k / per_cell];
+  const int cx = c % g.nx, cy = c / g.nx;
+  const T jx = (T)pcg_draw(s, 2ull * (unsigned long long)k);
+  const T jy = (T)pcg_draw(s, 2ull * (unsigned long long)k + 1ull);
+  const double tx = ((double)cx + 0.1) + (double)((T)0.8 * jx);
+  const double ty = ((double)cy + 0.1) + (double)((T)0.8 * jy);
+  st2(pos, k, (T)(g.x0 + tx * g.dx), (T)(g.y0 + ty * g.dy));
+}
+
+// ------------------------------------------------------------------ statistics
+
+// Reference cell ids (fb/core/particle_population.py:19-31): dtype arithmetic, clipped.
+template <class T>
+__device__ __forceinline__ int ref_cell(const PicGeom& g, T x, T y) {
+  T rx = floor((x - (T)g.x0) / (T)g.dx);
+  T ry = floor((y - (T)g.y0) / (T)g.dy);
+  double fx = fmin(fmax((double)rx, 0.0), (double)(g.nx - 1));
+  double fy = fmin(fmax((double)ry, 0.0), (double)(g.ny - 1));
+  if (!(fx == fx)) fx = 0.0;
+  if (!(fy == fy)) fy = 0.0;
+  return (int)fy * g.nx + (int)fx;
+}
+
+// keys: [0] max particle speed [1] max grid speed [2] particles inside [3] non-finite values
+//       [4] fluid cells [5] empty fluid cells [6] fluid cells with fewer than 4 particles
+template <class T>
+__global__ void __launch_bounds__(256) pic_particle_stats_kernel(PicGeom g, int n, const T* __restrict__ pos,
+                                                                 const T* __restrict__ vel, NacaDev foil,
+                                                                 int* __restrict__ counts,
+                                                                 unsigned long long* __restrict__ keys) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double sp = 0.0;
+  long long inside = 0, bad = 0;
+  if (i < n) {
+    T x, y, vx, vy;
+    ld2(pos, i, x, y);
+    ld2(vel, i, vx, vy);
+    if (!isfinite((double)x) || !isfinite((double)y)) bad++;
+    if (!isfinite((double)vx) || !isfinite((double)vy)) bad++;
+    T s = sqrt(vx * vx + vy * vy);
+    sp = (double)s;
+    if (!(sp == sp)) sp = 0.0;
+    if (naca_sdf_t<T>(foil, x, y) <= (T)0) inside = 1;
+    atomicAdd(&counts[ref_cell(g, x, y)], 1);
+  }
+  sp = wave_max(sp);
+  inside = wave_sum_i(inside);
+  bad = wave_sum_i(bad);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&keys[0], dkey(sp));
+    if (inside) atomicAdd(&keys[2], (unsigned long long)inside);
+    if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) pic_cell_stats_kernel(int ncell, const T* __restrict__ gu,
+                                                             const T* __restrict__ gv,
+                                                             const uint8_t* __restrict__ solid,
+                                                             const int* __restrict__ counts,
+                                                             unsigned long long* __restrict__ keys) {
+  int c = blockIdx.x * 256 + threadIdx.x;
+  double sp = 0.0;
+  long long bad = 0, fluid = 0, empty = 0, under = 0;
+  if (c < ncell) {
+    T a = gu[c], b = gv[c];
+    if (!isfinite((double)a) || !isfinite((double)b)) bad++;
+    sp = (double)sqrt(a * a + b * b);
+    if (!(sp == sp)) sp = 0.0;
+    if (!solid[c]) {
+      fluid = 1;
+      empty = counts[c] == 0;
+      under = counts[c] < 4;
+    }
+  }
+  sp = wave_max(sp);
+  bad = wave_sum_i(bad);
+  fluid = wave_sum_i(fluid);
+  empty = wave_sum_i(empty);
+  under = wave_sum_i(under);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&keys[1], dkey(sp));
+    if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
+    if (fluid) atomicAdd(&keys[4], (unsigned long long)fluid);
+    if (empty) atomicAdd(&keys[5], (unsigned long long)empty);
+    if (under) atomicAdd(&keys[6], (unsigned long long)under);
+  }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
+}
+
+}  // namespace fb
+
+// ====================================================================== host side
+
+namespace fb {
+
+struct SortBufs {
+  int ncell = 0, cap = 0;
+  int* cellof = nullptr;
+  int* sidx = nullptr;
+  int* counts = nullptr;  // ncell + 1
+  int* cstart = nullptr;  // ncell + 1
+  int* cursor = nullptr;  // ncell
+  int* bsum = nullptr;
+  int* total = nullptr;  // 4 ints
+  void* spos = nullptr;
+  void* sval = nullptr;
+};
+
+static int alloc_sort(SortBufs& b, int ncell, int cap, size_t sz, hipStream_t s, bool async) {
+  b.ncell = ncell;
+  b.cap = cap;
+  const size_t nb = (size_t)(std::max(ncell, cap) + 1023) / 1024 + 2;
+  auto A = [&](void** p, size_t bytes) -> hipError_t {
+    return async ? hipMallocAsync(p, std::max<size_t>(bytes, 16), s) : hipMalloc(p, std::max<size_t>(bytes, 16));
+  };
+  hipError_t e = hipSuccess;
+  void** ptrs[] = {(void**)&b.cellof, (void**)&b.sidx, (void**)&b.counts, (void**)&b.cstart, (void**)&b.cursor,
+                   (void**)&b.bsum, (void**)&b.total, &b.spos, &b.sval};
+  size_t bytes[] = {(size_t)cap * 4, (size_t)cap * 4, ((size_t)ncell + 1) * 4, ((size_t)ncell + 1) * 4,
+                    (size_t)ncell * 4, nb * 4, 16, (size_t)cap * 2 * sz, (size_t)cap * 2 * sz};
+  for (int k = 0; k < 9 && e == hipSuccess; ++k) e = A(ptrs[k], bytes[k]);
+  if (e != hipSuccess) {
+    set_error("device allocation failed: %s", hipGetErrorString(e));
+    return FB_ERR_NOMEM;
+  }
+  return FB_OK;
+}
+
+static void free_sort(SortBufs& b, hipStream_t s, bool async) {
+  for (void* p : {(void*)b.cellof, (void*)b.sidx, (void*)b.counts, (void*)b.cstart, (void*)b.cursor,
+                  (void*)b.bsum, (void*)b.total, b.spos, b.sval})
+    if (p) (void)(async ? hipFreeAsync(p, s) : hipFree(p));
+  b = SortBufs{};
+}
+
+// Exclusive scan of n ints: out[0..n) prefixes, out[n] = total (also written to *total).
+static int scan_ints(hipStream_t s, const int* in, int* out, int n, int* bsum, int* total) {
+  const int nb = std::max(1, (n + 1023) / 1024);
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, s, in, out, n, bsum);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, bsum, nb, total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, s, out, n, bsum, total);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+static inline unsigned blocks(long long n) { return (unsigned)std::max<long long>(1, (n + 255) / 256); }
+
+template <class T>
+static int sort_by_cell(hipStream_t s, const PicGeom& g, int n, const T* pos, const T* vel, SortBufs& b) {
+  const int nc = b.ncell;
+  hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc + 1)), dim3(256), 0, s, b.counts, nc + 1, 0);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc)), dim3(256), 0, s, b.cursor, nc, 0);
+  if (n > 0)
+    hipLaunchKernelGGL(bin_count_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, g, n, pos, b.cellof, b.counts);
+  FB_HIP(hipGetLastError());
+  int rc = scan_ints(s, b.counts, b.cstart, nc, b.bsum, b.total);
+  if (rc) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(blocks(n)), dim3(256), 0, s, n, b.cellof, b.cstart, b.cursor, b.sidx);
+    hipLaunchKernelGGL(cell_sort_kernel, dim3(blocks(nc)), dim3(256), 0, s, nc, b.cstart, b.sidx);
+    hipLaunchKernelGGL(sorted_copy_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, n, b.sidx, pos, vel, (T*)b.spos,
+                       (T*)b.sval);
+  }
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+template <class T>
+static int launch_p2g(hipStream_t s, const PicGeom& g, const SortBufs& b, const T* fu, const T* fv, T* tu, T* tv,
+                      unsigned long long* unsupported) {
+  for (int c = 0; c < 2; ++c) {
+    P2GArgs<T> a;
+    a.g = g;
+    a.c = c ? comp_v(g) : comp_u(g);
+    a.comp = c;
+    a.start = b.cstart;
+    a.spos = (const T*)b.spos;
+    a.sval = (const T*)b.sval;
+    a.fallback = c ? fv : fu;
+    a.out = c ? tv : tu;
+    a.unsupported = unsupported;
+    dim3 grid((a.c.W + 255) / 256, a.c.H);
+    hipLaunchKernelGGL(p2g_kernel<T>, grid, dim3(256), 0, s, a);
+  }
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+template <class T>
+static int launch_g2p(hipStream_t s, const PicGeom& g, int n, const int* idx, const T* pos, const T* u, const T* v,
+                      T* vel) {
+  if (n <= 0) return FB_OK;
+  hipLaunchKernelGGL(g2p_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, g, n, idx, pos, u, v, vel);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+static void fill_keys(hipStream_t s, unsigned long long* k, int n) {
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, k, n);
+}
+
+// Number of sweep-table entries: sample counts 2..16, samples 1..count.
+constexpr int kSweepEntries = 135;
+
+struct Pic {
+  Ctx* ctx = nullptr;
+  MacGrid* M = nullptr;
+  PicGeom g{};
+  fb_foil foil{};
+  int prec = FB_F64;
+  size_t sz = 8;
+  int n = 0, cap = 0;
+  void *pos = nullptr, *vel = nullptr, *start = nullptr;
+  uint8_t* cand = nullptr;
+  int* flag = nullptr;  // cap + 1
+  int* rank = nullptr;  // cap + 1
+  int* flist = nullptr;  // fluid cell list (ncell)
+  SortBufs sb;
+  void *GU = nullptr, *GV = nullptr;  // grid velocity planes
+  void *FU = nullptr, *FV = nullptr;  // fallback faces
+  void *TU = nullptr, *TV = nullptr;  // transferred (pre-projection) faces
+  Pcg* rng = nullptr;
+  unsigned long long* keys = nullptr;  // 16
+  unsigned long long* host_keys = nullptr;
+  int* cnt = nullptr;  // per-cell counts for statistics (ncell)
+  double* table = nullptr;
+  // checkpoint
+  void *ck_pos = nullptr, *ck_vel = nullptr, *ck_gu = nullptr, *ck_gv = nullptr;
+  Pcg* ck_rng = nullptr;
+  int ck_n = 0;
+  bool ck_valid = false;
+  bool grid_valid = false;
+};
+
+static int pic_alloc_particles(Pic& P, int cap) {
+  P.ck_valid = false;
+  for (void* p : {P.pos, P.vel, P.start, (void*)P.cand, (void*)P.flag, (void*)P.rank, P.ck_pos, P.ck_vel})
+    if (p) (void)hipFree(p);
+  P.pos = P.vel = P.start = P.ck_pos = P.ck_vel = nullptr;
+  P.cand = nullptr;
+  P.flag = P.rank = nullptr;
+  free_sort(P.sb, P.ctx->stream, false);
+  P.cap = cap;
+  const size_t pb = (size_t)std::max(cap, 1) * 2 * P.sz;
+  hipError_t e = hipSuccess;
+  auto A = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  A(&P.pos, pb);
+  A(&P.vel, pb);
+  A(&P.start, pb);
+  A(&P.ck_pos, pb);
+  A(&P.ck_vel, pb);
+  A((void**)&P.cand, (size_t)std::max(cap, 1));
+  A((void**)&P.flag, ((size_t)cap + 1) * 4);
+  A((void**)&P.rank, ((size_t)cap + 1) * 4);
+  if (e != hipSuccess) {
+    set_error("device allocation failed: %s", hipGetErrorString(e));
+    return FB_ERR_NOMEM;
+  }
+  return alloc_sort(P.sb, P.g.nx * P.g.ny, std::max(cap, 1), P.sz, P.ctx->stream, false);
+}
+
+static void pic_free(Pic* P) {
+  if (!P) return;
+  for (void* p : {P->pos, P->vel, P->start, (void*)P->cand, (void*)P->flag, (void*)P->rank, (void*)P->flist,
+                  P->GU, P->GV, P->FU, P->FV, P->TU, P->TV, (void*)P->rng, (void*)P->keys, (void*)P->cnt,
+                  (void*)P->table, P->ck_pos, P->ck_vel, P->ck_gu, P->ck_gv, (void*)P->ck_rng})
+    if (p) (void)hipFree(p);
+  free_sort(P->sb, nullptr, false);
+  if (P->host_keys) (void)hipHostFree(P->host_keys);
+  if (P->M) free_mac(P->M);
+  delete P;
+}
+
+static int pic_read_keys(Pic& P, int n) {
+  FB_HIP(hipMemcpyAsync(P.host_keys, P.keys, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, P.ctx->stream));
+  FB_HIP(hipStreamSynchronize(P.ctx->stream));
+  return FB_OK;
+}
+
+// Fallback faces from the grid velocity (fb/solvers/pic_flip.py:779-781).
+template <class T>
+static int pic_fallback(Pic& P) {
+  return launch_cell_to_faces<T>(*P.M, (const T*)P.GU, (const T*)P.GV, (T*)P.FU, (T*)P.FV);
+}
+
+template <class T>
+static int pic_collide(Pic& P, const fb_pose& pose, double omega) {
+  if (P.n <= 0) return FB_OK;
+  hipLaunchKernelGGL(collide_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, P.ctx->stream, P.n, (T*)P.pos, (T*)P.vel,
+                     make_naca(P.foil, pose), omega);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+// P2G with the grid velocity as fallback; the unsupported face count lands in keys[slot].
+template <class T>
+static int pic_p2g(Pic& P, int slot) {
+  hipStream_t s = P.ctx->stream;
+  int rc;
+  if ((rc = pic_fallback<T>(P))) return rc;
+  if ((rc = sort_by_cell<T>(s, P.g, P.n, (const T*)P.pos, (const T*)P.vel, P.sb))) return rc;
+  return launch_p2g<T>(s, P.g, P.sb, (const T*)P.FU, (const T*)P.FV, (T*)P.TU, (T*)P.TV, P.keys + slot);
+}
+
+template <class T>
+static int pic_seed(Pic& P, int per_cell, int64_t* count) {
+  hipStream_t s = P.ctx->stream;
+  MacGrid& M = *P.M;
+  const int nc = P.g.nx * P.g.ny;
+  int rc;
+  // fluid cells in row-major order
+  hipLaunchKernelGGL(fluid_flag_kernel, dim3(blocks(nc)), dim3(256), 0, s, nc, M.solid, P.sb.counts);
+  if ((rc = scan_ints(s, P.sb.counts, P.sb.cstart, nc, P.sb.bsum, P.sb.total))) return rc;
+  hipLaunchKernelGGL(fluid_list_kernel, dim3(blocks(nc)), dim3(256), 0, s, nc, M.solid, P.sb.cstart, P.flist);
+  FB_HIP(hipGetLastError());
+  int fluid = 0;
+  FB_HIP(hipMemcpyAsync(&fluid, P.sb.total, sizeof(int), hipMemcpyDeviceToHost, s));
+  FB_HIP(hipStreamSynchronize(s));
+  FB_REQUIRE(fluid > 0, FB_ERR_ARG, "particle seeding requires at least one fluid cell");
+  const long long n = (long long)fluid * per_cell;
+  FB_REQUIRE(n < (1ll << 30), FB_ERR_ARG, "particle count %lld exceeds the supported maximum", n);
+  if (n > P.cap) {
+    if ((rc = pic_alloc_particles(P, (int)n))) return rc;
+  }
+  P.n = (int)n;
+  hipLaunchKernelGGL(seed_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, P.g, P.n, per_cell, P.flist, P.rng,
+                     (T*)P.pos);
+  hipLaunchKernelGGL(rng_advance_kernel, dim3(1), dim3(64), 0, s, P.rng, (const int*)nullptr,
+                     2ull * (unsigned long long)n);
+  FB_HIP(hipGetLastError());
+  // velocities from the grid (fb/solvers/pic_flip.py:333, 379-387)
+  if ((rc = pic_fallback<T>(P))) return rc;
+  if ((rc = launch_g2p<T>(s, P.g, P.n, nullptr, (const T*)P.pos, (const T*)P.FU, (const T*)P.FV, (T*)P.vel)))
+    return rc;
+  *count = n;
+  FB_HIP(hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+template <class T>
+static int pic_substep(Pic& P, const fb_pic_params& p, const fb_pose& pose, const double* table,
+                       fb_pic_report* rep) {
+  hipStream_t s = P.ctx->stream;
+  MacGrid& M = *P.M;
+  int rc;
+  // sweep table first: the copy is staged before the host waits inside this substep
+  FB_HIP(hipMemcpyAsync(P.table, table, 2 * kSweepEntries * sizeof(double), hipMemcpyHostToDevice, s));
+  // keys: [0] unsupported faces [1] travel [2] swept hits
+  fill_keys(s, P.keys, 4);
+  if (!M.geom_valid || pose.angle_degrees != M.angle) {
+    if ((rc = mac_geometry(M, pose))) return rc;
+  }
+  if ((rc = pic_collide<T>(P, pose, p.wall_omega_rad))) return rc;
+  if ((rc = pic_p2g<T>(P, 0))) return rc;
+  // implicit viscosity on the faces (fb/core/grid.py:278-303)
+  const int nx = P.g.nx, ny = P.g.ny;
+  T* res[2];
+  if ((rc = diffuse<T>(M, (const T*)P.TU, (const T*)P.TV, nx + 1 - P.g.px, ny, nx + 1, nx, ny + 1 - P.g.py, nx,
+                       p.viscosity, p.dt, p.tolerance, p.max_iterations, res, &rep->grid)))
+    return rc;
+  if (!rep->grid.viscosity_converged) {
+    rep->grid.status = FB_SF_VISCOSITY_NOT_CONVERGED;
+    return FB_OK;
+  }
+  if ((rc = launch_dup_copy<T>(M, res[0], ny, nx + 1, nx + 1, P.g.px, 0))) return rc;
+  if ((rc = launch_dup_copy<T>(M, res[1], ny + 1, nx, nx, 0, P.g.py))) return rc;
+  FB_HIP(hipMemcpyAsync(M.U[M.cur], res[0], M.NU * M.sz, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(M.V[M.cur], res[1], M.NV * M.sz, hipMemcpyDeviceToDevice, s));
+  // projection (fb/solvers/pic_flip.py:436-479): grid.project_faces without the Stable Fluids CFL guard
+  fb_sf_params sp{};
+  sp.dt = p.dt;
+  sp.viscosity = p.viscosity;
+  sp.tolerance = p.tolerance;
+  sp.max_iterations = p.max_iterations;
+  sp.freestream_x = p.freestream_x;
+  sp.freestream_y = p.freestream_y;
+  sp.wall_omega_rad = p.wall_omega_rad;
+  sp.cfl_limit = INFINITY;
+  sp.channel_walls = p.channel_walls;
+  sp.mixed_poisson = p.mixed_poisson;
+  if ((rc = project<T>(M, sp, p.dt, &rep->grid))) return rc;
+  if (rep->grid.status) return FB_OK;
+  const T* qu = (const T*)M.U[M.cur];
+  const T* qv = (const T*)M.V[M.cur];
+  if ((rc = launch_faces_to_cell<T>(M, qu, qv, (T*)P.GU, (T*)P.GV))) return rc;
+  if (P.n > 0) {
+    FlipArgs fa;
+    fa.g = P.g;
+    fa.blend = p.blend;
+    fa.dt = p.dt;
+    fa.pivot_x = P.foil.pivot_x;
+    fa.pivot_y = P.foil.pivot_y;
+    const double margin = 0.05 * std::min(P.g.dx, P.g.dy);
+    fa.reach = p.maximum_radius + margin;
+    fa.travel_key = P.keys + 1;
+    hipLaunchKernelGGL(flip_advect_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, fa, P.n, (T*)P.pos, (T*)P.vel,
+                       (T*)P.start, P.cand, qu, qv, (const T*)P.TU, (const T*)P.TV);
+    SweptArgs sa;
+    sa.g = make_naca(P.foil, pose);
+    sa.table = P.table;
+    sa.omega = p.wall_omega_rad;
+    sa.wall_travel = p.wall_travel;
+    sa.spacing = 0.1 * std::min(P.g.dx, P.g.dy);
+    sa.margin = margin;
+    sa.travel_key = P.keys + 1;
+    sa.hits = P.keys + 2;
+    hipLaunchKernelGGL(swept_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, sa, P.n, (T*)P.pos, (T*)P.vel,
+                       (const T*)P.start, (const uint8_t*)P.cand);
+    hipLaunchKernelGGL(wrap_escape_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, P.g, P.n, (T*)P.pos, P.flag);
+    FB_HIP(hipGetLastError());
+    if ((rc = scan_ints(s, P.flag, P.rank, P.n, P.sb.bsum, P.sb.total))) return rc;
+    hipLaunchKernelGGL(respawn_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, P.g, P.n, (const int*)P.flag,
+                       (const int*)P.rank, (const int*)P.sb.total, (const Pcg*)P.rng, p.freestream_x,
+                       p.freestream_y, (T*)P.pos, (T*)P.vel);
+    hipLaunchKernelGGL(rng_advance_kernel, dim3(1), dim3(64), 0, s, P.rng, (const int*)P.sb.total, 0ull);
+    FB_HIP(hipGetLastError());
+    if ((rc = pic_collide<T>(P, pose, p.wall_omega_rad))) return rc;
+  }
+  int esc = 0;
+  FB_HIP(hipMemcpyAsync(&esc, P.sb.total, sizeof(int), hipMemcpyDeviceToHost, s));
+  if ((rc = pic_read_keys(P, 3))) return rc;
+  rep->unsupported_faces = (int64_t)P.host_keys[0];
+  rep->swept_hits = (int64_t)P.host_keys[2];
+  rep->escaped = P.n > 0 ? esc : 0;
+  return FB_OK;
+}
+
+template <class T>
+static int pic_stats(Pic& P, const fb_pose& pose, fb_pic_stats* st) {
+  hipStream_t s = P.ctx->stream;
+  const int nc = P.g.nx * P.g.ny;
+  fill_keys(s, P.keys, 8);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc)), dim3(256), 0, s, P.cnt, nc, 0);
+  if (P.n > 0)
+    hipLaunchKernelGGL(pic_particle_stats_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, P.g, P.n,
+                       (const T*)P.pos, (const T*)P.vel, make_naca(P.foil, pose), P.cnt, P.keys);
+  hipLaunchKernelGGL(pic_cell_stats_kernel<T>, dim3(blocks(nc)), dim3(256), 0, s, nc, (const T*)P.GU,
+                     (const T*)P.GV, (const uint8_t*)P.M->solid, (const int*)P.cnt, P.keys);
+  FB_HIP(hipGetLastError());
+  int rc = pic_read_keys(P, 7);
+  if (rc) return rc;
+  auto val = [&](int k) { return P.host_keys[k] ? dkey_inv(P.host_keys[k]) : 0.0; };
+  st->max_particle_speed = val(0);
+  st->max_grid_speed = val(1);
+  st->particles_inside = (int64_t)P.host_keys[2];
+  st->nonfinite = (int64_t)P.host_keys[3];
+  st->fluid_cells = (int64_t)P.host_keys[4];
+  st->empty_cells = (int64_t)P.host_keys[5];
+  st->underfilled_cells = (int64_t)P.host_keys[6];
+  st->particle_count = P.n;
+  return FB_OK;
+}
+
+}  // namespace fb
+
+using fb::Pic;
+
+#define PIC_DISPATCH(P, fn, ...) ((P)->prec == FB_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+int fb_pic_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, fb_pic** out) {
+  FB_REQUIRE(ctx && dom && foil && out, FB_ERR_ARG, "NULL argument");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::MacGrid* M = nullptr;
+  int rc = fb::create_mac(c, dom, foil, precision, &M);
+  if (rc) return rc;
+  auto* P = new Pic();
+  P->ctx = c;
+  P->M = M;
+  P->g = fb::make_pic_geom(*dom);
+  P->foil = *foil;
+  P->prec = precision;
+  P->sz = precision == FB_F64 ? 8 : 4;
+  const size_t nc = (size_t)dom->nx * dom->ny;
+  hipError_t e = hipSuccess;
+  auto A = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  A(&P->GU, nc * P->sz);
+  A(&P->GV, nc * P->sz);
+  A(&P->ck_gu, nc * P->sz);
+  A(&P->ck_gv, nc * P->sz);
+  A(&P->FU, M->NU * P->sz);
+  A(&P->FV, M->NV * P->sz);
+  A(&P->TU, M->NU * P->sz);
+  A(&P->TV, M->NV * P->sz);
+  A((void**)&P->flist, nc * 4);
+  A((void**)&P->cnt, nc * 4);
+  A((void**)&P->rng, sizeof(fb::Pcg));
+  A((void**)&P->ck_rng, sizeof(fb::Pcg));
+  A((void**)&P->keys, 16 * sizeof(unsigned long long));
+  A((void**)&P->table, 2 * fb::kSweepEntries * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&P->host_keys, 16 * sizeof(unsigned long long), 0);
+  if (e != hipSuccess) {
+    fb::set_error("device allocation failed: %s", hipGetErrorString(e));
+    fb::pic_free(P);
+    return FB_ERR_NOMEM;
+  }
+  if ((rc = fb::pic_alloc_particles(*P, 0))) {
+    fb::pic_free(P);
+    return rc;
+  }
+  fb::Pcg zero{0ull, 1ull};
+  FB_HIP(hipMemcpy(P->rng, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  *out = reinterpret_cast<fb_pic*>(P);
+  return FB_OK;
+}
+
+int fb_pic_destroy(fb_pic* pic) {
+  if (!pic) return FB_OK;
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  fb::pic_free(P);
+  return FB_OK;
+}
+
+int fb_pic_mac(fb_pic* pic, fb_mac** out) {
+  FB_REQUIRE(pic && out, FB_ERR_ARG, "NULL argument");
+  *out = reinterpret_cast<fb_mac*>(reinterpret_cast<Pic*>(pic)->M);
+  return FB_OK;
+}
+
+int fb_pic_set_pose(fb_pic* pic, const fb_pose* pose) {
+  FB_REQUIRE(pic && pose, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  return fb::mac_geometry(*P->M, *pose);
+}
+
+int fb_pic_set_grid(fb_pic* pic, const void* gu_host, const void* gv_host) {
+  FB_REQUIRE(pic && gu_host && gv_host, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  const size_t b = (size_t)P->g.nx * P->g.ny * P->sz;
+  FB_HIP(hipMemcpyAsync(P->GU, gu_host, b, hipMemcpyHostToDevice, P->ctx->stream));
+  FB_HIP(hipMemcpyAsync(P->GV, gv_host, b, hipMemcpyHostToDevice, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  P->grid_valid = true;
+  return FB_OK;
+}
+
+int fb_pic_get_grid(fb_pic* pic, void* gu_host, void* gv_host) {
+  FB_REQUIRE(pic && gu_host && gv_host, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  const size_t b = (size_t)P->g.nx * P->g.ny * P->sz;
+  FB_HIP(hipMemcpyAsync(gu_host, P->GU, b, hipMemcpyDeviceToHost, P->ctx->stream));
+  FB_HIP(hipMemcpyAsync(gv_host, P->GV, b, hipMemcpyDeviceToHost, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  return FB_OK;
+}
+
+int fb_pic_set_rng(fb_pic* pic, uint64_t state, uint64_t increment) {
+  FB_REQUIRE(pic, FB_ERR_ARG, "NULL handle");
+  FB_REQUIRE(increment & 1ull, FB_ERR_ARG, "PCG32 increment must be odd");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  fb::Pcg r{state, increment};
+  FB_HIP(hipMemcpyAsync(P->rng, &r, sizeof(r), hipMemcpyHostToDevice, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  return FB_OK;
+}
+
+int fb_pic_get_rng(fb_pic* pic, uint64_t* state, uint64_t* increment) {
+  FB_REQUIRE(pic && state && increment, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  fb::Pcg r{};
+  FB_HIP(hipMemcpyAsync(&r, P->rng, sizeof(r), hipMemcpyDeviceToHost, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  *state = r.state;
+  *increment = r.inc;
+  return FB_OK;
+}
+
+int fb_pic_seed(fb_pic* pic, int per_cell, int64_t* count) {
+  FB_REQUIRE(pic && count, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(per_cell >= 1 && per_cell <= 64, FB_ERR_ARG, "per_cell must be in [1, 64]");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(P->M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_REQUIRE(P->grid_valid, FB_ERR_STATE, "no grid velocity has been set");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  return PIC_DISPATCH(P, fb::pic_seed, *P, per_cell, count);
+}
+
+int fb_pic_count(fb_pic* pic, int64_t* count) {
+  FB_REQUIRE(pic && count, FB_ERR_ARG, "NULL argument");
+  *count = reinterpret_cast<Pic*>(pic)->n;
+  return FB_OK;
+}
+
+int fb_pic_set_particles(fb_pic* pic, int64_t n, const void* pos_host, const void* vel_host) {
+  FB_REQUIRE(pic && (n == 0 || (pos_host && vel_host)), FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(n >= 0 && n < (1ll << 30), FB_ERR_ARG, "particle count out of range");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  if (n > P->cap) {
+    int rc = fb::pic_alloc_particles(*P, (int)n);
+    if (rc) return rc;
+  }
+  P->n = (int)n;
+  const size_t b = (size_t)n * 2 * P->sz;
+  if (n) {
+    FB_HIP(hipMemcpyAsync(P->pos, pos_host, b, hipMemcpyHostToDevice, P->ctx->stream));
+    FB_HIP(hipMemcpyAsync(P->vel, vel_host, b, hipMemcpyHostToDevice, P->ctx->stream));
+  }
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  return FB_OK;
+}
+
+int fb_pic_get_particles(fb_pic* pic, void* pos_host, void* vel_host) {
+  FB_REQUIRE(pic, FB_ERR_ARG, "NULL handle");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  const size_t b = (size_t)P->n * 2 * P->sz;
+  if (P->n && pos_host) FB_HIP(hipMemcpyAsync(pos_host, P->pos, b, hipMemcpyDeviceToHost, P->ctx->stream));
+  if (P->n && vel_host) FB_HIP(hipMemcpyAsync(vel_host, P->vel, b, hipMemcpyDeviceToHost, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  return FB_OK;
+}
+
+int fb_pic_particle_to_grid(fb_pic* pic, int store_grid, double* unsupported_fraction) {
+  FB_REQUIRE(pic && unsupported_fraction, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(P->grid_valid, FB_ERR_STATE, "no grid velocity has been set");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  fb::fill_keys(P->ctx->stream, P->keys, 1);
+  int rc = PIC_DISPATCH(P, fb::pic_p2g, *P, 0);
+  if (rc) return rc;
+  if (store_grid) {
+    rc = P->prec == FB_F64
+             ? fb::launch_faces_to_cell<double>(*P->M, (const double*)P->TU, (const double*)P->TV,
+                                                (double*)P->GU, (double*)P->GV)
+             : fb::launch_faces_to_cell<float>(*P->M, (const float*)P->TU, (const float*)P->TV, (float*)P->GU,
+                                               (float*)P->GV);
+    if (rc) return rc;
+  }
+  if ((rc = fb::pic_read_keys(*P, 1))) return rc;
+  *unsupported_fraction = (double)P->host_keys[0] / (double)(P->M->NU + P->M->NV);
+  return FB_OK;
+}
+
+int fb_pic_regrid(fb_pic* pic, const int64_t* indices, int64_t count) {
+  FB_REQUIRE(pic && (count == 0 || indices), FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(count >= 0 && count <= P->n, FB_ERR_ARG, "index count out of range");
+  if (count == 0) return FB_OK;
+  FB_HIP(hipSetDevice(P->ctx->device));
+  std::vector<int> idx((size_t)count);
+  for (int64_t k = 0; k < count; ++k) {
+    FB_REQUIRE(indices[k] >= 0 && indices[k] < P->n, FB_ERR_ARG, "particle index %lld out of range",
+               (long long)indices[k]);
+    idx[(size_t)k] = (int)indices[k];
+  }
+  // the flag buffer holds cap + 1 ints and is free between substeps
+  FB_HIP(hipMemcpyAsync(P->flag, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, P->ctx->stream));
+  hipStream_t s = P->ctx->stream;
+  int rc;
+  if (P->prec == FB_F64) {
+    if ((rc = fb::pic_fallback<double>(*P))) return rc;
+    rc = fb::launch_g2p<double>(s, P->g, (int)count, P->flag, (const double*)P->pos, (const double*)P->FU,
+                                (const double*)P->FV, (double*)P->vel);
+  } else {
+    if ((rc = fb::pic_fallback<float>(*P))) return rc;
+    rc = fb::launch_g2p<float>(s, P->g, (int)count, P->flag, (const float*)P->pos, (const float*)P->FU,
+                               (const float*)P->FV, (float*)P->vel);
+  }
+  if (rc) return rc;
+  FB_HIP(hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+int fb_pic_resolve_collisions(fb_pic* pic, const fb_pose* pose, double wall_omega_rad) {
+  FB_REQUIRE(pic && pose, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  int rc = PIC_DISPATCH(P, fb::pic_collide, *P, *pose, wall_omega_rad);
+  if (rc) return rc;
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
+  return FB_OK;
+}
+
+int fb_pic_substep(fb_pic* pic, const fb_pic_params* params, const fb_pose* pose, const double* sweep_table,
+                   fb_pic_report* report) {
+  FB_REQUIRE(pic && params && pose && sweep_table && report, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(P->grid_valid, FB_ERR_STATE, "no grid velocity has been set");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  *report = fb_pic_report{};
+  return PIC_DISPATCH(P, fb::pic_substep, *P, *params, *pose, sweep_table, report);
+}
+
+int fb_pic_get_stats(fb_pic* pic, const fb_pose* pose, fb_pic_stats* stats) {
+  FB_REQUIRE(pic && pose && stats, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(P->M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  return PIC_DISPATCH(P, fb::pic_stats, *P, *pose, stats);
+}
+
+int fb_pic_checkpoint(fb_pic* pic) {
+  FB_REQUIRE(pic, FB_ERR_ARG, "NULL handle");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  hipStream_t s = P->ctx->stream;
+  const size_t pb = (size_t)P->n * 2 * P->sz, gb = (size_t)P->g.nx * P->g.ny * P->sz;
+  if (pb) {
+    FB_HIP(hipMemcpyAsync(P->ck_pos, P->pos, pb, hipMemcpyDeviceToDevice, s));
+    FB_HIP(hipMemcpyAsync(P->ck_vel, P->vel, pb, hipMemcpyDeviceToDevice, s));
+  }
+  FB_HIP(hipMemcpyAsync(P->ck_gu, P->GU, gb, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(P->ck_gv, P->GV, gb, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(P->ck_rng, P->rng, sizeof(fb::Pcg), hipMemcpyDeviceToDevice, s));
+  P->ck_n = P->n;
+  int rc = fb_mac_checkpoint(reinterpret_cast<fb_mac*>(P->M));
+  if (rc) return rc;
+  P->ck_valid = true;
+  return FB_OK;
+}
+
+int fb_pic_rollback(fb_pic* pic) {
+  FB_REQUIRE(pic, FB_ERR_ARG, "NULL handle");
+  auto* P = reinterpret_cast<Pic*>(pic);
+  FB_REQUIRE(P->ck_valid, FB_ERR_STATE, "no checkpoint to roll back to");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  hipStream_t s = P->ctx->stream;
+  P->n = P->ck_n;
+  const size_t pb = (size_t)P->n * 2 * P->sz, gb = (size_t)P->g.nx * P->g.ny * P->sz;
+  if (pb) {
+    FB_HIP(hipMemcpyAsync(P->pos, P->ck_pos, pb, hipMemcpyDeviceToDevice, s));
+    FB_HIP(hipMemcpyAsync(P->vel, P->ck_vel, pb, hipMemcpyDeviceToDevice, s));
+  }
+  FB_HIP(hipMemcpyAsync(P->GU, P->ck_gu, gb, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(P->GV, P->ck_gv, gb, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(P->rng, P->ck_rng, sizeof(fb::Pcg), hipMemcpyDeviceToDevice, s));
+  return fb_mac_rollback(reinterpret_cast<fb_mac*>(P->M));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ standalone transfers
+
+namespace fb {
+
+template <class T>
+static int standalone_p2f(Ctx* c, int64_t n, const void* pos, const void* vel, const void* fu, const void* fv,
+                          const PicGeom& g, void* uo, void* vo, double* frac) {
+  hipStream_t s = c->stream;
+  const size_t nu = (size_t)g.ny * (g.nx + 1), nv = (size_t)(g.ny + 1) * g.nx;
+  SortBufs b;
+  int rc = alloc_sort(b, g.nx * g.ny, (int)std::max<int64_t>(n, 1), sizeof(T), s, true);
+  if (rc) return rc;
+  void *dp = nullptr, *dv = nullptr, *dfu = nullptr, *dfv = nullptr, *du = nullptr, *dvv = nullptr;
+  unsigned long long* key = nullptr;
+  const size_t pb = (size_t)std::max<int64_t>(n, 1) * 2 * sizeof(T);
+  hipError_t e = hipSuccess;
+  for (auto pr : {std::make_pair(&dp, pb), std::make_pair(&dv, pb), std::make_pair(&dfu, nu * sizeof(T)),
+                  std::make_pair(&dfv, nv * sizeof(T)), std::make_pair(&du, nu * sizeof(T)),
+                  std::make_pair(&dvv, nv * sizeof(T))})
+    if (e == hipSuccess) e = hipMallocAsync(pr.first, pr.second, s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&key, 8, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dp, pos, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dv, vel, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dfu, fu, nu * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dfv, fv, nv * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    fill_keys(s, key, 1);
+    rc = sort_by_cell<T>(s, g, (int)n, (const T*)dp, (const T*)dv, b);
+    if (!rc) rc = launch_p2g<T>(s, g, b, (const T*)dfu, (const T*)dfv, (T*)du, (T*)dvv, key);
+    unsigned long long uns = 0;
+    if (!rc) {
+      e = hipMemcpyAsync(uo, du, nu * sizeof(T), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(vo, dvv, nv * sizeof(T), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(&uns, key, 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      *frac = (double)uns / (double)(nu + nv);
+    }
+  }
+  for (void* p : {dp, dv, dfu, dfv, du, dvv, (void*)key})
+    if (p) (void)hipFreeAsync(p, s);
+  free_sort(b, s, true);
+  if (e != hipSuccess) {
+    set_error("particle_to_faces failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  FB_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+template <class T>
+static int standalone_f2p(Ctx* c, int64_t n, const void* u, const void* v, const void* pos, const PicGeom& g,
+                          void* out) {
+  hipStream_t s = c->stream;
+  const size_t nu = (size_t)g.ny * (g.nx + 1), nv = (size_t)(g.ny + 1) * g.nx;
+  void *dp = nullptr, *du = nullptr, *dv = nullptr, *dout = nullptr;
+  const size_t pb = (size_t)std::max<int64_t>(n, 1) * 2 * sizeof(T);
+  hipError_t e = hipMallocAsync(&dp, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dout, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&du, nu * sizeof(T), s);
+  if (e == hipSuccess) e = hipMallocAsync(&dv, nv * sizeof(T), s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dp, pos, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(du, u, nu * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, v, nv * sizeof(T), hipMemcpyHostToDevice, s);
+  int rc = FB_OK;
+  if (e == hipSuccess) {
+    rc = launch_g2p<T>(s, g, (int)n, nullptr, (const T*)dp, (const T*)du, (const T*)dv, (T*)dout);
+    if (!rc && n) e = hipMemcpyAsync(out, dout, (size_t)n * 2 * sizeof(T), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  for (void* p : {dp, du, dv, dout})
+    if (p) (void)hipFreeAsync(p, s);
+  if (e != hipSuccess) {
+    set_error("faces_to_particle failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  FB_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+static PicGeom transfer_geom(int nx, int ny, double x0, double y0, double dx, double dy, int px, int py) {
+  PicGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.px = px;
+  g.py = py;
+  g.x0 = x0;
+  g.y0 = y0;
+  g.dx = dx;
+  g.dy = dy;
+  g.x1 = x0 + nx * dx;
+  g.y1 = y0 + ny * dy;
+  return g;
+}
+
+}  // namespace fb
+
+extern "C" {
+
+int fb_pic_particle_to_faces(fb_ctx* ctx, int precision, int64_t n, const void* positions, const void* velocities,
+                             const void* fallback_u, const void* fallback_v, int nx, int ny, double x0, double y0,
+                             double dx, double dy, int periodic_x, int periodic_y, void* u_out, void* v_out,
+                             double* unsupported_fraction) {
+  FB_REQUIRE(ctx && fallback_u && fallback_v && u_out && v_out && unsupported_fraction, FB_ERR_ARG,
+             "NULL argument");
+  FB_REQUIRE(n == 0 || (positions && velocities), FB_ERR_ARG, "NULL particle arrays");
+  FB_REQUIRE(n >= 0 && n < (1ll << 30), FB_ERR_ARG, "particle count out of range");
+  FB_REQUIRE(nx >= 4 && ny >= 4, FB_ERR_ARG, "each face grid axis must span at least four cells");
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  FB_REQUIRE(dx > 0.0 && dy > 0.0, FB_ERR_ARG, "cell sizes must be positive");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
+  return precision == FB_F64
+             ? fb::standalone_p2f<double>(c, n, positions, velocities, fallback_u, fallback_v, g, u_out, v_out,
+                                          unsupported_fraction)
+             : fb::standalone_p2f<float>(c, n, positions, velocities, fallback_u, fallback_v, g, u_out, v_out,
+                                         unsupported_fraction);
+}
+
+int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* u, const void* v,
+                             const void* positions, int nx, int ny, double x0, double y0, double dx, double dy,
+                             int periodic_x, int periodic_y, void* out) {
+  FB_REQUIRE(ctx && u && v, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(n == 0 || (positions && out), FB_ERR_ARG, "NULL particle arrays");
+  FB_REQUIRE(n >= 0 && n < (1ll << 30), FB_ERR_ARG, "particle count out of range");
+  FB_REQUIRE(nx >= 1 && ny >= 1, FB_ERR_ARG, "empty face grid");
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  FB_REQUIRE(dx > 0.0 && dy > 0.0, FB_ERR_ARG, "cell sizes must be positive");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
+  return precision == FB_F64 ? fb::standalone_f2p<double>(c, n, u, v, positions, g, out)
+                             : fb::standalone_f2p<float>(c, n, u, v, positions, g, out);
+}
+
+}  // extern "C"

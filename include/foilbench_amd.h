@@ -234,6 +234,93 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
 int fb_mac_project(fb_mac* mac, const fb_sf_params* params, fb_sf_report* report);
 int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats);
 
+/* ---------------------------------------------------------------------- PIC/FLIP ---- */
+/* Replaces the particle half of fb/solvers/pic_flip.py (seeding :317-333, collisions
+ * :527-618, advection :481-525, the FLIP update :765-840) and the quadratic B-spline
+ * transfers kern/pic.py:105-205 (via fb/solvers/_numba_adapter.py:108-160). The grid half
+ * (implicit viscosity on faces, projection) runs on the handle's MAC grid. Particles are
+ * (P, 2) arrays in the storage dtype; the grid velocity is two (ny, nx) planes. The PCG32
+ * stream (fb/core/rng.py) lives on the device with the particles. */
+
+typedef struct fb_pic fb_pic;
+
+typedef struct fb_pic_params {
+  double dt;               /* substep */
+  double viscosity;        /* U_ref * chord / Re */
+  double tolerance;        /* pressure_tolerance (also the Jacobi tolerance) */
+  int32_t max_iterations;  /* pressure_max_iterations */
+  int32_t channel_walls;   /* Poiseuille channel walls */
+  double freestream_x, freestream_y;
+  double wall_omega_rad;   /* boundary angular velocity of the substep control (rad/s) */
+  double blend;            /* FLIP fraction of this substep (0 while settling) */
+  double wall_travel;      /* |deg2rad(angle - start angle)| * maximum_radius */
+  double maximum_radius;   /* NacaFoil.maximum_radius (fb/core/geometry.py:28-33) */
+  int32_t mixed_poisson;   /* 1: FP32 CG sweeps inside FP64 residual refinement */
+  int32_t pad_;
+} fb_pic_params;
+
+typedef struct fb_pic_report {
+  fb_sf_report grid;          /* implicit viscosity and projection of the substep */
+  int64_t unsupported_faces;  /* u and v faces whose P2G weight is <= 1e-12 */
+  int64_t swept_hits;         /* particles resolved by the swept-collision test */
+  int64_t escaped;            /* particles respawned after leaving the domain */
+} fb_pic_report;
+
+typedef struct fb_pic_stats {
+  double max_particle_speed;  /* max |particle velocity| (pic_flip.py:701-706) */
+  double max_grid_speed;      /* max |grid velocity| */
+  int64_t particles_inside;   /* contains() at the pose (pic_flip.py:864-873) */
+  int64_t nonfinite;          /* non-finite positions, velocities or grid values */
+  int64_t fluid_cells;
+  int64_t empty_cells;        /* fluid cells without particles (pic_flip.py:954-961) */
+  int64_t underfilled_cells;  /* fluid cells with fewer than 4 particles */
+  int64_t particle_count;
+} fb_pic_stats;
+
+int fb_pic_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int precision, fb_pic** out);
+int fb_pic_destroy(fb_pic* pic);
+/* The handle's MAC grid (faces of the last projection, mask, pressure) for the fb_mac_* getters. */
+int fb_pic_mac(fb_pic* pic, fb_mac** out);
+int fb_pic_set_pose(fb_pic* pic, const fb_pose* pose);
+int fb_pic_set_grid(fb_pic* pic, const void* gu_host, const void* gv_host);
+int fb_pic_get_grid(fb_pic* pic, void* gu_host, void* gv_host);
+/* PCG32 stream words (fb/core/rng.py:33-44: checkpoint/restore). */
+int fb_pic_set_rng(fb_pic* pic, uint64_t state, uint64_t increment);
+int fb_pic_get_rng(fb_pic* pic, uint64_t* state, uint64_t* increment);
+/* _seed_particles (fb/solvers/pic_flip.py:317-333): per_cell particles per fluid cell of the
+ * current mask, jittered from the device stream; velocities gathered from the grid. */
+int fb_pic_seed(fb_pic* pic, int per_cell, int64_t* count);
+int fb_pic_count(fb_pic* pic, int64_t* count);
+int fb_pic_set_particles(fb_pic* pic, int64_t n, const void* pos_host, const void* vel_host);
+int fb_pic_get_particles(fb_pic* pic, void* pos_host, void* vel_host);
+/* _particle_to_grid (fb/solvers/pic_flip.py:361-377): P2G with the grid as fallback; the
+ * unsupported face fraction; store_grid = 1 replaces the grid with faces_to_cell of the result. */
+int fb_pic_particle_to_grid(fb_pic* pic, int store_grid, double* unsupported_fraction);
+/* velocities[indices] = G2P(cell_to_faces(grid)) at their positions (pic_flip.py:620-635). */
+int fb_pic_regrid(fb_pic* pic, const int64_t* indices, int64_t count);
+/* _resolve_particle_collisions (fb/solvers/pic_flip.py:604-618). */
+int fb_pic_resolve_collisions(fb_pic* pic, const fb_pose* pose, double wall_omega_rad);
+/* One PIC/FLIP substep (fb/solvers/pic_flip.py:765-840): mask at `pose`, collisions, P2G,
+ * implicit viscosity, projection, grid update, G2P blend, advection with swept collisions,
+ * wrap and respawn, collisions. sweep_table holds (cos, sin) of the swept-collision sample
+ * angles start + s/count * (angle - start) for count = 2..16 and s = 1..count (135 pairs). */
+int fb_pic_substep(fb_pic* pic, const fb_pic_params* params, const fb_pose* pose, const double* sweep_table,
+                   fb_pic_report* report);
+int fb_pic_get_stats(fb_pic* pic, const fb_pose* pose, fb_pic_stats* stats);
+int fb_pic_checkpoint(fb_pic* pic);
+int fb_pic_rollback(fb_pic* pic);
+
+/* particle_to_faces (fb/solvers/_numba_adapter.py:108-135, kern/pic.py:105-161) on host
+ * arrays: positions/velocities (n, 2), fallback and output faces u (ny, nx+1), v (ny+1, nx). */
+int fb_pic_particle_to_faces(fb_ctx* ctx, int precision, int64_t n, const void* positions, const void* velocities,
+                             const void* fallback_u, const void* fallback_v, int nx, int ny, double x0, double y0,
+                             double dx, double dy, int periodic_x, int periodic_y, void* u_out, void* v_out,
+                             double* unsupported_fraction);
+/* faces_to_particle (fb/solvers/_numba_adapter.py:138-160, kern/pic.py:164-205). */
+int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* u, const void* v,
+                             const void* positions, int nx, int ny, double x0, double y0, double dx, double dy,
+                             int periodic_x, int periodic_y, void* out);
+
 #ifdef __cplusplus
 }
 #endif

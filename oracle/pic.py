@@ -225,8 +225,8 @@ class PicFlipOracle:
         self.positions = seed_particles(self.solid, self.dom, self.rng, sc.dtype, per_cell)
         u, v = cell_to_faces(vel)
         self.velocities = self._f2p(u, v, self.positions)
-        self.grid = self._p2g_cell()
-        self.unsupported = 0.0
+        # fb/solvers/pic_flip.py:287: the P2G result is discarded; only the unsupported fraction is kept
+        self._p2g_cell()
 
     def _px(self):
         return "x" in self.dom.periodic_axes, "y" in self.dom.periodic_axes
