@@ -191,8 +191,13 @@ template <class T>
 __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
   const int c = blockIdx.z;
   if (a.k > 1) {
-    double prev = dkey_inv(a.resid[c * a.stride + a.k - 1]);
-    if (prev / a.scale[c] <= a.tol) return;
+    const unsigned long long key = a.resid[c * a.stride + a.k - 1];
+    if (key != 0ull && dkey_inv(key) / a.scale[c] <= a.tol) {
+      // converged: carry the residual forward so every later sweep of this solve skips too
+      // (a later sweep would otherwise read a ping-pong buffer this solve never wrote)
+      if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.resid[c * a.stride + a.k] = key;
+      return;
+    }
   }
   const int W = a.W[c], H = a.H[c], P = a.P[c];
   int x = blockIdx.x * blockDim.x + threadIdx.x;
