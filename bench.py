@@ -8,8 +8,14 @@ interpolated bounce-back on the foil, FP32 population storage.
 Prints one JSON line. Besides the headline FP32 value it reports the FP64 plain form and the
 shifted FP32/FP16 precision forms (throughput and relative velocity error against FP64),
 the roofline of the step kernel from HIP events, and the NumPy oracle timed on the host.
+Under "legs" it measures the other paths through their solver lifecycles, each with a model
+roofline and an oracle baseline on the host:
+  sf_plain   Stable Fluids FP64, NACA 0012 AoA 8, 256 x 128, Re 500, CG to 1e-6;
+  sf_mixed   Stable Fluids mixed (FP32 CG sweeps, FP64 refinement), NACA 0012 AoA 15,
+             2048 x 1024, Re 2000, CG to 1e-8, with its error against the FP64 plain form;
+  pic        PIC/FLIP FP32, NACA 4412 AoA 10, 1024 x 512, 8 particles per fluid cell.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-legs]
 """
 
 from __future__ import annotations
@@ -109,6 +115,33 @@ def _cpu_baseline(sample_steps: int):
     }
 
 
+def _legs(with_cpu: bool) -> dict:
+    """Stable Fluids and PIC/FLIP workloads; a failing leg is reported, not fatal."""
+    from foilbench_amd.benchmark import legs as L
+
+    plan = {
+        "sf_plain": (lambda: L.stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=30),
+                     lambda: L.sf_cpu_baseline(256, 128, -8.0, 500.0, 1e-6, 4)),
+        "sf_mixed": (lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=4, warmup=1,
+                                                 reference_form="plain"),
+                     lambda: L.sf_cpu_baseline(512, 256, -15.0, 2000.0, 1e-8, 1)),
+        "pic": (lambda: L.pic_leg(1024, 512, 8, steps=4),
+                lambda: L.pic_cpu_baseline(256, 128, 8, 1)),
+    }
+    out = {}
+    for name, (gpu, cpu) in plan.items():
+        t0 = time.perf_counter()
+        try:
+            out[name] = gpu()
+            if with_cpu:
+                out[name]["cpu_baseline"] = cpu()
+        except Exception as err:  # reported in the JSON line instead of aborting the headline
+            out[name] = {"error": f"{type(err).__name__}: {err}"}
+        print(f"[bench] leg {name}: {time.perf_counter() - t0:.1f} s "
+              f"{out[name].get('value', out[name].get('error'))}", file=sys.stderr, flush=True)
+    return out
+
+
 def _traffic():
     """HBM bytes per launch of the step kernel, from the committed rocprofv3 --pmc summary."""
     p = ROOT / "profiles" / "lbm_step_kernel_1024x512_f32_traffic.json"
@@ -126,6 +159,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--no-legs", action="store_true", help="measure only the LBM headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path does not shard "
@@ -136,6 +170,7 @@ def main() -> int:
     achieved = main_form["kernel_gbs"]
     traffic = _traffic()
     cpu = _cpu_baseline(args.cpu_steps) if args.cpu_steps > 0 else None
+    legs = None if args.no_legs else _legs(args.cpu_steps > 0)
     line = {
         "metric": "cell-steps/s/GPU",
         "value": main_form["cell_steps_per_s"],
@@ -167,6 +202,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "forms": forms,
+        "legs": legs,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -1,0 +1,201 @@
+"""Measured workloads of the Stable Fluids and PIC/FLIP paths beside the LBM headline.
+
+Each leg drives the solver lifecycle (initialize, then timed ``advance`` calls with inputs
+resident on the device) on one configuration and reports cell-steps/s, the iteration
+counts that set its algorithmic traffic, and a model roofline:
+
+* Stable Fluids, per cell-substep (FP64 s = 8, FP32 s = 4):
+  B = 4s (advection) + k_d * 6s (Jacobi: two components, read current and original,
+  write) + 8s (divergence/RHS 3s, gradient 5s) + k_p * 17s (PCG iteration: matvec and dot
+  2s, x/r update 6s, DST preconditioner as three read+write passes 6s, p update 3s).
+  The mixed form counts its FP32 sweeps at s = 4 and adds 3 * 8 bytes per FP64 refinement.
+* PIC/FLIP, per particle-substep: 56 bytes in FP32 (P2G reads position and velocity
+  16 B, the counting sort key and index 8 B, the fused G2P and advection read 16 B and
+  write 16 B; 112 B in FP64), plus the Stable Fluids grid terms without advection per cell.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from ..core.geometry import NacaFoil
+from ..core.models import ControlKeyframe, DomainSpec, FoilSpec, Scenario
+
+HBM_PEAK_GBS = 8000.0
+
+
+def sf_scenario(nx: int, ny: int, angle: float, reynolds: float, precision: str, tol: float,
+                naca: str = "0012") -> Scenario:
+    """Stable Fluids workload on bounds [-1.5, 2.5] x [-1, 1] with a unit-chord foil at the
+    origin; output_dt = 0.5 dx / U and stable_cfl = 1 give one substep per advance."""
+    dx = 4.0 / nx
+    return Scenario(
+        schema_version=1, id=f"sf-{nx}x{ny}", domain=DomainSpec(2, ((-1.5, 2.5), (-1.0, 1.0)), (nx, ny)),
+        reynolds=reynolds, freestream=(1.0, 0.0), foil=FoilSpec(naca, 1.0, (0.0, 0.0)),
+        controls=(ControlKeyframe(0.0, angle),), duration=1.0, output_dt=0.5 * dx, precision=precision, seed=0,
+        solver_options={"stable_advection": "maccormack", "pressure_tolerance": tol,
+                        "pressure_max_iterations": 640, "stable_cfl": 1.0},
+    )
+
+
+def pic_scenario(nx: int, ny: int, precision: str = "float32") -> Scenario:
+    """PIC/FLIP workload: NACA 4412 at AoA 10 on the LBM headline box, Re 1000, blend 0.95."""
+    dx = 6.4 / nx
+    return Scenario(
+        schema_version=1, id=f"pic-{nx}x{ny}", domain=DomainSpec(2, ((-1.6, 4.8), (-1.6, 1.6)), (nx, ny)),
+        reynolds=1000.0, freestream=(1.0, 0.0), foil=FoilSpec("4412", 1.0, (0.0, 0.0)),
+        controls=(ControlKeyframe(0.0, -10.0),), duration=1.0, output_dt=0.5 * dx, precision=precision, seed=42,
+        solver_options={"pic_flip_blend": 0.95},
+    )
+
+
+def sf_bytes_per_cell(s: int, k_d: float, k_p: float, refinements: float = 0.0) -> float:
+    return 4 * s + k_d * 6 * s + 8 * s + k_p * 17 * s + refinements * 3 * 8
+
+
+def _run_solver(solver, sc: Scenario, steps: int, warmup: int):
+    solver.initialize(sc, NacaFoil(sc.foil), sc.seed)
+    t = 0.0
+    for _ in range(warmup):
+        c = sc.control_at(t + sc.output_dt)
+        solver.advance(c, sc.output_dt)
+        t += sc.output_dt
+    reps = []
+    times = []
+    for _ in range(steps):
+        c = sc.control_at(t + sc.output_dt)
+        t0 = time.perf_counter()
+        r = solver.advance(c, sc.output_dt)
+        times.append(time.perf_counter() - t0)
+        reps.append(r)
+        t += sc.output_dt
+    return reps, np.asarray(times)
+
+
+def _latency(times: np.ndarray) -> dict:
+    return {"p50_ms": float(np.percentile(times, 50) * 1e3), "p99_ms": float(np.percentile(times, 99) * 1e3)}
+
+
+def stable_fluids_leg(nx: int, ny: int, angle: float, reynolds: float, form: str, tol: float, steps: int,
+                      warmup: int = 2, reference_form: str | None = None) -> dict:
+    from ..solvers.stable_fluids import StableFluidsSolver
+
+    sc = sf_scenario(nx, ny, angle, reynolds, "float64", tol)
+    solver = StableFluidsSolver(form)
+    reps, times = _run_solver(solver, sc, steps, warmup)
+    subs = sum(r.substeps for r in reps)
+    k_p = float(np.mean([r.evidence["pressure_iterations"] for r in reps]))
+    k_d = float(np.mean([r.evidence["viscosity_iterations"] for r in reps]))
+    cells = nx * ny
+    total = float(times.sum())
+    if form == "mixed":
+        refinements = float(np.mean([r.evidence.get("pressure_refinements", 0) for r in reps]))
+        bpc = sf_bytes_per_cell(8, k_d, 0.0) + k_p * 17 * 4 + refinements * 3 * 8
+    else:
+        bpc = sf_bytes_per_cell(8, k_d, k_p)
+    achieved = bpc * cells * subs / total / 1e9
+    out = {
+        "workload": f"stable-fluids {form} NACA{sc.foil.naca} AoA{-angle:g} {nx}x{ny} Re{reynolds:g} tol{tol:g}",
+        "value": cells * subs / total,
+        "unit": "cell-steps/s",
+        "ms_per_step": 1e3 * total / steps,
+        "steps": steps,
+        "substeps": subs,
+        "pressure_iterations_mean": k_p,
+        "viscosity_iterations_mean": k_d,
+        "pressure_refinements_mean": float(np.mean([r.evidence.get("pressure_refinements", 0) for r in reps])),
+        "latency": _latency(times),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_substep": bpc,
+                     "scope": "whole substep, model bytes"},
+    }
+    if reference_form is not None:
+        ref = StableFluidsSolver(reference_form)
+        _run_solver(ref, sc, steps, warmup)
+        u, v = solver.faces()
+        ur, vr = ref.faces()
+        p, pr = solver.pressure(), ref.pressure()
+        fl = ~solver.solid_mask()
+        out["error_vs_fp64"] = {
+            "u": float(np.linalg.norm(u - ur) / np.linalg.norm(ur)),
+            "v": float(np.linalg.norm(v - vr) / max(np.linalg.norm(vr), 1e-300)),
+            "p": float(np.linalg.norm((p - pr)[fl]) / max(np.linalg.norm(pr[fl]), 1e-300)),
+        }
+    return out
+
+
+def pic_leg(nx: int, ny: int, per_cell: int, steps: int, warmup: int = 1, precision: str = "float32") -> dict:
+    from ..solvers.pic_flip import PicFlipSolver
+
+    sc = pic_scenario(nx, ny, precision)
+    solver = PicFlipSolver(particles_per_cell=per_cell)
+    reps, times = _run_solver(solver, sc, steps, warmup)
+    subs = sum(r.substeps for r in reps)
+    npart = int(reps[-1].evidence["particle_count"])
+    k_p = float(np.mean([r.evidence["pressure_iterations"] for r in reps]))
+    k_d = float(np.mean([r.evidence["viscosity_iterations"] for r in reps]))
+    s = 4 if precision == "float32" else 8
+    cells = nx * ny
+    total = float(times.sum())
+    grid_bytes = sf_bytes_per_cell(s, k_d, 0.0) - 4 * s + k_p * 17 * 8
+    part_bytes = 14 * s
+    achieved = (grid_bytes * cells + part_bytes * npart) * subs / total / 1e9
+    return {
+        "workload": f"pic-flip {precision} NACA4412 AoA10 {nx}x{ny} {per_cell}/cell blend0.95 Re1000 seed42",
+        "value": cells * subs / total,
+        "unit": "cell-steps/s",
+        "particle_steps_per_s": npart * subs / total,
+        "particles": npart,
+        "ms_per_step": 1e3 * total / steps,
+        "steps": steps,
+        "substeps": subs,
+        "pressure_iterations_mean": k_p,
+        "viscosity_iterations_mean": k_d,
+        "latency": _latency(times),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle_substep": part_bytes, "bytes_per_cell_substep": grid_bytes,
+                     "scope": "whole substep, model bytes"},
+    }
+
+
+def sf_cpu_baseline(nx: int, ny: int, angle: float, reynolds: float, tol: float, steps: int) -> dict:
+    """The NumPy/SciPy oracle on the same workload, `steps` advances after one warm-up."""
+    from oracle.mac import StableFluidsOracle
+
+    sc = sf_scenario(nx, ny, angle, reynolds, "float64", tol)
+    o = StableFluidsOracle()
+    o.initialize(sc)
+    t = sc.output_dt
+    o.advance(sc.control_at(t), sc.output_dt)
+    subs = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += sc.output_dt
+        subs += o.advance(sc.control_at(t), sc.output_dt)["substeps"]
+    el = time.perf_counter() - t0
+    return {"value": nx * ny * subs / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/mac.py StableFluidsOracle, {nx}x{ny}, {steps} advances after 1 warm-up, "
+                      f"{el:.1f} s (NumPy/SciPy)"}
+
+
+def pic_cpu_baseline(nx: int, ny: int, per_cell: int, steps: int) -> dict:
+    from oracle.pic import PicFlipOracle
+
+    sc = pic_scenario(nx, ny, "float32")
+    o = PicFlipOracle()
+    o.initialize(sc, seed=42, per_cell=per_cell)
+    t = sc.output_dt
+    o.advance(sc.control_at(t), sc.output_dt)
+    subs = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += sc.output_dt
+        subs += o.advance(sc.control_at(t), sc.output_dt)["substeps"]
+    el = time.perf_counter() - t0
+    n = o.positions.shape[0]
+    return {"value": nx * ny * subs / el, "unit": "cell-steps/s", "particle_steps_per_s": n * subs / el, "cores": 1,
+            "kind": "port", "sample": f"oracle/pic.py PicFlipOracle, {nx}x{ny}, {per_cell}/cell ({n} particles), "
+                                      f"{steps} advances after 1 warm-up, {el:.1f} s (NumPy)"}

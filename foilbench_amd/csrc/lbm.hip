@@ -114,6 +114,13 @@ struct LbmArgs {
   double feq_u[9];
   int periodic_x, periodic_y, channel;
   int write_pre, do_reset, prime;
+  // launch shape: blocks [0, edge_blocks) cover the nedge edge cells, the next list_blocks
+  // stride over the flagged (solid or near-wall) cells of rows 1..ny-2 listed in `list`;
+  // the rest cover rows 1..ny-2 with int_gx blocks per row, whose edge and flagged lanes idle
+  int int_gx, edge_blocks, nedge, list_blocks;
+  const int* __restrict__ list;
+  const int* __restrict__ list_count;
+  const double* __restrict__ links;  // 8 Bouzidi fractions per list entry (0: straight link)
   unsigned long long* red;  // [min f key, max speed key, max excursion key, nonfinite f, bad rho, nonfinite u]
 };
 
@@ -187,6 +194,73 @@ __device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, int x, int
       g[d] = reflect<P>(a, x, y, D2Q9::opp[d], sx, sy);
     else
       g[d] = ld<S, C>(a.fin + (size_t)d * N + s);
+  }
+}
+
+// Pull-stream of a flagged interior cell from its precomputed links: q[d-1] > 0 marks the
+// direction d whose source cell is solid, holding the Bouzidi fraction reflect() would compute
+// (same expression, so the same bits). Every operand is loaded up front, in one round.
+template <class P>
+__device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, int x, int y, const double* __restrict__ q,
+                                            typename P::C g[9]) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const size_t N = (size_t)a.nx * a.ny;
+  const size_t i = (size_t)y * a.nx + x;
+  if (a.flags[i] & FL_SOLID) {
+    g[0] = ld<S, C>(a.fin + i);
+#pragma unroll
+    for (int d = 1; d < 9; ++d) g[d] = P::shift ? -(C)D2Q9::w[d] : (C)0;
+    return;
+  }
+  // operands stay in the storage type until used, to keep the register footprint small
+  S at[9], straight[9], up[9];
+  double qd[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) at[d] = a.fin[(size_t)d * N + i];
+#pragma unroll
+  for (int d = 1; d < 9; ++d) {
+    const int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
+    straight[d] = a.fin[(size_t)d * N + (size_t)sy * a.nx + sx];
+    // upstream of the reflected link e = opp(d): the cell (x - c_e) = (x + c_d)
+    const int ux = wrapi(x + D2Q9::cx[d], a.nx), uy = wrapi(y + D2Q9::cy[d], a.ny);
+    up[d] = a.fin[(size_t)D2Q9::opp[d] * N + (size_t)uy * a.nx + ux];
+    qd[d] = q[d - 1];
+  }
+  C rho_w = (C)0;
+  if (a.wall_omega != 0.0) {
+    rho_w = ld<S, C>(a.rho_in + i);
+    if constexpr (P::shift) rho_w = rho_w + (C)1;
+  }
+  g[0] = ld<S, C>(&at[0]);
+#pragma unroll
+  for (int d = 1; d < 9; ++d) {
+    const double qq = qd[d];
+    if (!(qq > 0.0)) {
+      g[d] = ld<S, C>(&straight[d]);
+      continue;
+    }
+    const int e = D2Q9::opp[d];
+    const C src = ld<S, C>(&at[e]);
+    C r;
+    if (qq < 0.5) {
+      C q2 = (C)(2.0 * qq);
+      C q12 = (C)(1.0 - 2.0 * qq);
+      r = q2 * src + q12 * ld<S, C>(&up[d]);
+    } else {
+      C q2 = (C)(2.0 * qq);
+      C q21 = (C)(2.0 * qq - 1.0);
+      r = src / q2 + q21 * ld<S, C>(&at[d]) / q2;
+    }
+    if (a.wall_omega != 0.0) {
+      double wx = a.x0 + ((double)x + 0.5) * a.dx + (qq * (double)D2Q9::cx[e]) * a.dx;
+      double wy = a.y0 + ((double)y + 0.5) * a.dy + (qq * (double)D2Q9::cy[e]) * a.dy;
+      double wu = (-a.wall_omega) * (wy - a.pivot_y);
+      double wv = a.wall_omega * (wx - a.pivot_x);
+      double wp = (double)D2Q9::cx[e] * wu + (double)D2Q9::cy[e] * wv;
+      r = r - (C)6.0 * (C)D2Q9::w[e] * rho_w * (C)wp * (C)a.wall_scale;
+    }
+    g[d] = rnd<P>(r);
   }
 }
 
@@ -431,52 +505,102 @@ __device__ __forceinline__ void cell_stats(const LbmArgs<typename P::S>& a, cons
 
 // ------------------------------------------------------------------ the step kernel
 
+// Edge cell e of the lattice: bottom row, top row, then the left and right columns without
+// their corners.
+__device__ __forceinline__ void edge_cell(int e, int nx, int ny, int& x, int& y) {
+  if (e < nx) {
+    x = e;
+    y = 0;
+  } else if (e < 2 * nx) {
+    x = e - nx;
+    y = ny - 1;
+  } else {
+    int k = e - 2 * nx;
+    if (k < ny - 2) {
+      x = 0;
+      y = 1 + k;
+    } else {
+      x = nx - 1;
+      y = 1 + (k - (ny - 2));
+    }
+  }
+}
+
+// One cell of a lattice step: pull (straight for interior cells, the boundary treatment for
+// edge and flagged cells), sponge, uncovered-cell reset, optional pre-collision write with the
+// postcondition statistics, TRT collision, write f*.
 template <class P>
-__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
+__device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x, int y, bool straight,
+                                          const double* links, double& mn, double& vmax, double& exc, long long& nf,
+                                          long long& bad, long long& nfu) {
   using S = typename P::S;
   using C = typename P::C;
-  const unsigned gx = gridDim.x;
-  const unsigned b = xcd_remap(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
-  const int x = (int)((b % gx) * blockDim.x + threadIdx.x);
-  const int y = (int)(b / gx);
-  const bool active = x < a.nx && y < a.ny;
+  const size_t N = (size_t)a.nx * a.ny;
+  const size_t i = (size_t)y * a.nx + x;
+  C g[9];
+  if (a.prime) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(a.fin + (size_t)d * N + i);
+  } else {
+    if (straight) {
+#pragma unroll
+      for (int d = 0; d < 9; ++d)
+        g[d] = ld<S, C>(a.fin + (size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d]));
+    } else if (links) {
+      pull_linked<P>(a, x, y, links, g);
+    } else {
+      boundary_cell<P>(a, x, y, g);
+    }
+    apply_sponge<P>(a, x, y, g);
+  }
+  const uint8_t fnext = a.flags_next[i];
+  if (a.do_reset && (a.flags[i] & FL_SOLID) && !(fnext & FL_SOLID)) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_u[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
+  }
+  if (a.write_pre) {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
+    cell_stats<P>(a, g, !(fnext & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
+  }
+  C post[9];
+  C rho;
+  if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
+  else rho = collide<P>(a, g, post);
+#pragma unroll
+  for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
+  if (fnext & FL_NEAR) st<S, C>(a.rho_out + i, rho);
+}
+
+// One lattice step. The edge cells and the flagged cells (foil interior and its neighbours)
+// run in the first blocks of the grid, so their longer dependent chains start early and do not
+// diverge the interior waves; interior blocks take the straight pull.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
   double mn = INFINITY, vmax = 0.0, exc = 0.0;
   long long nf = 0, bad = 0, nfu = 0;
-  if (active) {
-    const size_t N = (size_t)a.nx * a.ny;
-    const size_t i = (size_t)y * a.nx + x;
-    C g[9];
-    if (a.prime) {
-#pragma unroll
-      for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(a.fin + (size_t)d * N + i);
-    } else {
-      const bool interior = x > 0 && x < a.nx - 1 && y > 0 && y < a.ny - 1 && a.flags[i] == 0;
-      if (interior) {
-#pragma unroll
-        for (int d = 0; d < 9; ++d)
-          g[d] = ld<S, C>(a.fin + (size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d]));
-      } else {
-        boundary_cell<P>(a, x, y, g);
-      }
-      apply_sponge<P>(a, x, y, g);
+  const unsigned special = (unsigned)(a.edge_blocks + a.list_blocks);
+  if (blockIdx.x < (unsigned)a.edge_blocks) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e < a.nedge) {
+      int x, y;
+      edge_cell(e, a.nx, a.ny, x, y);
+      step_cell<P>(a, x, y, false, nullptr, mn, vmax, exc, nf, bad, nfu);
     }
-    const uint8_t fnext = a.flags_next[i];
-    if (a.do_reset && (a.flags[i] & FL_SOLID) && !(fnext & FL_SOLID)) {
-#pragma unroll
-      for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_u[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
+  } else if (blockIdx.x < special) {
+    const int count = *a.list_count;
+    const int stride = a.list_blocks * (int)blockDim.x;
+    for (int k = (int)((blockIdx.x - a.edge_blocks) * blockDim.x + threadIdx.x); k < count; k += stride) {
+      const int c = a.list[k];
+      step_cell<P>(a, c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, mn, vmax, exc, nf, bad, nfu);
     }
-    if (a.write_pre) {
-#pragma unroll
-      for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
-      cell_stats<P>(a, g, !(fnext & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
-    }
-    C post[9];
-    C rho;
-    if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
-    else rho = collide<P>(a, g, post);
-#pragma unroll
-    for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
-    if (fnext & FL_NEAR) st<S, C>(a.rho_out + i, rho);
+  } else {
+    const unsigned bi = xcd_remap(blockIdx.x - special, gridDim.x - special);
+    const unsigned gx = (unsigned)a.int_gx;
+    const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
+    const int y = 1 + (int)(bi / gx);
+    if (x > 0 && x < a.nx - 1 && a.flags[(size_t)y * a.nx + x] == 0)
+      step_cell<P>(a, x, y, true, nullptr, mn, vmax, exc, nf, bad, nfu);
   }
   if (a.write_pre) {
     // Block reduction: wave shuffles, then one atomic per wave.
@@ -495,6 +619,43 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
       if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
     }
   }
+}
+
+// Flagged cells of rows 1..ny-2, columns 1..nx-2, in any order (each cell's step is independent).
+__global__ void __launch_bounds__(256) lbm_list_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
+                                                       int* __restrict__ list, int* __restrict__ count) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y;
+  if (x < 1 || x >= nx - 1 || y < 1 || y >= ny - 1) return;
+  size_t i = (size_t)y * nx + x;
+  if (flags[i]) list[atomicAdd(count, 1)] = (int)i;
+}
+
+// Bouzidi fractions of the listed cells' solid links, with reflect()'s expression.
+__global__ void __launch_bounds__(256) lbm_links_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
+                                                        const double* __restrict__ sdf, const int* __restrict__ list,
+                                                        const int* __restrict__ count, double* __restrict__ links) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= *count) return;
+  const int c = list[k];
+  const int x = c % nx, y = c / nx;
+  const bool fluid_near = !(flags[c] & FL_SOLID) && (flags[c] & FL_NEAR);
+  const double sx = sdf[c];
+  for (int d = 1; d < 9; ++d) {
+    double q = 0.0;
+    const int px = wrapi(x - D2Q9::cx[d], nx), py = wrapi(y - D2Q9::cy[d], ny);
+    const size_t s = (size_t)py * nx + px;
+    if (fluid_near && (flags[s] & FL_SOLID)) {
+      double sn = sdf[s];
+      double den = fmax(sx - sn, 1.0e-12);
+      q = fmin(fmax(sx / den, 0.05), 1.0);
+    }
+    links[(size_t)k * 8 + d - 1] = q;
+  }
+}
+
+__global__ void lbm_zero_count_kernel(int* count) {
+  if (threadIdx.x == 0) *count = 0;
 }
 
 // Near-wall flags: fluid cells with a solid cell among their 8 periodic neighbours.
@@ -697,10 +858,16 @@ __global__ void lbm_outlet_convert(int ny, const typename P::B* __restrict__ in_
 
 // ------------------------------------------------------------------ host handle
 
+// Blocks that stride over the flagged-cell list of a lattice step.
+constexpr int kListBlocks = 32;
+
 struct Geometry {
   double* sdf = nullptr;
   uint8_t* solid = nullptr;
   uint8_t* flags = nullptr;
+  int* list = nullptr;        // flagged cells of rows 1..ny-2 (device-built, unordered)
+  int* list_count = nullptr;
+  double* links = nullptr;    // 8 Bouzidi fractions per list entry
   double angle = 0.0;
   bool valid = false;
 };
@@ -782,9 +949,15 @@ static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool pri
   a.do_reset = do_reset ? 1 : 0;
   a.prime = prime ? 1 : 0;
   a.red = L.red;
-  dim3 block(256);
-  dim3 grid((L.dom.nx + 255) / 256, L.dom.ny);
-  hipLaunchKernelGGL(lbm_step_kernel<P>, grid, block, 0, L.ctx->stream, a);
+  a.int_gx = (L.dom.nx + 255) / 256;
+  a.nedge = 2 * L.dom.nx + 2 * (L.dom.ny - 2);
+  a.edge_blocks = (a.nedge + 255) / 256;
+  a.list_blocks = kListBlocks;
+  a.list = L.G[g_cur].list;
+  a.list_count = L.G[g_cur].list_count;
+  a.links = L.G[g_cur].links;
+  const int blocks = a.edge_blocks + a.list_blocks + a.int_gx * (L.dom.ny - 2);
+  hipLaunchKernelGGL(lbm_step_kernel<P>, dim3(blocks), dim3(256), 0, L.ctx->stream, a);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
@@ -795,6 +968,11 @@ static int compute_geometry(Lattice& L, int gi, const fb_pose& pose) {
   if (rc) return rc;
   dim3 grid((L.dom.nx + 255) / 256, L.dom.ny);
   hipLaunchKernelGGL(lbm_flags_kernel, grid, dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.solid, G.flags);
+  hipLaunchKernelGGL(lbm_zero_count_kernel, dim3(1), dim3(64), 0, L.ctx->stream, G.list_count);
+  hipLaunchKernelGGL(lbm_list_kernel, grid, dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.flags, G.list,
+                     G.list_count);
+  hipLaunchKernelGGL(lbm_links_kernel, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
+                     L.dom.ny, G.flags, G.sdf, G.list, G.list_count, G.links);
   FB_HIP(hipGetLastError());
   G.angle = pose.angle_degrees;
   G.valid = true;
@@ -1080,6 +1258,9 @@ static void free_lattice(Lattice* L) {
     if (L->G[k].sdf) (void)hipFree(L->G[k].sdf);
     if (L->G[k].solid) (void)hipFree(L->G[k].solid);
     if (L->G[k].flags) (void)hipFree(L->G[k].flags);
+    if (L->G[k].list) (void)hipFree(L->G[k].list);
+    if (L->G[k].list_count) (void)hipFree(L->G[k].list_count);
+    if (L->G[k].links) (void)hipFree(L->G[k].links);
   }
   for (int k = 0; k < 3; ++k)
     if (L->PRE[k]) (void)hipFree(L->PRE[k]);
@@ -1126,6 +1307,9 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
     if ((e = hipMalloc((void**)&L->G[k].sdf, N * sizeof(double)))) return fail(e);
     if ((e = hipMalloc((void**)&L->G[k].solid, N))) return fail(e);
     if ((e = hipMalloc((void**)&L->G[k].flags, N))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].list, N * sizeof(int)))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].list_count, sizeof(int)))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].links, N * 8 * sizeof(double)))) return fail(e);
     if ((e = hipMemsetAsync(L->R[k], 0, N * L->ssize, c->stream))) return fail(e);
   }
   for (int k = 0; k < 3; ++k)

@@ -127,6 +127,7 @@ class StableFluidsSolver:
         self._face_advection = False
         self._reynolds = 1.0
         self._revision = 0
+        self._last_refinements = 0
 
     @property
     def reynolds(self) -> float:
@@ -295,6 +296,7 @@ class StableFluidsSolver:
         if projection:
             self._last_projection = ("relative-l2", tol, int(rep.pressure_iterations), float(rep.pressure_residual),
                                      bool(rep.pressure_converged))
+            self._last_refinements = int(rep.pressure_refinements)
         if viscosity:
             self._last_viscosity = ("update-linf", tol, int(rep.viscosity_iterations),
                                     float(rep.viscosity_residual), bool(rep.viscosity_converged))
@@ -445,6 +447,7 @@ class StableFluidsSolver:
             "requested_reynolds": self._reynolds,
             "effective_reynolds": self._reynolds,
             "degraded_motion": wall_speed == 0.0 and abs(control.angle_degrees - start_angle) > 1.0e-9,
+            **({"pressure_refinements": self._last_refinements} if self._form == "mixed" else {}),
         })
 
     def sample_velocity(self, points: np.ndarray) -> np.ndarray:
