@@ -197,17 +197,39 @@ __device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, int x, int
   }
 }
 
-// Pull-stream of a flagged interior cell from its precomputed links: q[d-1] > 0 marks the
-// direction d whose source cell is solid, holding the Bouzidi fraction reflect() would compute
-// (same expression, so the same bits). Every operand is loaded up front, in one round.
+// Pull-stream with every straight source loaded up front: cells whose own flags are clear
+// (no solid cell among the neighbours) need nothing else; flagged cells fall back to pull().
 template <class P>
-__device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, int x, int y, const double* __restrict__ q,
-                                            typename P::C g[9]) {
+__device__ __forceinline__ void pull_fast(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
   using S = typename P::S;
   using C = typename P::C;
   const size_t N = (size_t)a.nx * a.ny;
   const size_t i = (size_t)y * a.nx + x;
-  if (a.flags[i] & FL_SOLID) {
+  S v[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
+    v[d] = a.fin[(size_t)d * N + (size_t)sy * a.nx + sx];
+  }
+  if (a.flags[i] != 0) {
+    pull<P>(a, x, y, g);
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(&v[d]);
+}
+
+// Pull-stream of a flagged interior cell from its precomputed links: q[d-1] > 0 marks the
+// direction d whose source cell is solid, holding the Bouzidi fraction reflect() would compute
+// (same expression, so the same bits). Every operand is loaded up front, in one round.
+template <class P>
+__device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, int x, int y, bool solid,
+                                            const double* __restrict__ q, typename P::C g[9]) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const size_t N = (size_t)a.nx * a.ny;
+  const size_t i = (size_t)y * a.nx + x;
+  if (solid) {
     g[0] = ld<S, C>(a.fin + i);
 #pragma unroll
     for (int d = 1; d < 9; ++d) g[d] = P::shift ? -(C)D2Q9::w[d] : (C)0;
@@ -341,12 +363,12 @@ __device__ __forceinline__ void streamed_row(const LbmArgs<typename P::S>& a, in
   const bool edge = !a.periodic_y && (y == 0 || y == a.ny - 1);
   if (edge && a.channel) {
     C h[9];
-    pull<P>(a, x, y == 0 ? 1 : a.ny - 2, h);
+    pull_fast<P>(a, x, y == 0 ? 1 : a.ny - 2, h);
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = h[D2Q9::opp[d]];
     return;
   }
-  pull<P>(a, x, y, g);
+  pull_fast<P>(a, x, y, g);
   if (edge) {
     if (y == 0) bottom_bc<P>(a, g);
     else top_bc<P>(a, g);
@@ -361,12 +383,15 @@ __device__ __forceinline__ void boundary_cell(const LbmArgs<typename P::S>& a, i
   using C = typename P::C;
   using B = typename P::B;
   if (!a.periodic_x && x == a.nx - 1) {
+    S hist[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) hist[d] = a.outlet_in[(size_t)y * 9 + d];
     C g2[9];
     streamed_row<P>(a, a.nx - 2, y, g2);
     const B u = (B)a.u_lattice;
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-      B prev = (B)ld<S, C>(a.outlet_in + (size_t)y * 9 + d);
+      B prev = (B)ld<S, C>(&hist[d]);
       B v = prev + u * ((B)g2[d] - prev);
       g[d] = (C)v;
       st<S, C>(a.outlet_out + (size_t)y * 9 + d, (C)v);
@@ -531,8 +556,8 @@ __device__ __forceinline__ void edge_cell(int e, int nx, int ny, int& x, int& y)
 // postcondition statistics, TRT collision, write f*.
 template <class P>
 __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x, int y, bool straight,
-                                          const double* links, double& mn, double& vmax, double& exc, long long& nf,
-                                          long long& bad, long long& nfu) {
+                                          const double* links, bool solid, double& mn, double& vmax, double& exc,
+                                          long long& nf, long long& bad, long long& nfu) {
   using S = typename P::S;
   using C = typename P::C;
   const size_t N = (size_t)a.nx * a.ny;
@@ -547,7 +572,7 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
       for (int d = 0; d < 9; ++d)
         g[d] = ld<S, C>(a.fin + (size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d]));
     } else if (links) {
-      pull_linked<P>(a, x, y, links, g);
+      pull_linked<P>(a, x, y, solid, links, g);
     } else {
       boundary_cell<P>(a, x, y, g);
     }
@@ -585,14 +610,15 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
     if (e < a.nedge) {
       int x, y;
       edge_cell(e, a.nx, a.ny, x, y);
-      step_cell<P>(a, x, y, false, nullptr, mn, vmax, exc, nf, bad, nfu);
+      step_cell<P>(a, x, y, false, nullptr, false, mn, vmax, exc, nf, bad, nfu);
     }
   } else if (blockIdx.x < special) {
     const int count = *a.list_count;
     const int stride = a.list_blocks * (int)blockDim.x;
     for (int k = (int)((blockIdx.x - a.edge_blocks) * blockDim.x + threadIdx.x); k < count; k += stride) {
-      const int c = a.list[k];
-      step_cell<P>(a, c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, mn, vmax, exc, nf, bad, nfu);
+      const int entry = a.list[k];
+      const int c = entry & 0x7fffffff;
+      step_cell<P>(a, c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, entry < 0, mn, vmax, exc, nf, bad, nfu);
     }
   } else {
     const unsigned bi = xcd_remap(blockIdx.x - special, gridDim.x - special);
@@ -600,7 +626,7 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
     const int y = 1 + (int)(bi / gx);
     if (x > 0 && x < a.nx - 1 && a.flags[(size_t)y * a.nx + x] == 0)
-      step_cell<P>(a, x, y, true, nullptr, mn, vmax, exc, nf, bad, nfu);
+      step_cell<P>(a, x, y, true, nullptr, false, mn, vmax, exc, nf, bad, nfu);
   }
   if (a.write_pre) {
     // Block reduction: wave shuffles, then one atomic per wave.
@@ -628,7 +654,8 @@ __global__ void __launch_bounds__(256) lbm_list_kernel(int nx, int ny, const uin
   int y = blockIdx.y;
   if (x < 1 || x >= nx - 1 || y < 1 || y >= ny - 1) return;
   size_t i = (size_t)y * nx + x;
-  if (flags[i]) list[atomicAdd(count, 1)] = (int)i;
+  // the sign bit marks a solid cell
+  if (flags[i]) list[atomicAdd(count, 1)] = (int)i | ((flags[i] & FL_SOLID) ? (int)0x80000000u : 0);
 }
 
 // Bouzidi fractions of the listed cells' solid links, with reflect()'s expression.
@@ -637,7 +664,7 @@ __global__ void __launch_bounds__(256) lbm_links_kernel(int nx, int ny, const ui
                                                         const int* __restrict__ count, double* __restrict__ links) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= *count) return;
-  const int c = list[k];
+  const int c = list[k] & 0x7fffffff;
   const int x = c % nx, y = c / nx;
   const bool fluid_near = !(flags[c] & FL_SOLID) && (flags[c] & FL_NEAR);
   const double sx = sdf[c];
