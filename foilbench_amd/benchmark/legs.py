@@ -199,3 +199,25 @@ def pic_cpu_baseline(nx: int, ny: int, per_cell: int, steps: int) -> dict:
     return {"value": nx * ny * subs / el, "unit": "cell-steps/s", "particle_steps_per_s": n * subs / el, "cores": 1,
             "kind": "port", "sample": f"oracle/pic.py PicFlipOracle, {nx}x{ny}, {per_cell}/cell ({n} particles), "
                                       f"{steps} advances after 1 warm-up, {el:.1f} s (NumPy)"}
+
+
+def main() -> None:
+    """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|pic} [steps]``: one leg, one JSON line."""
+    import json
+    import sys
+
+    which = sys.argv[1] if len(sys.argv) > 1 else "sf_plain"
+    steps = int(sys.argv[2]) if len(sys.argv) > 2 else 4
+    if which == "sf_plain":
+        out = stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=steps)
+    elif which == "sf_mixed":
+        out = stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=steps, warmup=1)
+    elif which == "pic":
+        out = pic_leg(1024, 512, 8, steps=steps)
+    else:
+        raise SystemExit(f"unknown leg {which!r}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -407,9 +407,10 @@ __device__ __forceinline__ void boundary_cell(const LbmArgs<typename P::S>& a, i
 }
 
 template <class P>
-__device__ __forceinline__ void apply_sponge(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+__device__ __forceinline__ void apply_sponge(const LbmArgs<typename P::S>& a, double sy, double sx,
+                                             typename P::C g[9]) {
   using C = typename P::C;
-  double s = fmax(fmax(0.0, a.sponge_y[y]), a.sponge_x[x]);
+  double s = fmax(fmax(0.0, sy), sx);
   if (s == 0.0) return;
   if constexpr (P::shift) {
     float sf = (float)s, rf = (float)(1.0 - s);
@@ -562,6 +563,10 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
   using C = typename P::C;
   const size_t N = (size_t)a.nx * a.ny;
   const size_t i = (size_t)y * a.nx + x;
+  // operands that do not depend on the pull are loaded first, so the cell costs one round
+  const double spy = a.sponge_y[y], spx = a.sponge_x[x];
+  const uint8_t fnext = a.flags_next[i];
+  const uint8_t fcur = a.do_reset ? a.flags[i] : (uint8_t)0;
   C g[9];
   if (a.prime) {
 #pragma unroll
@@ -576,10 +581,9 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
     } else {
       boundary_cell<P>(a, x, y, g);
     }
-    apply_sponge<P>(a, x, y, g);
+    apply_sponge<P>(a, spy, spx, g);
   }
-  const uint8_t fnext = a.flags_next[i];
-  if (a.do_reset && (a.flags[i] & FL_SOLID) && !(fnext & FL_SOLID)) {
+  if (a.do_reset && (fcur & FL_SOLID) && !(fnext & FL_SOLID)) {
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_u[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
   }
@@ -647,15 +651,78 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
   }
 }
 
-// Flagged cells of rows 1..ny-2, columns 1..nx-2, in any order (each cell's step is independent).
-__global__ void __launch_bounds__(256) lbm_list_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
-                                                       int* __restrict__ list, int* __restrict__ count) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y;
-  if (x < 1 || x >= nx - 1 || y < 1 || y >= ny - 1) return;
-  size_t i = (size_t)y * nx + x;
-  // the sign bit marks a solid cell
-  if (flags[i]) list[atomicAdd(count, 1)] = (int)i | ((flags[i] & FL_SOLID) ? (int)0x80000000u : 0);
+// The flagged cells of rows 1..ny-2, columns 1..nx-2, listed in row-major order so that
+// neighbouring lanes of the list blocks touch neighbouring cells. Three passes: count per
+// row, scan over rows, fill. The sign bit of an entry marks a solid cell.
+__device__ __forceinline__ bool listed(const uint8_t* flags, int nx, int ny, int x, int y) {
+  return x >= 1 && x < nx - 1 && y >= 1 && y < ny - 1 && flags[(size_t)y * nx + x] != 0;
+}
+
+__global__ void __launch_bounds__(256) lbm_row_count_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
+                                                            int* __restrict__ row_count) {
+  __shared__ int part[4];
+  const int y = blockIdx.x;
+  int c = 0;
+  for (int x = threadIdx.x; x < nx; x += 256) c += listed(flags, nx, ny, x, y) ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) row_count[y] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of the row counts (one block); the total into *count.
+__global__ void __launch_bounds__(256) lbm_row_scan_kernel(int ny, const int* __restrict__ row_count,
+                                                           int* __restrict__ row_start, int* __restrict__ count) {
+  __shared__ int wsum[4];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int base = 0; base < ny; base += 256) {
+    const int y = base + (int)threadIdx.x;
+    const int v = y < ny ? row_count[y] : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int off = carry;
+    for (int j = 0; j < w; ++j) off += wsum[j];
+    if (y < ny) row_start[y] = off + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 255) carry = off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = carry;
+}
+
+__global__ void __launch_bounds__(256) lbm_row_fill_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
+                                                           const int* __restrict__ row_start,
+                                                           int* __restrict__ list) {
+  __shared__ int wsum[4];
+  const int y = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int run = row_start[y];
+  for (int base = 0; base < nx; base += 256) {
+    const int x = base + (int)threadIdx.x;
+    const bool on = x < nx && listed(flags, nx, ny, x, y);
+    const unsigned long long mask = __ballot(on);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(mask);
+    __syncthreads();
+    int off = run;
+    for (int j = 0; j < w; ++j) off += wsum[j];
+    if (on) {
+      const size_t i = (size_t)y * nx + x;
+      list[off + before] = (int)i | ((flags[i] & FL_SOLID) ? (int)0x80000000u : 0);
+    }
+    run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
 }
 
 // Bouzidi fractions of the listed cells' solid links, with reflect()'s expression.
@@ -679,10 +746,6 @@ __global__ void __launch_bounds__(256) lbm_links_kernel(int nx, int ny, const ui
     }
     links[(size_t)k * 8 + d - 1] = q;
   }
-}
-
-__global__ void lbm_zero_count_kernel(int* count) {
-  if (threadIdx.x == 0) *count = 0;
 }
 
 // Near-wall flags: fluid cells with a solid cell among their 8 periodic neighbours.
@@ -894,6 +957,7 @@ struct Geometry {
   uint8_t* flags = nullptr;
   int* list = nullptr;        // flagged cells of rows 1..ny-2 (device-built, unordered)
   int* list_count = nullptr;
+  int* rows = nullptr;        // per-row counts, then per-row starts, of the list
   double* links = nullptr;    // 8 Bouzidi fractions per list entry
   double angle = 0.0;
   bool valid = false;
@@ -995,9 +1059,12 @@ static int compute_geometry(Lattice& L, int gi, const fb_pose& pose) {
   if (rc) return rc;
   dim3 grid((L.dom.nx + 255) / 256, L.dom.ny);
   hipLaunchKernelGGL(lbm_flags_kernel, grid, dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.solid, G.flags);
-  hipLaunchKernelGGL(lbm_zero_count_kernel, dim3(1), dim3(64), 0, L.ctx->stream, G.list_count);
-  hipLaunchKernelGGL(lbm_list_kernel, grid, dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.flags, G.list,
-                     G.list_count);
+  hipLaunchKernelGGL(lbm_row_count_kernel, dim3(L.dom.ny), dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny,
+                     G.flags, G.rows);
+  hipLaunchKernelGGL(lbm_row_scan_kernel, dim3(1), dim3(256), 0, L.ctx->stream, L.dom.ny, G.rows,
+                     G.rows + L.dom.ny, G.list_count);
+  hipLaunchKernelGGL(lbm_row_fill_kernel, dim3(L.dom.ny), dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.flags,
+                     G.rows + L.dom.ny, G.list);
   hipLaunchKernelGGL(lbm_links_kernel, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
                      L.dom.ny, G.flags, G.sdf, G.list, G.list_count, G.links);
   FB_HIP(hipGetLastError());
@@ -1288,6 +1355,7 @@ static void free_lattice(Lattice* L) {
     if (L->G[k].list) (void)hipFree(L->G[k].list);
     if (L->G[k].list_count) (void)hipFree(L->G[k].list_count);
     if (L->G[k].links) (void)hipFree(L->G[k].links);
+    if (L->G[k].rows) (void)hipFree(L->G[k].rows);
   }
   for (int k = 0; k < 3; ++k)
     if (L->PRE[k]) (void)hipFree(L->PRE[k]);
@@ -1337,6 +1405,7 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
     if ((e = hipMalloc((void**)&L->G[k].list, N * sizeof(int)))) return fail(e);
     if ((e = hipMalloc((void**)&L->G[k].list_count, sizeof(int)))) return fail(e);
     if ((e = hipMalloc((void**)&L->G[k].links, N * 8 * sizeof(double)))) return fail(e);
+    if ((e = hipMalloc((void**)&L->G[k].rows, 2 * (size_t)dom->ny * sizeof(int)))) return fail(e);
     if ((e = hipMemsetAsync(L->R[k], 0, N * L->ssize, c->stream))) return fail(e);
   }
   for (int k = 0; k < 3; ++k)
