@@ -58,6 +58,15 @@ __host__ __device__ inline double dkey_inv(unsigned long long k) {
   return v;
 }
 
+// Max/min into a shared key with a read first: once the key holds the running extreme, most
+// waves skip the atomic, which keeps same-address contention low on large grids.
+__device__ __forceinline__ void key_max(unsigned long long* p, unsigned long long k) {
+  if (k > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, k);
+}
+__device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long k) {
+  if (k < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, k);
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) {
   i %= n;
   return i < 0 ? i + n : i;

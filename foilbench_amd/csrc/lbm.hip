@@ -641,9 +641,9 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
     bad = wave_sum_ll(bad);
     nfu = wave_sum_ll(nfu);
     if ((threadIdx.x & 63) == 0) {
-      atomicMin(&a.red[0], dkey(mn));
-      atomicMax(&a.red[1], dkey(vmax));
-      atomicMax(&a.red[2], dkey(exc));
+      key_min(&a.red[0], dkey(mn));
+      key_max(&a.red[1], dkey(vmax));
+      key_max(&a.red[2], dkey(exc));
       if (nf) atomicAdd(&a.red[3], (unsigned long long)nf);
       if (bad) atomicAdd(&a.red[4], (unsigned long long)bad);
       if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
@@ -1169,9 +1169,9 @@ __global__ void __launch_bounds__(256) lbm_stats_kernel(LbmArgs<typename P::S> a
   bad = wave_sum_ll(bad);
   nfu = wave_sum_ll(nfu);
   if ((threadIdx.x & 63) == 0) {
-    atomicMin(&a.red[0], dkey(mn));
-    atomicMax(&a.red[1], dkey(vmax));
-    atomicMax(&a.red[2], dkey(exc));
+    key_min(&a.red[0], dkey(mn));
+    key_max(&a.red[1], dkey(vmax));
+    key_max(&a.red[2], dkey(exc));
     if (nf) atomicAdd(&a.red[3], (unsigned long long)nf);
     if (bad) atomicAdd(&a.red[4], (unsigned long long)bad);
     if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);

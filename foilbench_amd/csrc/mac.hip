@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) absmax_kernel(size_t n, const T* __restri
   double m = 0.0;
   if (i < n) m = (double)fabs(a[i]);
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(&keys[slot], dkey(m));
+  if ((threadIdx.x & 63) == 0) key_max(&keys[slot], dkey(m));
 }
 
 // One Jacobi sweep of (I - nu dt Lap) u = u0 (fb/core/grid.py:245-267) on a logical W x H
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
     m = (double)fabs(upd - at(x, y));
   }
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(&a.resid[c * a.stride + a.k], dkey(m));
+  if ((threadIdx.x & 63) == 0) key_max(&a.resid[c * a.stride + a.k], dkey(m));
 }
 
 // Periodic duplicate endpoint copy after face diffusion (fb/core/grid.py:271-274).
@@ -441,11 +441,11 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
   nf = wave_sum_ll(nf);
   nfw = wave_sum_ll(nfw);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&s.keys[0], dkey(face));
-    atomicMax(&s.keys[1], dkey(wsp));
-    atomicMax(&s.keys[2], dkey(csp));
-    atomicMax(&s.keys[3], dkey(dv));
-    atomicMax(&s.keys[4], dkey(leak));
+    key_max(&s.keys[0], dkey(face));
+    key_max(&s.keys[1], dkey(wsp));
+    key_max(&s.keys[2], dkey(csp));
+    key_max(&s.keys[3], dkey(dv));
+    key_max(&s.keys[4], dkey(leak));
     if (nf) atomicAdd(&s.keys[5], (unsigned long long)nf);
     if (nfw) atomicAdd(&s.keys[6], (unsigned long long)nfw);
   }

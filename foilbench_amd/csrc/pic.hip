@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(256) flip_advect_kernel(FlipArgs a, int n, T* 
     if (c) travel = (double)sqrt(px * px + py * py);
   }
   travel = wave_max(travel);
-  if ((threadIdx.x & 63) == 0 && travel > 0.0) atomicMax(a.travel_key, dkey(travel));
+  if ((threadIdx.x & 63) == 0 && travel > 0.0) key_max(a.travel_key, dkey(travel));
   // a key of 0 (no candidate moved) reads as zero travel
 }
 
@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(256) pic_particle_stats_kernel(PicGeom g, int 
   inside = wave_sum_i(inside);
   bad = wave_sum_i(bad);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&keys[0], dkey(sp));
+    key_max(&keys[0], dkey(sp));
     if (inside) atomicAdd(&keys[2], (unsigned long long)inside);
     if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
   }
@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(256) pic_cell_stats_kernel(int ncell, const T*
   empty = wave_sum_i(empty);
   under = wave_sum_i(under);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&keys[1], dkey(sp));
+    key_max(&keys[1], dkey(sp));
     if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
     if (fluid) atomicAdd(&keys[4], (unsigned long long)fluid);
     if (empty) atomicAdd(&keys[5], (unsigned long long)empty);

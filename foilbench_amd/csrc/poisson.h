@@ -31,6 +31,10 @@ struct CgVecs {
   T* work = nullptr;
   T* scratch = nullptr;
   DstPlan* plan = nullptr;
+  // y-direction partition solver: per x-mode factors and spikes for chunks of tri_L rows
+  // (the last chunk has tri_Llast rows), tri_P chunks per column
+  T* tri = nullptr;
+  int tri_L = 0, tri_Llast = 0, tri_P = 0;
 };
 
 class Poisson {
@@ -63,6 +67,8 @@ class Poisson {
  private:
   template <class T>
   int alloc(CgVecs<T>& v);
+  template <class T>
+  int make_tri(CgVecs<T>& v);
   template <class T>
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>

@@ -445,6 +445,89 @@ __global__ void __launch_bounds__(64) thomas_kernel(int nx, int ny, T* __restric
   }
 }
 
+// Partition solve of (T_y/dy^2 + ex[i] I) z = w for every x-mode column i, in place: the same
+// tridiagonal systems the Thomas solve above handles, with short dependent chains. Block =
+// 64 modes (lanes) x P chunks (waves). Each thread eliminates its chunk of rows with the
+// chunk's precomputed LU factors (zero boundary values); the P chunk interfaces of a mode then
+// form a 2P-unknown system (one lane per mode, in LDS); finally each chunk adds its two
+// precomputed spikes scaled by the neighbouring interface values.
+// tab per mode: [invm, cp, alpha, beta] for chunks of L rows, then the same for Llast rows.
+template <class T>
+__global__ void __launch_bounds__(1024) tri_partition_kernel(int nx, int ny, T* __restrict__ w,
+                                                             const T* __restrict__ tab, int L, int Llast, int P, T a,
+                                                             const double* scal) {
+  if (scal && CG_DONE(scal)) return;
+  __shared__ T sF[16][64], sE[16][64];
+  const int lane = threadIdx.x & 63;
+  const int c = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const bool on = i < nx && c < P;
+  const int Lc = (c == P - 1) ? Llast : L;
+  const int s0 = c * L;
+  const size_t stride_mode = 4 * (size_t)(L + Llast);
+  const T* t = tab + (size_t)(on ? i : 0) * stride_mode + (c == P - 1 ? 4 * (size_t)L : 0);
+  const T* invm = t;
+  const T* cp = t + Lc;
+  const T* al = t + 2 * Lc;
+  const T* be = t + 3 * Lc;
+  if (on) {
+    // forward elimination within the chunk
+    T y = (T)0;
+    for (int j = 0; j < Lc; ++j) {
+      const size_t k = (size_t)(s0 + j) * nx + i;
+      y = (w[k] - a * y) * invm[j];
+      w[k] = y;
+    }
+    // back substitution within the chunk
+    T yn = y;
+    for (int j = Lc - 2; j >= 0; --j) {
+      const size_t k = (size_t)(s0 + j) * nx + i;
+      yn = w[k] - cp[j] * yn;
+      w[k] = yn;
+    }
+    sF[c][lane] = yn;  // y at the chunk's first row
+    sE[c][lane] = y;   // y at the chunk's last row
+  }
+  __syncthreads();
+  if (c == 0 && i < nx) {
+    // interfaces: F_c = z(first row of c), E_c = z(last row of c);
+    // F_c = yF + aF E_{c-1} + bF F_{c+1},  E_c = yE + aE E_{c-1} + bE F_{c+1}
+    T r[16], sv[16], pv[16], qv[16];
+    T pp = (T)0, qq = (T)0;
+    for (int k = 0; k < P; ++k) {
+      const int Lk = (k == P - 1) ? Llast : L;
+      const T* tk = tab + (size_t)i * stride_mode + (k == P - 1 ? 4 * (size_t)L : 0);
+      const T aF = tk[2 * Lk], bF = tk[3 * Lk], aE = tk[2 * Lk + Lk - 1], bE = tk[3 * Lk + Lk - 1];
+      const T den = (T)1 - aF * qq;
+      r[k] = (sF[k][lane] + aF * pp) / den;
+      sv[k] = bF / den;
+      const T pn = sE[k][lane] + aE * pp + aE * qq * r[k];
+      const T qn = aE * qq * sv[k] + bE;
+      pv[k] = pn;
+      qv[k] = qn;
+      pp = pn;
+      qq = qn;
+    }
+    T Fn = (T)0;
+    for (int k = P - 1; k >= 0; --k) {
+      const T F = r[k] + sv[k] * Fn;
+      const T E = pv[k] + qv[k] * Fn;
+      sF[k][lane] = F;
+      sE[k][lane] = E;
+      Fn = F;
+    }
+  }
+  __syncthreads();
+  if (on) {
+    const T El = c > 0 ? sE[c - 1][lane] : (T)0;
+    const T Fr = c < P - 1 ? sF[c + 1][lane] : (T)0;
+    for (int j = 0; j < Lc; ++j) {
+      const size_t k = (size_t)(s0 + j) * nx + i;
+      w[k] = w[k] + al[j] * El + be[j] * Fr;
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) copy_kernel(size_t N, const T* __restrict__ a, T* __restrict__ b,
                                                    const double* scal) {
@@ -535,7 +618,7 @@ __global__ void __launch_bounds__(256) rhs_prep_kernel(size_t N, const double* _
   double mw = m;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&stat[1], (unsigned long long)__double_as_longlong(mw));
+  if ((threadIdx.x & 63) == 0) key_max(&stat[1], (unsigned long long)__double_as_longlong(mw));
   double sv = block_sum<256>(v, lds);
   double sc = block_sum<256>(c, lds);
   if (threadIdx.x == 0) {
@@ -577,7 +660,7 @@ Poisson::~Poisson() {
   if (host_scal) (void)hipHostFree(host_scal);
   auto freev = [](auto& v) {
     for (void* ptr : {(void*)v.r, (void*)v.z, (void*)v.p, (void*)v.q, (void*)v.x, (void*)v.b, (void*)v.work,
-                      (void*)v.scratch})
+                      (void*)v.scratch, (void*)v.tri})
       if (ptr) (void)hipFree(ptr);
     free_plan(v.plan);
   };
@@ -590,7 +673,57 @@ int Poisson::alloc(CgVecs<T>& v) {
   if (v.r) return FB_OK;
   for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(hipMalloc((void**)a, N * sizeof(T)));
   FB_HIP(hipMalloc((void**)&v.p, 2 * N * sizeof(T)));
-  if (!periodic_x && !periodic_y) return make_plan<T>(v.plan, nx);
+  if (!periodic_x && !periodic_y) {
+    int rc = make_plan<T>(v.plan, nx);
+    if (rc) return rc;
+    return make_tri<T>(v);
+  }
+  return FB_OK;
+}
+
+// Factors and spikes of the y-direction partition solver (long double on the host).
+template <class T>
+int Poisson::make_tri(CgVecs<T>& v) {
+  const int P0 = std::min(16, ny);
+  const int L = (ny + P0 - 1) / P0;
+  const int P = (ny + L - 1) / L;
+  const int Ll = ny - (P - 1) * L;
+  v.tri_L = L;
+  v.tri_Llast = Ll;
+  v.tri_P = P;
+  const size_t stride = 4 * (size_t)(L + Ll);
+  std::vector<T> tab(stride * (size_t)nx);
+  const long double iy2 = 1.0L / ((long double)dy * (long double)dy);
+  const long double a = -iy2;
+  std::vector<double> exh(nx);
+  for (int i = 0; i < nx; ++i)
+    exh[i] = 2.0 * (1.0 - std::cos(M_PI * (double)(i + 1) / (double)(nx + 1))) * (1.0 / (dx * dx));
+  std::vector<long double> m, y;
+  for (int i = 0; i < nx; ++i) {
+    const long double b = 2.0L * iy2 + (long double)exh[i];
+    for (int part = 0; part < 2; ++part) {
+      const int Lc = part == 0 ? L : Ll;
+      T* t = tab.data() + (size_t)i * stride + (part == 0 ? 0 : 4 * (size_t)L);
+      m.assign(Lc, 0.0L);
+      y.assign(Lc, 0.0L);
+      m[0] = b;
+      for (int j = 1; j < Lc; ++j) m[j] = b - a * a / m[j - 1];
+      for (int j = 0; j < Lc; ++j) {
+        t[j] = (T)(1.0L / m[j]);
+        t[Lc + j] = (T)(a / m[j]);
+      }
+      // alpha: coupling to the value below the chunk (right-hand side -a in the first row)
+      for (int j = 0; j < Lc; ++j) y[j] = ((j == 0 ? -a : 0.0L) - (j ? a * y[j - 1] : 0.0L)) / m[j];
+      for (int j = Lc - 2; j >= 0; --j) y[j] = y[j] - (a / m[j]) * y[j + 1];
+      for (int j = 0; j < Lc; ++j) t[2 * Lc + j] = (T)y[j];
+      // beta: coupling to the value above the chunk (right-hand side -a in the last row)
+      for (int j = 0; j < Lc; ++j) y[j] = ((j == Lc - 1 ? -a : 0.0L) - (j ? a * y[j - 1] : 0.0L)) / m[j];
+      for (int j = Lc - 2; j >= 0; --j) y[j] = y[j] - (a / m[j]) * y[j + 1];
+      for (int j = 0; j < Lc; ++j) t[3 * Lc + j] = (T)y[j];
+    }
+  }
+  FB_HIP(hipMalloc((void**)&v.tri, tab.size() * sizeof(T)));
+  FB_HIP(hipMemcpy(v.tri, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice));
   return FB_OK;
 }
 
@@ -657,8 +790,8 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
                      (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 0,
                      partial, scal);
   FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(thomas_kernel<T>, dim3((nx + 63) / 64), dim3(64), 0, stream, nx, ny, v.work, v.scratch, ex,
-                     (T)(1.0 / (dy * dy)), scal);
+  hipLaunchKernelGGL(tri_partition_kernel<T>, dim3((nx + 63) / 64), dim3(64 * v.tri_P), 0, stream, nx, ny, v.work,
+                     (const T*)v.tri, v.tri_L, v.tri_Llast, v.tri_P, (T)(-(1.0 / (dy * dy))), scal);
   FB_HIP(hipGetLastError());
   hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(ny), dim3(threads), lds, stream, nx, ny, v.work, flags, rin, zout,
                      (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 1,
