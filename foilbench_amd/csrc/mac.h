@@ -62,8 +62,13 @@ struct MacGrid {
   unsigned long long* resid = nullptr;  // 2 x (maxit + 2)
   unsigned long long* host_resid = nullptr;
   int resid_cap = 0;
+  int last_sweeps = 0;  // Jacobi sweeps of the previous diffusion (sets the first polling chunk)
   Poisson poisson;
   bool poisson_ready = false;
+  // statistics of the faces after the last projection (valid until the faces change)
+  fb_mac_stats last_stats{};
+  double last_stats_omega = 0.0;
+  bool last_stats_valid = false;
 };
 
 int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision, MacGrid** out);

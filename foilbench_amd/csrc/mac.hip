@@ -180,19 +180,25 @@ struct JacobiArgs {
   int W[2], H[2], P[2];
   int px, py;
   double ax, ay, den;
-  double scale[2];
+  const unsigned long long* scale_keys;  // keys of max |original| per component
   double tol;
   int k;
   unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
   int stride;
 };
 
+// max(max |u0|, 1) of fb/core/grid.py:255 (NaN propagates).
+__device__ __forceinline__ double jacobi_scale(unsigned long long key) {
+  const double m = key ? dkey_inv(key) : 0.0;
+  return (m != m) ? m : fmax(m, 1.0);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
   const int c = blockIdx.z;
   if (a.k > 1) {
     const unsigned long long key = a.resid[c * a.stride + a.k - 1];
-    if (key != 0ull && dkey_inv(key) / a.scale[c] <= a.tol) {
+    if (key != 0ull && dkey_inv(key) / jacobi_scale(a.scale_keys[c]) <= a.tol) {
       // converged: carry the residual forward so every later sweep of this solve skips too
       // (a later sweep would otherwise read a ping-pong buffer this solve never wrote)
       if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.resid[c * a.stride + a.k] = key;
@@ -454,6 +460,7 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
 // ---------------------------------------------------------------- host handle
 
 int mac_geometry(MacGrid& M, const fb_pose& pose) {
+  M.last_stats_valid = false;
   int rc = launch_cell_sdf(M.ctx->stream, make_cell_grid(M.dom), make_naca(M.foil, pose), M.sdf, M.solid);
   if (rc) return rc;
   M.angle = pose.angle_degrees;
@@ -471,25 +478,43 @@ static int read_keys(MacGrid& M, int n) {
   return FB_OK;
 }
 
+// Statistics of the current faces into keys[base .. base+6] (enqueued only).
 template <class T>
-int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
-  FB_HIP(hipMemsetAsync(M.keys, 0, 8 * sizeof(unsigned long long), M.ctx->stream));
-  StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.keys};
+static int launch_stats(MacGrid& M, double w, int base) {
+  FB_HIP(hipMemsetAsync(M.keys + base, 0, 8 * sizeof(unsigned long long), M.ctx->stream));
+  StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.keys + base};
   dim3 grid((M.dom.nx + 1 + 255) / 256, M.dom.ny + 1);
   hipLaunchKernelGGL(mac_stats_kernel<T>, grid, dim3(256), 0, M.ctx->stream, s, (const T*)M.U[M.cur],
                      (const T*)M.V[M.cur]);
   FB_HIP(hipGetLastError());
-  int rc = read_keys(M, 7);
-  if (rc) return rc;
-  auto val = [&](int k) { return M.host_keys[k] ? dkey_inv(M.host_keys[k]) : 0.0; };
+  return FB_OK;
+}
+
+// Decode keys[base ..] from the host copy.
+static void stats_from_host(const MacGrid& M, int base, fb_mac_stats* st) {
+  const unsigned long long* k = M.host_keys + base;
+  auto val = [&](int j) { return k[j] ? dkey_inv(k[j]) : 0.0; };
   st->max_face_speed = val(0);
   st->max_wall_speed = val(1);
   st->max_cell_speed = val(2);
   st->divergence_linf = val(3);
   st->solid_leakage = val(4);
-  st->nonfinite_faces = (int64_t)M.host_keys[5];
-  st->nonfinite_wall = (int64_t)M.host_keys[6];
+  st->nonfinite_faces = (int64_t)k[5];
+  st->nonfinite_wall = (int64_t)k[6];
+}
+
+template <class T>
+int mac_stats(MacGrid& M, double w, fb_mac_stats* st) {
+  int rc = launch_stats<T>(M, w, 0);
+  if (rc) return rc;
+  if ((rc = read_keys(M, 7))) return rc;
+  stats_from_host(M, 0, st);
   return FB_OK;
+}
+
+static double host_jacobi_scale(unsigned long long key) {
+  const double m = key ? dkey_inv(key) : 0.0;
+  return (m != m) ? m : std::max(m, 1.0);
 }
 
 // Jacobi implicit viscosity on two fields (cell planes, or u/v faces for PIC).
@@ -514,15 +539,13 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
     FB_HIP(hipHostMalloc((void**)&M.host_resid, 2 * M.resid_cap * sizeof(unsigned long long), 0));
   }
   FB_HIP(hipMemsetAsync(M.resid, 0, 2 * M.resid_cap * sizeof(unsigned long long), s));
-  // scale = max(max |original|, 1)
+  // scale = max(max |original|, 1), kept on the device (keys 0 and 1) and read with the residuals
   FB_HIP(hipMemsetAsync(M.keys, 0, 2 * sizeof(unsigned long long), s));
   hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H0 * P0 + 255) / 256), dim3(256), 0, s,
                      (size_t)H0 * P0, in0, M.keys, 0);
   hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H1 * P1 + 255) / 256), dim3(256), 0, s,
                      (size_t)H1 * P1, in1, M.keys, 1);
   FB_HIP(hipGetLastError());
-  int rc = read_keys(M, 2);
-  if (rc) return rc;
   JacobiArgs<T> a{};
   a.orig[0] = in0;
   a.orig[1] = in1;
@@ -537,8 +560,7 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   a.ax = nu * dt / (M.g.dx * M.g.dx);
   a.ay = nu * dt / (M.g.dy * M.g.dy);
   a.den = 1.0 + 2.0 * a.ax + 2.0 * a.ay;
-  a.scale[0] = std::max(dkey_inv(M.host_keys[0]), 1.0);
-  a.scale[1] = std::max(dkey_inv(M.host_keys[1]), 1.0);
+  a.scale_keys = M.keys;
   a.tol = tol;
   a.resid = M.resid;
   a.stride = M.resid_cap;
@@ -548,7 +570,8 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   int done[2] = {0, 0};
   double fres[2] = {INFINITY, INFINITY};
   int k = 1;
-  int chunk = 8;
+  // the first read comes after about as many sweeps as the previous solve needed
+  int chunk = std::min(std::max(M.last_sweeps + 1, 4), 64);
   while (k <= maxit && !(done[0] && done[1])) {
     int kend = std::min(maxit, k + chunk - 1);
     for (int kk = k; kk <= kend; ++kk) {
@@ -562,11 +585,13 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
     }
     FB_HIP(hipMemcpyAsync(M.host_resid, M.resid, 2 * M.resid_cap * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
+    FB_HIP(hipMemcpyAsync(M.host_keys, M.keys, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FB_HIP(hipStreamSynchronize(s));
+    const double scale[2] = {host_jacobi_scale(M.host_keys[0]), host_jacobi_scale(M.host_keys[1])};
     for (int c = 0; c < 2; ++c) {
       if (done[c]) continue;
       for (int kk = k; kk <= kend; ++kk) {
-        double r = dkey_inv(M.host_resid[c * M.resid_cap + kk]) / a.scale[c];
+        double r = dkey_inv(M.host_resid[c * M.resid_cap + kk]) / scale[c];
         fres[c] = r;
         if (r <= tol) {
           done[c] = kk;
@@ -582,6 +607,7 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
     its[c] = done[c] ? done[c] : maxit;
     result[c] = buf[c][its[c] % 2];
   }
+  M.last_sweeps = std::max(its[0], its[1]);
   rep->viscosity_iterations = std::max(its[0], its[1]);
   rep->viscosity_residual = std::max(fres[0], fres[1]);
   rep->viscosity_converged = (fres[0] <= tol && fres[1] <= tol) ? 1 : 0;
@@ -590,21 +616,13 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
 
 template <class T>
 int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
+  // The reference checks the CFL guard and finiteness before solving (fb/solvers/stable_fluids.py:
+  // 256-330). Here the statistics are taken before the solve and read with everything else at the
+  // single synchronisation at the end; the status follows the reference's order of checks. The
+  // faces are modified either way, and the callers roll back on any non-zero status.
   hipStream_t s = M.ctx->stream;
   int rc;
-  // projection CFL guard (fb/solvers/stable_fluids.py:256-298)
-  fb_mac_stats st{};
-  if ((rc = mac_stats<T>(M, p.wall_omega_rad, &st))) return rc;
-  if (st.nonfinite_faces || st.nonfinite_wall) {
-    rep->status = FB_SF_NONFINITE_PROJECTION_INPUT;
-    return FB_OK;
-  }
-  double pc = std::max(st.max_face_speed, st.max_wall_speed) * dt / std::min(M.g.dx, M.g.dy);
-  rep->projection_cfl = pc;
-  if (pc > p.cfl_limit) {
-    rep->status = FB_SF_PROJECTION_CFL;
-    return FB_OK;
-  }
+  if ((rc = launch_stats<T>(M, p.wall_omega_rad, 0))) return rc;
   BcArgs b{M.g, p.freestream_x, p.freestream_y, p.channel_walls, p.wall_omega_rad, M.foil.pivot_x, M.foil.pivot_y,
            M.solid};
   const int a = M.cur, o = 1 - M.cur;
@@ -617,6 +635,29 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   FB_HIP(hipGetLastError());
   PoissonReport pr;
   if ((rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr))) return rc;
+  hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
+  hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
+  hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
+  hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)M.V[a], (T*)M.V[o]);
+  FB_HIP(hipGetLastError());
+  M.cur = o;
+  if ((rc = launch_stats<T>(M, p.wall_omega_rad, 8))) return rc;
+  FB_HIP(hipMemcpyAsync(M.host_keys, M.keys, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  FB_HIP(hipStreamSynchronize(s));
+  M.poisson.finish(&pr);
+  fb_mac_stats pre{}, post{};
+  stats_from_host(M, 0, &pre);
+  stats_from_host(M, 8, &post);
+  if (pre.nonfinite_faces || pre.nonfinite_wall) {
+    rep->status = FB_SF_NONFINITE_PROJECTION_INPUT;
+    return FB_OK;
+  }
+  const double pc = std::max(pre.max_face_speed, pre.max_wall_speed) * dt / std::min(M.g.dx, M.g.dy);
+  rep->projection_cfl = pc;
+  if (pc > p.cfl_limit) {
+    rep->status = FB_SF_PROJECTION_CFL;
+    return FB_OK;
+  }
   rep->pressure_iterations = pr.iterations;
   rep->pressure_residual = pr.relative_residual;
   rep->pressure_info = pr.info;
@@ -628,27 +669,24 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
                                     : FB_SF_CG_BREAKDOWN;
     return FB_OK;
   }
-  hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
-  hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
-  hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
-  hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)M.V[a], (T*)M.V[o]);
-  FB_HIP(hipGetLastError());
-  M.cur = o;
   rep->pressure_converged =
       (pr.info == 0 && pr.relative_residual <= p.tolerance * (1.0 + 1.0e-6)) ? 1 : 0;
   if (!rep->pressure_converged) {
     rep->status = FB_SF_PROJECTION_NOT_CONVERGED;
     return FB_OK;
   }
-  // non-finite velocity after projection
-  if ((rc = mac_stats<T>(M, p.wall_omega_rad, &st))) return rc;
-  if (st.nonfinite_faces) rep->status = FB_SF_NONFINITE_PROJECTION_OUTPUT;
+  if (post.nonfinite_faces) rep->status = FB_SF_NONFINITE_PROJECTION_OUTPUT;
+  // the statistics of the projected faces, for the caller's postconditions
+  M.last_stats = post;
+  M.last_stats_omega = p.wall_omega_rad;
+  M.last_stats_valid = true;
   return FB_OK;
 }
 
 template <class T>
 static int substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_report* rep) {
   hipStream_t s = M.ctx->stream;
+  M.last_stats_valid = false;
   int rc;
   const int nx = M.dom.nx, ny = M.dom.ny;
   dim3 gc((nx + 255) / 256, ny);
@@ -825,6 +863,7 @@ int fb_mac_destroy(fb_mac* mac) {
 int fb_mac_set_faces(fb_mac* mac, const void* u_host, const void* v_host) {
   FB_REQUIRE(mac && u_host && v_host, FB_ERR_ARG, "NULL argument");
   auto* M = reinterpret_cast<MacGrid*>(mac);
+  M->last_stats_valid = false;
   FB_HIP(hipSetDevice(M->ctx->device));
   FB_HIP(hipMemcpyAsync(M->U[M->cur], u_host, M->NU * M->sz, hipMemcpyHostToDevice, M->ctx->stream));
   FB_HIP(hipMemcpyAsync(M->V[M->cur], v_host, M->NV * M->sz, hipMemcpyHostToDevice, M->ctx->stream));
@@ -885,6 +924,7 @@ int fb_mac_checkpoint(fb_mac* mac) {
 int fb_mac_rollback(fb_mac* mac) {
   FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
   auto* M = reinterpret_cast<MacGrid*>(mac);
+  M->last_stats_valid = false;
   FB_REQUIRE(M->ck_valid, FB_ERR_STATE, "no checkpoint to roll back to");
   FB_HIP(hipSetDevice(M->ctx->device));
   hipStream_t s = M->ctx->stream;
@@ -927,6 +967,11 @@ int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats) {
   FB_REQUIRE(mac && stats, FB_ERR_ARG, "NULL argument");
   auto* M = reinterpret_cast<MacGrid*>(mac);
   FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  if (M->last_stats_valid && M->last_stats_omega == wall_omega_rad) {
+    // the faces have not changed since the projection that measured them
+    *stats = M->last_stats;
+    return FB_OK;
+  }
   FB_HIP(hipSetDevice(M->ctx->device));
   return MAC_DISPATCH(M, fb::mac_stats, *M, wall_omega_rad, stats);
 }

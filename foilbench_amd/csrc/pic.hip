@@ -1056,6 +1056,7 @@ static int pic_substep(Pic& P, const fb_pic_params& p, const fb_pose& pose, cons
   FB_HIP(hipMemcpyAsync(P.table, table, 2 * kSweepEntries * sizeof(double), hipMemcpyHostToDevice, s));
   // keys: [0] unsupported faces [1] travel [2] swept hits
   fill_keys(s, P.keys, 4);
+  M.last_stats_valid = false;
   if (!M.geom_valid || pose.angle_degrees != M.angle) {
     if ((rc = mac_geometry(M, pose))) return rc;
   }

@@ -47,6 +47,9 @@ class Poisson {
   // residual-refinement loop to the same relative tolerance.
   int solve(const double* d_rhs, double* d_x, double tolerance, int max_iterations, bool mixed,
             PoissonReport* rep);
+  // Completes `rep` (the relative residual) once the stream has been synchronised after solve():
+  // solve() leaves its last scalars in flight to host memory.
+  void finish(PoissonReport* rep) const;
 
   hipStream_t stream = nullptr;
   int nx = 0, ny = 0, periodic_x = 0, periodic_y = 0;
@@ -72,8 +75,11 @@ class Poisson {
   template <class T>
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>
-  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* iterations, int* info, int* failure);
+  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool check_rhs, int first_chunk, int* iterations,
+          int* info, int* failure);
   int reduce(const double* partial_in, int n, int mode);
+  int sum_to(const double* partial_in, int n, int slot);
+  int last_iterations_ = 0, last_inner_ = 0, pending_failure_ = 0;
   int read_scalars();
 };
 

@@ -24,7 +24,9 @@ namespace fb {
 enum : uint8_t { PF_SOLID = 1, PF_RIGHT = 2, PF_UP = 4, PF_WRAPX = 8, PF_WRAPY = 16 };
 // Device scalar slots.
 enum { S_RN2 = 0, S_RHO, S_RHOPREV, S_PQ, S_ALPHA, S_BETA, S_ATOL, S_DONE, S_ITER, S_FAIL, S_INFO, S_MAXIT, S_DOT,
-       S_COUNT = 16 };
+       S_COUNT = 16,
+       // after the two right-hand-side status words at S_COUNT
+       S_BSUM = S_COUNT + 2, S_BCNT, S_RES2, S_B2 };
 
 template <class T>
 struct cplx {
@@ -635,6 +637,38 @@ __global__ void __launch_bounds__(256) sub_mean_kernel(size_t N, const uint8_t* 
   if (i < N && !(fl[i] & PF_SOLID)) a[i] -= mean;
 }
 
+__global__ void __launch_bounds__(256) sub_mean_dev_kernel(size_t N, const uint8_t* __restrict__ fl,
+                                                           double* __restrict__ a, const double* __restrict__ scal) {
+  const double cnt = scal[S_BCNT];
+  if (!(cnt > 0.0)) return;
+  const double mean = scal[S_BSUM] / cnt;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N && !(fl[i] & PF_SOLID)) a[i] -= mean;
+}
+
+// Deterministic sum of n block partials into scal[slot].
+__global__ void __launch_bounds__(256) sum_slot_kernel(const double* __restrict__ partial, int n, double* scal,
+                                                       int slot) {
+  __shared__ double lds[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  s = block_sum<256>(s, lds);
+  if (threadIdx.x == 0) scal[slot] = s;
+}
+
+// Non-finite or out-of-range right-hand side: stop the CG before its first iteration.
+__global__ void rhs_check_kernel(double* scal, double safe) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const unsigned long long* st = reinterpret_cast<const unsigned long long*>(scal + S_COUNT);
+  if (st[0] != 0ull) {
+    scal[S_FAIL] = 1.0;
+    scal[S_DONE] = 1.0;
+  } else if (__longlong_as_double((long long)st[1]) > safe) {
+    scal[S_FAIL] = 2.0;
+    scal[S_DONE] = 1.0;
+  }
+}
+
 __global__ void __launch_bounds__(256) to_f32_scaled(size_t N, const double* __restrict__ a, float* __restrict__ b,
                                                      double s) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -801,7 +835,8 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
 }
 
 template <class T>
-int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* iterations, int* info, int* failure) {
+int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool check_rhs, int first_chunk,
+                 int* iterations, int* info, int* failure) {
   FB_HIP(hipMemsetAsync(x, 0, N * sizeof(T), stream));
   FB_HIP(hipMemcpyAsync(v.r, b, N * sizeof(T), hipMemcpyDeviceToDevice, stream));
   double init[S_COUNT] = {};
@@ -812,11 +847,18 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* it
   FB_HIP(hipGetLastError());
   int rc = reduce(partial, nblocks, 4);
   if (rc) return rc;
+  if (check_rhs) {
+    // rejections of fb/core/_scipy_adapter.py:113-120, decided on the device
+    const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
+    hipLaunchKernelGGL(rhs_check_kernel, dim3(1), dim3(64), 0, stream, scal, safe);
+    FB_HIP(hipGetLastError());
+  }
   const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
   T* pa = v.p;
   T* pb = v.p + N;
   int launched = 0;
-  int chunk = 4;
+  // the first poll comes after about as many iterations as the previous solve needed
+  int chunk = std::min(std::max(first_chunk, 4), 64);
   for (;;) {
     for (int c = 0; c < chunk && launched < maxit; ++c, ++launched) {
       if ((rc = precond(v, v.r, v.z))) return rc;
@@ -839,6 +881,12 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, int* it
   return FB_OK;
 }
 
+int Poisson::sum_to(const double* pin, int n, int slot) {
+  hipLaunchKernelGGL(sum_slot_kernel, dim3(1), dim3(256), 0, stream, pin, n, scal, slot);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
 int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* rep) {
   *rep = PoissonReport{};
   const int nb = nblocks;
@@ -850,44 +898,37 @@ int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit
   const bool singular = periodic_x && periodic_y;
   int rc;
   if (singular) {
-    if ((rc = reduce(partial, nb, 0))) return rc;
-    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
-    if ((rc = reduce(partial + nb, nb, 0))) return rc;
-    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT + 1, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
-  }
-  unsigned long long st[2];
-  FB_HIP(hipMemcpyAsync(st, stat, sizeof(st), hipMemcpyDeviceToHost, stream));
-  FB_HIP(hipStreamSynchronize(stream));
-  if (singular && host_scal[S_COUNT + 1] > 0.0) {
-    double mean = host_scal[S_COUNT] / host_scal[S_COUNT + 1];
-    hipLaunchKernelGGL(sub_mean_kernel, dim3(nb), dim3(256), 0, stream, N, flags, b64, mean);
+    // compatible right-hand side: remove the fluid mean (fb/core/_scipy_adapter.py:108-112)
+    if ((rc = sum_to(partial, nb, S_BSUM)) || (rc = sum_to(partial + nb, nb, S_BCNT))) return rc;
+    hipLaunchKernelGGL(sub_mean_dev_kernel, dim3(nb), dim3(256), 0, stream, N, flags, b64, (const double*)scal);
     FB_HIP(hipGetLastError());
-  }
-  if (st[0] != 0) {
-    rep->failure = 1;
-    return FB_OK;
-  }
-  double bmax;
-  {
-    long long bits = (long long)st[1];
-    memcpy(&bmax, &bits, sizeof(bmax));
-  }
-  const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
-  if (bmax > safe) {
-    rep->failure = 2;
-    return FB_OK;
   }
   const double rtol = 0.9 * tolerance;
   const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
+  pending_failure_ = 0;
   if (!mixed) {
     int its = 0, info = 0, fail = 0;
-    if ((rc = pcg<double>(v64, b64, d_x, rtol, maxit, &its, &info, &fail))) return rc;
+    if ((rc = pcg<double>(v64, b64, d_x, rtol, maxit, true, last_iterations_ + 1, &its, &info, &fail))) return rc;
+    last_iterations_ = its;
     rep->iterations = its;
     rep->info = info;
     rep->failure = fail;
   } else {
     // FP32 PCG sweeps on the scaled FP64 residual, FP64 correction and residual.
     if ((rc = alloc(v32))) return rc;
+    unsigned long long st[2];
+    FB_HIP(hipMemcpyAsync(st, stat, sizeof(st), hipMemcpyDeviceToHost, stream));
+    FB_HIP(hipStreamSynchronize(stream));
+    double bmax;
+    {
+      long long bits = (long long)st[1];
+      memcpy(&bmax, &bits, sizeof(bmax));
+    }
+    const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
+    if (st[0] != 0 || bmax > safe) {
+      rep->failure = st[0] != 0 ? 1 : 2;
+      return FB_OK;
+    }
     hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, N, d_x);
     FB_HIP(hipGetLastError());
     FB_HIP(hipMemcpyAsync(r64, b64, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
@@ -907,7 +948,9 @@ int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit
       FB_HIP(hipGetLastError());
       int its = 0, info = 0, fail = 0;
       // inner tolerance: a relative reduction FP32 reaches reliably
-      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, maxit - total, &its, &info, &fail))) return rc;
+      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, maxit - total, false, last_inner_ + 1, &its, &info, &fail)))
+        return rc;
+      last_inner_ = its;
       total += its;
       rep->refinements = outer + 1;
       if (fail) {
@@ -933,30 +976,26 @@ int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit
   if (singular) {
     hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb), dim3(256), 0, stream, N, d_x, flags, r64, partial, partial + nb,
                        stat);
-    if ((rc = reduce(partial, nb, 0))) return rc;
-    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
-    if ((rc = reduce(partial + nb, nb, 0))) return rc;
-    FB_HIP(hipMemcpyAsync(host_scal + S_COUNT + 1, scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, stream));
-    FB_HIP(hipStreamSynchronize(stream));
-    if (host_scal[S_COUNT + 1] > 0.0) {
-      hipLaunchKernelGGL(sub_mean_kernel, dim3(nb), dim3(256), 0, stream, N, flags, d_x,
-                         host_scal[S_COUNT] / host_scal[S_COUNT + 1]);
-      FB_HIP(hipGetLastError());
-    }
+    if ((rc = sum_to(partial, nb, S_BSUM)) || (rc = sum_to(partial + nb, nb, S_BCNT))) return rc;
+    hipLaunchKernelGGL(sub_mean_dev_kernel, dim3(nb), dim3(256), 0, stream, N, flags, d_x, (const double*)scal);
+    FB_HIP(hipGetLastError());
   }
-  // true relative residual ||b - A x|| / max(||b||, eps) in FP64
+  // true relative residual ||b - A x|| / max(||b||, eps) in FP64, read with the caller's next sync
   hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64, partial, ix2,
                      iy2);
   FB_HIP(hipGetLastError());
-  if ((rc = reduce(partial, nb, 0))) return rc;
-  if ((rc = read_scalars())) return rc;
-  const double rnorm = std::sqrt(host_scal[S_DOT]);
+  if ((rc = sum_to(partial, nb, S_RES2))) return rc;
   hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb), dim3(256), 0, stream, b64, b64, N, partial);
-  if ((rc = reduce(partial, nb, 0))) return rc;
-  if ((rc = read_scalars())) return rc;
-  const double bnorm = std::sqrt(host_scal[S_DOT]);
-  rep->relative_residual = rnorm / std::max(bnorm, std::numeric_limits<double>::epsilon());
+  FB_HIP(hipGetLastError());
+  if ((rc = sum_to(partial, nb, S_B2))) return rc;
+  FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_COUNT, 8 * sizeof(double), hipMemcpyDeviceToHost, stream));
   return FB_OK;
+}
+
+void Poisson::finish(PoissonReport* rep) const {
+  const double rnorm = std::sqrt(host_scal[S_RES2]);
+  const double bnorm = std::sqrt(host_scal[S_B2]);
+  rep->relative_residual = rnorm / std::max(bnorm, std::numeric_limits<double>::epsilon());
 }
 
 }  // namespace fb
