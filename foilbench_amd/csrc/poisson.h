@@ -77,8 +77,24 @@ class Poisson {
   template <class T>
   int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool check_rhs, int first_chunk, int* iterations,
           int* info, int* failure);
+  template <class T>
+  int iterations_chunk(CgVecs<T>& v, T* x, int n, int parity);
+  template <class T>
+  int enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity);
   int reduce(const double* partial_in, int n, int mode);
   int sum_to(const double* partial_in, int n, int slot);
+  struct GraphKey {
+    size_t elem;
+    const void* x;
+    int n, parity;
+    bool operator==(const GraphKey& o) const { return elem == o.elem && x == o.x && n == o.n && parity == o.parity; }
+  };
+  struct CachedGraph {
+    GraphKey key;
+    hipGraphExec_t exec;
+  };
+  std::vector<CachedGraph> graphs_;
+  bool use_graphs_ = true;
   int last_iterations_ = 0, last_inner_ = 0, pending_failure_ = 0;
   int read_scalars();
 };

@@ -689,6 +689,8 @@ __global__ void __launch_bounds__(256) fill_zero(size_t N, double* __restrict__ 
 // ---------------------------------------------------------------- host driver
 
 Poisson::~Poisson() {
+  for (auto& g : graphs_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (void* ptr : {(void*)flags, (void*)diag, (void*)ex, (void*)partial, (void*)scal, (void*)b64, (void*)r64})
     if (ptr) (void)hipFree(ptr);
   if (host_scal) (void)hipHostFree(host_scal);
@@ -853,24 +855,13 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool ch
     hipLaunchKernelGGL(rhs_check_kernel, dim3(1), dim3(64), 0, stream, scal, safe);
     FB_HIP(hipGetLastError());
   }
-  const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
-  T* pa = v.p;
-  T* pb = v.p + N;
   int launched = 0;
   // the first poll comes after about as many iterations as the previous solve needed
   int chunk = std::min(std::max(first_chunk, 4), 64);
   for (;;) {
-    for (int c = 0; c < chunk && launched < maxit; ++c, ++launched) {
-      if ((rc = precond(v, v.r, v.z))) return rc;
-      hipLaunchKernelGGL(cg_pq_kernel<T>, dim3(nblocks), dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q,
-                         partial, scal, ix2, iy2);
-      FB_HIP(hipGetLastError());
-      if ((rc = reduce(partial, nblocks, 2))) return rc;
-      hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal);
-      FB_HIP(hipGetLastError());
-      if ((rc = reduce(partial, nblocks, 3))) return rc;
-      std::swap(pa, pb);
-    }
+    const int n = std::min(chunk, maxit - launched);
+    if ((rc = iterations_chunk(v, x, n, launched & 1))) return rc;
+    launched += n;
     if ((rc = read_scalars())) return rc;
     if (host_scal[S_DONE] != 0.0 || launched >= maxit) break;
     chunk = std::min(2 * chunk, 32);
@@ -878,6 +869,64 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool ch
   *iterations = (int)host_scal[S_ITER];
   *info = (host_scal[S_DONE] != 0.0) ? (int)host_scal[S_INFO] : maxit;
   *failure = (int)host_scal[S_FAIL];
+  return FB_OK;
+}
+
+// `n` CG iterations starting with the search-direction buffers in order `parity`. The
+// iterations only read and write device memory, so a chunk is captured once as a HIP graph
+// per (precision, solution buffer, length, parity) and replayed; every kernel returns at once
+// when the device-side done flag is set.
+template <class T>
+int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
+  if (n <= 0) return FB_OK;
+  const GraphKey key{sizeof(T), (const void*)x, n, parity};
+  for (auto& g : graphs_)
+    if (g.key == key) {
+      FB_HIP(hipGraphLaunch(g.exec, stream));
+      return FB_OK;
+    }
+  if (!use_graphs_) return enqueue_iterations(v, x, n, parity);
+  hipGraph_t graph = nullptr;
+  FB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_iterations(v, x, n, parity);
+  hipError_t e = hipStreamEndCapture(stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error("CG graph capture failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    set_error("CG graph instantiation failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  graphs_.push_back({key, exec});
+  FB_HIP(hipGraphLaunch(exec, stream));
+  return FB_OK;
+}
+
+template <class T>
+int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
+  const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
+  T* pa = parity ? v.p + N : v.p;
+  T* pb = parity ? v.p : v.p + N;
+  int rc;
+  for (int c = 0; c < n; ++c) {
+    if ((rc = precond(v, v.r, v.z))) return rc;
+    hipLaunchKernelGGL(cg_pq_kernel<T>, dim3(nblocks), dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q,
+                       partial, scal, ix2, iy2);
+    FB_HIP(hipGetLastError());
+    if ((rc = reduce(partial, nblocks, 2))) return rc;
+    hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal);
+    FB_HIP(hipGetLastError());
+    if ((rc = reduce(partial, nblocks, 3))) return rc;
+    std::swap(pa, pb);
+  }
   return FB_OK;
 }
 
