@@ -17,12 +17,41 @@ namespace fb {
 void set_error(const char* fmt, ...);
 void clear_error();
 
+// Per-block reduction slots: a reduction kernel writes one value per block and quantity into
+// partials[q * kPartialBlocks + block]; finish_partials() folds them into the destination keys.
+// This keeps large grids off a single address (same-address atomics serialise at one channel).
+constexpr int kPartialBlocks = 8192;
+constexpr int kPartialKeys = 16;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int xcd_count = 8;
+  unsigned long long* partials = nullptr;  // kPartialKeys x kPartialBlocks, used in stream order
 };
+
+// What finish_partials() does with each quantity: fold it by sum (counts) or by max (keys
+// from dkey()) and merge it into keys[dest] with one atomic.
+struct FinishSpec {
+  int n = 0;
+  int dest[kPartialKeys];
+  int sum[kPartialKeys];
+  void add(int d, bool is_sum) {
+    dest[n] = d;
+    sum[n] = is_sum ? 1 : 0;
+    ++n;
+  }
+};
+
+// Enqueues the fold of `nblocks` per-block values of each quantity of `f` into `keys`.
+int finish_partials(Ctx* c, unsigned nblocks, const FinishSpec& f, unsigned long long* keys);
+
+// Grid of a grid-stride reduction kernel of 256-thread blocks over n items.
+inline unsigned reduction_blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > (size_t)kPartialBlocks ? (size_t)kPartialBlocks : b));
+}
 
 }  // namespace fb
 
@@ -87,6 +116,98 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Max that propagates NaN like NumPy's max (fmax would drop it).
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Key of a magnitude for max reductions: NaN maps above +inf, so a max over keys propagates it.
+__device__ __forceinline__ unsigned long long mag_key(double v) { return dkey(fabs(v)); }
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block reductions over a 256-thread block; the result is valid in thread 0. `lds` holds 4 values.
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v, unsigned long long* lds) {
+  v = wave_max_u64(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long r = v;
+  if (threadIdx.x == 0) {
+    r = lds[0] > lds[1] ? lds[0] : lds[1];
+    r = r > lds[2] ? r : lds[2];
+    r = r > lds[3] ? r : lds[3];
+  }
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long* lds) {
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long r = v;
+  if (threadIdx.x == 0) r = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return r;
+}
+
+// Block-reduce one quantity and store it in the block's partial slot q.
+__device__ __forceinline__ void put_max(unsigned long long* part, int q, unsigned long long v,
+                                        unsigned long long* lds) {
+  v = block_max_u64(v, lds);
+  if (threadIdx.x == 0) part[(size_t)q * kPartialBlocks + blockIdx.x] = v;
+}
+__device__ __forceinline__ void put_sum(unsigned long long* part, int q, unsigned long long v,
+                                        unsigned long long* lds) {
+  v = block_sum_u64(v, lds);
+  if (threadIdx.x == 0) part[(size_t)q * kPartialBlocks + blockIdx.x] = v;
+}
+
+// Adds 1 to count[c] for every active lane of the wave. Neighbouring lanes with the same c
+// share one atomic: particles are seeded in cell order and keep most of that order.
+// Every lane of the wave must call it.
+__device__ __forceinline__ void wave_run_add(int* count, int c, bool active) {
+  const int lane = threadIdx.x & 63;
+  const int cc = active ? c : -1;
+  const int prev = __shfl_up(cc, 1, 64);
+  const bool head = lane == 0 || prev != cc;
+  const unsigned long long heads = __ballot(head);
+  if (head && cc >= 0) {
+    const unsigned long long rest = lane == 63 ? 0ull : heads >> (lane + 1);
+    const int len = rest ? __ffsll((long long)rest) : 64 - lane;
+    atomicAdd(&count[cc], len);
+  }
+}
+
+// Claims a slot in cursor[c] for every active lane; lanes of one run get consecutive slots in
+// lane order. Every lane of the wave must call it. Returns the claimed slot (or -1).
+__device__ __forceinline__ int wave_run_claim(int* cursor, int c, bool active) {
+  const int lane = threadIdx.x & 63;
+  const int cc = active ? c : -1;
+  const int prev = __shfl_up(cc, 1, 64);
+  const bool head = lane == 0 || prev != cc;
+  const unsigned long long heads = __ballot(head);
+  int base = 0;
+  if (head && cc >= 0) {
+    const unsigned long long rest = lane == 63 ? 0ull : heads >> (lane + 1);
+    const int len = rest ? __ffsll((long long)rest) : 64 - lane;
+    base = atomicAdd(&cursor[cc], len);
+  }
+  const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+  const int hl = 63 - __clzll((long long)upto);
+  base = __shfl(base, hl, 64);
+  return cc >= 0 ? base + (lane - hl) : -1;
 }
 
 // XCD-aware block remap: blocks are dispatched round-robin over the XCDs; give each XCD

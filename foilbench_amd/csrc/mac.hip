@@ -159,14 +159,17 @@ __global__ void __launch_bounds__(256) advect_correct_kernel(MacGeom g, const T*
 }
 
 // Max |a| over a field into keys[slot] (order-free, exact).
+// Per-block max |a| into partial slot q (grid-stride over n values).
 template <class T>
-__global__ void __launch_bounds__(256) absmax_kernel(size_t n, const T* __restrict__ a, unsigned long long* keys,
-                                                     int slot) {
-  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  double m = 0.0;
-  if (i < n) m = (double)fabs(a[i]);
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) key_max(&keys[slot], dkey(m));
+__global__ void __launch_bounds__(256) absmax_kernel(size_t n, const T* __restrict__ a,
+                                                     unsigned long long* __restrict__ part, int q) {
+  __shared__ unsigned long long lds[4];
+  unsigned long long m = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const unsigned long long k = mag_key((double)a[i]);
+    m = k > m ? k : m;
+  }
+  put_max(part, q, m, lds);
 }
 
 // One Jacobi sweep of (I - nu dt Lap) u = u0 (fb/core/grid.py:245-267) on a logical W x H
@@ -183,8 +186,9 @@ struct JacobiArgs {
   const unsigned long long* scale_keys;  // keys of max |original| per component
   double tol;
   int k;
-  unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
+  const unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
   int stride;
+  unsigned long long* part;  // per-block maxima of this sweep, slot = component
 };
 
 // max(max |u0|, 1) of fb/core/grid.py:255 (NaN propagates).
@@ -195,35 +199,38 @@ __device__ __forceinline__ double jacobi_scale(unsigned long long key) {
 
 template <class T>
 __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
+  __shared__ unsigned long long lds[4];
   const int c = blockIdx.z;
   if (a.k > 1) {
     const unsigned long long key = a.resid[c * a.stride + a.k - 1];
     if (key != 0ull && dkey_inv(key) / jacobi_scale(a.scale_keys[c]) <= a.tol) {
       // converged: carry the residual forward so every later sweep of this solve skips too
       // (a later sweep would otherwise read a ping-pong buffer this solve never wrote)
-      if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.resid[c * a.stride + a.k] = key;
+      if (threadIdx.x == 0) a.part[(size_t)c * kPartialBlocks + blockIdx.x] = key;
       return;
     }
   }
   const int W = a.W[c], H = a.H[c], P = a.P[c];
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y;
-  double m = 0.0;
-  if (x < W && y < H) {
-    const T* r = a.in[c];
+  const T* __restrict__ r = a.in[c];
+  const T* __restrict__ orig = a.orig[c];
+  T* __restrict__ out = a.out[c];
+  const T ax = (T)a.ax, ay = (T)a.ay, den = (T)a.den;
+  unsigned long long m = 0;
+  const size_t n = (size_t)W * H;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256) {
+    const int x = (int)(idx % W), y = (int)(idx / W);
     auto at = [&](int xx, int yy) { return r[(size_t)yy * P + xx]; };
     int xl = a.px ? (x - 1 + W) % W : max(x - 1, 0);
     int xr = a.px ? (x + 1) % W : min(x + 1, W - 1);
     int yd = a.py ? (y - 1 + H) % H : max(y - 1, 0);
     int yu = a.py ? (y + 1) % H : min(y + 1, H - 1);
-    const T ax = (T)a.ax, ay = (T)a.ay, den = (T)a.den;
-    T o = a.orig[c][(size_t)y * P + x];
+    T o = orig[(size_t)y * P + x];
     T upd = (o + ax * (at(xl, y) + at(xr, y)) + ay * (at(x, yd) + at(x, yu))) / den;
-    a.out[c][(size_t)y * P + x] = upd;
-    m = (double)fabs(upd - at(x, y));
+    out[(size_t)y * P + x] = upd;
+    const unsigned long long k = mag_key((double)(upd - at(x, y)));
+    m = k > m ? k : m;
   }
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) key_max(&a.resid[c * a.stride + a.k], dkey(m));
+  put_max(a.part, c, m, lds);
 }
 
 // Periodic duplicate endpoint copy after face diffusion (fb/core/grid.py:271-274).
@@ -387,18 +394,27 @@ struct StatArgs {
   MacGeom g;
   double w, pvx, pvy;
   const uint8_t* solid;
-  unsigned long long* keys;
+  unsigned long long* part;
 };
 
+// Partial slots: [0] max |face| over finite faces [1] max wall speed at solid cells [2] max
+// cell-centred speed [3] max |div| over fluid cells [4] max solid leakage [5] non-finite faces
+// [6] non-finite wall velocities.
 template <class T>
 __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __restrict__ u, const T* __restrict__ v) {
+  __shared__ unsigned long long lds[4];
   const MacGeom& g = s.g;
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y;
-  double face = 0.0, wsp = 0.0, csp = 0.0, dv = 0.0, leak = 0.0;
-  long long nf = 0, nfw = 0;
+  double face = 0.0;
+  unsigned long long wsp = 0, csp = 0, dv = 0, leak = 0;
+  unsigned long long nf = 0, nfw = 0;
+  auto upd = [](unsigned long long& m, double val) {
+    const unsigned long long k = mag_key(val);
+    m = k > m ? k : m;
+  };
   const size_t PU = g.nx + 1;
-  if (x <= g.nx && y <= g.ny) {
+  const size_t total = (size_t)(g.nx + 1) * (g.ny + 1);
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const int x = (int)(idx % (g.nx + 1)), y = (int)(idx / (g.nx + 1));
     if (y < g.ny) {
       T a = u[(size_t)y * PU + x];
       if (!isfinite((double)a)) nf++;
@@ -415,46 +431,37 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
       double wu, wv;
       wall_at(g, x, y, s.w, s.pvx, s.pvy, wu, wv);
       if (!isfinite(wu) || !isfinite(wv)) nfw++;
-      if (sol) wsp = sqrt(wu * wu + wv * wv);
+      if (sol) upd(wsp, sqrt(wu * wu + wv * wv));
       T ul = u[(size_t)y * PU + x], ur = u[(size_t)y * PU + x + 1];
       T vb = v[i], va = v[(size_t)(y + 1) * g.nx + x];
       T cu = (T)0.5 * (ul + ur), cv = (T)0.5 * (vb + va);
-      csp = (double)sqrt(cu * cu + cv * cv);
+      upd(csp, (double)sqrt(cu * cu + cv * cv));
       if (!sol) {
         T div = (ur - ul) / (T)g.dx + (va - vb) / (T)g.dy;
-        dv = (double)fabs(div);
+        upd(dv, (double)div);
       }
       // leakage on the right u face and the upper v face of this cell (interior interfaces)
       if (x + 1 < g.nx && (sol != (bool)s.solid[i + 1])) {
         double w1u, w1v, w2u, w2v;
         wall_at(g, x, y, s.w, s.pvx, s.pvy, w1u, w1v);
         wall_at(g, x + 1, y, s.w, s.pvx, s.pvy, w2u, w2v);
-        leak = fmax(leak, fabs((double)ur - 0.5 * (w1u + w2u)));
+        upd(leak, (double)ur - 0.5 * (w1u + w2u));
       }
       if (y + 1 < g.ny && (sol != (bool)s.solid[i + g.nx])) {
         double w1u, w1v, w2u, w2v;
         wall_at(g, x, y, s.w, s.pvx, s.pvy, w1u, w1v);
         wall_at(g, x, y + 1, s.w, s.pvx, s.pvy, w2u, w2v);
-        leak = fmax(leak, fabs((double)va - 0.5 * (w1v + w2v)));
+        upd(leak, (double)va - 0.5 * (w1v + w2v));
       }
     }
   }
-  face = wave_max(face);
-  wsp = wave_max(wsp);
-  csp = wave_max(csp);
-  dv = wave_max(dv);
-  leak = wave_max(leak);
-  nf = wave_sum_ll(nf);
-  nfw = wave_sum_ll(nfw);
-  if ((threadIdx.x & 63) == 0) {
-    key_max(&s.keys[0], dkey(face));
-    key_max(&s.keys[1], dkey(wsp));
-    key_max(&s.keys[2], dkey(csp));
-    key_max(&s.keys[3], dkey(dv));
-    key_max(&s.keys[4], dkey(leak));
-    if (nf) atomicAdd(&s.keys[5], (unsigned long long)nf);
-    if (nfw) atomicAdd(&s.keys[6], (unsigned long long)nfw);
-  }
+  put_max(s.part, 0, dkey(face), lds);
+  put_max(s.part, 1, wsp, lds);
+  put_max(s.part, 2, csp, lds);
+  put_max(s.part, 3, dv, lds);
+  put_max(s.part, 4, leak, lds);
+  put_sum(s.part, 5, nf, lds);
+  put_sum(s.part, 6, nfw, lds);
 }
 
 // ---------------------------------------------------------------- host handle
@@ -482,12 +489,14 @@ static int read_keys(MacGrid& M, int n) {
 template <class T>
 static int launch_stats(MacGrid& M, double w, int base) {
   FB_HIP(hipMemsetAsync(M.keys + base, 0, 8 * sizeof(unsigned long long), M.ctx->stream));
-  StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.keys + base};
-  dim3 grid((M.dom.nx + 1 + 255) / 256, M.dom.ny + 1);
-  hipLaunchKernelGGL(mac_stats_kernel<T>, grid, dim3(256), 0, M.ctx->stream, s, (const T*)M.U[M.cur],
+  StatArgs s{M.g, w, M.foil.pivot_x, M.foil.pivot_y, M.solid, M.ctx->partials};
+  const unsigned grid = reduction_blocks((size_t)(M.dom.nx + 1) * (M.dom.ny + 1));
+  hipLaunchKernelGGL(mac_stats_kernel<T>, dim3(grid), dim3(256), 0, M.ctx->stream, s, (const T*)M.U[M.cur],
                      (const T*)M.V[M.cur]);
   FB_HIP(hipGetLastError());
-  return FB_OK;
+  FinishSpec f;
+  for (int q = 0; q < 7; ++q) f.add(base + q, q >= 5);
+  return finish_partials(M.ctx, grid, f, M.keys);
 }
 
 // Decode keys[base ..] from the host copy.
@@ -523,6 +532,7 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
                    double nu, double dt, double tol, int maxit, T** result, fb_sf_report* rep) {
   hipStream_t s = M.ctx->stream;
   const int px = M.dom.periodic_x, py = M.dom.periodic_y;
+  int rc;
   if (nu <= 0.0) {
     result[0] = (T*)in0;
     result[1] = (T*)in1;
@@ -541,11 +551,16 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   FB_HIP(hipMemsetAsync(M.resid, 0, 2 * M.resid_cap * sizeof(unsigned long long), s));
   // scale = max(max |original|, 1), kept on the device (keys 0 and 1) and read with the residuals
   FB_HIP(hipMemsetAsync(M.keys, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H0 * P0 + 255) / 256), dim3(256), 0, s,
-                     (size_t)H0 * P0, in0, M.keys, 0);
-  hipLaunchKernelGGL(absmax_kernel<T>, dim3((unsigned)((size_t)H1 * P1 + 255) / 256), dim3(256), 0, s,
-                     (size_t)H1 * P1, in1, M.keys, 1);
-  FB_HIP(hipGetLastError());
+  {
+    const unsigned nb = reduction_blocks(std::max((size_t)H0 * P0, (size_t)H1 * P1));
+    hipLaunchKernelGGL(absmax_kernel<T>, dim3(nb), dim3(256), 0, s, (size_t)H0 * P0, in0, M.ctx->partials, 0);
+    hipLaunchKernelGGL(absmax_kernel<T>, dim3(nb), dim3(256), 0, s, (size_t)H1 * P1, in1, M.ctx->partials, 1);
+    FB_HIP(hipGetLastError());
+    FinishSpec f;
+    f.add(0, false);
+    f.add(1, false);
+    if ((rc = finish_partials(M.ctx, nb, f, M.keys))) return rc;
+  }
   JacobiArgs<T> a{};
   a.orig[0] = in0;
   a.orig[1] = in1;
@@ -564,9 +579,9 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   a.tol = tol;
   a.resid = M.resid;
   a.stride = M.resid_cap;
+  a.part = M.ctx->partials;
   T* buf[2][2] = {{(T*)M.DU[1], (T*)M.DU[2]}, {(T*)M.DV[1], (T*)M.DV[2]}};
-  const int Wm = std::max(W0, W1), Hm = std::max(H0, H1);
-  dim3 grid((Wm + 255) / 256, Hm, 2);
+  dim3 grid(reduction_blocks(std::max((size_t)W0 * H0, (size_t)W1 * H1)), 1, 2);
   int done[2] = {0, 0};
   double fres[2] = {INFINITY, INFINITY};
   int k = 1;
@@ -582,6 +597,10 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
       a.k = kk;
       hipLaunchKernelGGL(jacobi_sweep_kernel<T>, grid, dim3(256), 0, s, a);
       FB_HIP(hipGetLastError());
+      FinishSpec f;
+      f.add(kk, false);
+      f.add(M.resid_cap + kk, false);
+      if ((rc = finish_partials(M.ctx, grid.x, f, M.resid))) return rc;
     }
     FB_HIP(hipMemcpyAsync(M.host_resid, M.resid, 2 * M.resid_cap * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));

@@ -304,22 +304,26 @@ __global__ void __launch_bounds__(256) fill_int_kernel(int* __restrict__ a, int 
 template <class T>
 __global__ void __launch_bounds__(256) bin_count_kernel(PicGeom g, int n, const T* __restrict__ pos,
                                                         int* __restrict__ cellof, int* __restrict__ count) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  T x, y;
-  ld2(pos, i, x, y);
-  int c = bin_cell(g, (double)x, (double)y);
-  cellof[i] = c;
-  atomicAdd(&count[c], 1);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < n;
+  int c = 0;
+  if (act) {
+    T x, y;
+    ld2(pos, i, x, y);
+    c = bin_cell(g, (double)x, (double)y);
+    cellof[i] = c;
+  }
+  wave_run_add(count, c, act);
 }
 
 __global__ void __launch_bounds__(256) bin_scatter_kernel(int n, const int* __restrict__ cellof,
                                                           const int* __restrict__ start, int* __restrict__ cursor,
                                                           int* __restrict__ sidx) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  int c = cellof[i];
-  sidx[start[c] + atomicAdd(&cursor[c], 1)] = i;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < n;
+  const int c = act ? cellof[i] : 0;
+  const int slot = wave_run_claim(cursor, c, act);
+  if (act) sidx[start[c] + slot] = i;
 }
 
 // Order each cell's particles by index (insertion sort; cells hold a handful of particles).
@@ -723,36 +727,40 @@ __device__ __forceinline__ int ref_cell(const PicGeom& g, T x, T y) {
   return (int)fy * g.nx + (int)fx;
 }
 
-// keys: [0] max particle speed [1] max grid speed [2] particles inside [3] non-finite values
-//       [4] fluid cells [5] empty fluid cells [6] fluid cells with fewer than 4 particles
+// Partial slots of the statistics (folded into the keys): [0] max particle speed, [1] particles
+// inside the foil, [2] non-finite particle values, [3] max grid speed, [4] non-finite grid values,
+// [5] fluid cells, [6] empty fluid cells, [7] fluid cells with fewer than 4 particles.
 template <class T>
 __global__ void __launch_bounds__(256) pic_particle_stats_kernel(PicGeom g, int n, const T* __restrict__ pos,
                                                                  const T* __restrict__ vel, NacaDev foil,
                                                                  int* __restrict__ counts,
-                                                                 unsigned long long* __restrict__ keys) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  double sp = 0.0;
-  long long inside = 0, bad = 0;
-  if (i < n) {
-    T x, y, vx, vy;
-    ld2(pos, i, x, y);
-    ld2(vel, i, vx, vy);
-    if (!isfinite((double)x) || !isfinite((double)y)) bad++;
-    if (!isfinite((double)vx) || !isfinite((double)vy)) bad++;
-    T s = sqrt(vx * vx + vy * vy);
-    sp = (double)s;
-    if (!(sp == sp)) sp = 0.0;
-    if (naca_sdf_t<T>(foil, x, y) <= (T)0) inside = 1;
-    atomicAdd(&counts[ref_cell(g, x, y)], 1);
+                                                                 unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long lds[4];
+  unsigned long long sp = 0, inside = 0, bad = 0;
+  // whole waves step together so wave_run_add sees every lane
+  for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+    const int i = base + threadIdx.x;
+    const bool act = i < n;
+    int c = 0;
+    if (act) {
+      T x, y, vx, vy;
+      ld2(pos, i, x, y);
+      ld2(vel, i, vx, vy);
+      if (!isfinite((double)x) || !isfinite((double)y)) bad++;
+      if (!isfinite((double)vx) || !isfinite((double)vy)) bad++;
+      const double s = (double)sqrt(vx * vx + vy * vy);
+      if (s == s) {
+        const unsigned long long k = dkey(s);
+        sp = k > sp ? k : sp;
+      }
+      if (naca_sdf_t<T>(foil, x, y) <= (T)0) inside++;
+      c = ref_cell(g, x, y);
+    }
+    wave_run_add(counts, c, act);
   }
-  sp = wave_max(sp);
-  inside = wave_sum_i(inside);
-  bad = wave_sum_i(bad);
-  if ((threadIdx.x & 63) == 0) {
-    key_max(&keys[0], dkey(sp));
-    if (inside) atomicAdd(&keys[2], (unsigned long long)inside);
-    if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
-  }
+  put_max(part, 0, sp, lds);
+  put_sum(part, 1, inside, lds);
+  put_sum(part, 2, bad, lds);
 }
 
 template <class T>
@@ -760,33 +768,28 @@ __global__ void __launch_bounds__(256) pic_cell_stats_kernel(int ncell, const T*
                                                              const T* __restrict__ gv,
                                                              const uint8_t* __restrict__ solid,
                                                              const int* __restrict__ counts,
-                                                             unsigned long long* __restrict__ keys) {
-  int c = blockIdx.x * 256 + threadIdx.x;
-  double sp = 0.0;
-  long long bad = 0, fluid = 0, empty = 0, under = 0;
-  if (c < ncell) {
+                                                             unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long lds[4];
+  unsigned long long sp = 0, bad = 0, fluid = 0, empty = 0, under = 0;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < ncell; c += gridDim.x * 256) {
     T a = gu[c], b = gv[c];
     if (!isfinite((double)a) || !isfinite((double)b)) bad++;
-    sp = (double)sqrt(a * a + b * b);
-    if (!(sp == sp)) sp = 0.0;
+    const double s = (double)sqrt(a * a + b * b);
+    if (s == s) {
+      const unsigned long long k = dkey(s);
+      sp = k > sp ? k : sp;
+    }
     if (!solid[c]) {
-      fluid = 1;
-      empty = counts[c] == 0;
-      under = counts[c] < 4;
+      fluid++;
+      empty += counts[c] == 0;
+      under += counts[c] < 4;
     }
   }
-  sp = wave_max(sp);
-  bad = wave_sum_i(bad);
-  fluid = wave_sum_i(fluid);
-  empty = wave_sum_i(empty);
-  under = wave_sum_i(under);
-  if ((threadIdx.x & 63) == 0) {
-    key_max(&keys[1], dkey(sp));
-    if (bad) atomicAdd(&keys[3], (unsigned long long)bad);
-    if (fluid) atomicAdd(&keys[4], (unsigned long long)fluid);
-    if (empty) atomicAdd(&keys[5], (unsigned long long)empty);
-    if (under) atomicAdd(&keys[6], (unsigned long long)under);
-  }
+  put_max(part, 3, sp, lds);
+  put_sum(part, 4, bad, lds);
+  put_sum(part, 5, fluid, lds);
+  put_sum(part, 6, empty, lds);
+  put_sum(part, 7, under, lds);
 }
 
 __global__ void fill_u64_kernel(unsigned long long* a, int n) {
@@ -1141,13 +1144,25 @@ static int pic_stats(Pic& P, const fb_pose& pose, fb_pic_stats* st) {
   const int nc = P.g.nx * P.g.ny;
   fill_keys(s, P.keys, 8);
   hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc)), dim3(256), 0, s, P.cnt, nc, 0);
-  if (P.n > 0)
-    hipLaunchKernelGGL(pic_particle_stats_kernel<T>, dim3(blocks(P.n)), dim3(256), 0, s, P.g, P.n,
-                       (const T*)P.pos, (const T*)P.vel, make_naca(P.foil, pose), P.cnt, P.keys);
-  hipLaunchKernelGGL(pic_cell_stats_kernel<T>, dim3(blocks(nc)), dim3(256), 0, s, nc, (const T*)P.GU,
-                     (const T*)P.GV, (const uint8_t*)P.M->solid, (const int*)P.cnt, P.keys);
+  // both kernels fill their slots over the same grid so that one fold covers them
+  const unsigned nb = reduction_blocks(std::max((size_t)P.n, (size_t)nc));
+  hipLaunchKernelGGL(pic_particle_stats_kernel<T>, dim3(nb), dim3(256), 0, s, P.g, P.n, (const T*)P.pos,
+                     (const T*)P.vel, make_naca(P.foil, pose), P.cnt, P.ctx->partials);
+  hipLaunchKernelGGL(pic_cell_stats_kernel<T>, dim3(nb), dim3(256), 0, s, nc, (const T*)P.GU, (const T*)P.GV,
+                     (const uint8_t*)P.M->solid, (const int*)P.cnt, P.ctx->partials);
   FB_HIP(hipGetLastError());
-  int rc = pic_read_keys(P, 7);
+  FinishSpec f;
+  f.add(0, false);  // max particle speed
+  f.add(2, true);   // particles inside
+  f.add(3, true);   // non-finite (particles)
+  f.add(1, false);  // max grid speed
+  f.add(3, true);   // non-finite (grid)
+  f.add(4, true);   // fluid cells
+  f.add(5, true);   // empty cells
+  f.add(6, true);   // underfilled cells
+  int rc = finish_partials(P.ctx, nb, f, P.keys);
+  if (rc) return rc;
+  rc = pic_read_keys(P, 7);
   if (rc) return rc;
   auto val = [&](int k) { return P.host_keys[k] ? dkey_inv(P.host_keys[k]) : 0.0; };
   st->max_particle_speed = val(0);

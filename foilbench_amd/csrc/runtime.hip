@@ -26,6 +26,45 @@ __global__ void __launch_bounds__(256) copy_f4_kernel(size_t n4, const float4* _
   for (; i < n4; i += stride) b[i] = a[i];
 }
 
+// Folds the per-block values of quantity blockIdx.x (fixed order; eight loads in flight per
+// thread, since the slots were just written and come from memory).
+__global__ void __launch_bounds__(256) finish_partials_kernel(const unsigned long long* __restrict__ part,
+                                                              unsigned nb, FinishSpec f, unsigned long long* keys) {
+  __shared__ unsigned long long lds[4];
+  const int q = blockIdx.x;
+  const unsigned long long* p = part + (size_t)q * kPartialBlocks;
+  const bool sum = f.sum[q] != 0;
+  unsigned long long v = 0;
+  for (unsigned i0 = 0; i0 < nb; i0 += 8 * 256) {
+    unsigned long long w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const unsigned i = i0 + u * 256 + threadIdx.x;
+      w[u] = i < nb ? p[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v = sum ? v + w[u] : (w[u] > v ? w[u] : v);
+  }
+  v = sum ? block_sum_u64(v, lds) : block_max_u64(v, lds);
+  if (threadIdx.x == 0) {
+    if (sum) {
+      if (v) atomicAdd(&keys[f.dest[q]], v);
+    } else {
+      key_max(&keys[f.dest[q]], v);
+    }
+  }
+}
+
+int finish_partials(Ctx* c, unsigned nblocks, const FinishSpec& f, unsigned long long* keys) {
+  FB_REQUIRE(nblocks <= (unsigned)kPartialBlocks && f.n <= kPartialKeys, FB_ERR_ARG,
+             "partial reduction over %u blocks / %d quantities exceeds the slots", nblocks, f.n);
+  if (f.n == 0) return FB_OK;
+  hipLaunchKernelGGL(finish_partials_kernel, dim3(f.n), dim3(256), 0, c->stream, (const unsigned long long*)c->partials,
+                     nblocks, f, keys);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
 }  // namespace fb
 
 extern "C" {
@@ -83,6 +122,12 @@ int fb_ctx_create(int device, fb_ctx** out) {
   c->cu_count = prop.multiProcessorCount;
   // Default-priority stream, non-blocking with respect to the null stream.
   FB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (hipMalloc(&c->partials, (size_t)fb::kPartialKeys * fb::kPartialBlocks * sizeof(unsigned long long)) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    fb::set_error("fb_ctx_create: cannot allocate the reduction slots");
+    return FB_ERR_NOMEM;
+  }
   *out = reinterpret_cast<fb_ctx*>(c);
   return FB_OK;
 }
@@ -92,6 +137,7 @@ int fb_ctx_destroy(fb_ctx* ctx) {
   auto* c = reinterpret_cast<fb::Ctx*>(ctx);
   FB_HIP(hipSetDevice(c->device));
   FB_HIP(hipStreamSynchronize(c->stream));
+  FB_HIP(hipFree(c->partials));
   FB_HIP(hipStreamDestroy(c->stream));
   delete c;
   return FB_OK;
