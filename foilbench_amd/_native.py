@@ -126,6 +126,22 @@ class fb_pic_stats(C.Structure):
     ]
 
 
+class fb_server_options(C.Structure):
+    _fields_ = [("sf_workers", C.c_int32), ("lbm_chunk", C.c_int32), ("use_graphs", C.c_int32)]
+
+
+FB_SERVE_SF = 0
+FB_SERVE_LBM = 1
+
+
+class fb_server_result(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64), ("kind", C.c_int32), ("status", C.c_int32), ("sf_status", C.c_int32),
+        ("steps_done", C.c_int32), ("submitted_ms", C.c_double), ("started_ms", C.c_double),
+        ("finished_ms", C.c_double), ("pressure_iterations", C.c_int64), ("lbm_stats", fb_lbm_stats),
+    ]
+
+
 _P = C.c_void_p
 _SIGNATURES: dict[str, tuple] = {
     "fb_last_error": (C.c_char_p, []),
@@ -155,6 +171,16 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_macroscopic": (C.c_int, [_P, C.c_double, _P, _P]),
     "fb_lbm_run_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int]),
     "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double)]),
+    "fb_lbm_run_batch": (C.c_int, [C.POINTER(_P), C.POINTER(fb_lbm_params), C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(fb_lbm_stats)]),
+    "fb_mem_info": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fb_server_create": (C.c_int, [C.c_int, C.POINTER(fb_server_options), C.POINTER(_P)]),
+    "fb_server_destroy": (C.c_int, [_P]),
+    "fb_server_submit_sf": (C.c_int, [_P, C.c_int64, _P, C.POINTER(fb_sf_params), C.POINTER(fb_pose), C.c_int]),
+    "fb_server_submit_lbm": (C.c_int, [_P, C.c_int64, _P, C.POINTER(fb_lbm_params), C.c_int]),
+    "fb_server_poll": (C.c_int, [_P, C.POINTER(fb_server_result), C.c_int, C.POINTER(C.c_int)]),
+    "fb_server_load": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fb_server_clock": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "fb_mac_create": (C.c_int, [_P, C.POINTER(fb_domain), C.POINTER(fb_foil), C.c_int, C.POINTER(_P)]),
     "fb_mac_destroy": (C.c_int, [_P]),
     "fb_mac_set_faces": (C.c_int, [_P, _P, _P]),

@@ -7,11 +7,36 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/foilbench_amd.h"
 
 namespace fb {
+
+// Graph capture on one thread and a call that may synchronise the device on another (allocation,
+// free, pinned-host allocation) do not mix: the library holds this lock across every capture and
+// every such call, so that handles can be created and freed while the request loop captures.
+std::recursive_mutex& hip_lock();
+
+template <class T>
+inline hipError_t dev_alloc(T** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipMalloc((void**)p, bytes);
+}
+inline hipError_t dev_free(void* p) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipFree(p);
+}
+template <class T>
+inline hipError_t host_alloc(T** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipHostMalloc((void**)p, bytes, 0);
+}
+inline hipError_t host_free(void* p) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipHostFree(p);
+}
 
 // Thread-local last error message returned by fb_last_error().
 void set_error(const char* fmt, ...);
@@ -54,6 +79,11 @@ inline unsigned reduction_blocks(size_t n) {
 }
 
 }  // namespace fb
+
+using fb::dev_alloc;
+using fb::dev_free;
+using fb::host_alloc;
+using fb::host_free;
 
 #define FB_HIP(call)                                                                   \
   do {                                                                                 \

@@ -12,6 +12,7 @@
 // so each lattice step reads 9 and writes 9 populations per cell. The pre-collision field
 // (the reference's solver state) is written only on the last substep of an advance, where
 // the postcondition reductions are fused in.
+#include "lbm_batch.h"
 #include <vector>
 
 #include "geometry.h"
@@ -118,6 +119,7 @@ struct LbmArgs {
   // stride over the flagged (solid or near-wall) cells of rows 1..ny-2 listed in `list`;
   // the rest cover rows 1..ny-2 with int_gx blocks per row, whose edge and flagged lanes idle
   int int_gx, edge_blocks, nedge, list_blocks;
+  int skip;  // batched launches: this lattice sits out the step (its blocks return at once)
   const int* __restrict__ list;
   const int* __restrict__ list_count;
   const double* __restrict__ links;  // 8 Bouzidi fractions per list entry (0: straight link)
@@ -604,28 +606,29 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
 // One lattice step. The edge cells and the flagged cells (foil interior and its neighbours)
 // run in the first blocks of the grid, so their longer dependent chains start early and do not
 // diverge the interior waves; interior blocks take the straight pull.
+// Block `blk` of the `nblk` blocks that cover lattice `a` for one step.
 template <class P>
-__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
+__device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsigned blk, unsigned nblk) {
   double mn = INFINITY, vmax = 0.0, exc = 0.0;
   long long nf = 0, bad = 0, nfu = 0;
   const unsigned special = (unsigned)(a.edge_blocks + a.list_blocks);
-  if (blockIdx.x < (unsigned)a.edge_blocks) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (blk < (unsigned)a.edge_blocks) {
+    const int e = (int)(blk * blockDim.x + threadIdx.x);
     if (e < a.nedge) {
       int x, y;
       edge_cell(e, a.nx, a.ny, x, y);
       step_cell<P>(a, x, y, false, nullptr, false, mn, vmax, exc, nf, bad, nfu);
     }
-  } else if (blockIdx.x < special) {
+  } else if (blk < special) {
     const int count = *a.list_count;
     const int stride = a.list_blocks * (int)blockDim.x;
-    for (int k = (int)((blockIdx.x - a.edge_blocks) * blockDim.x + threadIdx.x); k < count; k += stride) {
+    for (int k = (int)((blk - a.edge_blocks) * blockDim.x + threadIdx.x); k < count; k += stride) {
       const int entry = a.list[k];
       const int c = entry & 0x7fffffff;
       step_cell<P>(a, c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, entry < 0, mn, vmax, exc, nf, bad, nfu);
     }
   } else {
-    const unsigned bi = xcd_remap(blockIdx.x - special, gridDim.x - special);
+    const unsigned bi = xcd_remap(blk - special, nblk - special);
     const unsigned gx = (unsigned)a.int_gx;
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
     const int y = 1 + (int)(bi / gx);
@@ -649,6 +652,37 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a)
       if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
     }
   }
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
+  lbm_block<P>(a, blockIdx.x, gridDim.x);
+}
+
+// Blocks of one lattice step.
+template <class S>
+__host__ __device__ inline int lbm_step_blocks(const LbmArgs<S>& a) {
+  return a.edge_blocks + a.list_blocks + a.int_gx * (a.ny - 2);
+}
+
+// One step of many lattices in one launch. Lattice k's blocks start at start[k], a multiple of
+// the XCD count, so that its tiles map onto the XCDs as in a launch of its own; the padding
+// blocks between lattices return at once.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_batch_kernel(const LbmArgs<typename P::S>* __restrict__ args,
+                                                        const int* __restrict__ start, int n) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const LbmArgs<typename P::S>& a = args[lo];
+  const int nblk = lbm_step_blocks(a);
+  const int local = b - start[lo];
+  if (a.skip || local >= nblk) return;
+  lbm_block<P>(a, (unsigned)local, (unsigned)nblk);
 }
 
 // The flagged cells of rows 1..ny-2, columns 1..nx-2, listed in row-major order so that
@@ -996,10 +1030,9 @@ static int other3(int a, int b) {
   return 0;
 }
 
-template <class P>
-static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool prime, int fout_idx, int o_in, int o_out,
-                       int g_cur, int g_next, bool write_pre, int pre_out, bool do_reset) {
-  using S = typename P::S;
+template <class S>
+static LbmArgs<S> step_args(Lattice& L, const fb_lbm_params& p, int fin_idx, bool prime, int fout_idx, int o_in,
+                            int o_out, int g_cur, int g_next, bool write_pre, int pre_out, bool do_reset) {
   LbmArgs<S> a{};
   a.nx = L.dom.nx;
   a.ny = L.dom.ny;
@@ -1047,8 +1080,17 @@ static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool pri
   a.list = L.G[g_cur].list;
   a.list_count = L.G[g_cur].list_count;
   a.links = L.G[g_cur].links;
-  const int blocks = a.edge_blocks + a.list_blocks + a.int_gx * (L.dom.ny - 2);
-  hipLaunchKernelGGL(lbm_step_kernel<P>, dim3(blocks), dim3(256), 0, L.ctx->stream, a);
+  a.skip = 0;
+  return a;
+}
+
+template <class P>
+static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool prime, int fout_idx, int o_in, int o_out,
+                       int g_cur, int g_next, bool write_pre, int pre_out, bool do_reset) {
+  using S = typename P::S;
+  const LbmArgs<S> a =
+      step_args<S>(L, p, fin_idx, prime, fout_idx, o_in, o_out, g_cur, g_next, write_pre, pre_out, do_reset);
+  hipLaunchKernelGGL(lbm_step_kernel<P>, dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
@@ -1215,16 +1257,16 @@ static int macro_impl(Lattice& L, double vscale, void* rho_host, void* vel_host)
   using B = typename P::B;
   B* d_rho = nullptr;
   B* d_vel = nullptr;
-  FB_HIP(hipMalloc((void**)&d_rho, L.N * sizeof(B)));
-  FB_HIP(hipMalloc((void**)&d_vel, 2 * L.N * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&d_rho, L.N * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&d_vel, 2 * L.N * sizeof(B)));
   hipLaunchKernelGGL(lbm_macro_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
                      L.dom.nx, L.dom.ny, (const S*)L.PRE[L.pre], vscale, d_rho, d_vel);
   FB_HIP(hipGetLastError());
   if (rho_host) FB_HIP(hipMemcpyAsync(rho_host, d_rho, L.N * sizeof(B), hipMemcpyDeviceToHost, L.ctx->stream));
   if (vel_host) FB_HIP(hipMemcpyAsync(vel_host, d_vel, 2 * L.N * sizeof(B), hipMemcpyDeviceToHost, L.ctx->stream));
   FB_HIP(hipStreamSynchronize(L.ctx->stream));
-  FB_HIP(hipFree(d_rho));
-  FB_HIP(hipFree(d_vel));
+  FB_HIP(dev_free(d_rho));
+  FB_HIP(dev_free(d_vel));
   return FB_OK;
 }
 
@@ -1234,7 +1276,7 @@ static int set_state_impl(Lattice& L, const void* f_host, const void* outlet_hos
   using B = typename P::B;
   const size_t n9 = L.N * 9;
   B* stage = nullptr;
-  FB_HIP(hipMalloc((void**)&stage, n9 * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&stage, n9 * sizeof(B)));
   FB_HIP(hipMemcpyAsync(stage, f_host, n9 * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
   hipLaunchKernelGGL(lbm_aos_to_soa<P>, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
                      L.dom.ny, stage, (S*)L.PRE[L.pre]);
@@ -1244,7 +1286,7 @@ static int set_state_impl(Lattice& L, const void* f_host, const void* outlet_hos
                      stage, (S*)L.O[L.o], (const S*)nullptr, (B*)nullptr);
   FB_HIP(hipGetLastError());
   FB_HIP(hipStreamSynchronize(L.ctx->stream));
-  FB_HIP(hipFree(stage));
+  FB_HIP(dev_free(stage));
   L.fstar_valid = false;
   return FB_OK;
 }
@@ -1255,7 +1297,7 @@ static int get_state_impl(Lattice& L, void* f_host, void* outlet_host) {
   using B = typename P::B;
   const size_t n9 = L.N * 9;
   B* stage = nullptr;
-  FB_HIP(hipMalloc((void**)&stage, n9 * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&stage, n9 * sizeof(B)));
   if (f_host) {
     hipLaunchKernelGGL(lbm_soa_to_aos<P>, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
                        L.dom.ny, (const S*)L.PRE[L.pre], stage);
@@ -1271,7 +1313,7 @@ static int get_state_impl(Lattice& L, void* f_host, void* outlet_host) {
                           L.ctx->stream));
     FB_HIP(hipStreamSynchronize(L.ctx->stream));
   }
-  FB_HIP(hipFree(stage));
+  FB_HIP(dev_free(stage));
   return FB_OK;
 }
 
@@ -1281,8 +1323,8 @@ static int equilibrium_impl(Lattice& L, const void* rho_host, const void* vel_ho
   using B = typename P::B;
   B* d_rho = nullptr;
   B* d_vel = nullptr;
-  FB_HIP(hipMalloc((void**)&d_rho, L.N * sizeof(B)));
-  FB_HIP(hipMalloc((void**)&d_vel, 2 * L.N * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&d_rho, L.N * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&d_vel, 2 * L.N * sizeof(B)));
   FB_HIP(hipMemcpyAsync(d_rho, rho_host, L.N * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
   FB_HIP(hipMemcpyAsync(d_vel, vel_host, 2 * L.N * sizeof(B), hipMemcpyHostToDevice, L.ctx->stream));
   int po = other3(L.cvalid ? L.cpre : -1, L.pre);
@@ -1291,8 +1333,8 @@ static int equilibrium_impl(Lattice& L, const void* rho_host, const void* vel_ho
                      L.dom.nx, L.dom.ny, d_rho, d_vel, (S*)L.PRE[po], (S*)L.O[oo]);
   FB_HIP(hipGetLastError());
   FB_HIP(hipStreamSynchronize(L.ctx->stream));
-  FB_HIP(hipFree(d_rho));
-  FB_HIP(hipFree(d_vel));
+  FB_HIP(dev_free(d_rho));
+  FB_HIP(dev_free(d_vel));
   L.pre = po;
   L.o = oo;
   L.fstar_valid = false;
@@ -1346,24 +1388,262 @@ static int get_state_dispatch(Lattice& L, void* a, void* b) { FB_DISPATCH(L.prec
 
 static void free_lattice(Lattice* L) {
   for (int k = 0; k < 3; ++k) {
-    if (L->F[k]) (void)hipFree(L->F[k]);
-    if (L->R[k]) (void)hipFree(L->R[k]);
-    if (L->O[k]) (void)hipFree(L->O[k]);
-    if (L->G[k].sdf) (void)hipFree(L->G[k].sdf);
-    if (L->G[k].solid) (void)hipFree(L->G[k].solid);
-    if (L->G[k].flags) (void)hipFree(L->G[k].flags);
-    if (L->G[k].list) (void)hipFree(L->G[k].list);
-    if (L->G[k].list_count) (void)hipFree(L->G[k].list_count);
-    if (L->G[k].links) (void)hipFree(L->G[k].links);
-    if (L->G[k].rows) (void)hipFree(L->G[k].rows);
+    if (L->F[k]) (void)dev_free(L->F[k]);
+    if (L->R[k]) (void)dev_free(L->R[k]);
+    if (L->O[k]) (void)dev_free(L->O[k]);
+    if (L->G[k].sdf) (void)dev_free(L->G[k].sdf);
+    if (L->G[k].solid) (void)dev_free(L->G[k].solid);
+    if (L->G[k].flags) (void)dev_free(L->G[k].flags);
+    if (L->G[k].list) (void)dev_free(L->G[k].list);
+    if (L->G[k].list_count) (void)dev_free(L->G[k].list_count);
+    if (L->G[k].links) (void)dev_free(L->G[k].links);
+    if (L->G[k].rows) (void)dev_free(L->G[k].rows);
   }
   for (int k = 0; k < 3; ++k)
-    if (L->PRE[k]) (void)hipFree(L->PRE[k]);
-  if (L->sponge_x) (void)hipFree(L->sponge_x);
-  if (L->sponge_y) (void)hipFree(L->sponge_y);
-  if (L->red) (void)hipFree(L->red);
-  if (L->red_host) (void)hipHostFree(L->red_host);
+    if (L->PRE[k]) (void)dev_free(L->PRE[k]);
+  if (L->sponge_x) (void)dev_free(L->sponge_x);
+  if (L->sponge_y) (void)dev_free(L->sponge_y);
+  if (L->red) (void)dev_free(L->red);
+  if (L->red_host) (void)host_free(L->red_host);
   delete L;
+}
+
+// ------------------------------------------------------------------ batched stepping
+
+struct LbmMember {
+  Lattice* L = nullptr;
+  fb_lbm_params p{};
+  int64_t id = 0;
+  int total = 0, remaining = 0;
+  Ctx* home = nullptr;
+};
+
+struct LbmBatch {
+  Ctx* ctx = nullptr;
+  int prec = FB_F32;
+  int chunk = 32;
+  bool use_graph = true;
+  std::vector<LbmMember> m;
+  bool changed = true;       // membership changed since the graph was captured
+  void* d_args = nullptr;    // chunk x members LbmArgs
+  size_t args_bytes = 0;
+  int* d_start = nullptr;
+  size_t start_cap = 0;
+  std::vector<unsigned char> h_args, h_prev;
+  std::vector<int> h_start;
+  int total_blocks = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
+template <class P>
+static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
+  using S = typename P::S;
+  const int n = (int)B.m.size();
+  if (n == 0) return FB_OK;
+  const int K = B.chunk;
+  hipStream_t s = B.ctx->stream;
+  if (B.changed) {
+    B.h_start.assign(n + 1, 0);
+    int off = 0;
+    for (int i = 0; i < n; ++i) {
+      B.h_start[i] = off;
+      const Lattice& L = *B.m[i].L;
+      const int nb = (L.dom.nx + 255) / 256 * (L.dom.ny - 2) + kListBlocks +
+                     (2 * L.dom.nx + 2 * (L.dom.ny - 2) + 255) / 256;
+      off += (nb + 7) / 8 * 8;
+    }
+    B.h_start[n] = off;
+    B.total_blocks = off;
+    if ((size_t)(n + 1) > B.start_cap) {
+      if (B.d_start) FB_HIP(dev_free(B.d_start));
+      B.start_cap = (size_t)(n + 1) * 2;
+      FB_HIP(dev_alloc((void**)&B.d_start, B.start_cap * sizeof(int)));
+    }
+    FB_HIP(hipMemcpyAsync(B.d_start, B.h_start.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    const size_t need = (size_t)K * n * sizeof(LbmArgs<S>);
+    if (need > B.args_bytes) {
+      if (B.d_args) FB_HIP(dev_free(B.d_args));
+      B.args_bytes = need * 2;
+      FB_HIP(dev_alloc(&B.d_args, B.args_bytes));
+    }
+    B.h_prev.clear();
+    if (B.exec) {
+      (void)hipGraphExecDestroy(B.exec);
+      B.exec = nullptr;
+    }
+  }
+  // arguments of every step of the chunk; a lattice with fewer steps left sits the rest out
+  B.h_args.assign((size_t)K * n * sizeof(LbmArgs<S>), 0);
+  auto* args = reinterpret_cast<LbmArgs<S>*>(B.h_args.data());
+  std::vector<int> pre_out(n);
+  for (int i = 0; i < n; ++i) {
+    LbmMember& mb = B.m[i];
+    Lattice& L = *mb.L;
+    int f = L.f, o = L.o;
+    pre_out[i] = other3(L.cvalid ? L.cpre : -1, L.pre);
+    const int here = std::min(mb.remaining, K);
+    for (int j = 0; j < K; ++j) {
+      LbmArgs<S>& a = args[(size_t)j * n + i];
+      if (j < here) {
+        const int fo = other3(L.cvalid ? L.cf : -1, f);
+        const int oo = L.dom.periodic_x ? o : other3(L.cvalid ? L.co : -1, o);
+        const bool last = j == mb.remaining - 1;
+        a = step_args<S>(L, mb.p, f, false, fo, o, oo, L.g, L.g, last, pre_out[i], false);
+        f = fo;
+        o = oo;
+      } else {
+        a = step_args<S>(L, mb.p, f, false, f, o, o, L.g, L.g, false, pre_out[i], false);
+        a.skip = 1;
+      }
+    }
+    if (mb.remaining <= K) {
+      int rc = reset_red(L);
+      if (rc) return rc;
+    }
+  }
+  if (B.h_prev.size() != B.h_args.size() || memcmp(B.h_prev.data(), B.h_args.data(), B.h_args.size()) != 0) {
+    FB_HIP(hipMemcpyAsync(B.d_args, B.h_args.data(), B.h_args.size(), hipMemcpyHostToDevice, s));
+    // the copy reads pageable memory: the host buffer may be reused once the call returns
+    B.h_prev = B.h_args;
+  }
+  auto launch_all = [&]() {
+    for (int j = 0; j < K; ++j)
+      hipLaunchKernelGGL(lbm_batch_kernel<P>, dim3(B.total_blocks), dim3(256), 0, s,
+                         (const LbmArgs<S>*)B.d_args + (size_t)j * n, (const int*)B.d_start, n);
+  };
+  if (B.use_graph) {
+    if (!B.exec) {
+      hipGraph_t graph = nullptr;
+      std::lock_guard<std::recursive_mutex> guard(hip_lock());
+      FB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch_all();
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_error("lattice batch capture failed: %s", hipGetErrorString(e));
+        return FB_ERR_HIP;
+      }
+      e = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        B.exec = nullptr;
+        set_error("lattice batch graph instantiation failed: %s", hipGetErrorString(e));
+        return FB_ERR_HIP;
+      }
+    }
+    FB_HIP(hipGraphLaunch(B.exec, s));
+  } else {
+    launch_all();
+    FB_HIP(hipGetLastError());
+  }
+  B.changed = false;
+  FB_HIP(hipStreamSynchronize(s));
+  // bookkeeping: buffer indices after the chunk, finished lattices leave
+  std::vector<LbmMember> keep;
+  keep.reserve(n);
+  for (int i = 0; i < n; ++i) {
+    LbmMember& mb = B.m[i];
+    Lattice& L = *mb.L;
+    const int here = std::min(mb.remaining, K);
+    for (int j = 0; j < here; ++j) {
+      L.f = other3(L.cvalid ? L.cf : -1, L.f);
+      if (!L.dom.periodic_x) L.o = other3(L.cvalid ? L.co : -1, L.o);
+    }
+    mb.remaining -= here;
+    if (mb.remaining == 0) {
+      L.pre = pre_out[i];
+      LbmDone d;
+      d.id = mb.id;
+      d.steps = mb.total;
+      d.status = read_red(L, &d.stats);
+      L.ctx = mb.home;
+      done.push_back(d);
+      B.changed = true;
+    } else {
+      keep.push_back(mb);
+    }
+  }
+  B.m.swap(keep);
+  return FB_OK;
+}
+
+static int batch_chunk_dispatch(LbmBatch& B, std::vector<LbmDone>& done) { FB_DISPATCH(B.prec, batch_chunk, B, done); }
+
+template <class P>
+static int batch_prime(Lattice& L, const fb_lbm_params& p) {
+  // f* from the pre-collision state, as the first step of an advance does
+  if (L.fstar_valid) return FB_OK;
+  const int fo = other3(L.cvalid ? L.cf : -1, L.f);
+  int rc = launch_step<P>(L, p, L.f, true, fo, L.o, L.o, L.g, L.g, false, 0, false);
+  if (rc) return rc;
+  L.f = fo;
+  L.fstar_valid = true;
+  return FB_OK;
+}
+static int batch_prime_dispatch(Lattice& L, const fb_lbm_params& p) { FB_DISPATCH(L.prec, batch_prime, L, p); }
+
+int lbm_batch_create(Ctx* c, int prec, int chunk, int use_graph, LbmBatch** out) {
+  FB_REQUIRE(c && out, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(prec >= FB_F64 && prec <= FB_F16_MIXED, FB_ERR_ARG, "unknown precision %d", prec);
+  FB_REQUIRE(chunk >= 2 && chunk % 2 == 0 && chunk <= 1024, FB_ERR_ARG, "chunk must be even and in [2, 1024]");
+  auto* b = new LbmBatch();
+  b->ctx = c;
+  b->prec = prec;
+  b->chunk = chunk;
+  b->use_graph = use_graph != 0;
+  *out = b;
+  return FB_OK;
+}
+
+void lbm_batch_destroy(LbmBatch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  for (auto& mb : b->m) mb.L->ctx = mb.home;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->d_args) (void)dev_free(b->d_args);
+  if (b->d_start) (void)dev_free(b->d_start);
+  delete b;
+}
+
+int lbm_batch_add(LbmBatch* b, fb_lbm* lattice, const fb_lbm_params& p, int steps, int64_t id) {
+  FB_REQUIRE(b && lattice, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(steps >= 1, FB_ERR_ARG, "steps must be positive");
+  auto* L = reinterpret_cast<Lattice*>(lattice);
+  FB_REQUIRE(L->prec == b->prec, FB_ERR_ARG, "lattice precision %d differs from the batch's %d", L->prec, b->prec);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  for (auto& mb : b->m) FB_REQUIRE(mb.L != L, FB_ERR_STATE, "lattice is already in the batch");
+  FB_HIP(hipStreamSynchronize(L->ctx->stream));
+  LbmMember mb;
+  mb.L = L;
+  mb.p = p;
+  mb.id = id;
+  mb.total = steps;
+  mb.remaining = steps;
+  mb.home = L->ctx;
+  L->ctx = b->ctx;
+  int rc = batch_prime_dispatch(*L, p);
+  if (rc) {
+    L->ctx = mb.home;
+    return rc;
+  }
+  b->m.push_back(mb);
+  b->changed = true;
+  return FB_OK;
+}
+
+int lbm_batch_members(const LbmBatch* b) { return b ? (int)b->m.size() : 0; }
+int lbm_handle_precision(const fb_lbm* lattice) { return reinterpret_cast<const Lattice*>(lattice)->prec; }
+int lbm_batch_prec(const LbmBatch* b) { return b ? b->prec : -1; }
+int64_t lbm_batch_cells(const LbmBatch* b) {
+  int64_t c = 0;
+  if (b)
+    for (auto& mb : b->m) c += (int64_t)mb.L->N;
+  return c;
+}
+int lbm_batch_run_chunk(LbmBatch* b, std::vector<LbmDone>& done) {
+  FB_REQUIRE(b, FB_ERR_ARG, "NULL batch");
+  FB_HIP(hipSetDevice(b->ctx->device));
+  return batch_chunk_dispatch(*b, done);
 }
 
 }  // namespace fb
@@ -1396,26 +1676,30 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   };
   hipError_t e;
   for (int k = 0; k < 3; ++k) {
-    if ((e = hipMalloc(&L->F[k], 9 * N * L->ssize))) return fail(e);
-    if ((e = hipMalloc(&L->R[k], N * L->ssize))) return fail(e);
-    if ((e = hipMalloc(&L->O[k], (size_t)dom->ny * 9 * L->ssize))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].sdf, N * sizeof(double)))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].solid, N))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].flags, N))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].list, N * sizeof(int)))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].list_count, sizeof(int)))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].links, N * 8 * sizeof(double)))) return fail(e);
-    if ((e = hipMalloc((void**)&L->G[k].rows, 2 * (size_t)dom->ny * sizeof(int)))) return fail(e);
+    if ((e = dev_alloc(&L->F[k], 9 * N * L->ssize))) return fail(e);
+    if ((e = dev_alloc(&L->R[k], N * L->ssize))) return fail(e);
+    if ((e = dev_alloc(&L->O[k], (size_t)dom->ny * 9 * L->ssize))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].sdf, N * sizeof(double)))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].solid, N))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].flags, N))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].list, N * sizeof(int)))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].list_count, sizeof(int)))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].links, N * 8 * sizeof(double)))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].rows, 2 * (size_t)dom->ny * sizeof(int)))) return fail(e);
     if ((e = hipMemsetAsync(L->R[k], 0, N * L->ssize, c->stream))) return fail(e);
   }
   for (int k = 0; k < 3; ++k)
-    if ((e = hipMalloc(&L->PRE[k], 9 * N * L->ssize))) return fail(e);
-  if ((e = hipMalloc((void**)&L->sponge_y, dom->ny * sizeof(double)))) return fail(e);
-  if ((e = hipMalloc((void**)&L->sponge_x, dom->nx * sizeof(double)))) return fail(e);
-  if ((e = hipMalloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
-  if ((e = hipHostMalloc((void**)&L->red_host, 8 * sizeof(unsigned long long), 0))) return fail(e);
-  if ((e = hipMemcpy(L->sponge_y, sponge_y, dom->ny * sizeof(double), hipMemcpyHostToDevice))) return fail(e);
-  if ((e = hipMemcpy(L->sponge_x, sponge_x, dom->nx * sizeof(double), hipMemcpyHostToDevice))) return fail(e);
+    if ((e = dev_alloc(&L->PRE[k], 9 * N * L->ssize))) return fail(e);
+  if ((e = dev_alloc((void**)&L->sponge_y, dom->ny * sizeof(double)))) return fail(e);
+  if ((e = dev_alloc((void**)&L->sponge_x, dom->nx * sizeof(double)))) return fail(e);
+  if ((e = dev_alloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
+  if ((e = host_alloc((void**)&L->red_host, 8 * sizeof(unsigned long long)))) return fail(e);
+  // stream-ordered copies: a legacy-stream copy would conflict with graph capture on other threads
+  if ((e = hipMemcpyAsync(L->sponge_y, sponge_y, dom->ny * sizeof(double), hipMemcpyHostToDevice, c->stream)))
+    return fail(e);
+  if ((e = hipMemcpyAsync(L->sponge_x, sponge_x, dom->nx * sizeof(double), hipMemcpyHostToDevice, c->stream)))
+    return fail(e);
+  if ((e = hipStreamSynchronize(c->stream))) return fail(e);
   *out = reinterpret_cast<fb_lbm*>(L);
   return FB_OK;
 }
@@ -1567,6 +1851,28 @@ int fb_lbm_time_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps, doubl
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   return rc;
+}
+
+int fb_lbm_run_batch(fb_lbm* const* lattices, const fb_lbm_params* params, int n, int steps, int chunk,
+                     fb_lbm_stats* stats) {
+  FB_REQUIRE(lattices && params && n >= 1, FB_ERR_ARG, "NULL argument or empty batch");
+  FB_REQUIRE(steps >= 1, FB_ERR_ARG, "steps must be positive");
+  for (int k = 0; k < n; ++k) FB_REQUIRE(lattices[k] != nullptr, FB_ERR_ARG, "lattice %d is NULL", k);
+  auto* L0 = reinterpret_cast<Lattice*>(lattices[0]);
+  FB_HIP(hipSetDevice(L0->ctx->device));
+  fb::LbmBatch* b = nullptr;
+  int rc = fb::lbm_batch_create(L0->ctx, L0->prec, chunk, 1, &b);
+  if (rc) return rc;
+  for (int k = 0; k < n && rc == FB_OK; ++k) rc = fb::lbm_batch_add(b, lattices[k], params[k], steps, k);
+  std::vector<fb::LbmDone> done;
+  while (rc == FB_OK && fb::lbm_batch_members(b) > 0) rc = fb::lbm_batch_run_chunk(b, done);
+  fb::lbm_batch_destroy(b);
+  if (rc) return rc;
+  for (auto& d : done) {
+    if (d.status) return d.status;
+    if (stats) stats[d.id] = d.stats;
+  }
+  return FB_OK;
 }
 
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps) {

@@ -87,5 +87,10 @@ template <class T>
 int launch_cell_to_faces(MacGrid& M, const T* cu, const T* cv, T* u, T* v);
 template <class T>
 int launch_dup_copy(MacGrid& M, T* f, int rows, int cols, int pitch, int dup_x, int dup_y);
+// One Stable Fluids substep (the body of fb_sf_substep) on M's stream.
+int mac_substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_report* rep);
+// Moves M onto context c: its kernels, copies and reduction slots then use c's stream. The
+// caller makes sure no work of M is pending on the previous stream.
+void mac_bind(MacGrid& M, Ctx* c);
 
 }  // namespace fb

@@ -542,11 +542,11 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
     return FB_OK;
   }
   if (maxit + 2 > M.resid_cap) {
-    if (M.resid) (void)hipFree(M.resid);
-    if (M.host_resid) (void)hipHostFree(M.host_resid);
+    if (M.resid) (void)dev_free(M.resid);
+    if (M.host_resid) (void)host_free(M.host_resid);
     M.resid_cap = maxit + 2;
-    FB_HIP(hipMalloc((void**)&M.resid, 2 * M.resid_cap * sizeof(unsigned long long)));
-    FB_HIP(hipHostMalloc((void**)&M.host_resid, 2 * M.resid_cap * sizeof(unsigned long long), 0));
+    FB_HIP(dev_alloc((void**)&M.resid, 2 * M.resid_cap * sizeof(unsigned long long)));
+    FB_HIP(host_alloc((void**)&M.host_resid, 2 * M.resid_cap * sizeof(unsigned long long)));
   }
   FB_HIP(hipMemsetAsync(M.resid, 0, 2 * M.resid_cap * sizeof(unsigned long long), s));
   // scale = max(max |original|, 1), kept on the device (keys 0 and 1) and read with the residuals
@@ -789,9 +789,9 @@ void free_mac(MacGrid* M) {
                     M->CV[2], M->CV[3], M->DU[0], M->DU[1], M->DU[2], M->DV[0], M->DV[1], M->DV[2], M->CKU, M->CKV,
                     (void*)M->sdf, (void*)M->solid, (void*)M->ck_solid, (void*)M->rhs, (void*)M->pres,
                     (void*)M->keys, (void*)M->resid})
-    if (ptr) (void)hipFree(ptr);
-  if (M->host_keys) (void)hipHostFree(M->host_keys);
-  if (M->host_resid) (void)hipHostFree(M->host_resid);
+    if (ptr) (void)dev_free(ptr);
+  if (M->host_keys) (void)host_free(M->host_keys);
+  if (M->host_resid) (void)host_free(M->host_resid);
   delete M;
 }
 
@@ -801,6 +801,20 @@ using fb::MacGrid;
 
 #define MAC_DISPATCH(M, fn, ...) \
   ((M)->prec == FB_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+namespace fb {
+
+int mac_substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_report* rep) {
+  *rep = fb_sf_report{};
+  return MAC_DISPATCH(&M, substep, M, p, pose, rep);
+}
+
+void mac_bind(MacGrid& M, Ctx* c) {
+  M.ctx = c;
+  M.poisson.stream = c->stream;
+}
+
+}  // namespace fb
 
 extern "C" {
 
@@ -834,7 +848,7 @@ int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision,
   const size_t NF = std::max(M->NU, M->NV);
   hipError_t e = hipSuccess;
   auto A = [&](void** p, size_t bytes) {
-    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e == hipSuccess) e = dev_alloc(p, bytes);
   };
   for (int k = 0; k < 2; ++k) {
     A(&M->U[k], M->NU * M->sz);
@@ -856,7 +870,7 @@ int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision,
   A((void**)&M->rhs, M->N * sizeof(double));
   A((void**)&M->pres, M->N * sizeof(double));
   A((void**)&M->keys, 16 * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&M->host_keys, 16 * sizeof(unsigned long long), 0);
+  if (e == hipSuccess) e = host_alloc((void**)&M->host_keys, 16 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     set_error("device allocation failed: %s", hipGetErrorString(e));
     free_mac(M);

@@ -821,7 +821,7 @@ static int alloc_sort(SortBufs& b, int ncell, int cap, size_t sz, hipStream_t s,
   b.cap = cap;
   const size_t nb = (size_t)(std::max(ncell, cap) + 1023) / 1024 + 2;
   auto A = [&](void** p, size_t bytes) -> hipError_t {
-    return async ? hipMallocAsync(p, std::max<size_t>(bytes, 16), s) : hipMalloc(p, std::max<size_t>(bytes, 16));
+    return async ? hipMallocAsync(p, std::max<size_t>(bytes, 16), s) : dev_alloc(p, std::max<size_t>(bytes, 16));
   };
   hipError_t e = hipSuccess;
   void** ptrs[] = {(void**)&b.cellof, (void**)&b.sidx, (void**)&b.counts, (void**)&b.cstart, (void**)&b.cursor,
@@ -839,7 +839,7 @@ static int alloc_sort(SortBufs& b, int ncell, int cap, size_t sz, hipStream_t s,
 static void free_sort(SortBufs& b, hipStream_t s, bool async) {
   for (void* p : {(void*)b.cellof, (void*)b.sidx, (void*)b.counts, (void*)b.cstart, (void*)b.cursor,
                   (void*)b.bsum, (void*)b.total, b.spos, b.sval})
-    if (p) (void)(async ? hipFreeAsync(p, s) : hipFree(p));
+    if (p) (void)(async ? hipFreeAsync(p, s) : dev_free(p));
   b = SortBufs{};
 }
 
@@ -945,7 +945,7 @@ struct Pic {
 static int pic_alloc_particles(Pic& P, int cap) {
   P.ck_valid = false;
   for (void* p : {P.pos, P.vel, P.start, (void*)P.cand, (void*)P.flag, (void*)P.rank, P.ck_pos, P.ck_vel})
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
   P.pos = P.vel = P.start = P.ck_pos = P.ck_vel = nullptr;
   P.cand = nullptr;
   P.flag = P.rank = nullptr;
@@ -954,7 +954,7 @@ static int pic_alloc_particles(Pic& P, int cap) {
   const size_t pb = (size_t)std::max(cap, 1) * 2 * P.sz;
   hipError_t e = hipSuccess;
   auto A = [&](void** p, size_t bytes) {
-    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e == hipSuccess) e = dev_alloc(p, bytes);
   };
   A(&P.pos, pb);
   A(&P.vel, pb);
@@ -976,9 +976,9 @@ static void pic_free(Pic* P) {
   for (void* p : {P->pos, P->vel, P->start, (void*)P->cand, (void*)P->flag, (void*)P->rank, (void*)P->flist,
                   P->GU, P->GV, P->FU, P->FV, P->TU, P->TV, (void*)P->rng, (void*)P->keys, (void*)P->cnt,
                   (void*)P->table, P->ck_pos, P->ck_vel, P->ck_gu, P->ck_gv, (void*)P->ck_rng})
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
   free_sort(P->sb, nullptr, false);
-  if (P->host_keys) (void)hipHostFree(P->host_keys);
+  if (P->host_keys) (void)host_free(P->host_keys);
   if (P->M) free_mac(P->M);
   delete P;
 }
@@ -1201,7 +1201,7 @@ int fb_pic_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   const size_t nc = (size_t)dom->nx * dom->ny;
   hipError_t e = hipSuccess;
   auto A = [&](void** p, size_t bytes) {
-    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e == hipSuccess) e = dev_alloc(p, bytes);
   };
   A(&P->GU, nc * P->sz);
   A(&P->GV, nc * P->sz);
@@ -1217,7 +1217,7 @@ int fb_pic_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   A((void**)&P->ck_rng, sizeof(fb::Pcg));
   A((void**)&P->keys, 16 * sizeof(unsigned long long));
   A((void**)&P->table, 2 * fb::kSweepEntries * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&P->host_keys, 16 * sizeof(unsigned long long), 0);
+  if (e == hipSuccess) e = host_alloc((void**)&P->host_keys, 16 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     fb::set_error("device allocation failed: %s", hipGetErrorString(e));
     fb::pic_free(P);
@@ -1228,7 +1228,8 @@ int fb_pic_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
     return rc;
   }
   fb::Pcg zero{0ull, 1ull};
-  FB_HIP(hipMemcpy(P->rng, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  FB_HIP(hipMemcpyAsync(P->rng, &zero, sizeof(zero), hipMemcpyHostToDevice, P->ctx->stream));
+  FB_HIP(hipStreamSynchronize(P->ctx->stream));
   *out = reinterpret_cast<fb_pic*>(P);
   return FB_OK;
 }

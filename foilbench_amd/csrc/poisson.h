@@ -31,10 +31,14 @@ struct CgVecs {
   T* work = nullptr;
   T* scratch = nullptr;
   DstPlan* plan = nullptr;
-  // y-direction partition solver: per x-mode factors and spikes for chunks of tri_L rows
-  // (the last chunk has tri_Llast rows), tri_P chunks per column
+  // y-direction partition solver (chunks of kTriL rows, the last of tri_Llast, tri_P chunks):
+  // factors and spikes [chunk type][quantity][row][mode], interface recurrence coefficients
+  // [quantity][chunk][mode], and the interface values [chunk][mode]
   T* tri = nullptr;
-  int tri_L = 0, tri_Llast = 0, tri_P = 0;
+  T* tri_itf = nullptr;
+  T* tri_yF = nullptr;
+  T* tri_yE = nullptr;
+  int tri_Llast = 0, tri_P = 0;
 };
 
 class Poisson {

@@ -42,7 +42,7 @@ struct DstPlan {
   double scale = 0.0;
   void* chirp = nullptr;  // cplx<T>[n]: exp(-i pi m^2 / (2(n+1))), m = 1..n
   void* fb = nullptr;     // cplx<T>[M]: FFT of the cyclic exp(+i pi m^2 / (2(n+1)))
-  void* tw = nullptr;     // cplx<T>[M/2]: exp(-2 pi i k / M)
+  void* tw = nullptr;     // cplx<T>[M]: exp(-2 pi i k / M)
 };
 
 // ---------------------------------------------------------------- host table set-up
@@ -71,7 +71,7 @@ static void host_fft(std::vector<std::complex<long double>>& a) {
 }
 
 template <class T>
-static int make_plan(DstPlan*& plan, int n) {
+static int make_plan(DstPlan*& plan, int n, hipStream_t stream) {
   plan = new DstPlan();
   plan->n = n;
   int M = 1, logM = 0;
@@ -79,9 +79,14 @@ static int make_plan(DstPlan*& plan, int n) {
     M <<= 1;
     ++logM;
   }
-  if (M < 2) {
-    M = 2;
-    logM = 1;
+  // the radix-8 Stockham passes give each thread 16 points: at least one full thread per row
+  while (M < 16) {
+    M <<= 1;
+    ++logM;
+  }
+  if (M > 8192) {
+    set_error("DST preconditioner supports at most 4096 cells along x (got %d)", n);
+    return FB_ERR_ARG;
   }
   plan->M = M;
   plan->logM = logM;
@@ -92,7 +97,7 @@ static int make_plan(DstPlan*& plan, int n) {
     long long r = (m * m) % fourN;
     return pi * (long double)r / (long double)(2 * (n + 1));
   };
-  std::vector<cplx<T>> chirp(n), fbv(M), tw(M / 2);
+  std::vector<cplx<T>> chirp(n), fbv(M), tw(M);
   for (int m = 1; m <= n; ++m) {
     long double a = phase(m);
     chirp[m - 1] = {(T)cosl(a), (T)(-sinl(a))};
@@ -104,24 +109,26 @@ static int make_plan(DstPlan*& plan, int n) {
   }
   host_fft(b);
   for (int k = 0; k < M; ++k) fbv[k] = {(T)b[k].real(), (T)b[k].imag()};
-  for (int k = 0; k < M / 2; ++k) {
+  for (int k = 0; k < M; ++k) {
     long double a = -2.0L * pi * (long double)k / (long double)M;
     tw[k] = {(T)cosl(a), (T)sinl(a)};
   }
-  FB_HIP(hipMalloc(&plan->chirp, n * sizeof(cplx<T>)));
-  FB_HIP(hipMalloc(&plan->fb, M * sizeof(cplx<T>)));
-  FB_HIP(hipMalloc(&plan->tw, (M / 2) * sizeof(cplx<T>)));
-  FB_HIP(hipMemcpy(plan->chirp, chirp.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
-  FB_HIP(hipMemcpy(plan->fb, fbv.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice));
-  FB_HIP(hipMemcpy(plan->tw, tw.data(), (M / 2) * sizeof(cplx<T>), hipMemcpyHostToDevice));
+  FB_HIP(dev_alloc(&plan->chirp, n * sizeof(cplx<T>)));
+  FB_HIP(dev_alloc(&plan->fb, M * sizeof(cplx<T>)));
+  FB_HIP(dev_alloc(&plan->tw, M * sizeof(cplx<T>)));
+  // stream-ordered copies (a legacy-stream copy would conflict with graph capture on other threads)
+  FB_HIP(hipMemcpyAsync(plan->chirp, chirp.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipMemcpyAsync(plan->fb, fbv.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipMemcpyAsync(plan->tw, tw.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipStreamSynchronize(stream));
   return FB_OK;
 }
 
 static void free_plan(DstPlan* p) {
   if (!p) return;
-  if (p->chirp) (void)hipFree(p->chirp);
-  if (p->fb) (void)hipFree(p->fb);
-  if (p->tw) (void)hipFree(p->tw);
+  if (p->chirp) (void)dev_free(p->chirp);
+  if (p->fb) (void)dev_free(p->fb);
+  if (p->tw) (void)dev_free(p->tw);
   delete p;
 }
 
@@ -322,212 +329,330 @@ __global__ void __launch_bounds__(256) jacobi_kernel(size_t N, const double* __r
   if (threadIdx.x == 0 && partial) partial[blockIdx.x] = v;
 }
 
-__device__ __forceinline__ int bitrev(int j, int logM) { return (int)(__brev((unsigned)j) >> (32 - logM)); }
+// ---------------------------------------------------------------- DST-I along x
 
+// Radix-R DFT (forward, e^{-2 pi i / R}) of v[0..R) in registers, natural order in and out.
 template <class T>
-__device__ __forceinline__ void fft_dit(cplx<T>* buf, const cplx<T>* __restrict__ tw, int M, int logM) {
-  for (int s = 1; s <= logM; ++s) {
-    const int h = 1 << (s - 1);
-    const int ts = M >> s;
-    for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
-      const int k = b & (h - 1);
-      const int i = ((b >> (s - 1)) << s) + k;
-      const int j = i + h;
-      cplx<T> t = cmul(tw[k * ts], buf[j]);
-      cplx<T> a = buf[i];
-      buf[i] = {a.x + t.x, a.y + t.y};
-      buf[j] = {a.x - t.x, a.y - t.y};
-    }
-    __syncthreads();
+__device__ __forceinline__ void dft2(cplx<T>* v) {
+  const cplx<T> a = v[0], b = v[1];
+  v[0] = {a.x + b.x, a.y + b.y};
+  v[1] = {a.x - b.x, a.y - b.y};
+}
+template <class T>
+__device__ __forceinline__ void dft4(cplx<T>* v) {
+  const cplx<T> c0{v[0].x + v[2].x, v[0].y + v[2].y}, c1{v[0].x - v[2].x, v[0].y - v[2].y};
+  const cplx<T> c2{v[1].x + v[3].x, v[1].y + v[3].y};
+  const cplx<T> c3{v[1].y - v[3].y, v[3].x - v[1].x};  // (v1 - v3) * (-i)
+  v[0] = {c0.x + c2.x, c0.y + c2.y};
+  v[2] = {c0.x - c2.x, c0.y - c2.y};
+  v[1] = {c1.x + c3.x, c1.y + c3.y};
+  v[3] = {c1.x - c3.x, c1.y - c3.y};
+}
+template <class T>
+__device__ __forceinline__ void dft8(cplx<T>* v) {
+  const T h = (T)0.70710678118654752440084436210485;
+  // first radix-2 stage (pairs s, s+4), odd half twiddled by e^{-2 pi i s / 8}
+  cplx<T> e[4], o[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    e[s] = {v[s].x + v[s + 4].x, v[s].y + v[s + 4].y};
+    o[s] = {v[s].x - v[s + 4].x, v[s].y - v[s + 4].y};
+  }
+  o[1] = {(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};
+  o[2] = {o[2].y, -o[2].x};
+  o[3] = {(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};
+  dft4(e);
+  dft4(o);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    v[2 * m] = e[m];
+    v[2 * m + 1] = o[m];
   }
 }
-
-template <class T>
-__device__ __forceinline__ void fft_dif(cplx<T>* buf, const cplx<T>* __restrict__ tw, int M, int logM) {
-  for (int s = logM; s >= 1; --s) {
-    const int h = 1 << (s - 1);
-    const int ts = M >> s;
-    for (int b = threadIdx.x; b < (M >> 1); b += blockDim.x) {
-      const int k = b & (h - 1);
-      const int i = ((b >> (s - 1)) << s) + k;
-      const int j = i + h;
-      cplx<T> a = buf[i], c = buf[j];
-      buf[i] = {a.x + c.x, a.y + c.y};
-      buf[j] = cmul(cplx<T>{a.x - c.x, a.y - c.y}, tw[k * ts]);
-    }
-    __syncthreads();
-  }
+template <class T, int R>
+__device__ __forceinline__ void dft_r(cplx<T>* v) {
+  if constexpr (R == 8) dft8(v);
+  else if constexpr (R == 4) dft4(v);
+  else dft2(v);
 }
 
-// Orthonormal DST-I along x of every row (one row per block), Bluestein chirp-z.
-// mode 0: input r masked to fluid cells, output to `out`.
-// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row block.
-template <class T>
-__global__ void __launch_bounds__(256) dst_rows_kernel(int nx, int ny, const T* __restrict__ in,
-                                                       const uint8_t* __restrict__ fl, const T* __restrict__ rsrc,
-                                                       T* __restrict__ out, const cplx<T>* __restrict__ chirp,
-                                                       const cplx<T>* __restrict__ fbt, const cplx<T>* __restrict__ tw,
-                                                       int M, int logM, T scale, int mode, double* __restrict__ partial,
-                                                       const double* scal) {
-  extern __shared__ unsigned char smem_raw[];
-  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
-  __shared__ double lds[4];
-  if (scal && CG_DONE(scal)) return;
-  const int row = blockIdx.x;
-  const size_t base = (size_t)row * nx;
-  for (int j = threadIdx.x; j < M; j += blockDim.x) {
-    cplx<T> a{(T)0, (T)0};
-    if (j < nx) {
-      T v = in[base + j];
-      if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
-      cplx<T> c = chirp[j];
-      a = {v * c.x, v * c.y};
-    }
-    buf[bitrev(j, logM)] = a;
+// One radix-R Stockham pass (sub-transforms of length L -> L*R) over a row of M points in
+// LDS, in place: each of the row's M/16 threads reads its 16 points, the block synchronises,
+// then writes them to their autosorted places.
+template <class T, int R>
+__device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int L, int t,
+                                              int nthr) {
+  constexpr int NB = 16 / R;
+  const int MR = M / R;
+  cplx<T> v[NB][R];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * nthr;
+#pragma unroll
+    for (int s = 0; s < R; ++s) v[u][s] = buf[b + s * MR];
   }
   __syncthreads();
-  fft_dit(buf, tw, M, logM);
-  for (int k = threadIdx.x; k < M; k += blockDim.x) {
-    cplx<T> v = cmul(buf[k], fbt[k]);
+  const int stride = M / (L * R);
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * nthr;
+    const int k = b & (L - 1);
+    // one table load per butterfly; the higher twiddles are its powers
+    const cplx<T> w1 = roots[k * stride];
+    cplx<T> ws = w1;
+#pragma unroll
+    for (int s = 1; s < R; ++s) {
+      v[u][s] = cmul(v[u][s], ws);
+      if (s + 1 < R) ws = cmul(ws, w1);
+    }
+    dft_r<T, R>(v[u]);
+    const int base = (b - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[base + q * L] = v[u][q];
+  }
+  __syncthreads();
+}
+
+// Forward FFT of each row's M = 2^logM points (radix-8 passes, then one radix-2 or -4 pass).
+template <class T>
+__device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int logM, int t,
+                                         int nthr) {
+  int L = 1;
+  for (int p = 0; p < logM / 3; ++p, L *= 8) stockham_pass<T, 8>(buf, roots, M, L, t, nthr);
+  if (logM % 3 == 2) stockham_pass<T, 4>(buf, roots, M, L, t, nthr);
+  else if (logM % 3 == 1) stockham_pass<T, 2>(buf, roots, M, L, t, nthr);
+}
+
+// Orthonormal DST-I along x of every row, Bluestein chirp-z with Stockham FFTs in LDS. A block
+// holds `rows` rows of M points, M/16 threads each.
+// mode 0: input r masked to fluid cells, output to `out`.
+// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row.
+template <class T>
+__global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* __restrict__ in,
+                                                       const uint8_t* __restrict__ fl, const T* __restrict__ rsrc,
+                                                       T* __restrict__ out, const cplx<T>* __restrict__ chirp,
+                                                       const cplx<T>* __restrict__ fbt,
+                                                       const cplx<T>* __restrict__ roots, int M, int logM, int rows,
+                                                       T scale, int mode, double* __restrict__ partial,
+                                                       const double* scal) {
+  extern __shared__ unsigned char smem_raw[];
+  if (scal && CG_DONE(scal)) return;
+  const int nthr = M >> 4;
+  const int lr = threadIdx.x / nthr;  // row within the block
+  const int t = threadIdx.x - lr * nthr;
+  const int row = blockIdx.x * rows + lr;
+  const bool live = row < ny;
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw) + (size_t)lr * M;
+  double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * M * sizeof(cplx<T>));
+  const size_t base = (size_t)(live ? row : 0) * nx;
+  for (int j = t; j < M; j += nthr) {
+    cplx<T> a{(T)0, (T)0};
+    if (live && j < nx) {
+      T v = in[base + j];
+      if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
+      const cplx<T> c = chirp[j];
+      a = {v * c.x, v * c.y};
+    }
+    buf[j] = a;
+  }
+  __syncthreads();
+  fft_rows(buf, roots, M, logM, t, nthr);
+  // convolution with the chirp: inverse FFT as conj(FFT(conj(.)))
+  for (int k = t; k < M; k += nthr) {
+    const cplx<T> v = cmul(buf[k], fbt[k]);
     buf[k] = {v.x, -v.y};
   }
   __syncthreads();
-  fft_dif(buf, tw, M, logM);
+  fft_rows(buf, roots, M, logM, t, nthr);
   const T invM = (T)1 / (T)M;
   double acc = 0.0;
-  for (int k = threadIdx.x; k < nx; k += blockDim.x) {
-    cplx<T> w = buf[bitrev(k, logM)];
-    cplx<T> conv{w.x * invM, -w.y * invM};
-    cplx<T> y = cmul(chirp[k], conv);
-    T d = -y.y * scale;
-    if (mode == 0) {
-      out[base + k] = d;
-    } else {
-      T rv = rsrc[base + k];
-      T zv = (fl[base + k] & PF_SOLID) ? rv : d;
-      out[base + k] = zv;
-      acc += (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+  if (live) {
+    for (int k = t; k < nx; k += nthr) {
+      const cplx<T> w = buf[k];
+      const cplx<T> conv{w.x * invM, -w.y * invM};
+      const cplx<T> y = cmul(chirp[k], conv);
+      const T d = -y.y * scale;
+      if (mode == 0) {
+        out[base + k] = d;
+      } else {
+        const T rv = rsrc[base + k];
+        const T zv = (fl[base + k] & PF_SOLID) ? rv : d;
+        out[base + k] = zv;
+        acc += (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+      }
     }
   }
   if (mode == 1) {
+    // deterministic per-row partial: fixed tree over the row's threads
+    red[threadIdx.x] = acc;
     __syncthreads();
-    // deterministic per-row partial: fixed strided assignment and tree
-    double s = block_sum<256>(acc, lds);
-    if (threadIdx.x == 0) partial[row] = s;
+    for (int s2 = nthr >> 1; s2 > 0; s2 >>= 1) {
+      if (t < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+      __syncthreads();
+    }
+    if (t == 0 && live) partial[row] = red[threadIdx.x];
   }
 }
 
-// Thomas solve of (T_y/dy^2 + ex[i] I) z = rhs for every x-mode column i.
+// ---------------------------------------------------------------- tridiagonal solves along y
+
+// Partition solve of (T_y/dy^2 + ex[i] I) z = w for every x-mode column i, in place. The ny rows
+// are cut into P chunks of L rows (the last has Llast). Three stages:
+//  1. tri_chunk_kernel: one thread per (mode, chunk) eliminates its chunk with the chunk's LU
+//     factors and zero boundary values, keeping the chunk's first and last value;
+//  2. tri_interface_kernel: one thread per mode solves the 2P interface values (a recurrence
+//     over the chunks with precomputed coefficients);
+//  3. tri_spike_kernel: one thread per (mode, chunk) adds its two spikes scaled by the
+//     neighbouring interface values.
+// Tables are [chunk type][quantity][row][mode] so that a wave's 64 modes read 64 neighbours.
+constexpr int kTriL = 8;  // rows per chunk
+
 template <class T>
-__global__ void __launch_bounds__(64) thomas_kernel(int nx, int ny, T* __restrict__ w, T* __restrict__ cpb,
-                                                    const double* __restrict__ ex, T iy2, const double* scal) {
+__global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __restrict__ w,
+                                                        const T* __restrict__ fac, int Llast, int P, T a,
+                                                        T* __restrict__ yF, T* __restrict__ yE, const double* scal) {
   if (scal && CG_DONE(scal)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nx) return;
-  const T a = -iy2;
-  const T b = (T)2 * iy2 + (T)ex[i];
-  T cp = a / b;
-  T dp = w[i] / b;
-  cpb[i] = cp;
-  w[i] = dp;
-  for (int j = 1; j < ny; ++j) {
-    size_t k = (size_t)j * nx + i;
-    T m = b - a * cp;
-    T d = w[k];
-    cp = a / m;
-    dp = (d - a * dp) / m;
-    cpb[k] = cp;
-    w[k] = dp;
-  }
-  T zz = dp;
-  for (int j = ny - 2; j >= 0; --j) {
-    size_t k = (size_t)j * nx + i;
-    zz = w[k] - cpb[k] * zz;
-    w[k] = zz;
-  }
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.y * 4 + threadIdx.y;
+  if (i >= nx || c >= P) return;
+  const int last = c == P - 1;
+  const int Lc = last ? Llast : kTriL;
+  const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
+  const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
+  T* col = w + (size_t)c * kTriL * nx + i;
+  T v[kTriL];
+#pragma unroll
+  for (int j = 0; j < kTriL; ++j)
+    if (j < Lc) v[j] = col[(size_t)j * nx];
+  // forward elimination, then back substitution, within the chunk
+  T y = (T)0;
+#pragma unroll
+  for (int j = 0; j < kTriL; ++j)
+    if (j < Lc) {
+      y = (v[j] - a * y) * invm[(size_t)j * nx];
+      v[j] = y;
+    }
+  T yn = y;
+#pragma unroll
+  for (int j = kTriL - 2; j >= 0; --j)
+    if (j < Lc - 1) {
+      yn = v[j] - cp[(size_t)j * nx] * yn;
+      v[j] = yn;
+    }
+#pragma unroll
+  for (int j = 0; j < kTriL; ++j)
+    if (j < Lc) col[(size_t)j * nx] = v[j];
+  yF[(size_t)c * nx + i] = v[0];
+  yE[(size_t)c * nx + i] = y;
 }
 
-// Partition solve of (T_y/dy^2 + ex[i] I) z = w for every x-mode column i, in place: the same
-// tridiagonal systems the Thomas solve above handles, with short dependent chains. Block =
-// 64 modes (lanes) x P chunks (waves). Each thread eliminates its chunk of rows with the
-// chunk's precomputed LU factors (zero boundary values); the P chunk interfaces of a mode then
-// form a 2P-unknown system (one lane per mode, in LDS); finally each chunk adds its two
-// precomputed spikes scaled by the neighbouring interface values.
-// tab per mode: [invm, cp, alpha, beta] for chunks of L rows, then the same for Llast rows.
+// Interface recurrences per mode, as segmented scans: 16 modes x 16 segments of chunks per
+// block. Forward: r_k = g yF_k + h pp_{k-1}, pp_k = A pp_{k-1} + (yE_k + mg yF_k); back:
+// F_k = sv F_{k+1} + r_k, E_k = pp_k + qq F_{k+1} (F_P = 0). Each is an affine recurrence: every
+// thread composes its segment's maps, one thread per mode chains the 16 segment maps, then each
+// thread replays its segment from its start value. itf = [g, h, A, mg, sv, qq][chunk][mode].
 template <class T>
-__global__ void __launch_bounds__(1024) tri_partition_kernel(int nx, int ny, T* __restrict__ w,
-                                                             const T* __restrict__ tab, int L, int Llast, int P, T a,
-                                                             const double* scal) {
+__global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const T* __restrict__ itf,
+                                                            T* __restrict__ yF, T* __restrict__ yE,
+                                                            const double* scal) {
   if (scal && CG_DONE(scal)) return;
-  __shared__ T sF[16][64], sE[16][64];
-  const int lane = threadIdx.x & 63;
-  const int c = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + lane;
-  const bool on = i < nx && c < P;
-  const int Lc = (c == P - 1) ? Llast : L;
-  const int s0 = c * L;
-  const size_t stride_mode = 4 * (size_t)(L + Llast);
-  const T* t = tab + (size_t)(on ? i : 0) * stride_mode + (c == P - 1 ? 4 * (size_t)L : 0);
-  const T* invm = t;
-  const T* cp = t + Lc;
-  const T* al = t + 2 * Lc;
-  const T* be = t + 3 * Lc;
-  if (on) {
-    // forward elimination within the chunk
-    T y = (T)0;
-    for (int j = 0; j < Lc; ++j) {
-      const size_t k = (size_t)(s0 + j) * nx + i;
-      y = (w[k] - a * y) * invm[j];
-      w[k] = y;
+  __shared__ T sA[16][16], sB[16][16], sS[16][16];
+  const int m = threadIdx.x, gseg = threadIdx.y;
+  const int i = blockIdx.x * 16 + m;
+  const bool on = i < nx;
+  const int S = (P + 15) / 16;
+  const int k0 = min(P, gseg * S), k1 = min(P, k0 + S);
+  const size_t PQ = (size_t)P * nx;
+  const T* cg = itf;
+  const T* ch = itf + PQ;
+  const T* cA = itf + 2 * PQ;
+  const T* cmg = itf + 3 * PQ;
+  const T* csv = itf + 4 * PQ;
+  const T* cqq = itf + 5 * PQ;
+  // forward: compose pp_{k1-1} = Ac pp_{k0-1} + Bc
+  T Ac = (T)1, Bc = (T)0;
+  if (on)
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T A = cA[o];
+      Ac = A * Ac;
+      Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
     }
-    // back substitution within the chunk
-    T yn = y;
-    for (int j = Lc - 2; j >= 0; --j) {
-      const size_t k = (size_t)(s0 + j) * nx + i;
-      yn = w[k] - cp[j] * yn;
-      w[k] = yn;
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = 0; q < 16; ++q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
     }
-    sF[c][lane] = yn;  // y at the chunk's first row
-    sE[c][lane] = y;   // y at the chunk's last row
   }
   __syncthreads();
-  if (c == 0 && i < nx) {
-    // interfaces: F_c = z(first row of c), E_c = z(last row of c);
-    // F_c = yF + aF E_{c-1} + bF F_{c+1},  E_c = yE + aE E_{c-1} + bE F_{c+1}
-    T r[16], sv[16], pv[16], qv[16];
-    T pp = (T)0, qq = (T)0;
-    for (int k = 0; k < P; ++k) {
-      const int Lk = (k == P - 1) ? Llast : L;
-      const T* tk = tab + (size_t)i * stride_mode + (k == P - 1 ? 4 * (size_t)L : 0);
-      const T aF = tk[2 * Lk], bF = tk[3 * Lk], aE = tk[2 * Lk + Lk - 1], bE = tk[3 * Lk + Lk - 1];
-      const T den = (T)1 - aF * qq;
-      r[k] = (sF[k][lane] + aF * pp) / den;
-      sv[k] = bF / den;
-      const T pn = sE[k][lane] + aE * pp + aE * qq * r[k];
-      const T qn = aE * qq * sv[k] + bE;
-      pv[k] = pn;
-      qv[k] = qn;
-      pp = pn;
-      qq = qn;
+  if (on) {
+    T pp = sS[gseg][m];
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T f = yF[o];
+      const T r = cg[o] * f + ch[o] * pp;
+      pp = cA[o] * pp + (yE[o] + cmg[o] * f);
+      yF[o] = r;
+      yE[o] = pp;
     }
-    T Fn = (T)0;
-    for (int k = P - 1; k >= 0; --k) {
-      const T F = r[k] + sv[k] * Fn;
-      const T E = pv[k] + qv[k] * Fn;
-      sF[k][lane] = F;
-      sE[k][lane] = E;
+  }
+  // back: compose F_{k0} = Ac F_{k1} + Bc
+  Ac = (T)1;
+  Bc = (T)0;
+  if (on)
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T sv = csv[o];
+      Ac = sv * Ac;
+      Bc = sv * Bc + yF[o];
+    }
+  __syncthreads();
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = 15; q >= 0; --q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
+    }
+  }
+  __syncthreads();
+  if (on) {
+    T Fn = sS[gseg][m];
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T F = yF[o] + csv[o] * Fn;
+      const T E = yE[o] + cqq[o] * Fn;
+      yF[o] = F;
+      yE[o] = E;
       Fn = F;
     }
   }
-  __syncthreads();
-  if (on) {
-    const T El = c > 0 ? sE[c - 1][lane] : (T)0;
-    const T Fr = c < P - 1 ? sF[c + 1][lane] : (T)0;
-    for (int j = 0; j < Lc; ++j) {
-      const size_t k = (size_t)(s0 + j) * nx + i;
-      w[k] = w[k] + al[j] * El + be[j] * Fr;
-    }
-  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) tri_spike_kernel(int nx, int ny, T* __restrict__ w, const T* __restrict__ fac,
+                                                        int Llast, int P, const T* __restrict__ F,
+                                                        const T* __restrict__ E, const double* scal) {
+  if (scal && CG_DONE(scal)) return;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.y * 4 + threadIdx.y;
+  if (i >= nx || c >= P) return;
+  const int last = c == P - 1;
+  const int Lc = last ? Llast : kTriL;
+  const T* al = fac + (size_t)(last * 4 + 2) * kTriL * nx + i;
+  const T* be = fac + (size_t)(last * 4 + 3) * kTriL * nx + i;
+  const T El = c > 0 ? E[(size_t)(c - 1) * nx + i] : (T)0;
+  const T Fr = c < P - 1 ? F[(size_t)(c + 1) * nx + i] : (T)0;
+  T* col = w + (size_t)c * kTriL * nx + i;
+#pragma unroll
+  for (int j = 0; j < kTriL; ++j)
+    if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
 }
 
 template <class T>
@@ -692,12 +817,12 @@ Poisson::~Poisson() {
   for (auto& g : graphs_)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (void* ptr : {(void*)flags, (void*)diag, (void*)ex, (void*)partial, (void*)scal, (void*)b64, (void*)r64})
-    if (ptr) (void)hipFree(ptr);
-  if (host_scal) (void)hipHostFree(host_scal);
+    if (ptr) (void)dev_free(ptr);
+  if (host_scal) (void)host_free(host_scal);
   auto freev = [](auto& v) {
     for (void* ptr : {(void*)v.r, (void*)v.z, (void*)v.p, (void*)v.q, (void*)v.x, (void*)v.b, (void*)v.work,
-                      (void*)v.scratch, (void*)v.tri})
-      if (ptr) (void)hipFree(ptr);
+                      (void*)v.scratch, (void*)v.tri, (void*)v.tri_itf, (void*)v.tri_yF, (void*)v.tri_yE})
+      if (ptr) (void)dev_free(ptr);
     free_plan(v.plan);
   };
   freev(v64);
@@ -707,59 +832,89 @@ Poisson::~Poisson() {
 template <class T>
 int Poisson::alloc(CgVecs<T>& v) {
   if (v.r) return FB_OK;
-  for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(hipMalloc((void**)a, N * sizeof(T)));
-  FB_HIP(hipMalloc((void**)&v.p, 2 * N * sizeof(T)));
+  for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(dev_alloc((void**)a, N * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.p, 2 * N * sizeof(T)));
   if (!periodic_x && !periodic_y) {
-    int rc = make_plan<T>(v.plan, nx);
+    int rc = make_plan<T>(v.plan, nx, stream);
     if (rc) return rc;
     return make_tri<T>(v);
   }
   return FB_OK;
 }
 
-// Factors and spikes of the y-direction partition solver (long double on the host).
+// Factors, spikes and interface coefficients of the y-direction partition solver (long double
+// on the host), in the [chunk type][quantity][row][mode] and [quantity][chunk][mode] layouts.
 template <class T>
 int Poisson::make_tri(CgVecs<T>& v) {
-  const int P0 = std::min(16, ny);
-  const int L = (ny + P0 - 1) / P0;
+  const int L = kTriL;
   const int P = (ny + L - 1) / L;
   const int Ll = ny - (P - 1) * L;
-  v.tri_L = L;
   v.tri_Llast = Ll;
   v.tri_P = P;
-  const size_t stride = 4 * (size_t)(L + Ll);
-  std::vector<T> tab(stride * (size_t)nx);
+  std::vector<T> fac((size_t)8 * L * nx, (T)0);
+  std::vector<T> itf((size_t)6 * P * nx);
   const long double iy2 = 1.0L / ((long double)dy * (long double)dy);
   const long double a = -iy2;
   std::vector<double> exh(nx);
   for (int i = 0; i < nx; ++i)
     exh[i] = 2.0 * (1.0 - std::cos(M_PI * (double)(i + 1) / (double)(nx + 1))) * (1.0 / (dx * dx));
-  std::vector<long double> m, y;
+  std::vector<long double> m, al, be;
+  long double ends[2][4];  // per chunk type: alpha first, beta first, alpha last, beta last
   for (int i = 0; i < nx; ++i) {
     const long double b = 2.0L * iy2 + (long double)exh[i];
     for (int part = 0; part < 2; ++part) {
       const int Lc = part == 0 ? L : Ll;
-      T* t = tab.data() + (size_t)i * stride + (part == 0 ? 0 : 4 * (size_t)L);
+      auto at = [&](int q, int j) -> T& { return fac[((size_t)(part * 4 + q) * L + j) * nx + i]; };
       m.assign(Lc, 0.0L);
-      y.assign(Lc, 0.0L);
+      al.assign(Lc, 0.0L);
+      be.assign(Lc, 0.0L);
       m[0] = b;
       for (int j = 1; j < Lc; ++j) m[j] = b - a * a / m[j - 1];
       for (int j = 0; j < Lc; ++j) {
-        t[j] = (T)(1.0L / m[j]);
-        t[Lc + j] = (T)(a / m[j]);
+        at(0, j) = (T)(1.0L / m[j]);
+        at(1, j) = (T)(a / m[j]);
       }
       // alpha: coupling to the value below the chunk (right-hand side -a in the first row)
-      for (int j = 0; j < Lc; ++j) y[j] = ((j == 0 ? -a : 0.0L) - (j ? a * y[j - 1] : 0.0L)) / m[j];
-      for (int j = Lc - 2; j >= 0; --j) y[j] = y[j] - (a / m[j]) * y[j + 1];
-      for (int j = 0; j < Lc; ++j) t[2 * Lc + j] = (T)y[j];
+      for (int j = 0; j < Lc; ++j) al[j] = ((j == 0 ? -a : 0.0L) - (j ? a * al[j - 1] : 0.0L)) / m[j];
+      for (int j = Lc - 2; j >= 0; --j) al[j] = al[j] - (a / m[j]) * al[j + 1];
       // beta: coupling to the value above the chunk (right-hand side -a in the last row)
-      for (int j = 0; j < Lc; ++j) y[j] = ((j == Lc - 1 ? -a : 0.0L) - (j ? a * y[j - 1] : 0.0L)) / m[j];
-      for (int j = Lc - 2; j >= 0; --j) y[j] = y[j] - (a / m[j]) * y[j + 1];
-      for (int j = 0; j < Lc; ++j) t[3 * Lc + j] = (T)y[j];
+      for (int j = 0; j < Lc; ++j) be[j] = ((j == Lc - 1 ? -a : 0.0L) - (j ? a * be[j - 1] : 0.0L)) / m[j];
+      for (int j = Lc - 2; j >= 0; --j) be[j] = be[j] - (a / m[j]) * be[j + 1];
+      for (int j = 0; j < Lc; ++j) {
+        at(2, j) = (T)al[j];
+        at(3, j) = (T)be[j];
+      }
+      ends[part][0] = al[0];
+      ends[part][1] = be[0];
+      ends[part][2] = al[Lc - 1];
+      ends[part][3] = be[Lc - 1];
+    }
+    // interface recurrence coefficients (they do not depend on the right-hand side)
+    long double qq = 0.0L;
+    const size_t PQ = (size_t)P * nx;
+    for (int k = 0; k < P; ++k) {
+      const long double* e = ends[k == P - 1 ? 1 : 0];
+      const long double aF = e[0], bF = e[1], aE = e[2], bE = e[3];
+      const long double den = 1.0L - aF * qq;
+      const long double sv = bF / den;
+      const long double mm = aE * qq;
+      const size_t o = (size_t)k * nx + i;
+      itf[o] = (T)(1.0L / den);
+      itf[PQ + o] = (T)(aF / den);
+      itf[2 * PQ + o] = (T)(aE + mm * aF / den);
+      itf[3 * PQ + o] = (T)(mm / den);
+      itf[4 * PQ + o] = (T)sv;
+      qq = mm * sv + bE;
+      itf[5 * PQ + o] = (T)qq;
     }
   }
-  FB_HIP(hipMalloc((void**)&v.tri, tab.size() * sizeof(T)));
-  FB_HIP(hipMemcpy(v.tri, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice));
+  FB_HIP(dev_alloc((void**)&v.tri, fac.size() * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.tri_itf, itf.size() * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.tri_yF, (size_t)P * nx * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.tri_yE, (size_t)P * nx * sizeof(T)));
+  FB_HIP(hipMemcpyAsync(v.tri, fac.data(), fac.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipMemcpyAsync(v.tri_itf, itf.data(), itf.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipStreamSynchronize(stream));
   return FB_OK;
 }
 
@@ -778,18 +933,19 @@ int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int 
     periodic_y = py;
     N = (size_t)nx * ny;
     nblocks = (int)((N + 255) / 256);
-    FB_HIP(hipMalloc((void**)&flags, N));
-    FB_HIP(hipMalloc((void**)&diag, N * sizeof(double)));
-    FB_HIP(hipMalloc((void**)&partial, 2 * (size_t)(nblocks + ny + 64) * sizeof(double)));
-    FB_HIP(hipMalloc((void**)&scal, (S_COUNT + 8) * sizeof(double)));
-    FB_HIP(hipHostMalloc((void**)&host_scal, (S_COUNT + 8) * sizeof(double), 0));
-    FB_HIP(hipMalloc((void**)&b64, N * sizeof(double)));
-    FB_HIP(hipMalloc((void**)&r64, N * sizeof(double)));
-    FB_HIP(hipMalloc((void**)&ex, nx * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&flags, N));
+    FB_HIP(dev_alloc((void**)&diag, N * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&partial, 2 * (size_t)(nblocks + ny + 64) * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&scal, (S_COUNT + 8) * sizeof(double)));
+    FB_HIP(host_alloc((void**)&host_scal, (S_COUNT + 8) * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&b64, N * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&r64, N * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&ex, nx * sizeof(double)));
     std::vector<double> exh(nx);
     for (int i = 0; i < nx; ++i)
       exh[i] = 2.0 * (1.0 - std::cos(M_PI * (double)(i + 1) / (double)(nx + 1))) * (1.0 / (dx * dx));
-    FB_HIP(hipMemcpy(ex, exh.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpyAsync(ex, exh.data(), nx * sizeof(double), hipMemcpyHostToDevice, stream));
+    FB_HIP(hipStreamSynchronize(stream));
     int rc = alloc(v64);
     if (rc) return rc;
   }
@@ -820,18 +976,27 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
     return reduce(partial, nblocks, 1);
   }
   const DstPlan& P = *v.plan;
-  const int threads = std::min(256, std::max(64, P.M / 2));
-  const size_t lds = (size_t)P.M * sizeof(cplx<T>);
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(ny), dim3(threads), lds, stream, nx, ny, rin, flags, rin, v.work,
-                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 0,
-                     partial, scal);
+  const int nthr = P.M / 16;
+  // several short rows per block, but never fewer blocks than about two per CU
+  const int rows = std::max(1, std::min(256 / nthr, ny / 512));
+  const unsigned blocks = (unsigned)((ny + rows - 1) / rows);
+  const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * nthr * sizeof(double);
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(blocks), dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin,
+                     v.work, (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
+                     (T)P.scale, 0, partial, scal);
   FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tri_partition_kernel<T>, dim3((nx + 63) / 64), dim3(64 * v.tri_P), 0, stream, nx, ny, v.work,
-                     (const T*)v.tri, v.tri_L, v.tri_Llast, v.tri_P, (T)(-(1.0 / (dy * dy))), scal);
+  const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4);
+  const T a = (T)(-(1.0 / (dy * dy)));
+  hipLaunchKernelGGL(tri_chunk_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a,
+                     v.tri_yF, v.tri_yE, scal);
+  hipLaunchKernelGGL(tri_interface_kernel<T>, dim3((nx + 15) / 16), dim3(16, 16), 0, stream, nx, v.tri_P,
+                     (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
+  hipLaunchKernelGGL(tri_spike_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P,
+                     (const T*)v.tri_yF, (const T*)v.tri_yE, scal);
   FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(ny), dim3(threads), lds, stream, nx, ny, v.work, flags, rin, zout,
-                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, (T)P.scale, 1,
-                     partial, scal);
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(blocks), dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin,
+                     zout, (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
+                     (T)P.scale, 1, partial, scal);
   FB_HIP(hipGetLastError());
   return reduce(partial, ny, 1);
 }
@@ -887,6 +1052,7 @@ int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
     }
   if (!use_graphs_) return enqueue_iterations(v, x, n, parity);
   hipGraph_t graph = nullptr;
+  std::lock_guard<std::recursive_mutex> guard(hip_lock());
   FB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_iterations(v, x, n, parity);
   hipError_t e = hipStreamEndCapture(stream, &graph);

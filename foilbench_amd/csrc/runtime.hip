@@ -19,6 +19,11 @@ void set_error(const char* fmt, ...) {
 
 void clear_error() { g_error.clear(); }
 
+std::recursive_mutex& hip_lock() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 // Streaming copy used to calibrate HBM counters and the achievable bandwidth.
 __global__ void __launch_bounds__(256) copy_f4_kernel(size_t n4, const float4* __restrict__ a, float4* __restrict__ b) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -75,8 +80,8 @@ int fb_copy_benchmark(fb_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy)
   auto* c = reinterpret_cast<fb::Ctx*>(ctx);
   FB_HIP(hipSetDevice(c->device));
   void *a = nullptr, *b = nullptr;
-  FB_HIP(hipMalloc(&a, (size_t)bytes));
-  FB_HIP(hipMalloc(&b, (size_t)bytes));
+  FB_HIP(dev_alloc(&a, (size_t)bytes));
+  FB_HIP(dev_alloc(&b, (size_t)bytes));
   FB_HIP(hipMemsetAsync(a, 0, (size_t)bytes, c->stream));
   size_t n4 = (size_t)bytes / 16;
   unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, (size_t)c->cu_count * 32);
@@ -94,8 +99,8 @@ int fb_copy_benchmark(fb_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy)
   *ms_per_copy = ms / reps;
   FB_HIP(hipEventDestroy(e0));
   FB_HIP(hipEventDestroy(e1));
-  FB_HIP(hipFree(a));
-  FB_HIP(hipFree(b));
+  FB_HIP(dev_free(a));
+  FB_HIP(dev_free(b));
   return FB_OK;
 }
 
@@ -122,7 +127,7 @@ int fb_ctx_create(int device, fb_ctx** out) {
   c->cu_count = prop.multiProcessorCount;
   // Default-priority stream, non-blocking with respect to the null stream.
   FB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  if (hipMalloc(&c->partials, (size_t)fb::kPartialKeys * fb::kPartialBlocks * sizeof(unsigned long long)) != hipSuccess) {
+  if (dev_alloc(&c->partials, (size_t)fb::kPartialKeys * fb::kPartialBlocks * sizeof(unsigned long long)) != hipSuccess) {
     hipStreamDestroy(c->stream);
     delete c;
     fb::set_error("fb_ctx_create: cannot allocate the reduction slots");
@@ -137,7 +142,7 @@ int fb_ctx_destroy(fb_ctx* ctx) {
   auto* c = reinterpret_cast<fb::Ctx*>(ctx);
   FB_HIP(hipSetDevice(c->device));
   FB_HIP(hipStreamSynchronize(c->stream));
-  FB_HIP(hipFree(c->partials));
+  FB_HIP(dev_free(c->partials));
   FB_HIP(hipStreamDestroy(c->stream));
   delete c;
   return FB_OK;

@@ -525,10 +525,14 @@ class StableFluidsSolver:
         warnings = () if not self._projection_warning else (self._projection_warning,)
         return Diagnostics(values, warnings, self._revision)
 
+    def close(self) -> None:
+        """Free the device state now rather than at garbage collection."""
+        if self._handle is not None and self._ctx is not None:
+            self._ctx.lib.fb_mac_destroy(self._handle)
+            self._handle = None
+
     def __del__(self) -> None:  # pragma: no cover
         try:
-            if self._handle is not None and self._ctx is not None:
-                self._ctx.lib.fb_mac_destroy(self._handle)
-                self._handle = None
+            self.close()
         except Exception:
             pass

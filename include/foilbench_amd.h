@@ -321,6 +321,61 @@ int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
                              const void* positions, int nx, int ny, double x0, double y0, double dx, double dy,
                              int periodic_x, int periodic_y, void* out);
 
+/* ------------------------------------------------------------------- Serving ------ */
+/* No counterpart in the reference, which advances one scenario at a time in-process. These
+ * entry points run many scenarios of uneven size concurrently on one GPU. */
+
+/* Device memory in bytes (hipMemGetInfo) for memory-based admission. */
+int fb_mem_info(int device, int64_t* free_bytes, int64_t* total_bytes);
+
+/* `steps` lattice steps of each of n lattices (one precision) in batched launches: one
+ * launch per step covers every lattice, and chunks of `chunk` steps (even) are replayed
+ * from one captured HIP graph. Same arithmetic per cell as fb_lbm_advance; stats[k] (may
+ * be NULL) receives lattice k's postcondition reductions. Synchronous. */
+int fb_lbm_run_batch(fb_lbm* const* lattices, const fb_lbm_params* params, int n, int steps, int chunk,
+                     fb_lbm_stats* stats);
+
+typedef struct fb_server fb_server;
+
+typedef struct fb_server_options {
+  int sf_workers;  /* Stable Fluids requests in flight at once, one stream each (>= 1) */
+  int lbm_chunk;   /* lattice steps per captured graph launch (even, >= 2) */
+  int use_graphs;  /* 1: replay captured graphs for the lattice batches */
+} fb_server_options;
+
+enum fb_serve_kind { FB_SERVE_SF = 0, FB_SERVE_LBM = 1 };
+
+typedef struct fb_server_result {
+  int64_t id;
+  int kind;               /* fb_serve_kind */
+  int status;             /* FB_OK or the FB_ERR_* of the failing call */
+  int sf_status;          /* fb_sf_status of the first substep that did not finish cleanly */
+  int steps_done;
+  double submitted_ms;    /* server clock, ms since fb_server_create */
+  double started_ms;
+  double finished_ms;
+  int64_t pressure_iterations;  /* Stable Fluids: CG iterations over all substeps */
+  fb_lbm_stats lbm_stats;       /* lattice: postcondition reductions of the final state */
+} fb_server_result;
+
+/* A request loop on `device`: Stable Fluids requests run on a pool of worker streams, lattice
+ * requests join a continuous batch (one launch per step for all of them, graph-replayed in
+ * chunks; lattices join and leave at chunk boundaries). Handles submitted to it belong to the
+ * server until their result has been polled: the caller must not use them meanwhile. */
+int fb_server_create(int device, const fb_server_options* options, fb_server** out);
+/* Waits for the requests in flight, then stops the workers. */
+int fb_server_destroy(fb_server* srv);
+/* `steps` fixed substeps of params->dt at `pose` (each the body of fb_sf_substep). */
+int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_params* params, const fb_pose* pose,
+                        int steps);
+/* `steps` lattice steps with fixed parameters and pose. */
+int fb_server_submit_lbm(fb_server* srv, int64_t id, fb_lbm* lbm, const fb_lbm_params* params, int steps);
+/* Up to `max` finished requests, oldest first; *count receives how many were written. */
+int fb_server_poll(fb_server* srv, fb_server_result* out, int max, int* count);
+/* Requests queued (not yet started) and running. */
+int fb_server_load(fb_server* srv, int* queued, int* running);
+int fb_server_clock(fb_server* srv, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

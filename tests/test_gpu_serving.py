@@ -1,0 +1,190 @@
+"""Serving form: batched lattice launches, the request loop, admission, and results that do not
+depend on what else is in flight."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from foilbench_amd import _native as N
+
+    return N, N.Context.default()
+
+
+def _lattice(nx, ny, naca, aoa, re, form, steps=1):
+    from foilbench_amd.core.geometry import NacaFoil
+    from foilbench_amd.serving.mix import LBM, Request
+    from foilbench_amd.solvers.lbm import LBMSolver
+
+    N, ctx = _lib()
+    req = Request(0, 0.0, LBM, nx, ny, naca, aoa, re, steps)
+    sc = req.scenario("float32")
+    s = LBMSolver(form, ctx=ctx)
+    s.initialize(sc, NacaFoil(sc.foil), 0)
+    return s
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("form", ["f32-mixed", "f64"])
+def test_lbm_batch_matches_lattice_alone(form):
+    """One launch per step over three lattices of different sizes gives, per lattice, the same
+    populations as stepping it alone (chunk 8, 37 steps: partial last chunk, odd count)."""
+    N, ctx = _lib()
+    shapes = [(128, 64, "2412", 8.0, 800.0), (96, 48, "0012", -3.0, 150.0), (200, 100, "6318", 15.0, 5000.0)]
+    alone = [_lattice(*s, form) for s in shapes]
+    batch = [_lattice(*s, form) for s in shapes]
+    steps = 37
+    st_alone = []
+    for s in alone:
+        p = s._params(0.0)
+        N.check(ctx.lib.fb_lbm_run_batch((C.c_void_p * 1)(s._handle), C.byref(p), 1, steps, 8, None))
+        st_alone.append(s._stats_now())
+    params = (N.fb_lbm_params * 3)(*[s._params(0.0) for s in batch])
+    handles = (C.c_void_p * 3)(*[s._handle for s in batch])
+    stats = (N.fb_lbm_stats * 3)()
+    N.check(ctx.lib.fb_lbm_run_batch(handles, params, 3, steps, 8, stats))
+    for a, b, st in zip(alone, batch, stats):
+        fa, oa = a.populations()
+        fb_, ob = b.populations()
+        assert np.array_equal(fa, fb_)
+        assert np.array_equal(oa, ob)
+        assert st.nonfinite_populations == 0 and st.max_speed > 0.0
+
+
+def test_lbm_batch_matches_per_step_launches():
+    """The batched path and fb_lbm_run_fixed (one launch per step) agree bit for bit."""
+    N, ctx = _lib()
+    a = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
+    b = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
+    p = a._params(0.0)
+    N.check(ctx.lib.fb_lbm_run_fixed(a._handle, C.byref(p), 20))
+    ctx.synchronize()
+    N.check(ctx.lib.fb_lbm_run_batch((C.c_void_p * 1)(b._handle), C.byref(p), 1, 20, 4, None))
+    fa, _ = a.populations()
+    fb_, _ = b.populations()
+    assert np.array_equal(fa, fb_)
+
+
+def _small_mix():
+    from foilbench_amd.serving.mix import LBM, SF, Request
+
+    return [
+        Request(0, 0.00, SF, 64, 32, "0012", 8.0, 500.0, 6),
+        Request(1, 0.01, LBM, 128, 64, "2412", 12.0, 1000.0, 40),
+        Request(2, 0.02, SF, 128, 64, "4415", 3.0, 2000.0, 5),
+        Request(3, 0.02, LBM, 256, 128, "0009", -4.0, 300.0, 26),
+        Request(4, 0.03, SF, 96, 48, "6218", 15.0, 150.0, 4),
+        Request(5, 0.04, LBM, 64, 32, "3312", 0.0, 8000.0, 18),
+    ]
+
+
+def test_server_results_same_alone_and_in_mix():
+    """Each request's fields are the same served alone as in the mix (deterministic kernels on
+    every stream and batch composition)."""
+    from foilbench_amd.serving.engine import Server
+
+    mix = _small_mix()
+    srv = Server(sf_workers=4, lbm_chunk=8)
+    try:
+        together = srv.run(mix, keep_fields=True, realtime=False)
+        alone = {}
+        for r in mix:
+            alone[r.id] = srv.run([r], keep_fields=True, realtime=False)["outcomes"][0]
+    finally:
+        srv.close()
+    assert together["completed"] == len(mix), together["failed"]
+    for o in together["outcomes"]:
+        a = alone[o.id]
+        assert a.steps == o.steps
+        for k in o.fields:
+            assert np.array_equal(o.fields[k], a.fields[k]), (o.id, k)
+
+
+def test_serving_form_within_tolerance_of_plain_fp64():
+    """Serving form (FP32 storage, FP64 refinement / shifted FP32 populations) against the plain
+    FP64 form on the same requests: relative L2 error per field <= 1e-5."""
+    from foilbench_amd.serving.engine import Server, run_plain
+
+    mix = _small_mix()
+    srv = Server(sf_workers=4, lbm_chunk=8)
+    try:
+        served = srv.run(mix, keep_fields=True, realtime=False)
+    finally:
+        srv.close()
+    plain = run_plain(mix, keep_fields=True)
+    assert plain["completed"] == len(mix)
+    errs = {}
+    for o, p in zip(served["outcomes"], plain["outcomes"]):
+        for k in o.fields:
+            if k == "rho":
+                errs[(o.id, k)] = _rel(o.fields[k] - 1.0, p.fields[k] - 1.0)
+            else:
+                errs[(o.id, k)] = _rel(o.fields[k], p.fields[k])
+    worst = max(errs.values())
+    assert worst <= 1e-5, errs
+
+
+def test_memory_admission_bounds_device_state():
+    """A budget that holds one request at a time serialises admission; every request completes."""
+    from foilbench_amd.serving.engine import Server, estimate_bytes
+
+    mix = _small_mix()
+    budget = max(estimate_bytes(r, "serving") for r in mix)
+    srv = Server(sf_workers=2, lbm_chunk=4, memory_budget=budget)
+    try:
+        res = srv.run(mix, realtime=False)
+    finally:
+        srv.close()
+    assert res["completed"] == len(mix)
+    est = {r.id: estimate_bytes(r, "serving") for r in mix}
+    outs = res["outcomes"]
+    for o in outs:
+        # requests holding device state at the instant o was admitted
+        active = [p for p in outs if p.admitted_ms <= o.admitted_ms < p.finished_ms]
+        assert len(active) == 1 or sum(est[p.id] for p in active) <= budget
+
+
+def test_server_reports_failures_per_request():
+    """A request whose substep fails (a CFL limit no flow meets) ends with its status; the requests
+    beside it finish normally, and an invalid submission is refused at once."""
+    import time
+
+    from foilbench_amd.serving.engine import Server, _build, _close
+    from foilbench_amd.serving.mix import LBM, SF, Request
+
+    N, ctx = _lib()
+    reqs = [Request(0, 0.0, SF, 64, 32, "0012", 8.0, 500.0, 4), Request(1, 0.0, SF, 64, 32, "2412", 4.0, 300.0, 4),
+            Request(2, 0.0, LBM, 64, 32, "0012", 5.0, 400.0, 10)]
+    srv = Server(sf_workers=2, lbm_chunk=4)
+    built = {}
+    try:
+        for r in reqs:
+            solver, handle, params, pose = _build(r, "serving", ctx)
+            if r.id == 0:
+                params.cfl_limit = 1e-9
+            built[r.id] = solver
+            srv.submit(r, handle, params, pose)
+        bad = N.fb_sf_params()
+        assert ctx.lib.fb_server_submit_sf(srv.handle, 9, None, C.byref(bad), None, 3) == 1  # FB_ERR_ARG
+        got = {}
+        t0 = time.perf_counter()
+        while len(got) < len(reqs) and time.perf_counter() - t0 < 60:
+            for res in srv.poll():
+                got[res.id] = (res.status, res.sf_status, res.steps_done)
+            time.sleep(0.001)
+    finally:
+        srv.close()
+        for s_ in built.values():
+            _close(s_)
+    assert got[0][0] == 0 and got[0][1] == 3 and got[0][2] == 0  # projection CFL above its limit
+    assert got[1] == (0, 0, 4)
+    assert got[2] == (0, 0, 10)
