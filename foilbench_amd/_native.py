@@ -127,7 +127,8 @@ class fb_pic_stats(C.Structure):
 
 
 class fb_server_options(C.Structure):
-    _fields_ = [("sf_workers", C.c_int32), ("lbm_chunk", C.c_int32), ("use_graphs", C.c_int32)]
+    _fields_ = [("sf_workers", C.c_int32), ("lbm_chunk", C.c_int32), ("use_graphs", C.c_int32),
+                ("sf_batch", C.c_int32), ("sf_gather_us", C.c_double)]
 
 
 FB_SERVE_SF = 0

@@ -37,6 +37,18 @@ inline hipError_t host_free(void* p) {
   std::lock_guard<std::recursive_mutex> g(hip_lock());
   return hipHostFree(p);
 }
+inline hipError_t graph_exec_free(hipGraphExec_t e) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipGraphExecDestroy(e);
+}
+inline hipError_t stream_create(hipStream_t* s) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+inline hipError_t stream_free(hipStream_t s) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  return hipStreamDestroy(s);
+}
 
 // Thread-local last error message returned by fb_last_error().
 void set_error(const char* fmt, ...);
@@ -84,6 +96,9 @@ using fb::dev_alloc;
 using fb::dev_free;
 using fb::host_alloc;
 using fb::host_free;
+using fb::graph_exec_free;
+using fb::stream_create;
+using fb::stream_free;
 
 #define FB_HIP(call)                                                                   \
   do {                                                                                 \

@@ -1468,7 +1468,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
     }
     B.h_prev.clear();
     if (B.exec) {
-      (void)hipGraphExecDestroy(B.exec);
+      (void)graph_exec_free(B.exec);
       B.exec = nullptr;
     }
   }
@@ -1599,7 +1599,7 @@ void lbm_batch_destroy(LbmBatch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->ctx->stream);
   for (auto& mb : b->m) mb.L->ctx = mb.home;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->exec) (void)graph_exec_free(b->exec);
   if (b->d_args) (void)dev_free(b->d_args);
   if (b->d_start) (void)dev_free(b->d_start);
   delete b;
@@ -1612,7 +1612,6 @@ int lbm_batch_add(LbmBatch* b, fb_lbm* lattice, const fb_lbm_params& p, int step
   FB_REQUIRE(L->prec == b->prec, FB_ERR_ARG, "lattice precision %d differs from the batch's %d", L->prec, b->prec);
   FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
   for (auto& mb : b->m) FB_REQUIRE(mb.L != L, FB_ERR_STATE, "lattice is already in the batch");
-  FB_HIP(hipStreamSynchronize(L->ctx->stream));
   LbmMember mb;
   mb.L = L;
   mb.p = p;
@@ -1633,6 +1632,9 @@ int lbm_batch_add(LbmBatch* b, fb_lbm* lattice, const fb_lbm_params& p, int step
 
 int lbm_batch_members(const LbmBatch* b) { return b ? (int)b->m.size() : 0; }
 int lbm_handle_precision(const fb_lbm* lattice) { return reinterpret_cast<const Lattice*>(lattice)->prec; }
+hipError_t lbm_handle_sync(fb_lbm* lattice) {
+  return hipStreamSynchronize(reinterpret_cast<Lattice*>(lattice)->ctx->stream);
+}
 int lbm_batch_prec(const LbmBatch* b) { return b ? b->prec : -1; }
 int64_t lbm_batch_cells(const LbmBatch* b) {
   int64_t c = 0;
@@ -1860,6 +1862,7 @@ int fb_lbm_run_batch(fb_lbm* const* lattices, const fb_lbm_params* params, int n
   for (int k = 0; k < n; ++k) FB_REQUIRE(lattices[k] != nullptr, FB_ERR_ARG, "lattice %d is NULL", k);
   auto* L0 = reinterpret_cast<Lattice*>(lattices[0]);
   FB_HIP(hipSetDevice(L0->ctx->device));
+  for (int k = 0; k < n; ++k) FB_HIP(fb::lbm_handle_sync(lattices[k]));
   fb::LbmBatch* b = nullptr;
   int rc = fb::lbm_batch_create(L0->ctx, L0->prec, chunk, 1, &b);
   if (rc) return rc;

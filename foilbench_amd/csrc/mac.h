@@ -37,8 +37,18 @@ inline MacGeom make_geom(const fb_domain& d) {
 
 #pragma clang fp contract(fast)
 
+struct MacGrid;
+
+// Replacement for the grid's own pressure solve (the request loop batches solves of several grids):
+// solves the grid's rhs into its pres, with the report complete on return.
+struct SolveHook {
+  int (*fn)(void* user, MacGrid& M, double tolerance, int max_iterations, bool mixed, PoissonReport* rep) = nullptr;
+  void* user = nullptr;
+};
+
 struct MacGrid {
   Ctx* ctx = nullptr;
+  SolveHook hook{};
   fb_domain dom{};
   fb_foil foil{};
   MacGeom g{};
@@ -92,5 +102,7 @@ int mac_substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_re
 // Moves M onto context c: its kernels, copies and reduction slots then use c's stream. The
 // caller makes sure no work of M is pending on the previous stream.
 void mac_bind(MacGrid& M, Ctx* c);
+// Whether the pressure solve replays captured graphs for its CG iteration chunks.
+void mac_use_graphs(MacGrid& M, bool on);
 
 }  // namespace fb

@@ -653,7 +653,10 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
                      M.rhs);
   FB_HIP(hipGetLastError());
   PoissonReport pr;
-  if ((rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr))) return rc;
+  const bool hooked = M.hook.fn != nullptr;
+  if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr);
+  else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr);
+  if (rc) return rc;
   hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
   hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
   hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
@@ -663,7 +666,7 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   if ((rc = launch_stats<T>(M, p.wall_omega_rad, 8))) return rc;
   FB_HIP(hipMemcpyAsync(M.host_keys, M.keys, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   FB_HIP(hipStreamSynchronize(s));
-  M.poisson.finish(&pr);
+  if (!hooked) M.poisson.finish(&pr);
   fb_mac_stats pre{}, post{};
   stats_from_host(M, 0, &pre);
   stats_from_host(M, 8, &post);
@@ -813,6 +816,8 @@ void mac_bind(MacGrid& M, Ctx* c) {
   M.ctx = c;
   M.poisson.stream = c->stream;
 }
+
+void mac_use_graphs(MacGrid& M, bool on) { M.poisson.set_use_graphs(on); }
 
 }  // namespace fb
 

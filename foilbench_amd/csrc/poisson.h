@@ -41,25 +41,49 @@ struct CgVecs {
   int tri_Llast = 0, tri_P = 0;
 };
 
+// Most grids one batched solve handles.
+constexpr int kMaxBatch = 64;
+
 class Poisson {
  public:
   ~Poisson();
-  // Geometry from the solid mask (ny*nx bytes on the device). Periodic axes select Jacobi.
+  // Geometry from the solid mask (ny*nx bytes on the device; NULL keeps the current one).
+  // Periodic axes select Jacobi. `capacity` > 1 sizes the solver for batched solves of that
+  // many grids of this shape (non-periodic only), their masks loaded with load_slot().
   int setup(hipStream_t s, int nx, int ny, double dx, double dy, int periodic_x, int periodic_y,
-            const uint8_t* d_solid);
+            const uint8_t* d_solid, int capacity = 1);
+  // Copies the mask-derived operator of a single-grid solver of the same shape into slot k.
+  int load_slot(int k, const Poisson& src);
   // Solve A x = b (b FP64 on device, ny*nx). `mixed`: FP32 PCG sweeps inside an FP64
   // residual-refinement loop to the same relative tolerance.
   int solve(const double* d_rhs, double* d_x, double tolerance, int max_iterations, bool mixed,
             PoissonReport* rep);
+  // nb independent solves (grid g uses slot g) sharing every launch; each grid follows the
+  // same sequence as solve() and stops on its own. Synchronous; reports are complete.
+  int solve_batch(int nb, const double* const* rhs, double* const* x, double tolerance, int max_iterations,
+                  bool mixed, PoissonReport* reps);
   // Completes `rep` (the relative residual) once the stream has been synchronised after solve():
   // solve() leaves its last scalars in flight to host memory.
-  void finish(PoissonReport* rep) const;
+  void finish(PoissonReport* rep, int g = 0) const;
+  void set_use_graphs(bool on) { use_graphs_ = on; }
+  // Process-wide unique tag of the operator, renewed whenever setup() rebuilds it from a mask (a
+  // new solver at a recycled address gets a new tag).
+  unsigned long long version = 0;
 
   hipStream_t stream = nullptr;
   int nx = 0, ny = 0, periodic_x = 0, periodic_y = 0;
   double dx = 0, dy = 0;
-  size_t N = 0;
+  size_t N = 0;      // cells of one grid
   int nblocks = 0;
+  int cap = 1;       // grids the buffers hold
+  int nb_ = 1;       // grids of the current solve
+  int ps = 0;        // partial-sum slots per grid
+  double* bstage = nullptr;  // batched right-hand sides and solutions (cap > 1)
+  double* xstage = nullptr;
+  double* dpar = nullptr;    // per-grid parameters of a launch (cap), staged through host_par
+  double* host_par = nullptr;  // kParSlots x cap pinned staging slots
+  static constexpr int kParSlots = 16;
+  unsigned par_slot_ = 0;
   uint8_t* flags = nullptr;
   double* diag = nullptr;
   double* ex = nullptr;
@@ -79,8 +103,10 @@ class Poisson {
   template <class T>
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>
-  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool check_rhs, int first_chunk, int* iterations,
-          int* info, int* failure);
+  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, const int* maxit, bool check_rhs, int first_chunk,
+          int* iterations, int* info, int* failure);
+  int run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps);
+  int put_par(const double* v);
   template <class T>
   int iterations_chunk(CgVecs<T>& v, T* x, int n, int parity);
   template <class T>
@@ -90,8 +116,10 @@ class Poisson {
   struct GraphKey {
     size_t elem;
     const void* x;
-    int n, parity;
-    bool operator==(const GraphKey& o) const { return elem == o.elem && x == o.x && n == o.n && parity == o.parity; }
+    int n, parity, grids;
+    bool operator==(const GraphKey& o) const {
+      return elem == o.elem && x == o.x && n == o.n && parity == o.parity && grids == o.grids;
+    }
   };
   struct CachedGraph {
     GraphKey key;

@@ -17,6 +17,8 @@
 #include <cmath>
 #include <complex>
 
+#include <atomic>
+
 #include "poisson.h"
 
 namespace fb {
@@ -153,6 +155,10 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 
 #define CG_DONE(scal) ((scal)[S_DONE] != 0.0)
 
+// Several grids of one shape can share a launch: blockIdx.z selects the grid, whose vectors sit
+// `gs` values apart, its partials `ps` apart and its scalars kScalStride apart.
+constexpr int kScalStride = S_COUNT + 8;
+
 // Sum `n` partials in fixed order into scal[slot]; optional post-processing by `mode`.
 // mode 0: plain dot -> scal[S_DOT]
 // mode 1: rho = sum; beta = rho / rho_prev (iteration > 0)
@@ -160,8 +166,10 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 // mode 3: rn2 = sum; rho_prev = rho; iteration += 1; then the top-of-loop check
 // mode 4: rn2 = sum (initial) and the top-of-loop check
 __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restrict__ partial, int n, double* scal,
-                                                          int mode) {
+                                                          int mode, int ps) {
   __shared__ double lds[4];
+  partial += (size_t)blockIdx.z * ps;
+  scal += (size_t)blockIdx.z * kScalStride;
   if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
@@ -222,8 +230,11 @@ __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restri
 
 template <class T>
 __global__ void __launch_bounds__(256) dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, size_t n,
-                                                          double* __restrict__ partial) {
+                                                          double* __restrict__ partial, int ps) {
   __shared__ double lds[4];
+  a += (size_t)blockIdx.z * n;
+  b += (size_t)blockIdx.z * n;
+  partial += (size_t)blockIdx.z * ps;
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   double v = i < n ? (double)a[i] * (double)b[i] : 0.0;
   if constexpr (sizeof(T) == 4) v = i < n ? (double)(a[i] * b[i]) : 0.0;
@@ -262,13 +273,24 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
                                                     const double* __restrict__ diag, const T* __restrict__ z,
                                                     const T* __restrict__ pold, T* __restrict__ pnew,
                                                     T* __restrict__ q, double* __restrict__ partial,
-                                                    const double* __restrict__ scal, T ix2, T iy2) {
+                                                    const double* __restrict__ scal, T ix2, T iy2, int ps) {
   __shared__ double lds[4];
+  const size_t N = (size_t)nx * ny;
+  {
+    const size_t go = (size_t)blockIdx.z * N;
+    fl += go;
+    diag += go;
+    z += go;
+    pold += go;
+    pnew += go;
+    q += go;
+    partial += (size_t)blockIdx.z * ps;
+    scal += (size_t)blockIdx.z * kScalStride;
+  }
   if (CG_DONE(scal)) return;
   const bool first = scal[S_ITER] == 0.0;
   const T beta = (T)scal[S_BETA];
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const size_t N = (size_t)nx * ny;
   double v = 0.0;
   if (i < N) {
     int x = (int)(i % nx), y = (int)(i / nx);
@@ -290,8 +312,17 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
 template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t N, T* __restrict__ x, T* __restrict__ r,
                                                         const T* __restrict__ p, const T* __restrict__ q,
-                                                        double* __restrict__ partial, double* scal) {
+                                                        double* __restrict__ partial, double* scal, int ps) {
   __shared__ double lds[4];
+  {
+    const size_t go = (size_t)blockIdx.z * N;
+    x += go;
+    r += go;
+    p += go;
+    q += go;
+    partial += (size_t)blockIdx.z * ps;
+    scal += (size_t)blockIdx.z * kScalStride;
+  }
   if (CG_DONE(scal)) return;
   const T alpha = (T)scal[S_ALPHA];
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -434,8 +465,17 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
                                                        const cplx<T>* __restrict__ fbt,
                                                        const cplx<T>* __restrict__ roots, int M, int logM, int rows,
                                                        T scale, int mode, double* __restrict__ partial,
-                                                       const double* scal) {
+                                                       const double* scal, int ps) {
   extern __shared__ unsigned char smem_raw[];
+  {
+    const size_t go = (size_t)blockIdx.z * nx * ny;
+    in += go;
+    fl += go;
+    rsrc += go;
+    out += go;
+    partial += (size_t)blockIdx.z * ps;
+    if (scal) scal += (size_t)blockIdx.z * kScalStride;
+  }
   if (scal && CG_DONE(scal)) return;
   const int nthr = M >> 4;
   const int lr = threadIdx.x / nthr;  // row within the block
@@ -511,6 +551,10 @@ template <class T>
 __global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __restrict__ w,
                                                         const T* __restrict__ fac, int Llast, int P, T a,
                                                         T* __restrict__ yF, T* __restrict__ yE, const double* scal) {
+  w += (size_t)blockIdx.z * nx * ny;
+  yF += (size_t)blockIdx.z * P * nx;
+  yE += (size_t)blockIdx.z * P * nx;
+  if (scal) scal += (size_t)blockIdx.z * kScalStride;
   if (scal && CG_DONE(scal)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int c = blockIdx.y * 4 + threadIdx.y;
@@ -555,6 +599,9 @@ template <class T>
 __global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const T* __restrict__ itf,
                                                             T* __restrict__ yF, T* __restrict__ yE,
                                                             const double* scal) {
+  yF += (size_t)blockIdx.z * P * nx;
+  yE += (size_t)blockIdx.z * P * nx;
+  if (scal) scal += (size_t)blockIdx.z * kScalStride;
   if (scal && CG_DONE(scal)) return;
   __shared__ T sA[16][16], sB[16][16], sS[16][16];
   const int m = threadIdx.x, gseg = threadIdx.y;
@@ -639,6 +686,10 @@ template <class T>
 __global__ void __launch_bounds__(256) tri_spike_kernel(int nx, int ny, T* __restrict__ w, const T* __restrict__ fac,
                                                         int Llast, int P, const T* __restrict__ F,
                                                         const T* __restrict__ E, const double* scal) {
+  w += (size_t)blockIdx.z * nx * ny;
+  F += (size_t)blockIdx.z * P * nx;
+  E += (size_t)blockIdx.z * P * nx;
+  if (scal) scal += (size_t)blockIdx.z * kScalStride;
   if (scal && CG_DONE(scal)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int c = blockIdx.y * 4 + threadIdx.y;
@@ -704,8 +755,17 @@ __global__ void __launch_bounds__(256) setup_kernel(int nx, int ny, const uint8_
 __global__ void __launch_bounds__(256) residual_kernel(int nx, int ny, const uint8_t* __restrict__ fl,
                                                        const double* __restrict__ diag, const double* __restrict__ b,
                                                        const double* __restrict__ x, double* __restrict__ r,
-                                                       double* __restrict__ partial, double ix2, double iy2) {
+                                                       double* __restrict__ partial, double ix2, double iy2, int ps) {
   __shared__ double lds[4];
+  {
+    const size_t go = (size_t)blockIdx.z * nx * ny;
+    fl += go;
+    diag += go;
+    b += go;
+    x += go;
+    r += go;
+    partial += (size_t)blockIdx.z * ps;
+  }
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   double v = 0.0;
   if (i < (size_t)nx * ny) {
@@ -725,8 +785,17 @@ __global__ void __launch_bounds__(256) residual_kernel(int nx, int ny, const uin
 __global__ void __launch_bounds__(256) rhs_prep_kernel(size_t N, const double* __restrict__ rhs,
                                                        const uint8_t* __restrict__ fl, double* __restrict__ b,
                                                        double* __restrict__ partial, double* __restrict__ partial_n,
-                                                       unsigned long long* __restrict__ stat) {
+                                                       unsigned long long* __restrict__ stat, int ps) {
   __shared__ double lds[4];
+  {
+    const size_t go = (size_t)blockIdx.z * N;
+    rhs += go;
+    fl += go;
+    b += go;
+    partial += (size_t)blockIdx.z * ps;
+    partial_n += (size_t)blockIdx.z * ps;
+    stat += (size_t)blockIdx.z * kScalStride;  // the slots sit in the grid's scalar block
+  }
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   double v = 0.0, c = 0.0, m = 0.0;
   bool nf = false;
@@ -773,8 +842,10 @@ __global__ void __launch_bounds__(256) sub_mean_dev_kernel(size_t N, const uint8
 
 // Deterministic sum of n block partials into scal[slot].
 __global__ void __launch_bounds__(256) sum_slot_kernel(const double* __restrict__ partial, int n, double* scal,
-                                                       int slot) {
+                                                       int slot, int ps) {
   __shared__ double lds[4];
+  partial += (size_t)blockIdx.z * ps;
+  scal += (size_t)blockIdx.z * kScalStride;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
   s = block_sum<256>(s, lds);
@@ -784,6 +855,7 @@ __global__ void __launch_bounds__(256) sum_slot_kernel(const double* __restrict_
 // Non-finite or out-of-range right-hand side: stop the CG before its first iteration.
 __global__ void rhs_check_kernel(double* scal, double safe) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  scal += (size_t)blockIdx.z * kScalStride;
   const unsigned long long* st = reinterpret_cast<const unsigned long long*>(scal + S_COUNT);
   if (st[0] != 0ull) {
     scal[S_FAIL] = 1.0;
@@ -811,14 +883,39 @@ __global__ void __launch_bounds__(256) fill_zero(size_t N, double* __restrict__ 
   if (i < N) a[i] = 0.0;
 }
 
+// Per-grid scaled copies of the refinement loop: grid blockIdx.z uses scale s[blockIdx.z].
+__global__ void __launch_bounds__(256) to_f32_scaled_g(size_t N, const double* __restrict__ a, float* __restrict__ b,
+                                                       const double* __restrict__ s) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t o = (size_t)blockIdx.z * N + i;
+  if (i < N) b[o] = (float)(a[o] * s[blockIdx.z]);
+}
+__global__ void __launch_bounds__(256) add_f32_scaled_g(size_t N, double* __restrict__ x, const float* __restrict__ e,
+                                                        const double* __restrict__ s) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t o = (size_t)blockIdx.z * N + i;
+  if (i < N) x[o] += (double)e[o] * s[blockIdx.z];
+}
+
+// CG scalars of grid blockIdx.z: atol <- rtol (scaled by ||b|| later), maxiter <- maxit[g];
+// a grid with maxit 0 ends before its first iteration.
+__global__ void cg_init_kernel(double* scal, double rtol, const double* __restrict__ maxit) {
+  if (threadIdx.x >= S_COUNT) return;
+  double* sc = scal + (size_t)blockIdx.z * kScalStride;
+  const int k = threadIdx.x;
+  sc[k] = k == S_ATOL ? rtol : (k == S_MAXIT ? maxit[blockIdx.z] : 0.0);
+}
+
 // ---------------------------------------------------------------- host driver
 
 Poisson::~Poisson() {
   for (auto& g : graphs_)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  for (void* ptr : {(void*)flags, (void*)diag, (void*)ex, (void*)partial, (void*)scal, (void*)b64, (void*)r64})
+    if (g.exec) (void)graph_exec_free(g.exec);
+  for (void* ptr : {(void*)flags, (void*)diag, (void*)ex, (void*)partial, (void*)scal, (void*)b64, (void*)r64,
+                    (void*)bstage, (void*)xstage, (void*)dpar})
     if (ptr) (void)dev_free(ptr);
   if (host_scal) (void)host_free(host_scal);
+  if (host_par) (void)host_free(host_par);
   auto freev = [](auto& v) {
     for (void* ptr : {(void*)v.r, (void*)v.z, (void*)v.p, (void*)v.q, (void*)v.x, (void*)v.b, (void*)v.work,
                       (void*)v.scratch, (void*)v.tri, (void*)v.tri_itf, (void*)v.tri_yF, (void*)v.tri_yE})
@@ -832,8 +929,9 @@ Poisson::~Poisson() {
 template <class T>
 int Poisson::alloc(CgVecs<T>& v) {
   if (v.r) return FB_OK;
-  for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(dev_alloc((void**)a, N * sizeof(T)));
-  FB_HIP(dev_alloc((void**)&v.p, 2 * N * sizeof(T)));
+  const size_t n = (size_t)cap * N;
+  for (T** a : {&v.r, &v.z, &v.q, &v.x, &v.b, &v.work, &v.scratch}) FB_HIP(dev_alloc((void**)a, n * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.p, 2 * n * sizeof(T)));
   if (!periodic_x && !periodic_y) {
     int rc = make_plan<T>(v.plan, nx, stream);
     if (rc) return rc;
@@ -910,21 +1008,25 @@ int Poisson::make_tri(CgVecs<T>& v) {
   }
   FB_HIP(dev_alloc((void**)&v.tri, fac.size() * sizeof(T)));
   FB_HIP(dev_alloc((void**)&v.tri_itf, itf.size() * sizeof(T)));
-  FB_HIP(dev_alloc((void**)&v.tri_yF, (size_t)P * nx * sizeof(T)));
-  FB_HIP(dev_alloc((void**)&v.tri_yE, (size_t)P * nx * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.tri_yF, (size_t)cap * P * nx * sizeof(T)));
+  FB_HIP(dev_alloc((void**)&v.tri_yE, (size_t)cap * P * nx * sizeof(T)));
   FB_HIP(hipMemcpyAsync(v.tri, fac.data(), fac.size() * sizeof(T), hipMemcpyHostToDevice, stream));
   FB_HIP(hipMemcpyAsync(v.tri_itf, itf.data(), itf.size() * sizeof(T), hipMemcpyHostToDevice, stream));
   FB_HIP(hipStreamSynchronize(stream));
   return FB_OK;
 }
 
-int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int px, int py, const uint8_t* d_solid) {
+int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int px, int py, const uint8_t* d_solid,
+                   int capacity) {
   stream = s;
   if (flags == nullptr || nx_ != nx || ny_ != ny || px != periodic_x || py != periodic_y) {
     if (flags) {
       set_error("Poisson solver cannot change shape after set-up");
       return FB_ERR_STATE;
     }
+    FB_REQUIRE(capacity >= 1 && (capacity == 1 || (!px && !py)), FB_ERR_ARG,
+               "batched Poisson solves need a non-periodic box");
+    cap = capacity;
     nx = nx_;
     ny = ny_;
     dx = dx_;
@@ -933,13 +1035,20 @@ int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int 
     periodic_y = py;
     N = (size_t)nx * ny;
     nblocks = (int)((N + 255) / 256);
-    FB_HIP(dev_alloc((void**)&flags, N));
-    FB_HIP(dev_alloc((void**)&diag, N * sizeof(double)));
-    FB_HIP(dev_alloc((void**)&partial, 2 * (size_t)(nblocks + ny + 64) * sizeof(double)));
-    FB_HIP(dev_alloc((void**)&scal, (S_COUNT + 8) * sizeof(double)));
-    FB_HIP(host_alloc((void**)&host_scal, (S_COUNT + 8) * sizeof(double)));
-    FB_HIP(dev_alloc((void**)&b64, N * sizeof(double)));
-    FB_HIP(dev_alloc((void**)&r64, N * sizeof(double)));
+    ps = 2 * (nblocks + ny + 64);
+    FB_HIP(dev_alloc((void**)&flags, (size_t)cap * N));
+    FB_HIP(dev_alloc((void**)&diag, (size_t)cap * N * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&partial, (size_t)cap * ps * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&scal, (size_t)cap * kScalStride * sizeof(double)));
+    FB_HIP(host_alloc((void**)&host_scal, (size_t)cap * kScalStride * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&dpar, (size_t)cap * sizeof(double)));
+    FB_HIP(host_alloc((void**)&host_par, (size_t)kParSlots * cap * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&b64, (size_t)cap * N * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&r64, (size_t)cap * N * sizeof(double)));
+    if (cap > 1) {
+      FB_HIP(dev_alloc((void**)&bstage, (size_t)cap * N * sizeof(double)));
+      FB_HIP(dev_alloc((void**)&xstage, (size_t)cap * N * sizeof(double)));
+    }
     FB_HIP(dev_alloc((void**)&ex, nx * sizeof(double)));
     std::vector<double> exh(nx);
     for (int i = 0; i < nx; ++i)
@@ -949,21 +1058,43 @@ int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int 
     int rc = alloc(v64);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(setup_kernel, dim3(nblocks), dim3(256), 0, stream, nx, ny, d_solid, px, py, 1.0 / (dx * dx),
-                     1.0 / (dy * dy), flags, diag);
-  FB_HIP(hipGetLastError());
+  if (d_solid) {
+    hipLaunchKernelGGL(setup_kernel, dim3(nblocks), dim3(256), 0, stream, nx, ny, d_solid, px, py, 1.0 / (dx * dx),
+                       1.0 / (dy * dy), flags, diag);
+    FB_HIP(hipGetLastError());
+    static std::atomic<unsigned long long> tags{0};
+    version = ++tags;
+  }
+  return FB_OK;
+}
+
+int Poisson::load_slot(int k, const Poisson& src) {
+  FB_REQUIRE(k >= 0 && k < cap && src.nx == nx && src.ny == ny && src.flags, FB_ERR_ARG,
+             "slot %d does not fit a %dx%d grid", k, src.nx, src.ny);
+  FB_HIP(hipMemcpyAsync(flags + (size_t)k * N, src.flags, N, hipMemcpyDeviceToDevice, stream));
+  FB_HIP(hipMemcpyAsync(diag + (size_t)k * N, src.diag, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
   return FB_OK;
 }
 
 int Poisson::reduce(const double* pin, int n, int mode) {
-  hipLaunchKernelGGL(cg_finalize_kernel, dim3(1), dim3(256), 0, stream, pin, n, scal, mode);
+  hipLaunchKernelGGL(cg_finalize_kernel, dim3(1, 1, nb_), dim3(256), 0, stream, pin, n, scal, mode, ps);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
 
 int Poisson::read_scalars() {
-  FB_HIP(hipMemcpyAsync(host_scal, scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FB_HIP(hipMemcpyAsync(host_scal, scal, ((size_t)(nb_ - 1) * kScalStride + S_COUNT) * sizeof(double),
+                        hipMemcpyDeviceToHost, stream));
   FB_HIP(hipStreamSynchronize(stream));
+  return FB_OK;
+}
+
+// Per-grid launch parameters to dpar. The copy reads pinned memory when the stream reaches it,
+// so successive calls between synchronisations use successive host slots of a ring.
+int Poisson::put_par(const double* v) {
+  double* h = host_par + (size_t)(par_slot_++ % kParSlots) * cap;
+  for (int g = 0; g < nb_; ++g) h[g] = v[g];
+  FB_HIP(hipMemcpyAsync(dpar, h, nb_ * sizeof(double), hipMemcpyHostToDevice, stream));
   return FB_OK;
 }
 
@@ -978,73 +1109,86 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
   const DstPlan& P = *v.plan;
   const int nthr = P.M / 16;
   // several short rows per block, but never fewer blocks than about two per CU
-  const int rows = std::max(1, std::min(256 / nthr, ny / 512));
-  const unsigned blocks = (unsigned)((ny + rows - 1) / rows);
+  const int rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
+  const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * nthr * sizeof(double);
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(blocks), dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin,
-                     v.work, (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
-                     (T)P.scale, 0, partial, scal);
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dg, dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin, v.work,
+                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
+                     (T)P.scale, 0, partial, scal, ps);
   FB_HIP(hipGetLastError());
-  const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4);
+  const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
   const T a = (T)(-(1.0 / (dy * dy)));
   hipLaunchKernelGGL(tri_chunk_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a,
                      v.tri_yF, v.tri_yE, scal);
-  hipLaunchKernelGGL(tri_interface_kernel<T>, dim3((nx + 15) / 16), dim3(16, 16), 0, stream, nx, v.tri_P,
+  hipLaunchKernelGGL(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, stream, nx, v.tri_P,
                      (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
   hipLaunchKernelGGL(tri_spike_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P,
                      (const T*)v.tri_yF, (const T*)v.tri_yE, scal);
   FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dim3(blocks), dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin,
-                     zout, (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
-                     (T)P.scale, 1, partial, scal);
+  hipLaunchKernelGGL(dst_rows_kernel<T>, dg, dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin, zout,
+                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
+                     (T)P.scale, 1, partial, scal, ps);
   FB_HIP(hipGetLastError());
   return reduce(partial, ny, 1);
 }
 
 template <class T>
-int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, int maxit, bool check_rhs, int first_chunk,
+int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, const int* maxit, bool check_rhs, int first_chunk,
                  int* iterations, int* info, int* failure) {
-  FB_HIP(hipMemsetAsync(x, 0, N * sizeof(T), stream));
-  FB_HIP(hipMemcpyAsync(v.r, b, N * sizeof(T), hipMemcpyDeviceToDevice, stream));
-  double init[S_COUNT] = {};
-  init[S_ATOL] = rtol;
-  init[S_MAXIT] = (double)maxit;
-  FB_HIP(hipMemcpyAsync(scal, init, sizeof(init), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks), dim3(256), 0, stream, v.r, v.r, N, partial);
+  const size_t n = (size_t)nb_ * N;
+  FB_HIP(hipMemsetAsync(x, 0, n * sizeof(T), stream));
+  FB_HIP(hipMemcpyAsync(v.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+  int maxall = 0;
+  {
+    double mi[kMaxBatch];
+    for (int g = 0; g < nb_; ++g) {
+      mi[g] = (double)maxit[g];
+      maxall = std::max(maxall, maxit[g]);
+    }
+    int rc = put_par(mi);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, rtol, (const double*)dpar);
+  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks, 1, nb_), dim3(256), 0, stream, v.r, v.r, N, partial, ps);
   FB_HIP(hipGetLastError());
   int rc = reduce(partial, nblocks, 4);
   if (rc) return rc;
   if (check_rhs) {
     // rejections of fb/core/_scipy_adapter.py:113-120, decided on the device
     const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
-    hipLaunchKernelGGL(rhs_check_kernel, dim3(1), dim3(64), 0, stream, scal, safe);
+    hipLaunchKernelGGL(rhs_check_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, safe);
     FB_HIP(hipGetLastError());
   }
   int launched = 0;
   // the first poll comes after about as many iterations as the previous solve needed
   int chunk = std::min(std::max(first_chunk, 4), 64);
   for (;;) {
-    const int n = std::min(chunk, maxit - launched);
-    if ((rc = iterations_chunk(v, x, n, launched & 1))) return rc;
-    launched += n;
+    const int nn = std::min(chunk, maxall - launched);
+    if (nn > 0 && (rc = iterations_chunk(v, x, nn, launched & 1))) return rc;
+    launched += std::max(nn, 0);
     if ((rc = read_scalars())) return rc;
-    if (host_scal[S_DONE] != 0.0 || launched >= maxit) break;
+    bool all = true;
+    for (int g = 0; g < nb_; ++g) all = all && host_scal[(size_t)g * kScalStride + S_DONE] != 0.0;
+    if (all || launched >= maxall) break;
     chunk = std::min(2 * chunk, 32);
   }
-  *iterations = (int)host_scal[S_ITER];
-  *info = (host_scal[S_DONE] != 0.0) ? (int)host_scal[S_INFO] : maxit;
-  *failure = (int)host_scal[S_FAIL];
+  for (int g = 0; g < nb_; ++g) {
+    const double* h = host_scal + (size_t)g * kScalStride;
+    iterations[g] = (int)h[S_ITER];
+    info[g] = (h[S_DONE] != 0.0) ? (int)h[S_INFO] : maxit[g];
+    failure[g] = (int)h[S_FAIL];
+  }
   return FB_OK;
 }
 
-// `n` CG iterations starting with the search-direction buffers in order `parity`. The
+// A chunk of n CG iterations (with the preconditioner). Between host polls the
 // iterations only read and write device memory, so a chunk is captured once as a HIP graph
-// per (precision, solution buffer, length, parity) and replayed; every kernel returns at once
-// when the device-side done flag is set.
+// per (precision, solution buffer, length, parity, grids) and replayed; every kernel returns at
+// once when its grid's device-side done flag is set.
 template <class T>
 int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
   if (n <= 0) return FB_OK;
-  const GraphKey key{sizeof(T), (const void*)x, n, parity};
+  const GraphKey key{sizeof(T), (const void*)x, n, parity, nb_};
   for (auto& g : graphs_)
     if (g.key == key) {
       FB_HIP(hipGraphLaunch(g.exec, stream));
@@ -1079,16 +1223,18 @@ int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
 template <class T>
 int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
   const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
-  T* pa = parity ? v.p + N : v.p;
-  T* pb = parity ? v.p : v.p + N;
+  const size_t pn = (size_t)cap * N;
+  T* pa = parity ? v.p + pn : v.p;
+  T* pb = parity ? v.p : v.p + pn;
+  const dim3 grid(nblocks, 1, nb_);
   int rc;
   for (int c = 0; c < n; ++c) {
     if ((rc = precond(v, v.r, v.z))) return rc;
-    hipLaunchKernelGGL(cg_pq_kernel<T>, dim3(nblocks), dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q,
-                       partial, scal, ix2, iy2);
+    hipLaunchKernelGGL(cg_pq_kernel<T>, grid, dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q, partial,
+                       scal, ix2, iy2, ps);
     FB_HIP(hipGetLastError());
     if ((rc = reduce(partial, nblocks, 2))) return rc;
-    hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal);
+    hipLaunchKernelGGL(cg_update_kernel<T>, grid, dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal, ps);
     FB_HIP(hipGetLastError());
     if ((rc = reduce(partial, nblocks, 3))) return rc;
     std::swap(pa, pb);
@@ -1097,18 +1243,49 @@ int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
 }
 
 int Poisson::sum_to(const double* pin, int n, int slot) {
-  hipLaunchKernelGGL(sum_slot_kernel, dim3(1), dim3(256), 0, stream, pin, n, scal, slot);
+  hipLaunchKernelGGL(sum_slot_kernel, dim3(1, 1, nb_), dim3(256), 0, stream, pin, n, scal, slot, ps);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
 
 int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* rep) {
-  *rep = PoissonReport{};
+  nb_ = 1;
+  return run(d_rhs, d_x, tolerance, maxit, mixed, rep);
+}
+
+int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, double tolerance, int maxit, bool mixed,
+                         PoissonReport* reps) {
+  FB_REQUIRE(nb >= 1 && nb <= cap && nb <= kMaxBatch, FB_ERR_ARG, "batch of %d grids exceeds the capacity %d", nb,
+             cap);
+  if (nb == 1 && cap == 1) {
+    int rc = solve(rhs[0], x[0], tolerance, maxit, mixed, reps);
+    if (rc) return rc;
+    FB_HIP(hipStreamSynchronize(stream));
+    finish(reps);
+    return FB_OK;
+  }
+  nb_ = nb;
+  for (int g = 0; g < nb; ++g)
+    FB_HIP(hipMemcpyAsync(bstage + (size_t)g * N, rhs[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  int rc = run(bstage, xstage, tolerance, maxit, mixed, reps);
+  if (rc) return rc;
+  for (int g = 0; g < nb; ++g)
+    FB_HIP(hipMemcpyAsync(x[g], xstage + (size_t)g * N, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  FB_HIP(hipStreamSynchronize(stream));
+  for (int g = 0; g < nb; ++g) finish(reps + g, g);
+  return FB_OK;
+}
+
+// The solve of nb_ grids whose right-hand sides sit N apart at d_rhs; solutions go N apart to d_x.
+int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps) {
+  const int G = nb_;
+  for (int g = 0; g < G; ++g) reps[g] = PoissonReport{};
   const int nb = nblocks;
-  unsigned long long* stat = reinterpret_cast<unsigned long long*>(scal + S_COUNT);
-  FB_HIP(hipMemsetAsync(stat, 0, 2 * sizeof(unsigned long long), stream));
-  hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb), dim3(256), 0, stream, N, d_rhs, flags, b64, partial, partial + nb,
-                     stat);
+  const size_t n = (size_t)G * N;
+  const unsigned nbz = (unsigned)((n + 255) / 256);
+  FB_HIP(hipMemsetAsync(scal, 0, (size_t)G * kScalStride * sizeof(double), stream));
+  hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb, 1, G), dim3(256), 0, stream, N, d_rhs, flags, b64, partial,
+                     partial + nb, reinterpret_cast<unsigned long long*>(scal + S_COUNT), ps);
   FB_HIP(hipGetLastError());
   const bool singular = periodic_x && periodic_y;
   int rc;
@@ -1121,95 +1298,137 @@ int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit
   const double rtol = 0.9 * tolerance;
   const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
   pending_failure_ = 0;
+  int its[kMaxBatch], info[kMaxBatch], fail[kMaxBatch], mi[kMaxBatch];
   if (!mixed) {
-    int its = 0, info = 0, fail = 0;
-    if ((rc = pcg<double>(v64, b64, d_x, rtol, maxit, true, last_iterations_ + 1, &its, &info, &fail))) return rc;
-    last_iterations_ = its;
-    rep->iterations = its;
-    rep->info = info;
-    rep->failure = fail;
+    for (int g = 0; g < G; ++g) mi[g] = maxit;
+    if ((rc = pcg<double>(v64, b64, d_x, rtol, mi, true, last_iterations_ + 1, its, info, fail))) return rc;
+    int mx = 0;
+    for (int g = 0; g < G; ++g) {
+      reps[g].iterations = its[g];
+      reps[g].info = info[g];
+      reps[g].failure = fail[g];
+      mx = std::max(mx, its[g]);
+    }
+    last_iterations_ = mx;
   } else {
     // FP32 PCG sweeps on the scaled FP64 residual, FP64 correction and residual.
     if ((rc = alloc(v32))) return rc;
-    unsigned long long st[2];
-    FB_HIP(hipMemcpyAsync(st, stat, sizeof(st), hipMemcpyDeviceToHost, stream));
+    FB_HIP(hipMemcpyAsync(host_scal, scal, (size_t)G * kScalStride * sizeof(double), hipMemcpyDeviceToHost,
+                          stream));
     FB_HIP(hipStreamSynchronize(stream));
-    double bmax;
-    {
-      long long bits = (long long)st[1];
-      memcpy(&bmax, &bits, sizeof(bmax));
-    }
     const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
-    if (st[0] != 0 || bmax > safe) {
-      rep->failure = st[0] != 0 ? 1 : 2;
-      return FB_OK;
+    bool failed[kMaxBatch];
+    for (int g = 0; g < G; ++g) {
+      const unsigned long long* st = reinterpret_cast<const unsigned long long*>(host_scal + (size_t)g * kScalStride +
+                                                                                 S_COUNT);
+      double bmax;
+      {
+        long long bits = (long long)st[1];
+        memcpy(&bmax, &bits, sizeof(bmax));
+      }
+      failed[g] = st[0] != 0 || bmax > safe;
+      if (failed[g]) reps[g].failure = st[0] != 0 ? 1 : 2;
     }
-    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, N, d_x);
+    hipLaunchKernelGGL(fill_zero, dim3(nbz), dim3(256), 0, stream, n, d_x);
     FB_HIP(hipGetLastError());
-    FB_HIP(hipMemcpyAsync(r64, b64, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb), dim3(256), 0, stream, b64, b64, N, partial);
+    FB_HIP(hipMemcpyAsync(r64, b64, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb, 1, G), dim3(256), 0, stream, b64, b64, N, partial, ps);
     if ((rc = reduce(partial, nb, 0))) return rc;
     if ((rc = read_scalars())) return rc;
-    const double bn = std::sqrt(host_scal[S_DOT]);
-    double rn = bn;
-    int total = 0;
-    rep->info = maxit;
-    for (int outer = 0; outer < 12 && total < maxit; ++outer) {
-      if (bn == 0.0 || rn < rtol * bn) {
-        rep->info = 0;
-        break;
+    double bn[kMaxBatch], rn[kMaxBatch];
+    int total[kMaxBatch];
+    for (int g = 0; g < G; ++g) {
+      bn[g] = std::sqrt(host_scal[(size_t)g * kScalStride + S_DOT]);
+      rn[g] = bn[g];
+      total[g] = 0;
+      reps[g].info = maxit;
+    }
+    for (int outer = 0; outer < 12; ++outer) {
+      bool act[kMaxBatch], any = false;
+      for (int g = 0; g < G; ++g) {
+        act[g] = false;
+        if (failed[g]) continue;
+        if (bn[g] == 0.0 || rn[g] < rtol * bn[g]) {
+          reps[g].info = 0;
+          continue;
+        }
+        act[g] = total[g] < maxit;
+        any = any || act[g];
       }
-      hipLaunchKernelGGL(to_f32_scaled, dim3(nb), dim3(256), 0, stream, N, r64, v32.b, 1.0 / rn);
+      if (!any) break;
+      double sc[kMaxBatch];
+      for (int g = 0; g < G; ++g) {
+        sc[g] = act[g] ? 1.0 / rn[g] : 0.0;
+        mi[g] = act[g] ? maxit - total[g] : 0;
+      }
+      if ((rc = put_par(sc))) return rc;
+      hipLaunchKernelGGL(to_f32_scaled_g, dim3(nb, 1, G), dim3(256), 0, stream, N, r64, v32.b, (const double*)dpar);
       FB_HIP(hipGetLastError());
-      int its = 0, info = 0, fail = 0;
       // inner tolerance: a relative reduction FP32 reaches reliably
-      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, maxit - total, false, last_inner_ + 1, &its, &info, &fail)))
-        return rc;
-      last_inner_ = its;
-      total += its;
-      rep->refinements = outer + 1;
-      if (fail) {
-        rep->failure = fail;
-        break;
+      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, mi, false, last_inner_ + 1, its, info, fail))) return rc;
+      int mx = 0;
+      for (int g = 0; g < G; ++g) {
+        if (!act[g]) {
+          sc[g] = 0.0;
+          continue;
+        }
+        mx = std::max(mx, its[g]);
+        total[g] += its[g];
+        reps[g].refinements = outer + 1;
+        if (fail[g]) {
+          reps[g].failure = fail[g];
+          failed[g] = true;
+          sc[g] = 0.0;
+        } else {
+          sc[g] = rn[g];
+        }
       }
-      hipLaunchKernelGGL(add_f32_scaled, dim3(nb), dim3(256), 0, stream, N, d_x, v32.x, rn);
+      last_inner_ = mx;
+      if ((rc = put_par(sc))) return rc;
+      hipLaunchKernelGGL(add_f32_scaled_g, dim3(nb, 1, G), dim3(256), 0, stream, N, d_x, v32.x, (const double*)dpar);
       FB_HIP(hipGetLastError());
-      hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64, partial,
-                         ix2, iy2);
+      hipLaunchKernelGGL(residual_kernel, dim3(nb, 1, G), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64,
+                         partial, ix2, iy2, ps);
       FB_HIP(hipGetLastError());
       if ((rc = reduce(partial, nb, 0))) return rc;
       if ((rc = read_scalars())) return rc;
-      rn = std::sqrt(host_scal[S_DOT]);
-      if (!std::isfinite(rn)) {
-        rep->failure = 4;
-        break;
+      for (int g = 0; g < G; ++g) {
+        if (!act[g] || failed[g]) continue;
+        rn[g] = std::sqrt(host_scal[(size_t)g * kScalStride + S_DOT]);
+        if (!std::isfinite(rn[g])) {
+          reps[g].failure = 4;
+          failed[g] = true;
+        }
       }
     }
-    if (rep->info != 0 && rn < rtol * bn) rep->info = 0;
-    rep->iterations = total;
+    for (int g = 0; g < G; ++g) {
+      if (reps[g].info != 0 && !failed[g] && rn[g] < rtol * bn[g]) reps[g].info = 0;
+      reps[g].iterations = total[g];
+    }
   }
   if (singular) {
     hipLaunchKernelGGL(rhs_prep_kernel, dim3(nb), dim3(256), 0, stream, N, d_x, flags, r64, partial, partial + nb,
-                       stat);
+                       reinterpret_cast<unsigned long long*>(scal + S_COUNT), ps);
     if ((rc = sum_to(partial, nb, S_BSUM)) || (rc = sum_to(partial + nb, nb, S_BCNT))) return rc;
     hipLaunchKernelGGL(sub_mean_dev_kernel, dim3(nb), dim3(256), 0, stream, N, flags, d_x, (const double*)scal);
     FB_HIP(hipGetLastError());
   }
   // true relative residual ||b - A x|| / max(||b||, eps) in FP64, read with the caller's next sync
-  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64, partial, ix2,
-                     iy2);
+  hipLaunchKernelGGL(residual_kernel, dim3(nb, 1, G), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64,
+                     partial, ix2, iy2, ps);
   FB_HIP(hipGetLastError());
   if ((rc = sum_to(partial, nb, S_RES2))) return rc;
-  hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb), dim3(256), 0, stream, b64, b64, N, partial);
+  hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(nb, 1, G), dim3(256), 0, stream, b64, b64, N, partial, ps);
   FB_HIP(hipGetLastError());
   if ((rc = sum_to(partial, nb, S_B2))) return rc;
-  FB_HIP(hipMemcpyAsync(host_scal + S_COUNT, scal + S_COUNT, 8 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FB_HIP(hipMemcpyAsync(host_scal, scal, (size_t)G * kScalStride * sizeof(double), hipMemcpyDeviceToHost, stream));
   return FB_OK;
 }
 
-void Poisson::finish(PoissonReport* rep) const {
-  const double rnorm = std::sqrt(host_scal[S_RES2]);
-  const double bnorm = std::sqrt(host_scal[S_B2]);
+void Poisson::finish(PoissonReport* rep, int g) const {
+  const double* h = host_scal + (size_t)g * kScalStride;
+  const double rnorm = std::sqrt(h[S_RES2]);
+  const double bnorm = std::sqrt(h[S_B2]);
   rep->relative_residual = rnorm / std::max(bnorm, std::numeric_limits<double>::epsilon());
 }
 

@@ -126,9 +126,9 @@ int fb_ctx_create(int device, fb_ctx** out) {
   FB_HIP(hipGetDeviceProperties(&prop, device));
   c->cu_count = prop.multiProcessorCount;
   // Default-priority stream, non-blocking with respect to the null stream.
-  FB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  FB_HIP(stream_create(&c->stream));
   if (dev_alloc(&c->partials, (size_t)fb::kPartialKeys * fb::kPartialBlocks * sizeof(unsigned long long)) != hipSuccess) {
-    hipStreamDestroy(c->stream);
+    stream_free(c->stream);
     delete c;
     fb::set_error("fb_ctx_create: cannot allocate the reduction slots");
     return FB_ERR_NOMEM;
@@ -143,7 +143,7 @@ int fb_ctx_destroy(fb_ctx* ctx) {
   FB_HIP(hipSetDevice(c->device));
   FB_HIP(hipStreamSynchronize(c->stream));
   FB_HIP(dev_free(c->partials));
-  FB_HIP(hipStreamDestroy(c->stream));
+  FB_HIP(stream_free(c->stream));
   delete c;
   return FB_OK;
 }

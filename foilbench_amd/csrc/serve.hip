@@ -26,9 +26,9 @@ static int make_ctx(int device, Ctx** out) {
   hipDeviceProp_t prop;
   FB_HIP(hipGetDeviceProperties(&prop, device));
   c->cu_count = prop.multiProcessorCount;
-  FB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  FB_HIP(stream_create(&c->stream));
   if (dev_alloc(&c->partials, (size_t)kPartialKeys * kPartialBlocks * sizeof(unsigned long long)) != hipSuccess) {
-    (void)hipStreamDestroy(c->stream);
+    (void)stream_free(c->stream);
     delete c;
     set_error("cannot allocate reduction slots");
     return FB_ERR_NOMEM;
@@ -41,7 +41,7 @@ static void free_ctx(Ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   (void)dev_free(c->partials);
-  (void)hipStreamDestroy(c->stream);
+  (void)stream_free(c->stream);
   delete c;
 }
 
@@ -61,6 +61,44 @@ struct LbmJob {
   int steps;
   double submitted;
   int prec;
+};
+
+// ------------------------------------------------------------------ batched pressure solves
+
+// Stable Fluids workers of one grid shape meet here for their pressure solves: a worker hands its
+// grid over and waits; the batcher runs the waiting grids' solves in one batched solve (every CG
+// launch covers all of them, each grid stopping on its own), then releases them.
+struct SolveJob {
+  MacGrid* M = nullptr;
+  PoissonReport rep{};
+  int rc = FB_OK;
+  bool done = false;
+};
+
+struct SolveKey {
+  int nx, ny, mixed, maxit;
+  double dx, dy, tol;
+  bool operator==(const SolveKey& o) const {
+    return nx == o.nx && ny == o.ny && mixed == o.mixed && maxit == o.maxit && dx == o.dx && dy == o.dy &&
+           tol == o.tol;
+  }
+};
+
+struct Server;
+
+struct Batcher {
+  SolveKey key{};
+  Server* S = nullptr;
+  Ctx* ctx = nullptr;
+  Poisson P;
+  int capacity = 1;
+  std::vector<unsigned long long> slot_ver;  // operator tag loaded in each slot
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<SolveJob*> pending;
+  int members = 0;  // workers running a request of this shape (they may all join a solve)
+  bool stop = false;
+  std::thread thread;
 };
 
 struct LbmTiming {
@@ -83,11 +121,112 @@ struct Server {
   std::vector<Ctx*> sf_ctx;
   Ctx* lbm_ctx = nullptr;
   std::vector<std::thread> threads;
+  std::mutex bmu;  // guards `batchers`
+  std::vector<Batcher*> batchers;
 
   double now() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 };
+
+static void batcher_loop(Batcher* B) {
+  (void)hipSetDevice(B->S->device);
+  const auto gather = std::chrono::duration<double, std::micro>(B->S->opt.sf_gather_us);
+  for (;;) {
+    std::vector<SolveJob*> jobs;
+    {
+      std::unique_lock<std::mutex> lk(B->mu);
+      B->cv.wait(lk, [&] { return B->stop || !B->pending.empty(); });
+      if (B->pending.empty()) return;
+      // give the other workers of this shape a moment to arrive
+      const auto t0 = std::chrono::steady_clock::now();
+      B->cv.wait_until(lk, t0 + gather, [&] {
+        return B->stop || (int)B->pending.size() >= std::min(B->members, B->capacity);
+      });
+      const int take = std::min((int)B->pending.size(), B->capacity);
+      jobs.assign(B->pending.begin(), B->pending.begin() + take);
+      B->pending.erase(B->pending.begin(), B->pending.begin() + take);
+    }
+    // stable slots: the same grids keep the same operator slots from one solve to the next
+    std::sort(jobs.begin(), jobs.end(), [](const SolveJob* a, const SolveJob* b) { return a->M < b->M; });
+    const int nb = (int)jobs.size();
+    int rc = FB_OK;
+    std::vector<const double*> rhs(nb);
+    std::vector<double*> x(nb);
+    std::vector<PoissonReport> reps(nb);
+    for (int k = 0; k < nb && rc == FB_OK; ++k) {
+      const Poisson& src = jobs[k]->M->poisson;
+      if (B->slot_ver[k] != src.version) {
+        rc = B->P.load_slot(k, src);
+        B->slot_ver[k] = src.version;
+      }
+      rhs[k] = jobs[k]->M->rhs;
+      x[k] = jobs[k]->M->pres;
+    }
+    if (rc == FB_OK)
+      rc = B->P.solve_batch(nb, rhs.data(), x.data(), B->key.tol, B->key.maxit, B->key.mixed != 0, reps.data());
+    if (rc) {
+      // the slots may hold partial copies: reload them next time
+      for (auto& v : B->slot_ver) v = 0;
+    }
+    {
+      std::lock_guard<std::mutex> lk(B->mu);
+      for (int k = 0; k < nb; ++k) {
+        jobs[k]->rc = rc;
+        jobs[k]->rep = reps[k];
+        jobs[k]->done = true;
+      }
+    }
+    B->cv.notify_all();
+  }
+}
+
+static Batcher* batcher_for(Server* S, const SolveKey& key, int* rc) {
+  std::lock_guard<std::mutex> lk(S->bmu);
+  for (auto* b : S->batchers)
+    if (b->key == key) return b;
+  auto* B = new Batcher();
+  B->key = key;
+  B->S = S;
+  B->capacity = std::min(S->opt.sf_batch, kMaxBatch);
+  *rc = make_ctx(S->device, &B->ctx);
+  if (*rc == FB_OK)
+    *rc = B->P.setup(B->ctx->stream, key.nx, key.ny, key.dx, key.dy, 0, 0, nullptr, B->capacity);
+  if (*rc) {
+    free_ctx(B->ctx);
+    delete B;
+    return nullptr;
+  }
+  B->slot_ver.assign(B->capacity, 0ull);
+  B->thread = std::thread(batcher_loop, B);
+  S->batchers.push_back(B);
+  return B;
+}
+
+// SolveHook of a served grid: hand the solve to the grid's batcher and wait for it.
+static int batched_solve(void* user, MacGrid& M, double tol, int maxit, bool mixed, PoissonReport* rep) {
+  auto* B = static_cast<Batcher*>(user);
+  if (B->key.tol != tol || B->key.maxit != maxit || (B->key.mixed != 0) != mixed) {
+    // not the batch's solve: the grid's own solver, completed here as the hook requires
+    int rc = M.poisson.solve(M.rhs, M.pres, tol, maxit, mixed, rep);
+    if (rc) return rc;
+    FB_HIP(hipStreamSynchronize(M.ctx->stream));
+    M.poisson.finish(rep);
+    return FB_OK;
+  }
+  // the right-hand side was written on the worker's stream; the batcher reads it on its own
+  FB_HIP(hipStreamSynchronize(M.ctx->stream));
+  SolveJob job;
+  job.M = &M;
+  {
+    std::unique_lock<std::mutex> lk(B->mu);
+    B->pending.push_back(&job);
+    B->cv.notify_all();
+    B->cv.wait(lk, [&] { return job.done; });
+  }
+  *rep = job.rep;
+  return job.rc;
+}
 
 static void sf_worker(Server* S, int w) {
   (void)hipSetDevice(S->device);
@@ -108,11 +247,23 @@ static void sf_worker(Server* S, int w) {
     r.submitted_ms = job.submitted;
     r.started_ms = S->now();
     MacGrid& M = *job.M;
-    Ctx* home = M.ctx;
+    Ctx* home = M.ctx;  // its stream was synchronised at submission, by the submitting thread
     int rc = FB_OK;
-    if (hipStreamSynchronize(home->stream) != hipSuccess) rc = FB_ERR_HIP;
+    Batcher* B = nullptr;
+    if (S->opt.sf_batch > 1 && !M.dom.periodic_x && !M.dom.periodic_y) {
+      const SolveKey key{M.dom.nx, M.dom.ny, job.p.mixed_poisson != 0 ? 1 : 0, job.p.max_iterations, M.g.dx, M.g.dy,
+                         job.p.tolerance};
+      B = batcher_for(S, key, &rc);
+    }
     if (rc == FB_OK) {
       mac_bind(M, c);
+      mac_use_graphs(M, S->opt.use_graphs != 0);
+      if (B) {
+        std::lock_guard<std::mutex> lk(B->mu);
+        B->members++;
+        M.hook.fn = batched_solve;
+        M.hook.user = B;
+      }
       for (int k = 0; k < job.steps; ++k) {
         fb_sf_report rep;
         rc = mac_substep(M, job.p, job.pose, &rep);
@@ -125,6 +276,13 @@ static void sf_worker(Server* S, int w) {
         r.steps_done = k + 1;
       }
       if (hipStreamSynchronize(c->stream) != hipSuccess && rc == FB_OK) rc = FB_ERR_HIP;
+      if (B) {
+        std::lock_guard<std::mutex> lk(B->mu);
+        B->members--;
+        M.hook = SolveHook{};
+        B->cv.notify_all();
+      }
+      mac_use_graphs(M, true);
       mac_bind(M, home);
     }
     r.status = rc;
@@ -252,6 +410,8 @@ int fb_server_create(int device, const fb_server_options* options, fb_server** o
   FB_REQUIRE(options->sf_workers >= 1 && options->sf_workers <= 64, FB_ERR_ARG, "sf_workers must be in [1, 64]");
   FB_REQUIRE(options->lbm_chunk >= 2 && options->lbm_chunk % 2 == 0 && options->lbm_chunk <= 1024, FB_ERR_ARG,
              "lbm_chunk must be even and in [2, 1024]");
+  FB_REQUIRE(options->sf_batch >= 0 && options->sf_batch <= fb::kMaxBatch && options->sf_gather_us >= 0.0,
+             FB_ERR_ARG, "sf_batch must be in [0, %d] and sf_gather_us non-negative", fb::kMaxBatch);
   int n = 0;
   FB_HIP(hipGetDeviceCount(&n));
   FB_REQUIRE(device >= 0 && device < n, FB_ERR_ARG, "device %d out of range (%d devices)", device, n);
@@ -287,6 +447,16 @@ int fb_server_destroy(fb_server* srv) {
   }
   S->cv.notify_all();
   for (auto& t : S->threads) t.join();
+  for (auto* B : S->batchers) {
+    {
+      std::lock_guard<std::mutex> lk(B->mu);
+      B->stop = true;
+    }
+    B->cv.notify_all();
+    B->thread.join();
+    fb::free_ctx(B->ctx);
+    delete B;
+  }
   for (auto* c : S->sf_ctx) fb::free_ctx(c);
   fb::free_ctx(S->lbm_ctx);
   delete S;
@@ -301,6 +471,10 @@ int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_par
   auto* S = reinterpret_cast<Server*>(srv);
   auto* M = reinterpret_cast<fb::MacGrid*>(mac);
   FB_REQUIRE(M->ctx->device == S->device, FB_ERR_ARG, "handle lives on another device");
+  // work pending on the handle's own stream completes here, in the caller's thread: the worker
+  // must not wait on a stream that the caller may be capturing into meanwhile
+  FB_HIP(hipSetDevice(S->device));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
   {
     std::lock_guard<std::mutex> lk(S->mu);
     FB_REQUIRE(!S->stop, FB_ERR_STATE, "server is stopping");
@@ -316,6 +490,8 @@ int fb_server_submit_lbm(fb_server* srv, int64_t id, fb_lbm* lbm, const fb_lbm_p
   auto* S = reinterpret_cast<Server*>(srv);
   const int prec = fb::lbm_handle_precision(lbm);
   FB_REQUIRE(prec >= FB_F64 && prec <= FB_F16_MIXED, FB_ERR_ARG, "invalid lattice handle");
+  FB_HIP(hipSetDevice(S->device));
+  FB_HIP(fb::lbm_handle_sync(lbm));
   {
     std::lock_guard<std::mutex> lk(S->mu);
     FB_REQUIRE(!S->stop, FB_ERR_STATE, "server is stopping");

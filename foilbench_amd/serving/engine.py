@@ -99,10 +99,12 @@ class Server:
     """The native request loop (fb_server) with memory-based admission in front of it."""
 
     def __init__(self, sf_workers: int = 8, lbm_chunk: int = 32, use_graphs: bool = True,
-                 memory_budget: int | None = None, max_inflight: int = 64, ctx: N.Context | None = None) -> None:
+                 memory_budget: int | None = None, max_inflight: int = 64, sf_batch: int | None = None,
+                 sf_gather_us: float = 300.0, ctx: N.Context | None = None) -> None:
         self.ctx = ctx or N.Context.default()
         self.lib = self.ctx.lib
-        opt = N.fb_server_options(sf_workers, lbm_chunk, 1 if use_graphs else 0)
+        sf_batch = sf_workers if sf_batch is None else sf_batch
+        opt = N.fb_server_options(sf_workers, lbm_chunk, 1 if use_graphs else 0, sf_batch, sf_gather_us)
         h = C.c_void_p()
         N.check(self.lib.fb_server_create(self.ctx.device, C.byref(opt), C.byref(h)))
         self.handle = h
@@ -114,6 +116,7 @@ class Server:
         self.memory_budget = int(memory_budget)
         self.max_inflight = int(max_inflight)
         self.options = {"sf_workers": sf_workers, "lbm_chunk": lbm_chunk, "use_graphs": bool(use_graphs),
+                        "sf_batch": sf_batch, "sf_gather_us": sf_gather_us,
                         "memory_budget": self.memory_budget, "max_inflight": self.max_inflight}
 
     def clock_ms(self) -> float:

@@ -340,7 +340,9 @@ typedef struct fb_server fb_server;
 typedef struct fb_server_options {
   int sf_workers;  /* Stable Fluids requests in flight at once, one stream each (>= 1) */
   int lbm_chunk;   /* lattice steps per captured graph launch (even, >= 2) */
-  int use_graphs;  /* 1: replay captured graphs for the lattice batches */
+  int use_graphs;  /* 1: replay captured graphs (lattice batches, CG iteration chunks) */
+  int sf_batch;    /* > 1: pressure solves of up to this many same-shape requests share launches */
+  double sf_gather_us; /* how long a batched solve waits for more same-shape requests to join */
 } fb_server_options;
 
 enum fb_serve_kind { FB_SERVE_SF = 0, FB_SERVE_LBM = 1 };

@@ -77,6 +77,7 @@ def test_lbm_batch_matches_per_step_launches():
 def _small_mix():
     from foilbench_amd.serving.mix import LBM, SF, Request
 
+    # same-shape Stable Fluids requests share batched pressure solves
     return [
         Request(0, 0.00, SF, 64, 32, "0012", 8.0, 500.0, 6),
         Request(1, 0.01, LBM, 128, 64, "2412", 12.0, 1000.0, 40),
@@ -84,6 +85,9 @@ def _small_mix():
         Request(3, 0.02, LBM, 256, 128, "0009", -4.0, 300.0, 26),
         Request(4, 0.03, SF, 96, 48, "6218", 15.0, 150.0, 4),
         Request(5, 0.04, LBM, 64, 32, "3312", 0.0, 8000.0, 18),
+        Request(6, 0.04, SF, 64, 32, "2310", -3.0, 900.0, 7),
+        Request(7, 0.05, SF, 64, 32, "4412", 18.0, 120.0, 5),
+        Request(8, 0.05, SF, 128, 64, "0015", 10.0, 4000.0, 6),
     ]
 
 
