@@ -118,6 +118,7 @@ def _cpu_baseline(sample_steps: int):
 def _legs(with_cpu: bool) -> dict:
     """Stable Fluids and PIC/FLIP workloads; a failing leg is reported, not fatal."""
     from foilbench_amd.benchmark import legs as L
+    from foilbench_amd.benchmark.serving import serving_leg
 
     plan = {
         "sf_plain": (lambda: L.stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=30),
@@ -127,13 +128,14 @@ def _legs(with_cpu: bool) -> dict:
                      lambda: L.sf_cpu_baseline(512, 256, -15.0, 2000.0, 1e-8, 1)),
         "pic": (lambda: L.pic_leg(1024, 512, 8, steps=4),
                 lambda: L.pic_cpu_baseline(256, 128, 8, 1)),
+        "serving": (lambda: serving_leg(), None),
     }
     out = {}
     for name, (gpu, cpu) in plan.items():
         t0 = time.perf_counter()
         try:
             out[name] = gpu()
-            if with_cpu:
+            if with_cpu and cpu is not None:
                 out[name]["cpu_baseline"] = cpu()
         except Exception as err:  # reported in the JSON line instead of aborting the headline
             out[name] = {"error": f"{type(err).__name__}: {err}"}
@@ -204,6 +206,10 @@ def main() -> int:
         "forms": forms,
         "legs": legs,
     }
+    sv = (legs or {}).get("serving") or {}
+    if "value" in sv:
+        line["serving"] = {k: sv[k] for k in ("metric", "value", "unit", "plain_value", "ratio_vs_plain", "p50_ms",
+                                              "p99_ms", "max_rel_field_error", "workload")}
     print(json.dumps(line), flush=True)
     return 0
 

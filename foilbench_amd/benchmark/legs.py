@@ -202,7 +202,8 @@ def pic_cpu_baseline(nx: int, ny: int, per_cell: int, steps: int) -> dict:
 
 
 def main() -> None:
-    """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|pic} [steps]``: one leg, one JSON line."""
+    """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|pic|serving} [steps]``: one leg, one JSON
+    line (for ``serving`` the second argument is the number of requests)."""
     import json
     import sys
 
@@ -214,6 +215,10 @@ def main() -> None:
         out = stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=steps, warmup=1)
     elif which == "pic":
         out = pic_leg(1024, 512, 8, steps=steps)
+    elif which == "serving":
+        from .serving import serving_leg
+
+        out = serving_leg(n=int(sys.argv[2]) if len(sys.argv) > 2 else 256)
     else:
         raise SystemExit(f"unknown leg {which!r}")
     print(json.dumps(out), flush=True)

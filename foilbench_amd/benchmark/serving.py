@@ -1,0 +1,68 @@
+"""Serving leg: the request mix served by the native request loop against the plain form.
+
+Three runs over the same mix (PCG32 seed 42, stream 1001; step counts scaled so that the leg
+finishes in about a minute):
+* capacity: every request queued at t = 0, served with continuous batching (FP32 storage, FP32
+  CG sweeps in FP64 refinement, shifted FP32 populations, batched lattice launches and batched
+  same-shape pressure solves) -> aggregate cell-steps/s;
+* plain: the drop-in FP64 form, one request after another -> aggregate cell-steps/s;
+* latency: Poisson arrivals at the mix's rate on the wall clock -> p50/p99 completion latency.
+The relative L2 field error of the served results against the plain FP64 results is measured on
+the first requests of each solver (velocity faces for Stable Fluids; density deviation and
+velocity for the lattice).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ..serving.engine import Server, run_plain
+from ..serving.mix import LBM, SF, serving_mix
+
+
+def _rel(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def serving_leg(n: int = 256, step_scale: float = 0.02, rate: float = 50.0, sf_workers: int = 16,
+                lbm_chunk: int = 32, check_per_solver: int = 4) -> dict:
+    mix = serving_mix(n, rate=rate, step_scale=step_scale)
+    check = set()
+    for solver in (SF, LBM):
+        check.update([r.id for r in mix if r.solver == solver][:check_per_solver])
+    srv = Server(sf_workers=sf_workers, lbm_chunk=lbm_chunk)
+    try:
+        cap = srv.run(mix, keep_fields=check, realtime=False)
+        lat = srv.run(mix, realtime=True)
+    finally:
+        srv.close()
+    plain = run_plain(mix, keep_fields=check)
+    errors = {}
+    served = {o.id: o for o in cap["outcomes"]}
+    for o in plain["outcomes"]:
+        if o.id not in check or o.id not in served or not served[o.id].fields or not o.fields:
+            continue
+        s = served[o.id]
+        for k, ref in o.fields.items():
+            if k == "rho":
+                errors[f"{o.solver}:{o.id}:rho-1"] = _rel(s.fields[k] - 1.0, ref - 1.0)
+            else:
+                errors[f"{o.solver}:{o.id}:{k}"] = _rel(s.fields[k], ref)
+    strip = lambda d: {k: v for k, v in d.items() if k != "outcomes"}  # noqa: E731
+    ratio = cap["cell_steps_per_s"] / plain["cell_steps_per_s"]
+    return {
+        "metric": "aggregate cell-steps/s, serving form vs plain form",
+        "value": cap["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "plain_value": plain["cell_steps_per_s"],
+        "ratio_vs_plain": ratio,
+        "p50_ms": lat["p50_ms"],
+        "p99_ms": lat["p99_ms"],
+        "max_rel_field_error": max(errors.values()) if errors else None,
+        "field_errors": errors,
+        "workload": (f"serving mix: {n} requests, Poisson arrivals {rate:g}/s, grids 128x64..1024x512, "
+                     f"SF/LBM 50/50, steps 200..5000 x {step_scale:g}"),
+        "capacity_run": strip(cap),
+        "latency_run": strip(lat),
+        "plain_run": strip(plain),
+    }

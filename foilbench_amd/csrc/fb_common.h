@@ -19,15 +19,20 @@ namespace fb {
 // every such call, so that handles can be created and freed while the request loop captures.
 std::recursive_mutex& hip_lock();
 
+// Device memory through a block cache: a freed block goes back to a per-size free list instead of
+// hipFree (which synchronises the whole device and would stall every other stream), and the next
+// allocation of that size class takes it. A block is freed only after the work of its owner has
+// completed (the owners synchronise their streams first), so any stream may reuse it.
+hipError_t cached_alloc(void** p, size_t bytes);
+hipError_t cached_free(void* p);
+// Returns the cached free blocks to HIP (for example before measuring free memory).
+void release_cached_blocks();
+
 template <class T>
 inline hipError_t dev_alloc(T** p, size_t bytes) {
-  std::lock_guard<std::recursive_mutex> g(hip_lock());
-  return hipMalloc((void**)p, bytes);
+  return cached_alloc((void**)p, bytes);
 }
-inline hipError_t dev_free(void* p) {
-  std::lock_guard<std::recursive_mutex> g(hip_lock());
-  return hipFree(p);
-}
+inline hipError_t dev_free(void* p) { return cached_free(p); }
 template <class T>
 inline hipError_t host_alloc(T** p, size_t bytes) {
   std::lock_guard<std::recursive_mutex> g(hip_lock());

@@ -80,8 +80,8 @@ class Poisson {
   int ps = 0;        // partial-sum slots per grid
   double* bstage = nullptr;  // batched right-hand sides and solutions (cap > 1)
   double* xstage = nullptr;
-  double* dpar = nullptr;    // per-grid parameters of a launch (cap), staged through host_par
-  double* host_par = nullptr;  // kParSlots x cap pinned staging slots
+  double* dpar = nullptr;    // per-grid parameters of a launch (2 cap), staged through host_par
+  double* host_par = nullptr;  // kParSlots x 2 cap pinned staging slots
   static constexpr int kParSlots = 16;
   unsigned par_slot_ = 0;
   uint8_t* flags = nullptr;
@@ -103,10 +103,10 @@ class Poisson {
   template <class T>
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>
-  int pcg(CgVecs<T>& v, const T* b, T* x, double rtol, const int* maxit, bool check_rhs, int first_chunk,
+  int pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs, int first_chunk,
           int* iterations, int* info, int* failure);
   int run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps);
-  int put_par(const double* v);
+  int put_par(const double* v, int count);
   template <class T>
   int iterations_chunk(CgVecs<T>& v, T* x, int n, int parity);
   template <class T>
@@ -126,6 +126,7 @@ class Poisson {
     hipGraphExec_t exec;
   };
   std::vector<CachedGraph> graphs_;
+  std::vector<GraphKey> seen_;  // chunk shapes run once without a graph
   bool use_graphs_ = true;
   int last_iterations_ = 0, last_inner_ = 0, pending_failure_ = 0;
   int read_scalars();

@@ -897,13 +897,13 @@ __global__ void __launch_bounds__(256) add_f32_scaled_g(size_t N, double* __rest
   if (i < N) x[o] += (double)e[o] * s[blockIdx.z];
 }
 
-// CG scalars of grid blockIdx.z: atol <- rtol (scaled by ||b|| later), maxiter <- maxit[g];
-// a grid with maxit 0 ends before its first iteration.
-__global__ void cg_init_kernel(double* scal, double rtol, const double* __restrict__ maxit) {
+// CG scalars of grid g = blockIdx.z: atol <- rtol[g] (scaled by ||b|| later), maxiter <-
+// maxit[g] (par = [maxit x nb, rtol x nb]); a grid with maxit 0 ends before its first iteration.
+__global__ void cg_init_kernel(double* scal, const double* __restrict__ par, int nb) {
   if (threadIdx.x >= S_COUNT) return;
   double* sc = scal + (size_t)blockIdx.z * kScalStride;
   const int k = threadIdx.x;
-  sc[k] = k == S_ATOL ? rtol : (k == S_MAXIT ? maxit[blockIdx.z] : 0.0);
+  sc[k] = k == S_ATOL ? par[nb + blockIdx.z] : (k == S_MAXIT ? par[blockIdx.z] : 0.0);
 }
 
 // ---------------------------------------------------------------- host driver
@@ -1041,8 +1041,8 @@ int Poisson::setup(hipStream_t s, int nx_, int ny_, double dx_, double dy_, int 
     FB_HIP(dev_alloc((void**)&partial, (size_t)cap * ps * sizeof(double)));
     FB_HIP(dev_alloc((void**)&scal, (size_t)cap * kScalStride * sizeof(double)));
     FB_HIP(host_alloc((void**)&host_scal, (size_t)cap * kScalStride * sizeof(double)));
-    FB_HIP(dev_alloc((void**)&dpar, (size_t)cap * sizeof(double)));
-    FB_HIP(host_alloc((void**)&host_par, (size_t)kParSlots * cap * sizeof(double)));
+    FB_HIP(dev_alloc((void**)&dpar, (size_t)2 * cap * sizeof(double)));
+    FB_HIP(host_alloc((void**)&host_par, (size_t)kParSlots * 2 * cap * sizeof(double)));
     FB_HIP(dev_alloc((void**)&b64, (size_t)cap * N * sizeof(double)));
     FB_HIP(dev_alloc((void**)&r64, (size_t)cap * N * sizeof(double)));
     if (cap > 1) {
@@ -1091,10 +1091,10 @@ int Poisson::read_scalars() {
 
 // Per-grid launch parameters to dpar. The copy reads pinned memory when the stream reaches it,
 // so successive calls between synchronisations use successive host slots of a ring.
-int Poisson::put_par(const double* v) {
-  double* h = host_par + (size_t)(par_slot_++ % kParSlots) * cap;
-  for (int g = 0; g < nb_; ++g) h[g] = v[g];
-  FB_HIP(hipMemcpyAsync(dpar, h, nb_ * sizeof(double), hipMemcpyHostToDevice, stream));
+int Poisson::put_par(const double* v, int count) {
+  double* h = host_par + (size_t)(par_slot_++ % kParSlots) * 2 * cap;
+  for (int g = 0; g < count; ++g) h[g] = v[g];
+  FB_HIP(hipMemcpyAsync(dpar, h, count * sizeof(double), hipMemcpyHostToDevice, stream));
   return FB_OK;
 }
 
@@ -1133,22 +1133,23 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
 }
 
 template <class T>
-int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, double rtol, const int* maxit, bool check_rhs, int first_chunk,
-                 int* iterations, int* info, int* failure) {
+int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs,
+                 int first_chunk, int* iterations, int* info, int* failure) {
   const size_t n = (size_t)nb_ * N;
   FB_HIP(hipMemsetAsync(x, 0, n * sizeof(T), stream));
   FB_HIP(hipMemcpyAsync(v.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
   int maxall = 0;
   {
-    double mi[kMaxBatch];
+    double par[2 * kMaxBatch];
     for (int g = 0; g < nb_; ++g) {
-      mi[g] = (double)maxit[g];
+      par[g] = (double)maxit[g];
+      par[nb_ + g] = rtol[g];
       maxall = std::max(maxall, maxit[g]);
     }
-    int rc = put_par(mi);
+    int rc = put_par(par, 2 * nb_);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, rtol, (const double*)dpar);
+  hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, (const double*)dpar, nb_);
   hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks, 1, nb_), dim3(256), 0, stream, v.r, v.r, N, partial, ps);
   FB_HIP(hipGetLastError());
   int rc = reduce(partial, nblocks, 4);
@@ -1195,6 +1196,15 @@ int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
       return FB_OK;
     }
   if (!use_graphs_) return enqueue_iterations(v, x, n, parity);
+  // capture a chunk shape the second time it comes up: a one-off solve (such as a set-up
+  // projection) does not pay for capture and instantiation
+  bool seen = false;
+  for (auto& k : seen_)
+    if (k == key) seen = true;
+  if (!seen) {
+    seen_.push_back(key);
+    return enqueue_iterations(v, x, n, parity);
+  }
   hipGraph_t graph = nullptr;
   std::lock_guard<std::recursive_mutex> guard(hip_lock());
   FB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1299,9 +1309,13 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
   const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
   pending_failure_ = 0;
   int its[kMaxBatch], info[kMaxBatch], fail[kMaxBatch], mi[kMaxBatch];
+  double rt[kMaxBatch];
   if (!mixed) {
-    for (int g = 0; g < G; ++g) mi[g] = maxit;
-    if ((rc = pcg<double>(v64, b64, d_x, rtol, mi, true, last_iterations_ + 1, its, info, fail))) return rc;
+    for (int g = 0; g < G; ++g) {
+      mi[g] = maxit;
+      rt[g] = rtol;
+    }
+    if ((rc = pcg<double>(v64, b64, d_x, rt, mi, true, last_iterations_ + 1, its, info, fail))) return rc;
     int mx = 0;
     for (int g = 0; g < G; ++g) {
       reps[g].iterations = its[g];
@@ -1360,12 +1374,14 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
       for (int g = 0; g < G; ++g) {
         sc[g] = act[g] ? 1.0 / rn[g] : 0.0;
         mi[g] = act[g] ? maxit - total[g] : 0;
+        // inner tolerance: the reduction still missing (with a factor 2 of margin), but no tighter
+        // than a relative reduction FP32 reaches reliably
+        rt[g] = act[g] ? std::min(0.5, std::max(2.0e-5, 0.5 * rtol * bn[g] / rn[g])) : 1.0;
       }
-      if ((rc = put_par(sc))) return rc;
+      if ((rc = put_par(sc, G))) return rc;
       hipLaunchKernelGGL(to_f32_scaled_g, dim3(nb, 1, G), dim3(256), 0, stream, N, r64, v32.b, (const double*)dpar);
       FB_HIP(hipGetLastError());
-      // inner tolerance: a relative reduction FP32 reaches reliably
-      if ((rc = pcg<float>(v32, v32.b, v32.x, 2.0e-5, mi, false, last_inner_ + 1, its, info, fail))) return rc;
+      if ((rc = pcg<float>(v32, v32.b, v32.x, rt, mi, false, last_inner_ + 1, its, info, fail))) return rc;
       int mx = 0;
       for (int g = 0; g < G; ++g) {
         if (!act[g]) {
@@ -1384,7 +1400,7 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
         }
       }
       last_inner_ = mx;
-      if ((rc = put_par(sc))) return rc;
+      if ((rc = put_par(sc, G))) return rc;
       hipLaunchKernelGGL(add_f32_scaled_g, dim3(nb, 1, G), dim3(256), 0, stream, N, d_x, v32.x, (const double*)dpar);
       FB_HIP(hipGetLastError());
       hipLaunchKernelGGL(residual_kernel, dim3(nb, 1, G), dim3(256), 0, stream, nx, ny, flags, diag, b64, d_x, r64,

@@ -1,6 +1,8 @@
 // Context, error state and device selection of libfoilbench_amd.
 #include <algorithm>
 #include <cstdarg>
+#include <map>
+#include <unordered_map>
 
 #include "fb_common.h"
 
@@ -22,6 +24,77 @@ void clear_error() { g_error.clear(); }
 std::recursive_mutex& hip_lock() {
   static std::recursive_mutex m;
   return m;
+}
+
+namespace {
+struct BlockCache {
+  std::unordered_map<void*, std::pair<int, size_t>> live;     // block -> (device, size class)
+  std::multimap<std::pair<int, size_t>, void*> idle;          // (device, size class) -> free block
+  size_t idle_bytes = 0;
+};
+BlockCache& cache() {
+  static BlockCache* c = new BlockCache();  // never destroyed: blocks may be freed at exit
+  return *c;
+}
+// Size classes: powers of two up to 1 MiB, then whole MiB.
+size_t size_class(size_t b) {
+  if (b <= 512) return 512;
+  if (b <= (size_t(1) << 20)) {
+    size_t c = 512;
+    while (c < b) c <<= 1;
+    return c;
+  }
+  const size_t mib = size_t(1) << 20;
+  return (b + mib - 1) / mib * mib;
+}
+constexpr size_t kMaxIdleBytes = size_t(32) << 30;
+}  // namespace
+
+void release_cached_blocks() {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  BlockCache& c = cache();
+  for (auto& kv : c.idle) (void)hipFree(kv.second);
+  c.idle.clear();
+  c.idle_bytes = 0;
+}
+
+hipError_t cached_alloc(void** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  BlockCache& c = cache();
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const size_t sz = size_class(bytes);
+  auto it = c.idle.find({dev, sz});
+  if (it != c.idle.end()) {
+    *p = it->second;
+    c.idle.erase(it);
+    c.idle_bytes -= sz;
+    c.live[*p] = {dev, sz};
+    return hipSuccess;
+  }
+  e = hipMalloc(p, sz);
+  if (e == hipErrorOutOfMemory && c.idle_bytes > 0) {
+    (void)hipGetLastError();
+    release_cached_blocks();
+    e = hipMalloc(p, sz);
+  }
+  if (e == hipSuccess) c.live[*p] = {dev, sz};
+  return e;
+}
+
+hipError_t cached_free(void* p) {
+  if (!p) return hipSuccess;
+  std::lock_guard<std::recursive_mutex> g(hip_lock());
+  BlockCache& c = cache();
+  auto it = c.live.find(p);
+  if (it == c.live.end()) return hipFree(p);
+  const auto key = it->second;
+  c.live.erase(it);
+  if (c.idle_bytes + key.second > kMaxIdleBytes) return hipFree(p);
+  c.idle.insert({key, p});
+  c.idle_bytes += key.second;
+  return hipSuccess;
 }
 
 // Streaming copy used to calibrate HBM counters and the achievable bandwidth.

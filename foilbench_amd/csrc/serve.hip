@@ -398,6 +398,7 @@ extern "C" {
 int fb_mem_info(int device, int64_t* free_bytes, int64_t* total_bytes) {
   FB_REQUIRE(free_bytes && total_bytes, FB_ERR_ARG, "NULL argument");
   FB_HIP(hipSetDevice(device));
+  fb::release_cached_blocks();
   size_t f = 0, t = 0;
   FB_HIP(hipMemGetInfo(&f, &t));
   *free_bytes = (int64_t)f;

@@ -18,7 +18,9 @@ Forms:
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -100,9 +102,12 @@ class Server:
 
     def __init__(self, sf_workers: int = 8, lbm_chunk: int = 32, use_graphs: bool = True,
                  memory_budget: int | None = None, max_inflight: int = 64, sf_batch: int | None = None,
-                 sf_gather_us: float = 300.0, ctx: N.Context | None = None) -> None:
+                 sf_gather_us: float = 300.0, builders: int = 4, ctx: N.Context | None = None) -> None:
         self.ctx = ctx or N.Context.default()
         self.lib = self.ctx.lib
+        self.builders = max(1, int(builders))
+        self._local = threading.local()
+        self._builder_ctxs: list[N.Context] = []
         sf_batch = sf_workers if sf_batch is None else sf_batch
         opt = N.fb_server_options(sf_workers, lbm_chunk, 1 if use_graphs else 0, sf_batch, sf_gather_us)
         h = C.c_void_p()
@@ -116,7 +121,7 @@ class Server:
         self.memory_budget = int(memory_budget)
         self.max_inflight = int(max_inflight)
         self.options = {"sf_workers": sf_workers, "lbm_chunk": lbm_chunk, "use_graphs": bool(use_graphs),
-                        "sf_batch": sf_batch, "sf_gather_us": sf_gather_us,
+                        "sf_batch": sf_batch, "sf_gather_us": sf_gather_us, "builders": self.builders,
                         "memory_budget": self.memory_budget, "max_inflight": self.max_inflight}
 
     def clock_ms(self) -> float:
@@ -148,55 +153,93 @@ class Server:
         except Exception:
             pass
 
-    def run(self, requests: list[Request], keep_fields: bool = False, realtime: bool = True) -> dict:
-        """Serve `requests` (arrivals on the wall clock when `realtime`, else all at once)."""
+    def _builder_ctx(self) -> N.Context:
+        c = getattr(self._local, "ctx", None)
+        if c is None:
+            c = N.Context(self.ctx.device)
+            self._local.ctx = c
+            self._builder_ctxs.append(c)
+        return c
+
+    def _build_and_submit(self, r: Request):
+        t0 = self.clock_ms()
+        solver, handle, params, pose = _build(r, "serving", self._builder_ctx())
+        self.submit(r, handle, params, pose)
+        return solver, self.clock_ms() - t0
+
+    def run(self, requests: list[Request], keep_fields=False, realtime: bool = True) -> dict:
+        """Serve `requests` (arrivals on the wall clock when `realtime`, else all at once). `keep_fields`:
+        True, or a set of request ids whose final fields are read back. Admitted requests are built
+        (solver initialisation) by `builders` threads, each with its own context, and handed to the
+        server as soon as they are ready."""
         order = sorted(requests, key=lambda r: (r.arrival_s, r.id))
         by_id = {r.id: r for r in order}
         c0 = self.clock_ms()
         arrival = {r.id: (c0 + 1e3 * r.arrival_s if realtime else c0) for r in order}
         waiting: list[Request] = []
-        inflight: dict[int, tuple] = {}
+        inflight: dict[int, list] = {}  # id -> [future, need, t_admitted]
         used = 0
         outcomes: dict[int, Outcome] = {}
         nxt = 0
         peak_used = 0
-        while nxt < len(order) or waiting or inflight:
-            now = self.clock_ms()
-            while nxt < len(order) and arrival[order[nxt].id] <= now:
-                waiting.append(order[nxt])
-                nxt += 1
-            progressed = False
-            # admission in arrival order, by memory and slots
-            while waiting and len(inflight) < self.max_inflight:
-                r = waiting[0]
-                need = estimate_bytes(r, "serving")
-                if used + need > self.memory_budget and inflight:
-                    break
-                waiting.pop(0)
-                t_adm = self.clock_ms()
-                solver, handle, params, pose = _build(r, "serving", self.ctx)
-                self.submit(r, handle, params, pose)
-                inflight[r.id] = (solver, need, t_adm)
-                used += need
-                peak_used = max(peak_used, used)
-                progressed = True
-            for res in self.poll():
-                r = by_id[res.id]
-                solver, need, t_adm = inflight.pop(res.id)
-                out = Outcome(r.id, r.solver, r.cells, int(res.steps_done), int(res.status), int(res.sf_status),
-                              arrival[r.id], t_adm, res.started_ms, res.finished_ms, int(res.pressure_iterations))
-                if keep_fields and res.status == 0:
-                    out.fields = _fields(r, solver)
-                _close(solver)
-                used -= need
-                outcomes[r.id] = out
-                progressed = True
-            if not progressed:
-                time.sleep(0.0005)
-        return summarize([outcomes[r.id] for r in order], self.options)
+        build_ms = 0.0
+        pool = ThreadPoolExecutor(max_workers=self.builders)
+        try:
+            while nxt < len(order) or waiting or inflight:
+                now = self.clock_ms()
+                while nxt < len(order) and arrival[order[nxt].id] <= now:
+                    waiting.append(order[nxt])
+                    nxt += 1
+                progressed = False
+                # admission in arrival order, by memory and slots
+                while waiting and len(inflight) < self.max_inflight:
+                    r = waiting[0]
+                    need = estimate_bytes(r, "serving")
+                    if used + need > self.memory_budget and inflight:
+                        break
+                    waiting.pop(0)
+                    inflight[r.id] = [pool.submit(self._build_and_submit, r), need, self.clock_ms()]
+                    used += need
+                    peak_used = max(peak_used, used)
+                    progressed = True
+                # a build that failed never reaches the server: report it here
+                for rid, (fut, need, t_adm) in list(inflight.items()):
+                    if fut.done() and fut.exception() is not None:
+                        r = by_id[rid]
+                        inflight.pop(rid)
+                        used -= need
+                        t = self.clock_ms()
+                        outcomes[rid] = Outcome(r.id, r.solver, r.cells, 0, -1, 0, arrival[rid], t_adm,
+                                                t, t)
+                        progressed = True
+                for res in self.poll():
+                    r = by_id[res.id]
+                    fut, need, t_adm = inflight.pop(res.id)
+                    solver, bms = fut.result()
+                    build_ms += bms
+                    out = Outcome(r.id, r.solver, r.cells, int(res.steps_done), int(res.status), int(res.sf_status),
+                                  arrival[r.id], t_adm, res.started_ms, res.finished_ms, int(res.pressure_iterations))
+                    if _keep(keep_fields, r.id) and res.status == 0:
+                        out.fields = _fields(r, solver)
+                    _close(solver)
+                    used -= need
+                    outcomes[r.id] = out
+                    progressed = True
+                if not progressed:
+                    time.sleep(0.0005)
+        finally:
+            pool.shutdown(wait=True)
+        out = summarize([outcomes[r.id] for r in order], self.options)
+        out["host_build_s"] = build_ms / 1e3
+        out["peak_admitted_bytes"] = peak_used
+        return out
 
 
-def run_plain(requests: list[Request], keep_fields: bool = False, ctx: N.Context | None = None) -> dict:
+def _keep(keep_fields, rid: int) -> bool:
+    return bool(keep_fields) if isinstance(keep_fields, bool) else rid in keep_fields
+
+
+def run_plain(requests: list[Request], keep_fields=False, ctx: N.Context | None = None) -> dict:
     """The drop-in form, one request after another with no overlap (FP64; substep calls for Stable
     Fluids, one launch per lattice step)."""
     ctx = ctx or N.Context.default()
@@ -224,7 +267,7 @@ def run_plain(requests: list[Request], keep_fields: bool = False, ctx: N.Context
         ctx.synchronize()
         t1 = (time.perf_counter() - t_start) * 1e3
         out = Outcome(r.id, r.solver, r.cells, done, int(status), sf_status, t_adm, t_adm, t0, t1, its)
-        if keep_fields and status == 0:
+        if _keep(keep_fields, r.id) and status == 0:
             out.fields = _fields(r, solver)
         _close(solver)
         outcomes.append(out)

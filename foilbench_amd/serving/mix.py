@@ -5,16 +5,19 @@ The mix draws, per request: a grid from {128x64, 256x128, 512x256, 1024x512}; a 
 foil with camber 0-6 %, camber position 20-50 % and thickness 8-18 %; an angle of attack uniform
 in [-5, 20] degrees; a Reynolds number log-uniform in [1e2, 1e4]; the Stable Fluids or the D2Q9
 solver with equal probability; and a step count uniform in [200, 5000] (scaled by
-`step_scale`). Inter-arrival times are exponential with mean 1 / rate.
+`step_scale`). Inter-arrival times are exponential with mean 1 / rate. All draws come from the
+project's PCG32 stream (seed 42, stream 1001 by default), nine 32-bit words per request, each
+read as u = word / 2^32.
 """
 
 from __future__ import annotations
 
 from dataclasses import dataclass
 
-import numpy as np
+import math
 
 from ..core.models import ControlKeyframe, DomainSpec, FoilSpec, Scenario
+from ..core.rng import PCG32
 
 GRIDS = ((128, 64), (256, 128), (512, 256), (1024, 512))
 SF = "stable-fluids"
@@ -74,19 +77,22 @@ def naca_code(camber_pct: int, position_tenths: int, thickness_pct: int) -> str:
     return f"{camber_pct}{position_tenths}{thickness_pct:02d}"
 
 
-def serving_mix(n: int = 256, rate: float = 50.0, seed: int = 2024, step_scale: float = 1.0,
+def serving_mix(n: int = 256, rate: float = 50.0, seed: int = 42, stream: int = 1001, step_scale: float = 1.0,
                 grids=GRIDS, solvers=(SF, LBM), min_steps: int = 200, max_steps: int = 5000) -> list[Request]:
-    rng = np.random.default_rng(seed)
+    words = PCG32(seed, stream).uint32(9 * n)
+    u = [float(w) / 4294967296.0 for w in words]
+    pick = lambda x, k: min(int(x * k), k - 1)  # noqa: E731
     t = 0.0
     out = []
     for k in range(n):
-        t += float(rng.exponential(1.0 / rate))
-        nx, ny = grids[int(rng.integers(0, len(grids)))]
-        code = naca_code(int(rng.integers(0, 7)), int(rng.integers(2, 6)), int(rng.integers(8, 19)))
-        aoa = float(rng.uniform(-5.0, 20.0))
-        re = float(10.0 ** rng.uniform(2.0, 4.0))
-        solver = solvers[int(rng.integers(0, len(solvers)))]
-        steps = int(rng.integers(min_steps, max_steps + 1))
+        d = u[9 * k:9 * k + 9]
+        t += -math.log1p(-d[0]) / rate
+        nx, ny = grids[pick(d[1], len(grids))]
+        code = naca_code(pick(d[2], 7), 2 + pick(d[3], 4), 8 + pick(d[4], 11))
+        aoa = -5.0 + 25.0 * d[5]
+        re = 10.0 ** (2.0 + 2.0 * d[6])
+        solver = solvers[pick(d[7], len(solvers))]
+        steps = min_steps + pick(d[8], max_steps - min_steps + 1)
         steps = max(2, int(round(steps * step_scale)))
         out.append(Request(k, t, solver, nx, ny, code, aoa, re, steps))
     return out
