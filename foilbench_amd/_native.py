@@ -133,6 +133,8 @@ class fb_server_options(C.Structure):
 
 FB_SERVE_SF = 0
 FB_SERVE_LBM = 1
+FB_TUNE_DST_ROWS = 0
+FB_TUNE_LBM_LIST_BLOCKS = 1
 
 
 class fb_server_result(C.Structure):
@@ -175,6 +177,10 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_run_batch": (C.c_int, [C.POINTER(_P), C.POINTER(fb_lbm_params), C.c_int, C.c_int, C.c_int,
                                    C.POINTER(fb_lbm_stats)]),
     "fb_mem_info": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fb_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fb_tune_get": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "fb_poisson_time_precond": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, C.POINTER(C.c_double)]),
     "fb_server_create": (C.c_int, [C.c_int, C.POINTER(fb_server_options), C.POINTER(_P)]),
     "fb_server_destroy": (C.c_int, [_P]),
     "fb_server_submit_sf": (C.c_int, [_P, C.c_int64, _P, C.POINTER(fb_sf_params), C.POINTER(fb_pose), C.c_int]),

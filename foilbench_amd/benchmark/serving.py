@@ -7,7 +7,8 @@ finishes in about a minute):
   same-shape pressure solves) -> aggregate cell-steps/s;
 * plain: the drop-in FP64 form, one request after another -> aggregate cell-steps/s;
 * latency: Poisson arrivals at the mix's rate on the wall clock -> p50/p99 completion latency.
-The relative L2 field error of the served results against the plain FP64 results is measured on
+Launch shapes and the lattice chunk are picked by the autotuner (`foilbench_amd.tuning`) for
+the mix's grid shapes before the runs. The relative L2 field error of the served results against the plain FP64 results is measured on
 the first requests of each solver (velocity faces for Stable Fluids; density deviation and
 velocity for the lattice).
 """
@@ -25,12 +26,12 @@ def _rel(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def serving_leg(n: int = 256, step_scale: float = 0.02, rate: float = 50.0, sf_workers: int = 16,
-                lbm_chunk: int = 32, check_per_solver: int = 4) -> dict:
+                lbm_chunk: int = 32, check_per_solver: int = 4, autotune: bool = True) -> dict:
     mix = serving_mix(n, rate=rate, step_scale=step_scale)
     check = set()
     for solver in (SF, LBM):
         check.update([r.id for r in mix if r.solver == solver][:check_per_solver])
-    srv = Server(sf_workers=sf_workers, lbm_chunk=lbm_chunk)
+    srv = Server.tuned(mix, sf_workers=sf_workers) if autotune else Server(sf_workers=sf_workers, lbm_chunk=lbm_chunk)
     try:
         cap = srv.run(mix, keep_fields=check, realtime=False)
         lat = srv.run(mix, realtime=True)
@@ -62,6 +63,7 @@ def serving_leg(n: int = 256, step_scale: float = 0.02, rate: float = 50.0, sf_w
         "field_errors": errors,
         "workload": (f"serving mix: {n} requests, Poisson arrivals {rate:g}/s, grids 128x64..1024x512, "
                      f"SF/LBM 50/50, steps 200..5000 x {step_scale:g}"),
+        "tuning": srv.tuning,
         "capacity_run": strip(cap),
         "latency_run": strip(lat),
         "plain_run": strip(plain),

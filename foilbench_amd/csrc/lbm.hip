@@ -1024,6 +1024,13 @@ struct Lattice {
   bool cvalid = false, cfstar = false;
 };
 
+// Blocks striding over the flagged-cell list (tuned per lattice shape; each flagged cell is
+// stepped by the same arithmetic whichever block takes it).
+static int list_blocks(const Lattice& L) {
+  const int t = tune_get(TUNE_LBM_LIST_BLOCKS, L.dom.nx, L.dom.ny, L.prec, 1);
+  return (t >= 1 && t <= 1024) ? t : kListBlocks;
+}
+
 static int other3(int a, int b) {
   for (int k = 0; k < 3; ++k)
     if (k != a && k != b) return k;
@@ -1076,7 +1083,7 @@ static LbmArgs<S> step_args(Lattice& L, const fb_lbm_params& p, int fin_idx, boo
   a.int_gx = (L.dom.nx + 255) / 256;
   a.nedge = 2 * L.dom.nx + 2 * (L.dom.ny - 2);
   a.edge_blocks = (a.nedge + 255) / 256;
-  a.list_blocks = kListBlocks;
+  a.list_blocks = list_blocks(L);
   a.list = L.G[g_cur].list;
   a.list_count = L.G[g_cur].list_count;
   a.links = L.G[g_cur].links;
@@ -1416,6 +1423,7 @@ struct LbmMember {
   int64_t id = 0;
   int total = 0, remaining = 0;
   Ctx* home = nullptr;
+  int list_blocks = 0;  // fixed at join, so the block table and the arguments agree
 };
 
 struct LbmBatch {
@@ -1448,7 +1456,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
     for (int i = 0; i < n; ++i) {
       B.h_start[i] = off;
       const Lattice& L = *B.m[i].L;
-      const int nb = (L.dom.nx + 255) / 256 * (L.dom.ny - 2) + kListBlocks +
+      const int nb = (L.dom.nx + 255) / 256 * (L.dom.ny - 2) + B.m[i].list_blocks +
                      (2 * L.dom.nx + 2 * (L.dom.ny - 2) + 255) / 256;
       off += (nb + 7) / 8 * 8;
     }
@@ -1495,6 +1503,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
         a = step_args<S>(L, mb.p, f, false, f, o, o, L.g, L.g, false, pre_out[i], false);
         a.skip = 1;
       }
+      a.list_blocks = mb.list_blocks;
     }
     if (mb.remaining <= K) {
       int rc = reset_red(L);
@@ -1619,6 +1628,7 @@ int lbm_batch_add(LbmBatch* b, fb_lbm* lattice, const fb_lbm_params& p, int step
   mb.total = steps;
   mb.remaining = steps;
   mb.home = L->ctx;
+  mb.list_blocks = list_blocks(*L);
   L->ctx = b->ctx;
   int rc = batch_prime_dispatch(*L, p);
   if (rc) {

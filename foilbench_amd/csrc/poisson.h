@@ -66,6 +66,8 @@ class Poisson {
   // solve() leaves its last scalars in flight to host memory.
   void finish(PoissonReport* rep, int g = 0) const;
   void set_use_graphs(bool on) { use_graphs_ = on; }
+  // Average milliseconds of one preconditioner application to nb grids (FP32 or FP64 sweeps).
+  int time_precond(bool f32, int nb, int reps, double* ms);
   // Process-wide unique tag of the operator, renewed whenever setup() rebuilds it from a mask (a
   // new solver at a recycled address gets a new tag).
   unsigned long long version = 0;

@@ -1108,8 +1108,12 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
   }
   const DstPlan& P = *v.plan;
   const int nthr = P.M / 16;
-  // several short rows per block, but never fewer blocks than about two per CU
-  const int rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
+  // several short rows per block, but never fewer blocks than about two per CU (unless tuned)
+  const int tp = sizeof(T) == 8 ? FB_F64 : FB_F32;
+  int rows = tune_get(TUNE_DST_ROWS, nx, ny, tp, nb_);
+  if (rows < 1) rows = tune_get(TUNE_DST_ROWS, nx, ny, tp, 0);  // tuned for any batch size
+  if (rows < 1 || rows * nthr > 512 || (size_t)rows * P.M * sizeof(cplx<T>) > 96 * 1024)
+    rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * nthr * sizeof(double);
   hipLaunchKernelGGL(dst_rows_kernel<T>, dg, dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin, v.work,
@@ -1439,6 +1443,32 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
   if ((rc = sum_to(partial, nb, S_B2))) return rc;
   FB_HIP(hipMemcpyAsync(host_scal, scal, (size_t)G * kScalStride * sizeof(double), hipMemcpyDeviceToHost, stream));
   return FB_OK;
+}
+
+// Average duration of one preconditioner application to nb grids (timing for the autotuner; the
+// vectors hold whatever they hold, which does not change the work).
+int Poisson::time_precond(bool f32, int nb, int reps, double* ms) {
+  FB_REQUIRE(nb >= 1 && nb <= cap && reps >= 1 && !periodic_x && !periodic_y, FB_ERR_ARG, "bad timing request");
+  nb_ = nb;
+  int rc = f32 ? alloc(v32) : alloc(v64);
+  if (rc) return rc;
+  FB_HIP(hipMemsetAsync(scal, 0, (size_t)nb * kScalStride * sizeof(double), stream));
+  FB_HIP(hipMemsetAsync(flags, 0, (size_t)nb * N, stream));
+  hipEvent_t e0, e1;
+  FB_HIP(hipEventCreate(&e0));
+  FB_HIP(hipEventCreate(&e1));
+  auto once = [&]() { return f32 ? precond(v32, v32.r, v32.z) : precond(v64, v64.r, v64.z); };
+  if ((rc = once())) return rc;  // warm-up
+  FB_HIP(hipEventRecord(e0, stream));
+  for (int k = 0; k < reps && rc == FB_OK; ++k) rc = once();
+  FB_HIP(hipEventRecord(e1, stream));
+  FB_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  FB_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return rc;
 }
 
 void Poisson::finish(PoissonReport* rep, int g) const {

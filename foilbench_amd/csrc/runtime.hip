@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <map>
+#include <tuple>
 #include <unordered_map>
 
 #include "fb_common.h"
@@ -24,6 +25,36 @@ void clear_error() { g_error.clear(); }
 std::recursive_mutex& hip_lock() {
   static std::recursive_mutex m;
   return m;
+}
+
+namespace {
+struct TuneKey {
+  int param, nx, ny, prec, grids;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(param, nx, ny, prec, grids) < std::tie(o.param, o.nx, o.ny, o.prec, o.grids);
+  }
+};
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+std::map<TuneKey, int>& tune_table() {
+  static std::map<TuneKey, int> t;
+  return t;
+}
+}  // namespace
+
+int tune_get(int param, int nx, int ny, int precision, int grids) {
+  std::lock_guard<std::mutex> g(tune_mutex());
+  auto& t = tune_table();
+  auto it = t.find({param, nx, ny, precision, grids});
+  return it == t.end() ? -1 : it->second;
+}
+
+void tune_set(int param, int nx, int ny, int precision, int grids, int value) {
+  std::lock_guard<std::mutex> g(tune_mutex());
+  if (value < 0) tune_table().erase({param, nx, ny, precision, grids});
+  else tune_table()[{param, nx, ny, precision, grids}] = value;
 }
 
 namespace {
@@ -178,6 +209,18 @@ int fb_copy_benchmark(fb_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy)
 }
 
 const char* fb_last_error(void) { return fb::g_error.c_str(); }
+
+int fb_tune_set(int param, int nx, int ny, int precision, int grids, int value) {
+  FB_REQUIRE(param >= 0 && param < fb::TUNE_COUNT, FB_ERR_ARG, "unknown tuning parameter %d", param);
+  fb::tune_set(param, nx, ny, precision, grids, value);
+  return FB_OK;
+}
+
+int fb_tune_get(int param, int nx, int ny, int precision, int grids, int* value) {
+  FB_REQUIRE(value && param >= 0 && param < fb::TUNE_COUNT, FB_ERR_ARG, "bad tuning query");
+  *value = fb::tune_get(param, nx, ny, precision, grids);
+  return FB_OK;
+}
 
 int fb_version(void) { return 1; }
 

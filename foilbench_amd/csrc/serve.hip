@@ -16,6 +16,7 @@
 #include "fb_common.h"
 #include "lbm_batch.h"
 #include "mac.h"
+#include "poisson.h"
 
 namespace fb {
 
@@ -404,6 +405,24 @@ int fb_mem_info(int device, int64_t* free_bytes, int64_t* total_bytes) {
   *free_bytes = (int64_t)f;
   *total_bytes = (int64_t)t;
   return FB_OK;
+}
+
+int fb_poisson_time_precond(int device, int nx, int ny, double dx, double dy, int precision, int grids, int reps,
+                            double* ms) {
+  FB_REQUIRE(ms && nx >= 2 && ny >= 2 && grids >= 1 && grids <= fb::kMaxBatch && reps >= 1, FB_ERR_ARG,
+             "bad preconditioner timing request");
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  fb::Ctx* c = nullptr;
+  int rc = fb::make_ctx(device, &c);
+  if (rc) return rc;
+  {
+    fb::Poisson P;
+    rc = P.setup(c->stream, nx, ny, dx, dy, 0, 0, nullptr, grids);
+    if (rc == FB_OK) rc = P.time_precond(precision == FB_F32, grids, reps, ms);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  fb::free_ctx(c);
+  return rc;
 }
 
 int fb_server_create(int device, const fb_server_options* options, fb_server** out) {

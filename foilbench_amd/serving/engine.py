@@ -106,6 +106,7 @@ class Server:
         self.ctx = ctx or N.Context.default()
         self.lib = self.ctx.lib
         self.builders = max(1, int(builders))
+        self.tuning: dict = {}
         self._local = threading.local()
         self._builder_ctxs: list[N.Context] = []
         sf_batch = sf_workers if sf_batch is None else sf_batch
@@ -123,6 +124,19 @@ class Server:
         self.options = {"sf_workers": sf_workers, "lbm_chunk": lbm_chunk, "use_graphs": bool(use_graphs),
                         "sf_batch": sf_batch, "sf_gather_us": sf_gather_us, "builders": self.builders,
                         "memory_budget": self.memory_budget, "max_inflight": self.max_inflight}
+
+    @classmethod
+    def tuned(cls, requests: list[Request], sf_workers: int = 16, tuner=None, **kw) -> "Server":
+        """A server whose launch shapes and lattice chunk were picked by the autotuner for the grid
+        shapes of `requests` (the choices are in `server.tuning`)."""
+        from ..tuning import Autotuner
+
+        tuner = tuner or Autotuner(kw.get("ctx"))
+        choices = tuner.for_mix(requests, kw.get("sf_batch") or sf_workers)
+        srv = cls(sf_workers=sf_workers, lbm_chunk=int(choices["lbm_chunk"]), **kw)
+        srv.tuning = choices
+        srv.options["tuning"] = choices
+        return srv
 
     def clock_ms(self) -> float:
         ms = C.c_double()

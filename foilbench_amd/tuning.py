@@ -1,0 +1,155 @@
+"""Autotuner: picks launch shapes and schedules on the device by timing candidates.
+
+Tuned here, per grid shape and precision, and stored in the library's tuning table
+(`fb_tune_set`), which the launches consult:
+* rows per block of the DST-I kernel (the tile of the preconditioner's x-direction transform),
+  timed as whole preconditioner applications on stacked grids (`fb_poisson_time_precond`);
+* blocks striding over a lattice's flagged cells (`lbm_step_kernel`'s special-cell tile), timed
+  as lattice steps (`fb_lbm_time_fixed`).
+For the request loop it also picks the lattice-batch chunk (steps per captured graph) by timing
+batched steps of the mix's lattice shapes (`fb_lbm_run_batch`).
+
+Every tuned choice leaves each result bit unchanged (rows are independent transforms; a
+flagged cell is stepped by the same arithmetic whichever block takes it), so tuning changes only
+speed. Results can be saved to and loaded from a JSON file.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+from pathlib import Path
+
+from . import _native as N
+
+_PREC = {"f64": N.FB_F64, "f32": N.FB_F32, "f32-mixed": N.FB_F32_MIXED, "f16-mixed": N.FB_F16_MIXED}
+
+
+class Autotuner:
+    def __init__(self, ctx: N.Context | None = None, path: str | Path | None = None, reps: int = 20) -> None:
+        self.ctx = ctx or N.Context.default()
+        self.lib = self.ctx.lib
+        self.reps = reps
+        self.path = Path(path) if path else None
+        self.choices: dict[str, dict] = {}
+        if self.path and self.path.exists():
+            self.choices = json.loads(self.path.read_text())
+            for rec in self.choices.values():
+                if "param" in rec:
+                    N.check(self.lib.fb_tune_set(rec["param"], rec["nx"], rec["ny"], rec["precision"], rec["grids"],
+                                                 rec["value"]))
+
+    def save(self) -> None:
+        if self.path:
+            self.path.parent.mkdir(parents=True, exist_ok=True)
+            self.path.write_text(json.dumps(self.choices, indent=1, sort_keys=True))
+
+    def _set(self, param, nx, ny, prec, grids, value) -> None:
+        N.check(self.lib.fb_tune_set(param, nx, ny, prec, grids, value))
+
+    # ------------------------------------------------------------------ Poisson preconditioner
+    def poisson(self, nx: int, ny: int, dx: float, dy: float, precision: int = N.FB_F32, grids: int = 1,
+                key_grids: int | None = None) -> dict:
+        """Rows per DST block for `grids` stacked nx x ny grids; stored under batch size `key_grids`
+        (default `grids`; 0 = any batch size)."""
+        kg = grids if key_grids is None else key_grids
+        key = f"dst_rows:{nx}x{ny}:p{precision}:g{kg}"
+        if key in self.choices:
+            return self.choices[key]
+        m = 16
+        while m < 2 * nx - 1:
+            m *= 2
+        nthr = m // 16
+        esize = 8 if precision == N.FB_F32 else 16  # bytes of one complex value in LDS
+        cands = [r for r in (1, 2, 4, 8, 16, 32) if r * nthr <= 512 and r * m * esize <= 96 * 1024 and r <= ny * grids]
+        times = {}
+        for r in cands:
+            self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, r)
+            ms = C.c_double()
+            N.check(self.lib.fb_poisson_time_precond(self.ctx.device, nx, ny, dx, dy, precision, grids, self.reps,
+                                                     C.byref(ms)))
+            times[r] = ms.value
+        self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, -1)
+        best = min(times, key=times.get)
+        self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, kg, best)
+        rec = {"param": N.FB_TUNE_DST_ROWS, "nx": nx, "ny": ny, "precision": precision, "grids": kg, "value": best,
+               "ms": times}
+        self.choices[key] = rec
+        return rec
+
+    # ------------------------------------------------------------------ lattice special-cell blocks
+    def lbm(self, nx: int, ny: int, form: str = "f32-mixed", steps: int = 50) -> dict:
+        from .benchmark.lbm_harness import Lattice, LatticeCase
+
+        prec = _PREC[form]
+        key = f"lbm_list_blocks:{nx}x{ny}:p{prec}"
+        if key in self.choices:
+            return self.choices[key]
+        lat = Lattice(LatticeCase(nx=nx, ny=ny), prec, self.ctx)
+        times = {}
+        try:
+            for b in (8, 16, 32, 64, 128):
+                self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, b)
+                lat.time(4)
+                times[b] = lat.time(steps) / steps
+        finally:
+            lat.close()
+        best = min(times, key=times.get)
+        self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, best)
+        rec = {"param": N.FB_TUNE_LBM_LIST_BLOCKS, "nx": nx, "ny": ny, "precision": prec, "grids": 1, "value": best,
+               "ms": times}
+        self.choices[key] = rec
+        return rec
+
+    # ------------------------------------------------------------------ lattice batch chunk
+    def lbm_chunk(self, shapes: list[tuple[int, int]], form: str = "f32-mixed", steps: int = 64) -> dict:
+        from .benchmark.lbm_harness import Lattice, LatticeCase
+
+        key = "lbm_chunk:" + ",".join(f"{nx}x{ny}" for nx, ny in shapes) + f":{form}"
+        if key in self.choices:
+            return self.choices[key]
+        prec = _PREC[form]
+        lats = [Lattice(LatticeCase(nx=nx, ny=ny), prec, self.ctx) for nx, ny in shapes]
+        times = {}
+        try:
+            handles = (C.c_void_p * len(lats))(*[l.handle for l in lats])
+            params = (N.fb_lbm_params * len(lats))(*[l.params for l in lats])
+            for chunk in (8, 16, 32, 64):
+                N.check(self.lib.fb_lbm_run_batch(handles, params, len(lats), chunk, chunk, None))
+                t0 = time.perf_counter()
+                N.check(self.lib.fb_lbm_run_batch(handles, params, len(lats), steps, chunk, None))
+                times[chunk] = (time.perf_counter() - t0) * 1e3 / steps
+        finally:
+            for l in lats:
+                l.close()
+        best = min(times, key=times.get)
+        rec = {"value": best, "ms_per_step": times}
+        self.choices[key] = rec
+        return rec
+
+    # ------------------------------------------------------------------ a request mix
+    def for_mix(self, requests, sf_batch: int) -> dict:
+        """Tunes every grid shape of a serving mix; returns the request loop's lattice chunk."""
+        from .serving.mix import LBM, SF
+
+        sf_shapes: dict[tuple[int, int], int] = {}
+        lbm_shapes: set[tuple[int, int]] = set()
+        for r in requests:
+            if r.solver == SF:
+                sf_shapes[(r.nx, r.ny)] = sf_shapes.get((r.nx, r.ny), 0) + 1
+            elif r.solver == LBM:
+                lbm_shapes.add((r.nx, r.ny))
+        out = {"dst_rows": {}, "lbm_list_blocks": {}}
+        for (nx, ny), count in sorted(sf_shapes.items()):
+            dx = 4.0 / nx
+            g = max(1, min(sf_batch, count, N_MAX_BATCH))
+            out["dst_rows"][f"{nx}x{ny}"] = self.poisson(nx, ny, dx, dx, N.FB_F32, grids=g, key_grids=0)["value"]
+        for nx, ny in sorted(lbm_shapes):
+            out["lbm_list_blocks"][f"{nx}x{ny}"] = self.lbm(nx, ny)["value"]
+        out["lbm_chunk"] = self.lbm_chunk(sorted(lbm_shapes))["value"] if lbm_shapes else 32
+        self.save()
+        return out
+
+
+N_MAX_BATCH = 64

@@ -325,6 +325,18 @@ int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
 /* No counterpart in the reference, which advances one scenario at a time in-process. These
  * entry points run many scenarios of uneven size concurrently on one GPU. */
 
+/* Autotuner table: launch-shape choices that leave every result bit unchanged, per (parameter,
+ * nx, ny, precision, grids); value -1 clears an entry, and unset entries use built-in defaults.
+ * FB_TUNE_DST_ROWS: rows per block of the DST-I kernel (grids = batch size, 0 = any);
+ * FB_TUNE_LBM_LIST_BLOCKS: blocks striding over a lattice's flagged cells (grids = 1). */
+enum fb_tune_param { FB_TUNE_DST_ROWS = 0, FB_TUNE_LBM_LIST_BLOCKS = 1 };
+int fb_tune_set(int param, int nx, int ny, int precision, int grids, int value);
+int fb_tune_get(int param, int nx, int ny, int precision, int grids, int* value);
+/* Average milliseconds of one DST-I preconditioner application (FP32 or FP64) to `grids` stacked
+ * nx x ny grids under the current tuning table, for the autotuner. */
+int fb_poisson_time_precond(int device, int nx, int ny, double dx, double dy, int precision, int grids, int reps,
+                            double* ms);
+
 /* Device memory in bytes (hipMemGetInfo) for memory-based admission. */
 int fb_mem_info(int device, int64_t* free_bytes, int64_t* total_bytes);
 

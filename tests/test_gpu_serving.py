@@ -192,3 +192,58 @@ def test_server_reports_failures_per_request():
     assert got[0][0] == 0 and got[0][1] == 3 and got[0][2] == 0  # projection CFL above its limit
     assert got[1] == (0, 0, 4)
     assert got[2] == (0, 0, 10)
+
+
+def test_autotuned_launch_shapes_leave_results_unchanged():
+    """The autotuner's parameters only change speed: a Stable Fluids run with another DST tile and
+    a lattice with another special-cell block count give the same bits as the defaults."""
+    from foilbench_amd.core.geometry import NacaFoil
+    from foilbench_amd.serving.mix import SF, Request
+    from foilbench_amd.solvers.stable_fluids import StableFluidsSolver
+    from foilbench_amd.tuning import Autotuner
+
+    N, ctx = _lib()
+
+    def run_sf():
+        req = Request(0, 0.0, SF, 128, 64, "2412", 6.0, 700.0, 3)
+        sc = req.scenario("float32")
+        s = StableFluidsSolver("mixed", ctx=ctx)
+        s.initialize(sc, NacaFoil(sc.foil), 0)
+        t = 0.0
+        for _ in range(3):
+            t += sc.output_dt
+            s.advance(sc.control_at(t), sc.output_dt)
+        u, v = s.faces()
+        s.close()
+        return u, v
+
+    base = run_sf()
+    for prec in (N.FB_F32, N.FB_F64):
+        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, 4))
+    try:
+        tuned = run_sf()
+    finally:
+        for prec in (N.FB_F32, N.FB_F64):
+            N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, -1))
+    assert np.array_equal(base[0], tuned[0]) and np.array_equal(base[1], tuned[1])
+
+    a = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
+    b = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
+    p = a._params(0.0)
+    N.check(ctx.lib.fb_lbm_run_fixed(a._handle, C.byref(p), 12))
+    ctx.synchronize()
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_LIST_BLOCKS, 160, 80, N.FB_F32_MIXED, 1, 8))
+    try:
+        N.check(ctx.lib.fb_lbm_run_fixed(b._handle, C.byref(p), 12))
+        ctx.synchronize()
+    finally:
+        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_LIST_BLOCKS, 160, 80, N.FB_F32_MIXED, 1, -1))
+    assert np.array_equal(a.populations()[0], b.populations()[0])
+
+    tuner = Autotuner(ctx, reps=5)
+    rec = tuner.poisson(128, 64, 4.0 / 128, 4.0 / 128, N.FB_F32, grids=4, key_grids=0)
+    assert rec["value"] in rec["ms"] and all(t > 0 for t in rec["ms"].values())
+    v = C.c_int()
+    N.check(ctx.lib.fb_tune_get(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, C.byref(v)))
+    assert v.value == rec["value"]
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, -1))
