@@ -174,6 +174,9 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_macroscopic": (C.c_int, [_P, C.c_double, _P, _P]),
     "fb_lbm_run_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int]),
     "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double)]),
+    "fb_lbm_forces": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "fb_mac_forces": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    "fb_mac_set_pressure": (C.c_int, [_P, _P]),
     "fb_lbm_run_batch": (C.c_int, [C.POINTER(_P), C.POINTER(fb_lbm_params), C.c_int, C.c_int, C.c_int,
                                    C.POINTER(fb_lbm_stats)]),
     "fb_mem_info": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

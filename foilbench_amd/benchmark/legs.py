@@ -55,7 +55,9 @@ def sf_bytes_per_cell(s: int, k_d: float, k_p: float, refinements: float = 0.0) 
     return 4 * s + k_d * 6 * s + 8 * s + k_p * 17 * s + refinements * 3 * 8
 
 
-def _run_solver(solver, sc: Scenario, steps: int, warmup: int):
+def _run_solver(solver, sc: Scenario, steps: int, warmup: int, forces: list | None = None):
+    """Warm-up and `steps` timed advances; with `forces`, the lift and drag coefficients after each
+    timed advance are appended to it (outside the timed region)."""
     solver.initialize(sc, NacaFoil(sc.foil), sc.seed)
     t = 0.0
     for _ in range(warmup):
@@ -71,6 +73,9 @@ def _run_solver(solver, sc: Scenario, steps: int, warmup: int):
         times.append(time.perf_counter() - t0)
         reps.append(r)
         t += sc.output_dt
+        if forces is not None:
+            f = solver.forces()
+            forces.append((round(t, 9), f["lift_coefficient"], f["drag_coefficient"]))
     return reps, np.asarray(times)
 
 
@@ -84,7 +89,8 @@ def stable_fluids_leg(nx: int, ny: int, angle: float, reynolds: float, form: str
 
     sc = sf_scenario(nx, ny, angle, reynolds, "float64", tol)
     solver = StableFluidsSolver(form)
-    reps, times = _run_solver(solver, sc, steps, warmup)
+    history: list = []
+    reps, times = _run_solver(solver, sc, steps, warmup, forces=history)
     subs = sum(r.substeps for r in reps)
     k_p = float(np.mean([r.evidence["pressure_iterations"] for r in reps]))
     k_d = float(np.mean([r.evidence["viscosity_iterations"] for r in reps]))
@@ -110,6 +116,7 @@ def stable_fluids_leg(nx: int, ny: int, angle: float, reynolds: float, form: str
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_substep": bpc,
                      "scope": "whole substep, model bytes"},
+        "lift_drag_history": {"columns": ["time", "lift_coefficient", "drag_coefficient"], "rows": history},
     }
     if reference_form is not None:
         ref = StableFluidsSolver(reference_form)

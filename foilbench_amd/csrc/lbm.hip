@@ -1415,6 +1415,84 @@ static void free_lattice(Lattice* L) {
   delete L;
 }
 
+// ------------------------------------------------------------------ force on the foil
+
+// Momentum exchange on the pre-collision state (build-defined; the reference computes none): for
+// every fluid cell x and direction e whose neighbour x + c_e is solid, the link carries
+// c_e (f_e(x) + f_opp(e)(x)) into the wall. Lattice units; FP64 sums per block in a fixed order.
+template <class P>
+__global__ void __launch_bounds__(256) lbm_force_kernel(int nx, int ny, int px, int py,
+                                                        const typename P::S* __restrict__ f,
+                                                        const uint8_t* __restrict__ flags, double* __restrict__ part) {
+  using S = typename P::S;
+  using C = typename P::C;
+  __shared__ double lds[2][4];
+  double fx = 0.0, fy = 0.0;
+  const size_t N = (size_t)nx * ny;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t)gridDim.x * 256) {
+    if (flags[i] & FL_SOLID) continue;
+    const int x = (int)(i % nx), y = (int)(i / nx);
+#pragma unroll
+    for (int e = 1; e < 9; ++e) {
+      int xn = x + D2Q9::cx[e], yn = y + D2Q9::cy[e];
+      if (px) xn = (xn + nx) % nx;
+      else if (xn < 0 || xn >= nx) continue;
+      if (py) yn = (yn + ny) % ny;
+      else if (yn < 0 || yn >= ny) continue;
+      if (!(flags[(size_t)yn * nx + xn] & FL_SOLID)) continue;
+      const int o = D2Q9::opp[e];
+      double fe = (double)ld<S, C>(f + (size_t)e * N + i), fo = (double)ld<S, C>(f + (size_t)o * N + i);
+      if constexpr (P::shift) {
+        fe += D2Q9::w[e];
+        fo += D2Q9::w[o];
+      }
+      fx += D2Q9::cx[e] * (fe + fo);
+      fy += D2Q9::cy[e] * (fe + fo);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    lds[0][threadIdx.x >> 6] = fx;
+    lds[1][threadIdx.x >> 6] = fy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ((lds[0][0] + lds[0][1]) + lds[0][2]) + lds[0][3];
+    part[2 * blockIdx.x + 1] = ((lds[1][0] + lds[1][1]) + lds[1][2]) + lds[1][3];
+  }
+}
+
+__global__ void __launch_bounds__(64) lbm_sum_pairs_kernel(const double* __restrict__ part, int n,
+                                                           double* __restrict__ out) {
+  if (threadIdx.x >= 2) return;
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s += part[2 * k + threadIdx.x];
+  out[threadIdx.x] = s;
+}
+
+template <class P>
+static int forces_impl(Lattice& L, double* force) {
+  hipStream_t s = L.ctx->stream;
+  const unsigned nb = reduction_blocks(L.N);
+  double* part = reinterpret_cast<double*>(L.ctx->partials);
+  double* out = part + 2 * (size_t)kPartialBlocks;
+  hipLaunchKernelGGL(lbm_force_kernel<P>, dim3(nb), dim3(256), 0, s, L.dom.nx, L.dom.ny, L.dom.periodic_x,
+                     L.dom.periodic_y, (const typename P::S*)L.PRE[L.pre], (const uint8_t*)L.G[L.g].flags, part);
+  hipLaunchKernelGGL(lbm_sum_pairs_kernel, dim3(1), dim3(64), 0, s, part, (int)nb, out);
+  FB_HIP(hipGetLastError());
+  double h[2];
+  FB_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+  FB_HIP(hipStreamSynchronize(s));
+  force[0] = h[0];
+  force[1] = h[1];
+  return FB_OK;
+}
+static int forces_dispatch(Lattice& L, double* f) { FB_DISPATCH(L.prec, forces_impl, L, f); }
+
 // ------------------------------------------------------------------ batched stepping
 
 struct LbmMember {
@@ -1886,6 +1964,14 @@ int fb_lbm_run_batch(fb_lbm* const* lattices, const fb_lbm_params* params, int n
     if (stats) stats[d.id] = d.stats;
   }
   return FB_OK;
+}
+
+int fb_lbm_forces(fb_lbm* lbm, double* force) {
+  FB_REQUIRE(lbm && force, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::forces_dispatch(*L, force);
 }
 
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps) {

@@ -464,6 +464,69 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
   put_sum(s.part, 6, nfw, lds);
 }
 
+// Force on the foil per unit depth at unit density (build-defined; the reference computes none):
+// on every interface between a fluid and a solid cell, the fluid cell's pressure pushes along the
+// interface normal, and the tangential velocity of the fluid cell centre relative to the wall
+// velocity of the solid cell, over half a cell, gives the viscous shear. FP64 sums per block, in a
+// fixed order, into part[2 block], part[2 block + 1].
+template <class T>
+__global__ void __launch_bounds__(256) mac_force_kernel(MacGeom g, double nu, double w, double pvx, double pvy,
+                                                        const uint8_t* __restrict__ solid,
+                                                        const double* __restrict__ pres, const T* __restrict__ u,
+                                                        const T* __restrict__ v, double* __restrict__ part) {
+  __shared__ double lds[2][4];
+  double fx = 0.0, fy = 0.0;
+  const size_t PU = g.nx + 1;
+  const size_t total = (size_t)g.nx * g.ny;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const int x = (int)(i % g.nx), y = (int)(i / g.nx);
+    const bool s0 = solid[i] != 0;
+    if (x + 1 < g.nx && s0 != (solid[i + 1] != 0)) {
+      const size_t f = s0 ? i + 1 : i;
+      const int xf = s0 ? x + 1 : x, xs = s0 ? x : x + 1;
+      const double sign = s0 ? -1.0 : 1.0;  // normal from the fluid cell into the solid one
+      fx += sign * pres[f] * g.dy;
+      double wu, wv;
+      wall_at(g, xs, y, w, pvx, pvy, wu, wv);
+      const double vc = 0.5 * ((double)v[(size_t)y * g.nx + xf] + (double)v[(size_t)(y + 1) * g.nx + xf]);
+      fy += nu * (vc - wv) / (0.5 * g.dx) * g.dy;
+    }
+    if (y + 1 < g.ny && s0 != (solid[i + g.nx] != 0)) {
+      const size_t f = s0 ? i + g.nx : i;
+      const int yf = s0 ? y + 1 : y, ys = s0 ? y : y + 1;
+      const double sign = s0 ? -1.0 : 1.0;
+      fy += sign * pres[f] * g.dx;
+      double wu, wv;
+      wall_at(g, x, ys, w, pvx, pvy, wu, wv);
+      const double uc = 0.5 * ((double)u[(size_t)yf * PU + x] + (double)u[(size_t)yf * PU + x + 1]);
+      fx += nu * (uc - wu) / (0.5 * g.dy) * g.dx;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    lds[0][threadIdx.x >> 6] = fx;
+    lds[1][threadIdx.x >> 6] = fy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ((lds[0][0] + lds[0][1]) + lds[0][2]) + lds[0][3];
+    part[2 * blockIdx.x + 1] = ((lds[1][0] + lds[1][1]) + lds[1][2]) + lds[1][3];
+  }
+}
+
+// Fixed-order sum of n pairs of block partials into out[0..1].
+__global__ void __launch_bounds__(64) sum_pairs_kernel(const double* __restrict__ part, int n,
+                                                       double* __restrict__ out) {
+  if (threadIdx.x >= 2) return;
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s += part[2 * k + threadIdx.x];
+  out[threadIdx.x] = s;
+}
+
 // ---------------------------------------------------------------- host handle
 
 int mac_geometry(MacGrid& M, const fb_pose& pose) {
@@ -943,6 +1006,40 @@ int fb_mac_get_pressure(fb_mac* mac, double* p_host) {
   FB_HIP(hipSetDevice(M->ctx->device));
   FB_HIP(hipMemcpyAsync(p_host, M->pres, M->N * sizeof(double), hipMemcpyDeviceToHost, M->ctx->stream));
   FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_set_pressure(fb_mac* mac, const double* p_host) {
+  FB_REQUIRE(mac && p_host, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  FB_HIP(hipMemcpyAsync(M->pres, p_host, M->N * sizeof(double), hipMemcpyHostToDevice, M->ctx->stream));
+  FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  return FB_OK;
+}
+
+int fb_mac_forces(fb_mac* mac, double viscosity, double wall_omega_rad, double* force) {
+  FB_REQUIRE(mac && force, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  hipStream_t s = M->ctx->stream;
+  const unsigned nb = fb::reduction_blocks(M->N);
+  double* part = reinterpret_cast<double*>(M->ctx->partials);  // 2 doubles per block fit the slots
+  double* out = part + 2 * (size_t)fb::kPartialBlocks;
+  if (M->prec == FB_F64)
+    hipLaunchKernelGGL(fb::mac_force_kernel<double>, dim3(nb), dim3(256), 0, s, M->g, viscosity, wall_omega_rad,
+                       M->foil.pivot_x, M->foil.pivot_y, M->solid, M->pres, (const double*)M->U[M->cur],
+                       (const double*)M->V[M->cur], part);
+  else
+    hipLaunchKernelGGL(fb::mac_force_kernel<float>, dim3(nb), dim3(256), 0, s, M->g, viscosity, wall_omega_rad,
+                       M->foil.pivot_x, M->foil.pivot_y, M->solid, M->pres, (const float*)M->U[M->cur],
+                       (const float*)M->V[M->cur], part);
+  hipLaunchKernelGGL(fb::sum_pairs_kernel, dim3(1), dim3(64), 0, s, part, (int)nb, out);
+  FB_HIP(hipGetLastError());
+  FB_HIP(hipMemcpyAsync(M->host_keys, out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FB_HIP(hipStreamSynchronize(s));
+  memcpy(force, M->host_keys, 2 * sizeof(double));
   return FB_OK;
 }
 

@@ -14,6 +14,7 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
 from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
@@ -533,6 +534,18 @@ class LBMSolver:
         warnings = ((f"LBM relaxation clamp active: effective Re={self._effective_reynolds:.1f}",)
                     if self._viscosity_clamped else ())
         return Diagnostics(values, warnings, self._revision)
+
+    def forces(self) -> dict:
+        """Force on the foil by momentum exchange, as physical force per unit depth (reference density 1)
+        with drag, lift and their coefficients."""
+        sc, _ = self._require()
+        f = (C.c_double * 2)()
+        N.check(self._lib().fb_lbm_forces(self._handle, f))
+        chord_cells = sc.foil.chord / sc.domain.dx
+        ul = self._lattice_speed
+        # lattice force / (0.5 u_lat^2 chord_cells) is the coefficient; scale to 0.5 U^2 chord
+        scale = (self._reference_speed ** 2 * sc.foil.chord) / (ul * ul * chord_cells)
+        return lift_drag(f[0] * scale, f[1] * scale, sc.freestream[:2], sc.foil.chord, speed=self._reference_speed)
 
     def __del__(self) -> None:  # pragma: no cover
         try:

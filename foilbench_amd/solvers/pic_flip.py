@@ -16,6 +16,7 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
 from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
@@ -573,6 +574,15 @@ class PicFlipSolver:
         report = ImportReport(state.source_solver, self.info.id, ("solver particles", "FLIP velocity delta history"),
                               ("The first imported step is PIC-dominant while FLIP history is rebuilt.",))
         return ImportOutcome("accepted", "none", report, report.warnings)
+
+    def forces(self) -> dict:
+        """Force on the foil from the MAC grid's pressure and shear (unit density), with drag, lift and
+        their coefficients."""
+        sc, _ = self._require()
+        f = (C.c_double * 2)()
+        nu = sc.reference_speed * sc.foil.chord / self._reynolds
+        N.check(self._lib().fb_mac_forces(self._mac, nu, float(np.deg2rad(self._control.angular_velocity_degrees)), f))
+        return lift_drag(f[0], f[1], sc.freestream[:2], sc.foil.chord)
 
     def diagnostics(self) -> Diagnostics:
         # fb/solvers/pic_flip.py:1115-1169

@@ -14,6 +14,7 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
 from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
@@ -524,6 +525,15 @@ class StableFluidsSolver:
             raise NumericalFailure("nonfinite_state", "Stable Fluids produced non-finite diagnostics")
         warnings = () if not self._projection_warning else (self._projection_warning,)
         return Diagnostics(values, warnings, self._revision)
+
+    def forces(self) -> dict:
+        """Force on the foil per unit depth (unit density) with drag, lift and their coefficients."""
+        sc, _ = self._require()
+        f = (C.c_double * 2)()
+        nu = sc.reference_speed * sc.foil.chord / self._reynolds
+        N.check(self._lib().fb_mac_forces(self._handle, nu, float(np.deg2rad(self._control.angular_velocity_degrees)),
+                                          f))
+        return lift_drag(f[0], f[1], sc.freestream[:2], sc.foil.chord)
 
     def close(self) -> None:
         """Free the device state now rather than at garbage collection."""

@@ -150,6 +150,11 @@ int fb_lbm_stats_now(fb_lbm* lbm, const fb_lbm_params* params, fb_lbm_stats* sta
 /* Density (ny, nx) and physical velocity (ny, nx, 2) in the host dtype
  * (fb/solvers/lbm.py:447-461). */
 int fb_lbm_macroscopic(fb_lbm* lbm, double velocity_scale, void* rho_host, void* vel_host);
+/* Force on the foil per unit depth in lattice units, by momentum exchange on the pre-collision
+ * state: the sum over fluid cells x and directions e with a solid neighbour x + c_e of
+ * c_e (f_e(x) + f_opp(e)(x)). Build-defined: the reference computes no lift or drag.
+ * force[0], force[1]: x and y components. Coefficients: C = 2 F / (u_lattice^2 * chord cells). */
+int fb_lbm_forces(fb_lbm* lbm, double* force);
 /* Timing harness: run `steps` lattice steps with fixed parameters and no pose change,
  * without postcondition reductions; returns once queued (use fb_ctx_synchronize). */
 int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps);
@@ -233,6 +238,13 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
 /* Projection only (fb/solvers/stable_fluids.py:256-330; initialize and import). */
 int fb_mac_project(fb_mac* mac, const fb_sf_params* params, fb_sf_report* report);
 int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats);
+/* Pressure (ny, nx) FP64 of the last projection, replaced (import, tests). */
+int fb_mac_set_pressure(fb_mac* mac, const double* pressure_host);
+/* Force on the foil per unit depth at unit density, in FP64 (build-defined: the reference computes
+ * no lift or drag). On every fluid-solid cell interface the fluid cell's pressure acts along the
+ * interface normal, and viscosity * (tangential fluid velocity at the cell centre - wall velocity)
+ * / (half a cell) acts along it. force[0], force[1]: x and y components. */
+int fb_mac_forces(fb_mac* mac, double viscosity, double wall_omega_rad, double* force);
 
 /* ---------------------------------------------------------------------- PIC/FLIP ---- */
 /* Replaces the particle half of fb/solvers/pic_flip.py (seeding :317-333, collisions
