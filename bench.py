@@ -39,13 +39,18 @@ def _gpu_forms(steps: int, warmup: int):
     from foilbench_amd import _native as N
     from foilbench_amd.benchmark.lbm_harness import BYTES_PER_CELL, Lattice, LatticeCase
 
+    from foilbench_amd.tuning import Autotuner
+
     case = LatticeCase()
     ctx = N.Context.default()
+    tuner = Autotuner(ctx)
     out = {}
     velocities = {}
     solid = None
     for name, prec in (("f32", N.FB_F32), ("f64", N.FB_F64), ("f32-mixed", N.FB_F32_MIXED),
                        ("f16-mixed", N.FB_F16_MIXED)):
+        # the special-cell tile is the autotuner's pick for this lattice and storage form
+        tuned = tuner.lbm(case.nx, case.ny, name)["value"]
         lat = Lattice(case, prec, ctx)
         lat.run(warmup)
         ctx.synchronize()
@@ -63,6 +68,7 @@ def _gpu_forms(steps: int, warmup: int):
             "kernel_ms": avg_ms,
             "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
             "bytes_per_cell": BYTES_PER_CELL[prec],
+            "tuned_list_blocks": tuned,
         }
         velocities[name] = lat.velocity().astype(np.float64)
         if solid is None:

@@ -441,6 +441,134 @@ __global__ void __launch_bounds__(256) p2g_kernel(P2GArgs<T> a) {
   if ((threadIdx.x & 63) == 0 && uns) atomicAdd(a.unsupported, (unsigned long long)uns);
 }
 
+// The same gather for a non-periodic box, staged through LDS: a block owns a tile of
+// kTileW x kTileH faces, loads once the particles of the (kTileW + 3) x (kTileH + 3) cells that can
+// reach it (each cell row's slice of the sorted arrays is contiguous), with their grid coordinates
+// computed once per particle, and every face then scans its 4 x 4 cells in LDS. Candidate order,
+// weights and sums are those of p2g_kernel, so the results are the same bits. A tile whose cells
+// hold more particles than the LDS capacity gathers from global memory instead.
+constexpr int kTileW = 32, kTileH = 4;
+constexpr int kTileCap = 3200;  // staged particles per tile (24 B each)
+
+template <class T>
+__global__ void __launch_bounds__(kTileW* kTileH) p2g_tile_kernel(P2GArgs<T> a) {
+  __shared__ double sgx[kTileCap], sgy[kTileCap], sv[kTileCap];
+  __shared__ int cell_off[kTileH + 3][kTileW + 4];
+  __shared__ int row_base[kTileH + 4];
+  __shared__ int row_first[kTileH + 3];
+  const Comp& c = a.c;
+  const PicGeom& g = a.g;
+  const int tid = threadIdx.y * kTileW + threadIdx.x;
+  const int X0 = blockIdx.x * kTileW, Y0 = blockIdx.y * kTileH;
+  constexpr int NR = kTileH + 3, NC = kTileW + 3;
+  // per cell row: the particle slice of cells X0-2 .. X0+kTileW (clipped to the box)
+  if (tid < NR) {
+    const int cy = Y0 - 2 + tid;
+    int first = 0, cnt = 0;
+    if (cy >= 0 && cy < g.ny) {
+      const int cx0 = max(X0 - 2, 0), cx1 = min(X0 + kTileW, g.nx - 1);
+      if (cx0 <= cx1) {
+        first = a.start[cy * g.nx + cx0];
+        cnt = a.start[cy * g.nx + cx1 + 1] - first;
+      }
+    }
+    row_first[tid] = first;
+    row_base[tid + 1] = cnt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    row_base[0] = 0;
+    for (int r = 0; r < NR; ++r) row_base[r + 1] += row_base[r];
+  }
+  __syncthreads();
+  const int total = row_base[NR];
+  const bool staged = total <= kTileCap;
+  // local cell offsets: cell (r, lc) spans [cell_off[r][lc], cell_off[r][lc + 1])
+  for (int k = tid; k < NR * (NC + 1); k += kTileW * kTileH) {
+    const int r = k / (NC + 1), lc = k % (NC + 1);
+    const int cy = Y0 - 2 + r;
+    int off = row_base[r];
+    if (cy >= 0 && cy < g.ny) {
+      const int cx0 = max(X0 - 2, 0), cx1 = min(X0 + kTileW, g.nx - 1);
+      const int cx = min(max(X0 - 2 + lc, cx0), cx1 + 1);
+      if (cx0 <= cx1) off += a.start[cy * g.nx + cx] - row_first[r];
+    }
+    cell_off[r][lc] = off;
+  }
+  if (staged) {
+    for (int r = 0; r < NR; ++r) {
+      const int n = row_base[r + 1] - row_base[r];
+      for (int k = tid; k < n; k += kTileW * kTileH) {
+        const size_t p = (size_t)row_first[r] + k;
+        T x, y, vx, vy;
+        ld2(a.spos, p, x, y);
+        ld2(a.sval, p, vx, vy);
+        const int o = row_base[r] + k;
+        sgx[o] = ((double)x - g.x0) / g.dx + c.gxo;
+        sgy[o] = ((double)y - g.y0) / g.dy + c.gyo;
+        sv[o] = (double)(a.comp ? vy : vx);
+      }
+    }
+  }
+  __syncthreads();
+  const int fx = X0 + threadIdx.x, fy = Y0 + threadIdx.y;
+  long long uns = 0;
+  if (fx < c.W && fy < c.H) {
+    T wsum = (T)0, msum = (T)0;
+    for (int wy = -2; wy <= 1; ++wy) {
+      const int cy = fy + wy;
+      if (cy < 0 || cy >= g.ny) continue;
+      const int r = threadIdx.y + wy + 2;
+      for (int wx = -2; wx <= 1; ++wx) {
+        const int cx = fx + wx;
+        if (cx < 0 || cx >= g.nx) continue;
+        const int lc = threadIdx.x + wx + 2;
+        int k0, k1;
+        if (staged) {
+          k0 = cell_off[r][lc];
+          k1 = cell_off[r][lc + 1];
+        } else {
+          k0 = a.start[cy * g.nx + cx];
+          k1 = a.start[cy * g.nx + cx + 1];
+        }
+        for (int k = k0; k < k1; ++k) {
+          double gx, gy, val;
+          if (staged) {
+            gx = sgx[k];
+            gy = sgy[k];
+          } else {
+            T x, y;
+            ld2(a.spos, k, x, y);
+            gx = ((double)x - g.x0) / g.dx + c.gxo;
+            gy = ((double)y - g.y0) / g.dy + c.gyo;
+          }
+          const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+          if (fx < bx || fx > bx + 2) continue;
+          if (fy < by || fy > by + 2) continue;
+          const double w = bspline(gy - (double)fy) * bspline(gx - (double)fx);
+          if (staged) {
+            val = sv[k];
+          } else {
+            T vx, vy;
+            ld2(a.sval, k, vx, vy);
+            val = (double)(a.comp ? vy : vx);
+          }
+          wsum = (T)((double)wsum + w);
+          msum = (T)((double)msum + w * val);
+        }
+      }
+    }
+    T outv;
+    const bool supported = (double)wsum > 1.0e-12;
+    if (supported) outv = msum / wsum;
+    else outv = a.fallback[(size_t)fy * c.W + fx];
+    uns = supported ? 0 : 1;
+    a.out[(size_t)fy * c.W + fx] = outv;
+  }
+  uns = wave_sum_i(uns);
+  if ((tid & 63) == 0 && uns) atomicAdd(a.unsupported, (unsigned long long)uns);
+}
+
 // ------------------------------------------------------------------ G2P and the FLIP update
 
 template <class T>
@@ -889,8 +1017,13 @@ static int launch_p2g(hipStream_t s, const PicGeom& g, const SortBufs& b, const 
     a.fallback = c ? fv : fu;
     a.out = c ? tv : tu;
     a.unsupported = unsupported;
-    dim3 grid((a.c.W + 255) / 256, a.c.H);
-    hipLaunchKernelGGL(p2g_kernel<T>, grid, dim3(256), 0, s, a);
+    if (!g.px && !g.py) {
+      dim3 grid((a.c.W + kTileW - 1) / kTileW, (a.c.H + kTileH - 1) / kTileH);
+      hipLaunchKernelGGL(p2g_tile_kernel<T>, grid, dim3(kTileW, kTileH), 0, s, a);
+    } else {
+      dim3 grid((a.c.W + 255) / 256, a.c.H);
+      hipLaunchKernelGGL(p2g_kernel<T>, grid, dim3(256), 0, s, a);
+    }
   }
   FB_HIP(hipGetLastError());
   return FB_OK;
