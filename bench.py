@@ -47,10 +47,13 @@ def _gpu_forms(steps: int, warmup: int):
     out = {}
     velocities = {}
     solid = None
-    for name, prec in (("f32", N.FB_F32), ("f64", N.FB_F64), ("f32-mixed", N.FB_F32_MIXED),
-                       ("f16-mixed", N.FB_F16_MIXED)):
-        # the special-cell tile is the autotuner's pick for this lattice and storage form
-        tuned = tuner.lbm(case.nx, case.ny, name)["value"]
+    for name, prec, pname in (("f32", N.FB_F32, "FB_F32"), ("f64", N.FB_F64, "FB_F64"),
+                              ("f32-mixed", N.FB_F32_MIXED, "FB_F32_MIXED"),
+                              ("f16-mixed", N.FB_F16_MIXED, "FB_F16_MIXED")):
+        # fused pairs, interior rows per thread and the special-cell blocks are the autotuner's
+        # picks for this lattice and storage form (none of them changes a result bit)
+        rec = tuner.lbm(case.nx, case.ny, name)
+        tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks")}
         lat = Lattice(case, prec, ctx)
         lat.run(warmup)
         ctx.synchronize()
@@ -62,13 +65,18 @@ def _gpu_forms(steps: int, warmup: int):
         ev_steps = max(1, min(steps, 5000))
         avg_ms = lat.time(ev_steps) / ev_steps
         bytes_launch = BYTES_PER_CELL[prec] * lat.cells
+        per_launch = 2 if tuned["fuse"] else 1  # lattice steps per launch of the step kernel
         out[name] = {
             "cell_steps_per_s": lat.cells * steps / wall,
             "ms_per_step": 1e3 * wall / steps,
             "kernel_ms": avg_ms,
             "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
             "bytes_per_cell": BYTES_PER_CELL[prec],
-            "tuned_list_blocks": tuned,
+            "tuned_step_shape": tuned,
+            "kernel": (f"lbm_pair_kernel<Prec<{pname}>, 16>" if tuned["fuse"]
+                       else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}>"),
+            "steps_per_launch": per_launch,
+            "launch_ms": avg_ms * per_launch,
         }
         velocities[name] = lat.velocity().astype(np.float64)
         if solid is None:
@@ -150,13 +158,15 @@ def _legs(with_cpu: bool) -> dict:
     return out
 
 
-def _traffic():
-    """HBM bytes per launch of the step kernel, from the committed rocprofv3 --pmc summary."""
+def _traffic(kernel: str):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 --pmc summary when it was
+    measured on that kernel (otherwise None)."""
     p = ROOT / "profiles" / "lbm_step_kernel_1024x512_f32_traffic.json"
     if not p.exists():
         return None
     try:
-        return float(json.loads(p.read_text())["bytes_per_launch"])
+        rec = json.loads(p.read_text())
+        return float(rec["bytes_per_launch"]) if rec.get("kernel") == kernel else None
     except Exception:
         return None
 
@@ -176,7 +186,7 @@ def main() -> int:
     case, forms = _gpu_forms(args.steps, args.warmup)
     main_form = forms["f32"]
     achieved = main_form["kernel_gbs"]
-    traffic = _traffic()
+    traffic = _traffic(main_form["kernel"])
     cpu = _cpu_baseline(args.cpu_steps) if args.cpu_steps > 0 else None
     legs = None if args.no_legs else _legs(args.cpu_steps > 0)
     line = {
@@ -205,7 +215,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "lbm_step_kernel<Prec<FB_F32>>",
+            "kernel": main_form["kernel"],
+            "steps_per_launch": main_form["steps_per_launch"],
+            "launch_ms": main_form["launch_ms"],
             "bytes_per_cell": main_form["bytes_per_cell"],
         },
         "cpu_baseline": cpu,

@@ -135,6 +135,8 @@ FB_SERVE_SF = 0
 FB_SERVE_LBM = 1
 FB_TUNE_DST_ROWS = 0
 FB_TUNE_LBM_LIST_BLOCKS = 1
+FB_TUNE_LBM_FUSE = 2
+FB_TUNE_LBM_ROWS = 3
 
 
 class fb_server_result(C.Structure):

@@ -33,6 +33,7 @@ class LatticeCase:
     u_lattice: float = 0.05
     reynolds: float = 1000.0
     magic: float = 3.0 / 16.0
+    channel: int = 0  # bounce-back rows at the top and bottom instead of Zou-He rims
 
     @property
     def name(self) -> str:
@@ -61,11 +62,11 @@ class Lattice:
         self.ctx = ctx or N.Context.default()
         lib = self.ctx.lib
         dom = case.domain()
-        sy, sx = sponge_profiles(dom.nx, dom.ny, False, False, False)
+        sy, sx = sponge_profiles(dom.nx, dom.ny, False, False, bool(case.channel))
         h = C.c_void_p()
         foil = FoilSpec(case.naca, case.chord, case.pivot)
         N.check(lib.fb_lbm_create(self.ctx.handle, C.byref(domain_struct(dom)), C.byref(foil_struct(foil)),
-                                  precision, 0, N.ptr(sy), N.ptr(sx), C.byref(h)))
+                                  precision, int(case.channel), N.ptr(sy), N.ptr(sx), C.byref(h)))
         self.handle = h
         N.check(lib.fb_lbm_set_pose(h, C.byref(pose_struct(case.angle_degrees))))
         hd = np.float64 if precision == N.FB_F64 else np.float32
@@ -99,6 +100,22 @@ class Lattice:
 
     def run(self, steps: int) -> None:
         N.check(self.ctx.lib.fb_lbm_run_fixed(self.handle, C.byref(self.params), steps))
+
+    def advance(self, steps: int) -> N.fb_lbm_stats:
+        """`steps` lattice steps at the case's fixed pose, with the postcondition reductions of the
+        last step's pre-collision state."""
+        poses = (N.fb_pose * steps)(*[pose_struct(self.case.angle_degrees)] * steps)
+        st = N.fb_lbm_stats()
+        N.check(self.ctx.lib.fb_lbm_advance(self.handle, C.byref(self.params), steps, poses, C.byref(st)))
+        return st
+
+    def state(self) -> tuple[np.ndarray, np.ndarray]:
+        """Pre-collision populations (ny, nx, 9) and outlet history (ny, 9) after the last advance."""
+        hd = np.float64 if self.precision == N.FB_F64 else np.float32
+        f = np.empty((self.case.ny, self.case.nx, 9), dtype=hd)
+        out = np.empty((self.case.ny, 9), dtype=hd)
+        N.check(self.ctx.lib.fb_lbm_get_state(self.handle, N.ptr(f), N.ptr(out)))
+        return f, out
 
     def time(self, steps: int) -> float:
         """Milliseconds of `steps` back-to-back lattice steps, from one HIP event pair on the stream

@@ -117,14 +117,67 @@ struct LbmArgs {
   int write_pre, do_reset, prime;
   // launch shape: blocks [0, edge_blocks) cover the nedge edge cells, the next list_blocks
   // stride over the flagged (solid or near-wall) cells of rows 1..ny-2 listed in `list`;
-  // the rest cover rows 1..ny-2 with int_gx blocks per row, whose edge and flagged lanes idle
-  int int_gx, edge_blocks, nedge, list_blocks;
+  // the rest cover rows 1..ny-2, int_rows rows per block (1 or 2) and int_gx blocks across,
+  // whose edge and flagged lanes idle
+  int int_gx, int_rows, edge_blocks, nedge, list_blocks;
   int skip;  // batched launches: this lattice sits out the step (its blocks return at once)
   const int* __restrict__ list;
   const int* __restrict__ list_count;
   const double* __restrict__ links;  // 8 Bouzidi fractions per list entry (0: straight link)
+  const int* __restrict__ lidx;      // list position of each listed cell (other cells: unset)
   unsigned long long* red;  // [min f key, max speed key, max excursion key, nonfinite f, bad rho, nonfinite u]
 };
+
+// Where a lattice step reads the previous post-collision state f*, the collision density of the
+// near-wall cells and the outlet history, and where it writes the outlet history it produces.
+// PlaneSrc: the global planes. TileSrc: a block's LDS tile holding the first step of a fused
+// pair over the block's cells and a one-cell halo.
+template <class S>
+struct PlaneSrc {
+  const S* __restrict__ f;
+  const S* __restrict__ rho;
+  const S* __restrict__ hist;
+  S* hist_out;
+  int hist_y0;  // hist_out holds rows hist_y0.. (0 for the global history)
+  int nx;
+  size_t N;
+  __device__ __forceinline__ S at(int d, int x, int y) const { return f[(size_t)d * N + (size_t)y * nx + x]; }
+  __device__ __forceinline__ S at_in(int d, int x, int y) const { return at(d, x, y); }
+  __device__ __forceinline__ S rho_at(int x, int y) const { return rho[(size_t)y * nx + x]; }
+  __device__ __forceinline__ S hist_at(int y, int d) const { return hist[(size_t)y * 9 + d]; }
+  __device__ __forceinline__ void put_hist(int y, int d, S v) const { hist_out[(size_t)(y - hist_y0) * 9 + d] = v; }
+};
+
+// LDS tile of W x H cells whose corner is the domain cell (ox, oy). A read outside the tile is a
+// periodic-wrap read of a non-periodic lattice, whose value the boundary treatment overwrites
+// (Zou-He unknowns, corner equilibria); it returns 0.
+template <class S, int W, int H>
+struct TileSrc {
+  const S* f;     // [9][H][W]
+  const S* rho;   // [H][W]
+  const S* hist;  // [H][9]: outlet history written by the first step, tile rows
+  S* hist_out;    // global outlet history of the second step
+  int ox, oy;
+  __device__ __forceinline__ S at(int d, int x, int y) const {
+    const unsigned lx = (unsigned)(x - ox), ly = (unsigned)(y - oy);
+    return (lx < (unsigned)W && ly < (unsigned)H) ? f[d * (W * H) + (int)ly * W + (int)lx] : (S)0.0f;
+  }
+  // a read known to fall inside the tile
+  __device__ __forceinline__ S at_in(int d, int x, int y) const { return f[d * (W * H) + (y - oy) * W + (x - ox)]; }
+  __device__ __forceinline__ S rho_at(int x, int y) const { return rho[(y - oy) * W + (x - ox)]; }
+  __device__ __forceinline__ S hist_at(int y, int d) const { return hist[(y - oy) * 9 + d]; }
+  __device__ __forceinline__ void put_hist(int y, int d, S v) const { hist_out[(size_t)y * 9 + d] = v; }
+};
+
+// Interior blocks of the single-step kernel with `rows` interior rows per block.
+__host__ __device__ inline int interior_blocks(int nx, int ny, int rows) {
+  return (nx + 255) / 256 * ((ny - 2 + rows - 1) / rows);
+}
+
+template <class S>
+__device__ __forceinline__ PlaneSrc<S> plane_src(const LbmArgs<S>& a) {
+  return PlaneSrc<S>{a.fin, a.rho_in, a.outlet_in, a.outlet_out, 0, a.nx, (size_t)a.nx * a.ny};
+}
 
 // ------------------------------------------------------------------ streaming pieces
 
@@ -132,29 +185,31 @@ struct LbmArgs {
 
 // Bouzidi reflection at fluid cell (x,y) for the link e pointing into solid cell n;
 // the result lands in direction opp(e) at (x,y) (kern/lbm.py:276-308).
-template <class P>
-__device__ __forceinline__ typename P::C reflect(const LbmArgs<typename P::S>& a, int x, int y, int e, int nxs,
-                                                 int nys) {
+template <class P, class Src>
+__device__ __forceinline__ typename P::C reflect(const LbmArgs<typename P::S>& a, const Src& src_f, int x, int y,
+                                                 int e, int nxs, int nys) {
   using S = typename P::S;
   using C = typename P::C;
-  const size_t N = (size_t)a.nx * a.ny;
   const size_t i = (size_t)y * a.nx + x;
   double sx = a.sdf[i];
   double sn = a.sdf[(size_t)nys * a.nx + nxs];
   double den = fmax(sx - sn, 1.0e-12);
   double q = fmin(fmax(sx / den, 0.05), 1.0);
-  C src = ld<S, C>(a.fin + (size_t)e * N + i);
+  const S sv = src_f.at(e, x, y);
+  C src = ld<S, C>(&sv);
   C r;
   if (q < 0.5) {
     int ux = wrapi(x - D2Q9::cx[e], a.nx), uy = wrapi(y - D2Q9::cy[e], a.ny);
-    C up = ld<S, C>(a.fin + (size_t)e * N + (size_t)uy * a.nx + ux);
+    const S uv = src_f.at(e, ux, uy);
+    C up = ld<S, C>(&uv);
     C q2 = (C)(2.0 * q);
     C q12 = (C)(1.0 - 2.0 * q);
     r = q2 * src + q12 * up;
   } else {
     C q2 = (C)(2.0 * q);
     C q21 = (C)(2.0 * q - 1.0);
-    C fo = ld<S, C>(a.fin + (size_t)D2Q9::opp[e] * N + i);
+    const S ov = src_f.at(D2Q9::opp[e], x, y);
+    C fo = ld<S, C>(&ov);
     r = src / q2 + q21 * fo / q2;
   }
   if (a.wall_omega != 0.0) {
@@ -163,7 +218,8 @@ __device__ __forceinline__ typename P::C reflect(const LbmArgs<typename P::S>& a
     double wu = (-a.wall_omega) * (wy - a.pivot_y);
     double wv = a.wall_omega * (wx - a.pivot_x);
     double wp = (double)D2Q9::cx[e] * wu + (double)D2Q9::cy[e] * wv;
-    C rho = ld<S, C>(a.rho_in + i);
+    const S rv = src_f.rho_at(x, y);
+    C rho = ld<S, C>(&rv);
     if constexpr (P::shift) rho = rho + (C)1;
     r = r - (C)6.0 * (C)D2Q9::w[e] * rho * (C)wp * (C)a.wall_scale;
   }
@@ -171,13 +227,14 @@ __device__ __forceinline__ typename P::C reflect(const LbmArgs<typename P::S>& a
 }
 
 // Pull-stream of cell (x,y): the value each direction receives (kern/lbm.py:262-308).
-template <class P>
-__device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+template <class P, class Src>
+__device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, const Src& src, int x, int y,
+                                     typename P::C g[9]) {
   using S = typename P::S;
   using C = typename P::C;
-  const size_t N = (size_t)a.nx * a.ny;
   const size_t i = (size_t)y * a.nx + x;
-  g[0] = ld<S, C>(a.fin + i);
+  const S v0 = src.at(0, x, y);
+  g[0] = ld<S, C>(&v0);
   if (a.flags[i] & FL_SOLID) {
 #pragma unroll
     for (int d = 1; d < 9; ++d) g[d] = (C)0;
@@ -192,29 +249,31 @@ __device__ __forceinline__ void pull(const LbmArgs<typename P::S>& a, int x, int
   for (int d = 1; d < 9; ++d) {
     int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
     size_t s = (size_t)sy * a.nx + sx;
-    if (near && (a.flags[s] & FL_SOLID))
-      g[d] = reflect<P>(a, x, y, D2Q9::opp[d], sx, sy);
-    else
-      g[d] = ld<S, C>(a.fin + (size_t)d * N + s);
+    if (near && (a.flags[s] & FL_SOLID)) {
+      g[d] = reflect<P>(a, src, x, y, D2Q9::opp[d], sx, sy);
+    } else {
+      const S v = src.at(d, sx, sy);
+      g[d] = ld<S, C>(&v);
+    }
   }
 }
 
 // Pull-stream with every straight source loaded up front: cells whose own flags are clear
 // (no solid cell among the neighbours) need nothing else; flagged cells fall back to pull().
-template <class P>
-__device__ __forceinline__ void pull_fast(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+template <class P, class Src>
+__device__ __forceinline__ void pull_fast(const LbmArgs<typename P::S>& a, const Src& src, int x, int y,
+                                          typename P::C g[9]) {
   using S = typename P::S;
   using C = typename P::C;
-  const size_t N = (size_t)a.nx * a.ny;
   const size_t i = (size_t)y * a.nx + x;
   S v[9];
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
     const int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
-    v[d] = a.fin[(size_t)d * N + (size_t)sy * a.nx + sx];
+    v[d] = src.at(d, sx, sy);
   }
   if (a.flags[i] != 0) {
-    pull<P>(a, x, y, g);
+    pull<P>(a, src, x, y, g);
     return;
   }
 #pragma unroll
@@ -224,15 +283,14 @@ __device__ __forceinline__ void pull_fast(const LbmArgs<typename P::S>& a, int x
 // Pull-stream of a flagged interior cell from its precomputed links: q[d-1] > 0 marks the
 // direction d whose source cell is solid, holding the Bouzidi fraction reflect() would compute
 // (same expression, so the same bits). Every operand is loaded up front, in one round.
-template <class P>
-__device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, int x, int y, bool solid,
-                                            const double* __restrict__ q, typename P::C g[9]) {
+template <class P, class Src>
+__device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, const Src& src, int x, int y,
+                                            bool solid, const double* __restrict__ q, typename P::C g[9]) {
   using S = typename P::S;
   using C = typename P::C;
-  const size_t N = (size_t)a.nx * a.ny;
-  const size_t i = (size_t)y * a.nx + x;
   if (solid) {
-    g[0] = ld<S, C>(a.fin + i);
+    const S v0 = src.at(0, x, y);
+    g[0] = ld<S, C>(&v0);
 #pragma unroll
     for (int d = 1; d < 9; ++d) g[d] = P::shift ? -(C)D2Q9::w[d] : (C)0;
     return;
@@ -241,19 +299,20 @@ __device__ __forceinline__ void pull_linked(const LbmArgs<typename P::S>& a, int
   S at[9], straight[9], up[9];
   double qd[9];
 #pragma unroll
-  for (int d = 0; d < 9; ++d) at[d] = a.fin[(size_t)d * N + i];
+  for (int d = 0; d < 9; ++d) at[d] = src.at(d, x, y);
 #pragma unroll
   for (int d = 1; d < 9; ++d) {
     const int sx = wrapi(x - D2Q9::cx[d], a.nx), sy = wrapi(y - D2Q9::cy[d], a.ny);
-    straight[d] = a.fin[(size_t)d * N + (size_t)sy * a.nx + sx];
+    straight[d] = src.at(d, sx, sy);
     // upstream of the reflected link e = opp(d): the cell (x - c_e) = (x + c_d)
     const int ux = wrapi(x + D2Q9::cx[d], a.nx), uy = wrapi(y + D2Q9::cy[d], a.ny);
-    up[d] = a.fin[(size_t)D2Q9::opp[d] * N + (size_t)uy * a.nx + ux];
+    up[d] = src.at(D2Q9::opp[d], ux, uy);
     qd[d] = q[d - 1];
   }
   C rho_w = (C)0;
   if (a.wall_omega != 0.0) {
-    rho_w = ld<S, C>(a.rho_in + i);
+    const S rv = src.rho_at(x, y);
+    rho_w = ld<S, C>(&rv);
     if constexpr (P::shift) rho_w = rho_w + (C)1;
   }
   g[0] = ld<S, C>(&at[0]);
@@ -358,54 +417,74 @@ __device__ __forceinline__ void left_bc(const LbmArgs<typename P::S>& a, typenam
   reshift<P>(g);
 }
 
-// Streamed values of (x,y) after the y-boundary treatment (fb/solvers/lbm.py:612-627).
-template <class P>
-__device__ __forceinline__ void streamed_row(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
-  using C = typename P::C;
-  const bool edge = !a.periodic_y && (y == 0 || y == a.ny - 1);
-  if (edge && a.channel) {
-    C h[9];
-    pull_fast<P>(a, x, y == 0 ? 1 : a.ny - 2, h);
-#pragma unroll
-    for (int d = 0; d < 9; ++d) g[d] = h[D2Q9::opp[d]];
-    return;
-  }
-  pull_fast<P>(a, x, y, g);
-  if (edge) {
-    if (y == 0) bottom_bc<P>(a, g);
-    else top_bc<P>(a, g);
-  }
+// The cell whose pull an edge cell's boundary treatment starts from (fb/solvers/lbm.py:612-640):
+// the cell itself; for the rows of channel walls, the row inside (mirrored); for the outlet
+// column, the column before it (with the same row rule).
+__device__ __forceinline__ void edge_source(int nx, int ny, int periodic_x, int periodic_y, int channel, int x,
+                                            int y, int& px, int& py, bool& mirror) {
+  px = (!periodic_x && x == nx - 1) ? nx - 2 : x;
+  mirror = channel && !periodic_y && (y == 0 || y == ny - 1);
+  py = mirror ? (y == 0 ? 1 : ny - 2) : y;
 }
 
-// Final pre-collision value of cell (x,y) for this step, before the sponge
-// (fb/solvers/lbm.py:612-645).
-template <class P>
-__device__ __forceinline__ void boundary_cell(const LbmArgs<typename P::S>& a, int x, int y, typename P::C g[9]) {
+// Boundary treatment of edge cell (x,y) (fb/solvers/lbm.py:612-645) from h, the pull of its
+// source cell (edge_source), and for the outlet column its history `hist`: mirrored wall rows or
+// Zou-He rows, then the convective outlet (whose new history goes out through src.put_hist) or the
+// Zou-He inlet, then the corner equilibria.
+template <class P, class Src>
+__device__ __forceinline__ void edge_fixup(const LbmArgs<typename P::S>& a, const Src& src, int x, int y,
+                                           bool mirror, const typename P::C h[9], const typename P::S hist[9],
+                                           typename P::C g[9]) {
   using S = typename P::S;
   using C = typename P::C;
   using B = typename P::B;
-  if (!a.periodic_x && x == a.nx - 1) {
-    S hist[9];
+  if (mirror) {
 #pragma unroll
-    for (int d = 0; d < 9; ++d) hist[d] = a.outlet_in[(size_t)y * 9 + d];
-    C g2[9];
-    streamed_row<P>(a, a.nx - 2, y, g2);
+    for (int d = 0; d < 9; ++d) g[d] = h[D2Q9::opp[d]];
+  } else {
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = h[d];
+    if (!a.periodic_y && (y == 0 || y == a.ny - 1)) {
+      if (y == 0) bottom_bc<P>(a, g);
+      else top_bc<P>(a, g);
+    }
+  }
+  if (!a.periodic_x && x == a.nx - 1) {
     const B u = (B)a.u_lattice;
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
       B prev = (B)ld<S, C>(&hist[d]);
-      B v = prev + u * ((B)g2[d] - prev);
+      B v = prev + u * ((B)g[d] - prev);
       g[d] = (C)v;
-      st<S, C>(a.outlet_out + (size_t)y * 9 + d, (C)v);
+      S sv;
+      st<S, C>(&sv, (C)v);
+      src.put_hist(y, d, sv);
     }
-  } else {
-    streamed_row<P>(a, x, y, g);
-    if (!a.periodic_x && x == 0) left_bc<P>(a, g);
+  } else if (!a.periodic_x && x == 0) {
+    left_bc<P>(a, g);
   }
   if (!a.periodic_x && !a.periodic_y && (x == 0 || x == a.nx - 1) && (y == 0 || y == a.ny - 1)) {
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_b[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
   }
+}
+
+// Final pre-collision value of edge cell (x,y) for this step, before the sponge.
+template <class P, class Src>
+__device__ __forceinline__ void boundary_cell(const LbmArgs<typename P::S>& a, const Src& src, int x, int y,
+                                              typename P::C g[9]) {
+  using S = typename P::S;
+  using C = typename P::C;
+  int px, py;
+  bool mirror;
+  edge_source(a.nx, a.ny, a.periodic_x, a.periodic_y, a.channel, x, y, px, py, mirror);
+  S hist[9];
+  const bool outlet = !a.periodic_x && x == a.nx - 1;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) hist[d] = outlet ? src.hist_at(y, d) : (S)0.0f;
+  C h[9];
+  pull_fast<P>(a, src, px, py, h);
+  edge_fixup<P>(a, src, x, y, mirror, h, hist, g);
 }
 
 template <class P>
@@ -554,13 +633,53 @@ __device__ __forceinline__ void edge_cell(int e, int nx, int ny, int& x, int& y)
   }
 }
 
-// One cell of a lattice step: pull (straight for interior cells, the boundary treatment for
-// edge and flagged cells), sponge, uncovered-cell reset, optional pre-collision write with the
-// postcondition statistics, TRT collision, write f*.
-template <class P>
-__device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x, int y, bool straight,
-                                          const double* links, bool solid, double& mn, double& vmax, double& exc,
-                                          long long& nf, long long& bad, long long& nfu) {
+// Where a lattice step writes f* and the collision density of the near-wall cells: the global
+// planes, or the LDS tile of the first step of a fused pair.
+template <class S>
+struct PlaneSink {
+  S* f;
+  S* rho;
+  int nx;
+  size_t N;
+  template <class C>
+  __device__ __forceinline__ void put(int d, int x, int y, C v) const {
+    st<S, C>(f + (size_t)d * N + (size_t)y * nx + x, v);
+  }
+  template <class C>
+  __device__ __forceinline__ void put_rho(int x, int y, C v) const {
+    st<S, C>(rho + (size_t)y * nx + x, v);
+  }
+};
+
+template <class S, int W, int H>
+struct TileSink {
+  S* f;    // [9][H][W]
+  S* rho;  // [H][W]
+  int ox, oy;
+  template <class C>
+  __device__ __forceinline__ void put(int d, int x, int y, C v) const {
+    st<S, C>(f + d * (W * H) + (y - oy) * W + (x - ox), v);
+  }
+  template <class C>
+  __device__ __forceinline__ void put_rho(int x, int y, C v) const {
+    st<S, C>(rho + (y - oy) * W + (x - ox), v);
+  }
+};
+
+template <class S>
+__device__ __forceinline__ PlaneSink<S> plane_sink(const LbmArgs<S>& a) {
+  return PlaneSink<S>{a.fout, a.rho_out, a.nx, (size_t)a.nx * a.ny};
+}
+
+// One cell of a lattice step: pull (straight for interior cells whose neighbours are all fluid,
+// the boundary treatment for edge cells, the precomputed links for flagged cells), sponge,
+// uncovered-cell reset, optional pre-collision write with the postcondition statistics, TRT
+// collision, write f*. `prime` collides the pre-collision state in place instead of pulling.
+template <class P, class Src, class Sink>
+__device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, const Src& src, const Sink& out, int x,
+                                          int y, bool straight, const double* links, bool solid, bool prime,
+                                          bool reset, bool pre, double& mn, double& vmax, double& exc, long long& nf,
+                                          long long& bad, long long& nfu) {
   using S = typename P::S;
   using C = typename P::C;
   const size_t N = (size_t)a.nx * a.ny;
@@ -568,28 +687,33 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
   // operands that do not depend on the pull are loaded first, so the cell costs one round
   const double spy = a.sponge_y[y], spx = a.sponge_x[x];
   const uint8_t fnext = a.flags_next[i];
-  const uint8_t fcur = a.do_reset ? a.flags[i] : (uint8_t)0;
+  const uint8_t fcur = reset ? a.flags[i] : (uint8_t)0;
   C g[9];
-  if (a.prime) {
+  if (prime) {
 #pragma unroll
-    for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(a.fin + (size_t)d * N + i);
+    for (int d = 0; d < 9; ++d) {
+      const S v = src.at(d, x, y);
+      g[d] = ld<S, C>(&v);
+    }
   } else {
     if (straight) {
 #pragma unroll
-      for (int d = 0; d < 9; ++d)
-        g[d] = ld<S, C>(a.fin + (size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d]));
+      for (int d = 0; d < 9; ++d) {
+        const S v = src.at_in(d, x - D2Q9::cx[d], y - D2Q9::cy[d]);
+        g[d] = ld<S, C>(&v);
+      }
     } else if (links) {
-      pull_linked<P>(a, x, y, solid, links, g);
+      pull_linked<P>(a, src, x, y, solid, links, g);
     } else {
-      boundary_cell<P>(a, x, y, g);
+      boundary_cell<P>(a, src, x, y, g);
     }
     apply_sponge<P>(a, spy, spx, g);
   }
-  if (a.do_reset && (fcur & FL_SOLID) && !(fnext & FL_SOLID)) {
+  if (reset && (fcur & FL_SOLID) && !(fnext & FL_SOLID)) {
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = (C)a.feq_u[d] - (P::shift ? (C)D2Q9::w[d] : (C)0);
   }
-  if (a.write_pre) {
+  if (pre) {
 #pragma unroll
     for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
     cell_stats<P>(a, g, !(fnext & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
@@ -599,15 +723,88 @@ __device__ __forceinline__ void step_cell(const LbmArgs<typename P::S>& a, int x
   if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
   else rho = collide<P>(a, g, post);
 #pragma unroll
-  for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
-  if (fnext & FL_NEAR) st<S, C>(a.rho_out + i, rho);
+  for (int d = 0; d < 9; ++d) out.put(d, x, y, post[d]);
+  if (fnext & FL_NEAR) out.put_rho(x, y, rho);
+}
+
+// Block reduction of the postcondition statistics: wave shuffles, then one atomic per wave.
+__device__ __forceinline__ void put_step_stats(unsigned long long* red, double mn, double vmax, double exc,
+                                               long long nf, long long bad, long long nfu) {
+  mn = wave_min(mn);
+  vmax = wave_max(vmax);
+  exc = wave_max(exc);
+  nf = wave_sum_ll(nf);
+  bad = wave_sum_ll(bad);
+  nfu = wave_sum_ll(nfu);
+  if ((threadIdx.x & 63) == 0) {
+    key_min(&red[0], dkey(mn));
+    key_max(&red[1], dkey(vmax));
+    key_max(&red[2], dkey(exc));
+    if (nf) atomicAdd(&red[3], (unsigned long long)nf);
+    if (bad) atomicAdd(&red[4], (unsigned long long)bad);
+    if (nfu) atomicAdd(&red[5], (unsigned long long)nfu);
+  }
+}
+
+// ROWS interior cells of column x from row y0 (rows past ny-2 and columns 0, nx-1 idle): the
+// straight pull of every cell, its flags, next-step flags and sponge weights are loaded before any
+// cell is computed, so the cells share one round of loads; flagged cells are left to the list
+// blocks. step_cell's arithmetic for straight cells (whose own flags are clear, so no reset).
+template <class P, int ROWS>
+__device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, int x, int y0, double& mn,
+                                               double& vmax, double& exc, long long& nf, long long& bad,
+                                               long long& nfu) {
+  using S = typename P::S;
+  using C = typename P::C;
+  const size_t N = (size_t)a.nx * a.ny;
+  const bool xin = x > 0 && x < a.nx - 1;
+  S v[ROWS][9];
+  uint8_t fl[ROWS], fnext[ROWS];
+  double spy[ROWS];
+  const double spx = xin ? a.sponge_x[x] : 0.0;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int y = y0 + r;
+    fl[r] = 1;
+    if (xin && y <= a.ny - 2) {
+      const size_t i = (size_t)y * a.nx + x;
+#pragma unroll
+      for (int d = 0; d < 9; ++d)
+        v[r][d] = a.fin[(size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d])];
+      fl[r] = a.flags[i];
+      fnext[r] = a.flags_next[i];
+      spy[r] = a.sponge_y[y];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    if (fl[r] != 0) continue;
+    const int y = y0 + r;
+    const size_t i = (size_t)y * a.nx + x;
+    C g[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(&v[r][d]);
+    apply_sponge<P>(a, spy[r], spx, g);
+    if (a.write_pre) {
+#pragma unroll
+      for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
+      cell_stats<P>(a, g, !(fnext[r] & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
+    }
+    C post[9];
+    C rho;
+    if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
+    else rho = collide<P>(a, g, post);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
+    if (fnext[r] & FL_NEAR) st<S, C>(a.rho_out + i, rho);
+  }
 }
 
 // One lattice step. The edge cells and the flagged cells (foil interior and its neighbours)
 // run in the first blocks of the grid, so their longer dependent chains start early and do not
 // diverge the interior waves; interior blocks take the straight pull.
 // Block `blk` of the `nblk` blocks that cover lattice `a` for one step.
-template <class P>
+template <class P, int ROWS>
 __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsigned blk, unsigned nblk) {
   double mn = INFINITY, vmax = 0.0, exc = 0.0;
   long long nf = 0, bad = 0, nfu = 0;
@@ -617,7 +814,8 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
     if (e < a.nedge) {
       int x, y;
       edge_cell(e, a.nx, a.ny, x, y);
-      step_cell<P>(a, x, y, false, nullptr, false, mn, vmax, exc, nf, bad, nfu);
+      step_cell<P>(a, plane_src(a), plane_sink(a), x, y, false, nullptr, false, a.prime, a.do_reset, a.write_pre, mn,
+                   vmax, exc, nf, bad, nfu);
     }
   } else if (blk < special) {
     const int count = *a.list_count;
@@ -625,49 +823,45 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
     for (int k = (int)((blk - a.edge_blocks) * blockDim.x + threadIdx.x); k < count; k += stride) {
       const int entry = a.list[k];
       const int c = entry & 0x7fffffff;
-      step_cell<P>(a, c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, entry < 0, mn, vmax, exc, nf, bad, nfu);
+      step_cell<P>(a, plane_src(a), plane_sink(a), c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, entry < 0,
+                   a.prime, a.do_reset, a.write_pre, mn, vmax, exc, nf, bad, nfu);
     }
   } else {
     const unsigned bi = xcd_remap(blk - special, nblk - special);
     const unsigned gx = (unsigned)a.int_gx;
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
-    const int y = 1 + (int)(bi / gx);
-    if (x > 0 && x < a.nx - 1 && a.flags[(size_t)y * a.nx + x] == 0)
-      step_cell<P>(a, x, y, true, nullptr, false, mn, vmax, exc, nf, bad, nfu);
-  }
-  if (a.write_pre) {
-    // Block reduction: wave shuffles, then one atomic per wave.
-    mn = wave_min(mn);
-    vmax = wave_max(vmax);
-    exc = wave_max(exc);
-    nf = wave_sum_ll(nf);
-    bad = wave_sum_ll(bad);
-    nfu = wave_sum_ll(nfu);
-    if ((threadIdx.x & 63) == 0) {
-      key_min(&a.red[0], dkey(mn));
-      key_max(&a.red[1], dkey(vmax));
-      key_max(&a.red[2], dkey(exc));
-      if (nf) atomicAdd(&a.red[3], (unsigned long long)nf);
-      if (bad) atomicAdd(&a.red[4], (unsigned long long)bad);
-      if (nfu) atomicAdd(&a.red[5], (unsigned long long)nfu);
+    const int y0 = 1 + ROWS * (int)(bi / gx);
+    if (ROWS == 1 || a.prime) {
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const int y = y0 + r;
+        if (x > 0 && x < a.nx - 1 && y <= a.ny - 2 && a.flags[(size_t)y * a.nx + x] == 0)
+          step_cell<P>(a, plane_src(a), plane_sink(a), x, y, true, nullptr, false, a.prime, a.do_reset, a.write_pre,
+                       mn, vmax, exc, nf, bad, nfu);
+      }
+    } else {
+      interior_cells<P, ROWS>(a, x, y0, mn, vmax, exc, nf, bad, nfu);
     }
   }
+  if (a.write_pre) put_step_stats(a.red, mn, vmax, exc, nf, bad, nfu);
 }
 
-template <class P>
+// One lattice step with ROWS interior rows per block (a kernel of its own per ROWS, so that each
+// gets its own register allocation).
+template <class P, int ROWS>
 __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
-  lbm_block<P>(a, blockIdx.x, gridDim.x);
+  lbm_block<P, ROWS>(a, blockIdx.x, gridDim.x);
 }
 
 // Blocks of one lattice step.
 template <class S>
 __host__ __device__ inline int lbm_step_blocks(const LbmArgs<S>& a) {
-  return a.edge_blocks + a.list_blocks + a.int_gx * (a.ny - 2);
+  return a.edge_blocks + a.list_blocks + interior_blocks(a.nx, a.ny, a.int_rows);
 }
 
-// One step of many lattices in one launch. Lattice k's blocks start at start[k], a multiple of
-// the XCD count, so that its tiles map onto the XCDs as in a launch of its own; the padding
-// blocks between lattices return at once.
+// One step of many lattices in one launch (one interior row per block). Lattice k's blocks start
+// at start[k], a multiple of the XCD count, so that its tiles map onto the XCDs as in a launch of
+// its own; the padding blocks between lattices return at once.
 template <class P>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const LbmArgs<typename P::S>* __restrict__ args,
                                                         const int* __restrict__ start, int n) {
@@ -682,7 +876,235 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const LbmArgs<typename P
   const int nblk = lbm_step_blocks(a);
   const int local = b - start[lo];
   if (a.skip || local >= nblk) return;
-  lbm_block<P>(a, (unsigned)local, (unsigned)nblk);
+  lbm_block<P, 1>(a, (unsigned)local, (unsigned)nblk);
+}
+
+// Postcondition statistics of the cells a thread has stepped.
+struct StepStats {
+  double mn = INFINITY, vmax = 0.0, exc = 0.0;
+  long long nf = 0, bad = 0, nfu = 0;
+};
+
+// p - c wrapped onto [0, n) for |c| <= 1 (wrapi for these arguments, without a division).
+__device__ __forceinline__ int wrap1(int p, int c, int n) {
+  int s = p - c;
+  s += s < 0 ? n : 0;
+  s -= s >= n ? n : 0;
+  return s;
+}
+
+// A cell of a fused pair that is not near the wall (the geometry unchanged over the pair): an edge
+// cell from the pull v of its source cell (and the outlet history), a solid cell from its own rest
+// population v[0], or a fluid cell from its straight pull v. step_cell's arithmetic for such cells.
+template <class P, class Src, class Sink>
+__device__ __forceinline__ void plain_cell(const LbmArgs<typename P::S>& a, const Src& src, const Sink& out, int x,
+                                           int y, const typename P::S v[9], bool edge, bool mirror,
+                                           const typename P::S hist[9], bool solid, double spy, double spx, bool pre,
+                                           StepStats& acc) {
+  using S = typename P::S;
+  using C = typename P::C;
+  C g[9];
+  if (edge) {
+    C h[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) h[d] = ld<S, C>(&v[d]);
+    edge_fixup<P>(a, src, x, y, mirror, h, hist, g);
+  } else {
+    g[0] = ld<S, C>(&v[0]);
+#pragma unroll
+    for (int d = 1; d < 9; ++d) g[d] = solid ? (P::shift ? -(C)D2Q9::w[d] : (C)0) : ld<S, C>(&v[d]);
+  }
+  apply_sponge<P>(a, spy, spx, g);
+  if (pre) {
+    const size_t N = (size_t)a.nx * a.ny;
+    const size_t i = (size_t)y * a.nx + x;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
+    cell_stats<P>(a, g, !solid, acc.mn, acc.vmax, acc.exc, acc.nf, acc.bad, acc.nfu);
+  }
+  C post[9];
+  if constexpr (P::shift) (void)collide_shifted<P>(a, g, post);
+  else (void)collide<P>(a, g, post);
+#pragma unroll
+  for (int d = 0; d < 9; ++d) out.put(d, x, y, post[d]);
+}
+
+__device__ __forceinline__ void merge_stats(StepStats& a, const StepStats& b) {
+  a.mn = fmin(a.mn, b.mn);
+  a.vmax = fmax(a.vmax, b.vmax);
+  a.exc = fmax(a.exc, b.exc);
+  a.nf += b.nf;
+  a.bad += b.bad;
+  a.nfu += b.nfu;
+}
+
+// Appends the lanes with `on` set to the block's LDS list: the cell's tile index and its list
+// position k in the flagged-cell list.
+__device__ __forceinline__ void list_append(bool on, int code, int k, unsigned short* codes, int* ks, int* count) {
+  const unsigned long long m = __ballot(on);
+  if (m == 0) return;
+  const int lane = (int)threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, 0, 64);
+  if (on) {
+    const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    codes[slot] = (unsigned short)code;
+    ks[slot] = k;
+  }
+}
+
+// Two lattice steps in one launch, for a non-periodic lattice whose geometry stays the same over
+// both steps and whose flagged cells keep off the three outermost rows and columns. A block owns a
+// tile of 64 x TH cells. The first step runs over the tile and a one-cell halo, reading the global
+// f* planes, and leaves its f*, near-wall collision densities and outlet history in LDS; the second
+// step runs over the tile from LDS and writes the global planes (and the statistics, on the last
+// step of an advance). Every cell takes the single-step kernel's arithmetic, so the results are the
+// same bits, and each step's f* crosses HBM once per two steps. The halo cells are stepped by this
+// block and by their own: (TH + 2) * 66 first-step cells per 64 * TH output cells.
+// In each step, one tile row per wave, every lane issues its pull (the straight pull, or for an
+// edge cell the wrapped pull of its source cell) together with its flags, so fluid, solid and edge
+// cells cost one round of loads; the near-wall cells, which need their Bouzidi links, are listed in
+// LDS with their list positions and then run densely.
+template <int TH>
+struct PairTile {
+  static constexpr int W = 66, H = TH + 2, A = W * H;
+};
+constexpr int kPairWaves = 8;
+template <class S>
+constexpr int pair_rows() {
+  return sizeof(S) == 8 ? 8 : 16;
+}
+
+template <class P, int TH>
+__global__ void __launch_bounds__(64 * kPairWaves) __attribute__((amdgpu_waves_per_eu(4, 8)))
+lbm_pair_kernel(LbmArgs<typename P::S> a) {
+  using S = typename P::S;
+  constexpr int W = PairTile<TH>::W, H = PairTile<TH>::H, A = PairTile<TH>::A;
+  constexpr int R2 = (TH + kPairWaves - 1) / kPairWaves;  // second-step rows per wave
+  __shared__ S lf[9 * A];
+  __shared__ S lrho[A];
+  __shared__ S lhist[H * 9];
+  __shared__ unsigned short lcode[A];
+  __shared__ int lk[A];
+  __shared__ int nlist[2];
+  const unsigned ntx = (unsigned)(a.nx + 63) / 64;
+  const unsigned bi = xcd_remap(blockIdx.x, gridDim.x);
+  const int X0 = (int)(bi % ntx) * 64, Y0 = (int)(bi / ntx) * TH;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int nx = a.nx, ny = a.ny;
+  const size_t N = (size_t)nx * ny;
+  if (threadIdx.x < 2) nlist[threadIdx.x] = 0;
+  // second-step operands that do not depend on the first step, loaded ahead
+  uint8_t fl2[R2];
+  int k2[R2];
+  double spy2[R2];
+  const int x2 = X0 + lane;
+  const double spx2 = x2 < nx ? a.sponge_x[x2] : 0.0;
+#pragma unroll
+  for (int r = 0; r < R2; ++r) {
+    const int y = Y0 + wv + r * kPairWaves;
+    const bool in = wv + r * kPairWaves < TH && x2 < nx && y < ny;
+    const bool edge = x2 == 0 || y == 0 || x2 == nx - 1 || y == ny - 1;
+    fl2[r] = (in && !edge) ? a.flags[(size_t)y * nx + x2] : (uint8_t)0;
+    spy2[r] = in ? a.sponge_y[y] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < R2; ++r) {
+    const int y = Y0 + wv + r * kPairWaves;
+    k2[r] = (fl2[r] & FL_NEAR) ? a.lidx[(size_t)y * nx + x2] : 0;
+  }
+  StepStats st;
+  __syncthreads();
+  {
+    // first step: H rows of the 64 tile columns, then the 2 H halo-column cells
+    const PlaneSrc<S> src{a.fin, a.rho_in, a.outlet_in, lhist, Y0 - 1, nx, N};
+    const TileSink<S, W, H> out{lf, lrho, X0 - 1, Y0 - 1};
+    constexpr int jobs = H + (2 * H + 63) / 64;
+    for (int j = wv; j < jobs; j += kPairWaves) {
+      int lx, ly;
+      bool in = true;
+      if (j < H) {
+        lx = 1 + lane;
+        ly = j;
+      } else {
+        const int h = (j - H) * 64 + lane;
+        in = h < 2 * H;
+        ly = h >> 1;
+        lx = (h & 1) ? W - 1 : 0;
+      }
+      const int x = X0 - 1 + lx, y = Y0 - 1 + ly;
+      in = in && x >= 0 && x < nx && y >= 0 && y < ny;
+      const bool edge = x == 0 || y == 0 || x == nx - 1 || y == ny - 1;
+      int px = x, py = y;
+      bool mirror = false;
+      if (edge) edge_source(nx, ny, 0, 0, a.channel, x, y, px, py, mirror);
+      S v[9], hist[9];
+      uint8_t fl = 0;
+      double spy = 0.0, spx = 0.0;
+      if (in) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+          v[d] = a.fin[(size_t)d * N + (size_t)wrap1(py, D2Q9::cy[d], ny) * nx + wrap1(px, D2Q9::cx[d], nx)];
+        const bool outlet = x == nx - 1;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) hist[d] = outlet ? a.outlet_in[(size_t)y * 9 + d] : (S)0.0f;
+        if (!edge) fl = a.flags[(size_t)y * nx + x];
+        spy = a.sponge_y[y];
+        spx = a.sponge_x[x];
+      }
+      const bool near = (fl & FL_NEAR) != 0;
+      const int k = near ? a.lidx[(size_t)y * nx + x] : 0;
+      if (in && !near) plain_cell<P>(a, src, out, x, y, v, edge, mirror, hist, (fl & FL_SOLID) != 0, spy, spx, false, st);
+      list_append(in && near, ly * W + lx, k, lcode, lk, &nlist[0]);
+    }
+    __syncthreads();
+    const int n = nlist[0];
+    for (int m = (int)threadIdx.x; m < n; m += 64 * kPairWaves) {
+      const int code = lcode[m];
+      const int x = X0 - 1 + code % W, y = Y0 - 1 + code / W;
+      step_cell<P>(a, src, out, x, y, false, a.links + (size_t)lk[m] * 8, false, false, false, false, st.mn, st.vmax,
+                   st.exc, st.nf, st.bad, st.nfu);
+    }
+  }
+  __syncthreads();
+  {
+    const TileSrc<S, W, H> src{lf, lrho, lhist, a.outlet_out, X0 - 1, Y0 - 1};
+    const PlaneSink<S> out = plane_sink(a);
+    const bool pre = a.write_pre != 0;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      const int ry = wv + r * kPairWaves;
+      const int x = x2, y = Y0 + ry;
+      const bool in = ry < TH && x < nx && y < ny;
+      const bool edge = x == 0 || y == 0 || x == nx - 1 || y == ny - 1;
+      const bool near = (fl2[r] & FL_NEAR) != 0;
+      if (in && !near) {
+        int px = x, py = y;
+        bool mirror = false;
+        if (edge) edge_source(nx, ny, 0, 0, a.channel, x, y, px, py, mirror);
+        S v[9], hist[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+          const int sx = wrap1(px, D2Q9::cx[d], nx), sy = wrap1(py, D2Q9::cy[d], ny);
+          v[d] = edge ? src.at(d, sx, sy) : src.at_in(d, sx, sy);
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) hist[d] = x == nx - 1 ? src.hist_at(y, d) : (S)0.0f;
+        plain_cell<P>(a, src, out, x, y, v, edge, mirror, hist, (fl2[r] & FL_SOLID) != 0, spy2[r], spx2, pre, st);
+      }
+      list_append(in && near, (ry + 1) * W + lane + 1, k2[r], lcode, lk, &nlist[1]);
+    }
+    __syncthreads();
+    const int n = nlist[1];
+    for (int m = (int)threadIdx.x; m < n; m += 64 * kPairWaves) {
+      const int code = lcode[m];
+      const int x = X0 - 1 + code % W, y = Y0 - 1 + code / W;
+      step_cell<P>(a, src, out, x, y, false, a.links + (size_t)lk[m] * 8, false, false, false, pre, st.mn, st.vmax,
+                   st.exc, st.nf, st.bad, st.nfu);
+    }
+  }
+  if (a.write_pre) put_step_stats(a.red, st.mn, st.vmax, st.exc, st.nf, st.bad, st.nfu);
 }
 
 // The flagged cells of rows 1..ny-2, columns 1..nx-2, listed in row-major order so that
@@ -736,7 +1158,7 @@ __global__ void __launch_bounds__(256) lbm_row_scan_kernel(int ny, const int* __
 
 __global__ void __launch_bounds__(256) lbm_row_fill_kernel(int nx, int ny, const uint8_t* __restrict__ flags,
                                                            const int* __restrict__ row_start,
-                                                           int* __restrict__ list) {
+                                                           int* __restrict__ list, int* __restrict__ lidx) {
   __shared__ int wsum[4];
   const int y = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -753,6 +1175,7 @@ __global__ void __launch_bounds__(256) lbm_row_fill_kernel(int nx, int ny, const
     if (on) {
       const size_t i = (size_t)y * nx + x;
       list[off + before] = (int)i | ((flags[i] & FL_SOLID) ? (int)0x80000000u : 0);
+      lidx[i] = off + before;
     }
     run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
@@ -993,6 +1416,7 @@ struct Geometry {
   int* list_count = nullptr;
   int* rows = nullptr;        // per-row counts, then per-row starts, of the list
   double* links = nullptr;    // 8 Bouzidi fractions per list entry
+  int* lidx = nullptr;        // list position of each listed cell
   double angle = 0.0;
   bool valid = false;
 };
@@ -1029,6 +1453,15 @@ struct Lattice {
 static int list_blocks(const Lattice& L) {
   const int t = tune_get(TUNE_LBM_LIST_BLOCKS, L.dom.nx, L.dom.ny, L.prec, 1);
   return (t >= 1 && t <= 1024) ? t : kListBlocks;
+}
+
+// Interior rows per thread of the single-step kernel: two cells per thread give each wave more
+// loads in flight, which pays on large lattices; small lattices want the extra waves. Tuned per
+// shape (1 or 2), default 2 from 2 Mi cells; the choice changes no result.
+static int int_rows(const Lattice& L) {
+  const int t = tune_get(TUNE_LBM_ROWS, L.dom.nx, L.dom.ny, L.prec, 1);
+  if (t == 1 || t == 2) return t;
+  return L.N >= ((size_t)1 << 21) ? 2 : 1;
 }
 
 static int other3(int a, int b) {
@@ -1081,12 +1514,14 @@ static LbmArgs<S> step_args(Lattice& L, const fb_lbm_params& p, int fin_idx, boo
   a.prime = prime ? 1 : 0;
   a.red = L.red;
   a.int_gx = (L.dom.nx + 255) / 256;
+  a.int_rows = int_rows(L);
   a.nedge = 2 * L.dom.nx + 2 * (L.dom.ny - 2);
   a.edge_blocks = (a.nedge + 255) / 256;
   a.list_blocks = list_blocks(L);
   a.list = L.G[g_cur].list;
   a.list_count = L.G[g_cur].list_count;
   a.links = L.G[g_cur].links;
+  a.lidx = L.G[g_cur].lidx;
   a.skip = 0;
   return a;
 }
@@ -1097,7 +1532,43 @@ static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool pri
   using S = typename P::S;
   const LbmArgs<S> a =
       step_args<S>(L, p, fin_idx, prime, fout_idx, o_in, o_out, g_cur, g_next, write_pre, pre_out, do_reset);
-  hipLaunchKernelGGL(lbm_step_kernel<P>, dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
+  if (a.int_rows == 2)
+    hipLaunchKernelGGL((lbm_step_kernel<P, 2>), dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
+  else
+    hipLaunchKernelGGL((lbm_step_kernel<P, 1>), dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
+  FB_HIP(hipGetLastError());
+  return FB_OK;
+}
+
+// Whether two steps of this lattice may run as one fused launch (lbm_pair_kernel): a
+// non-periodic lattice whose flagged cells keep off the three outermost rows and columns, and
+// whose last tile column and row hold at least two cells. A NACA four-digit section lies within
+// 0.96 chord of its pivot (quarter chord; |y| <= (m + t/2) c), so a pivot more than a chord plus
+// eight cells inside every edge suffices. Fusion halves the HBM traffic per step, but a block
+// then runs three dependent rounds of loads and the edge and foil tiles run longer than the rest,
+// so it is chosen per shape by the autotuner (tuning value 1) and off otherwise; the choice
+// changes no result.
+template <class S>
+static bool pair_ok(const Lattice& L) {
+  if (tune_get(TUNE_LBM_FUSE, L.dom.nx, L.dom.ny, L.prec, 1) != 1) return false;
+  if (L.dom.periodic_x || L.dom.periodic_y) return false;
+  if (L.dom.nx % 64 == 1 || L.dom.ny % pair_rows<S>() == 1) return false;
+  const double dx = (L.dom.x1 - L.dom.x0) / L.dom.nx, dy = (L.dom.y1 - L.dom.y0) / L.dom.ny;
+  const double r = L.foil.chord;
+  return L.foil.pivot_x - r - 8.0 * dx > L.dom.x0 && L.foil.pivot_x + r + 8.0 * dx < L.dom.x1 &&
+         L.foil.pivot_y - r - 8.0 * dy > L.dom.y0 && L.foil.pivot_y + r + 8.0 * dy < L.dom.y1;
+}
+
+// Steps k and k+1 of an advance as one launch: f* from F[fin_idx] to F[fout_idx] (the first
+// step's f* stays in LDS), outlet history from O[o_in] to O[o_out].
+template <class P>
+static int launch_pair(Lattice& L, const fb_lbm_params& p, int fin_idx, int fout_idx, int o_in, int o_out, int g,
+                       bool write_pre, int pre_out) {
+  using S = typename P::S;
+  constexpr int th = pair_rows<S>();
+  const LbmArgs<S> a = step_args<S>(L, p, fin_idx, false, fout_idx, o_in, o_out, g, g, write_pre, pre_out, false);
+  const unsigned blocks = (unsigned)((L.dom.nx + 63) / 64) * (unsigned)((L.dom.ny + th - 1) / th);
+  hipLaunchKernelGGL((lbm_pair_kernel<P, th>), dim3(blocks), dim3(64 * kPairWaves), 0, L.ctx->stream, a);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
@@ -1113,7 +1584,7 @@ static int compute_geometry(Lattice& L, int gi, const fb_pose& pose) {
   hipLaunchKernelGGL(lbm_row_scan_kernel, dim3(1), dim3(256), 0, L.ctx->stream, L.dom.ny, G.rows,
                      G.rows + L.dom.ny, G.list_count);
   hipLaunchKernelGGL(lbm_row_fill_kernel, dim3(L.dom.ny), dim3(256), 0, L.ctx->stream, L.dom.nx, L.dom.ny, G.flags,
-                     G.rows + L.dom.ny, G.list);
+                     G.rows + L.dom.ny, G.list, G.lidx);
   hipLaunchKernelGGL(lbm_links_kernel, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, L.ctx->stream, L.dom.nx,
                      L.dom.ny, G.flags, G.sdf, G.list, G.list_count, G.links);
   FB_HIP(hipGetLastError());
@@ -1167,10 +1638,15 @@ static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const 
   }
   if ((rc = reset_red(L))) return rc;
   int pre_out = other3(L.cvalid ? L.cpre : -1, L.pre);
+  const bool pairs = pair_ok<typename P::S>(L);
+  // the geometry changes after step j (the pose of step j + 1 differs)
+  auto moves_after = [&](int j) {
+    return poses && j + 1 < substeps && poses[j + 1].angle_degrees != poses[j].angle_degrees;
+  };
   for (int k = 0; k < substeps; ++k) {
     bool last = (k == substeps - 1);
     int g_next = L.g;
-    bool moving = poses && !last && poses[k + 1].angle_degrees != poses[k].angle_degrees;
+    bool moving = moves_after(k);
     if (moving) {
       g_next = other3(L.cvalid ? L.cg : -1, L.g);
       if ((rc = compute_geometry(L, g_next, poses[k + 1]))) return rc;
@@ -1179,8 +1655,13 @@ static int advance_impl(Lattice& L, const fb_lbm_params& p, int substeps, const 
     // the outlet history only evolves with a non-periodic x axis (fb/solvers/lbm.py:628-640)
     int oo = L.dom.periodic_x ? L.o : other3(L.cvalid ? L.co : -1, L.o);
     if (ev && k == 0) FB_HIP(hipEventRecord(ev[0], L.ctx->stream));
-    if ((rc = launch_step<P>(L, p, fcur, false, fo, L.o, oo, L.g, g_next, last, pre_out, moving)))
+    if (pairs && !moving && k + 1 < substeps && !moves_after(k + 1)) {
+      ++k;
+      last = (k == substeps - 1);
+      if ((rc = launch_pair<P>(L, p, fcur, fo, L.o, oo, L.g, last, pre_out))) return rc;
+    } else if ((rc = launch_step<P>(L, p, fcur, false, fo, L.o, oo, L.g, g_next, last, pre_out, moving))) {
       return rc;
+    }
     if (ev && last) FB_HIP(hipEventRecord(ev[1], L.ctx->stream));
     fcur = fo;
     L.f = fo;
@@ -1405,6 +1886,7 @@ static void free_lattice(Lattice* L) {
     if (L->G[k].list_count) (void)dev_free(L->G[k].list_count);
     if (L->G[k].links) (void)dev_free(L->G[k].links);
     if (L->G[k].rows) (void)dev_free(L->G[k].rows);
+    if (L->G[k].lidx) (void)dev_free(L->G[k].lidx);
   }
   for (int k = 0; k < 3; ++k)
     if (L->PRE[k]) (void)dev_free(L->PRE[k]);
@@ -1534,7 +2016,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
     for (int i = 0; i < n; ++i) {
       B.h_start[i] = off;
       const Lattice& L = *B.m[i].L;
-      const int nb = (L.dom.nx + 255) / 256 * (L.dom.ny - 2) + B.m[i].list_blocks +
+      const int nb = interior_blocks(L.dom.nx, L.dom.ny, 1) + B.m[i].list_blocks +
                      (2 * L.dom.nx + 2 * (L.dom.ny - 2) + 255) / 256;
       off += (nb + 7) / 8 * 8;
     }
@@ -1582,6 +2064,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
         a.skip = 1;
       }
       a.list_blocks = mb.list_blocks;
+      a.int_rows = 1;
     }
     if (mb.remaining <= K) {
       int rc = reset_red(L);
@@ -1776,6 +2259,7 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
     if ((e = dev_alloc((void**)&L->G[k].list_count, sizeof(int)))) return fail(e);
     if ((e = dev_alloc((void**)&L->G[k].links, N * 8 * sizeof(double)))) return fail(e);
     if ((e = dev_alloc((void**)&L->G[k].rows, 2 * (size_t)dom->ny * sizeof(int)))) return fail(e);
+    if ((e = dev_alloc((void**)&L->G[k].lidx, N * sizeof(int)))) return fail(e);
     if ((e = hipMemsetAsync(L->R[k], 0, N * L->ssize, c->stream))) return fail(e);
   }
   for (int k = 0; k < 3; ++k)

@@ -34,7 +34,7 @@ from .mix import LBM, SF, Request
 # Device bytes per cell of a request's state, from the allocations of fb_mac_create/Poisson set-up
 # and fb_lbm_create, with a margin; plus a fixed allowance per request.
 _SF_BYTES_PER_CELL = 420
-_LBM_BYTES_PER_CELL = {"f64": 720, "f32-mixed": 480, "f16-mixed": 380, "f32": 480}
+_LBM_BYTES_PER_CELL = {"f64": 750, "f32-mixed": 500, "f16-mixed": 400, "f32": 500}
 _FIXED_BYTES = 8 << 20
 
 

@@ -4,14 +4,15 @@ Tuned here, per grid shape and precision, and stored in the library's tuning tab
 (`fb_tune_set`), which the launches consult:
 * rows per block of the DST-I kernel (the tile of the preconditioner's x-direction transform),
   timed as whole preconditioner applications on stacked grids (`fb_poisson_time_precond`);
-* blocks striding over a lattice's flagged cells (`lbm_step_kernel`'s special-cell tile), timed
-  as lattice steps (`fb_lbm_time_fixed`).
+* for a lattice, whether two steps run per launch (`lbm_pair_kernel`), the interior rows per
+  thread of the single-step kernel and the blocks striding over its flagged cells, timed as
+  lattice steps (`fb_lbm_time_fixed`).
 For the request loop it also picks the lattice-batch chunk (steps per captured graph) by timing
 batched steps of the mix's lattice shapes (`fb_lbm_run_batch`).
 
-Every tuned choice leaves each result bit unchanged (rows are independent transforms; a
-flagged cell is stepped by the same arithmetic whichever block takes it), so tuning changes only
-speed. Results can be saved to and loaded from a JSON file.
+Every tuned choice leaves each result bit unchanged (rows are independent transforms; a lattice
+cell is stepped by the same arithmetic whichever block, launch shape or fused pair takes it), so
+tuning changes only speed. Results can be saved to and loaded from a JSON file.
 """
 
 from __future__ import annotations
@@ -39,6 +40,8 @@ class Autotuner:
                 if "param" in rec:
                     N.check(self.lib.fb_tune_set(rec["param"], rec["nx"], rec["ny"], rec["precision"], rec["grids"],
                                                  rec["value"]))
+                for entry in rec.get("set", []):
+                    N.check(self.lib.fb_tune_set(*entry))
 
     def save(self) -> None:
         if self.path:
@@ -78,27 +81,47 @@ class Autotuner:
         self.choices[key] = rec
         return rec
 
-    # ------------------------------------------------------------------ lattice special-cell blocks
+    # ------------------------------------------------------------------ lattice step shape
     def lbm(self, nx: int, ny: int, form: str = "f32-mixed", steps: int = 50) -> dict:
+        """Fused pairs on or off and interior rows per thread, then (single steps) the blocks
+        striding over the flagged cells, for one lattice shape and storage form."""
         from .benchmark.lbm_harness import Lattice, LatticeCase
 
         prec = _PREC[form]
-        key = f"lbm_list_blocks:{nx}x{ny}:p{prec}"
+        key = f"lbm:{nx}x{ny}:p{prec}"
         if key in self.choices:
             return self.choices[key]
         lat = Lattice(LatticeCase(nx=nx, ny=ny), prec, self.ctx)
+
+        def timed() -> float:
+            lat.time(4)
+            return min(lat.time(steps) / steps for _ in range(2))
+
         times = {}
         try:
-            for b in (8, 16, 32, 64, 128):
-                self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, b)
-                lat.time(4)
-                times[b] = lat.time(steps) / steps
+            for fuse in (0, 1):
+                for rows in (1, 2):
+                    self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
+                    self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
+                    times[f"fuse{fuse}:rows{rows}"] = timed()
+            best = min(times, key=times.get)
+            fuse, rows = int(best[4]), int(best[-1])
+            self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
+            self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
+            list_blocks = -1
+            if not fuse:
+                lt = {}
+                for b in (8, 16, 32, 64, 128):
+                    self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, b)
+                    lt[b] = timed()
+                list_blocks = min(lt, key=lt.get)
+                times.update({f"list{b}": v for b, v in lt.items()})
+                self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, list_blocks)
         finally:
             lat.close()
-        best = min(times, key=times.get)
-        self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, best)
-        rec = {"param": N.FB_TUNE_LBM_LIST_BLOCKS, "nx": nx, "ny": ny, "precision": prec, "grids": 1, "value": best,
-               "ms": times}
+        rec = {"fuse": fuse, "rows": rows, "list_blocks": list_blocks, "ms": times,
+               "set": [[N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse], [N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows],
+                       [N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, list_blocks]]}
         self.choices[key] = rec
         return rec
 
@@ -140,13 +163,14 @@ class Autotuner:
                 sf_shapes[(r.nx, r.ny)] = sf_shapes.get((r.nx, r.ny), 0) + 1
             elif r.solver == LBM:
                 lbm_shapes.add((r.nx, r.ny))
-        out = {"dst_rows": {}, "lbm_list_blocks": {}}
+        out = {"dst_rows": {}, "lbm": {}}
         for (nx, ny), count in sorted(sf_shapes.items()):
             dx = 4.0 / nx
             g = max(1, min(sf_batch, count, N_MAX_BATCH))
             out["dst_rows"][f"{nx}x{ny}"] = self.poisson(nx, ny, dx, dx, N.FB_F32, grids=g, key_grids=0)["value"]
         for nx, ny in sorted(lbm_shapes):
-            out["lbm_list_blocks"][f"{nx}x{ny}"] = self.lbm(nx, ny)["value"]
+            r = self.lbm(nx, ny)
+            out["lbm"][f"{nx}x{ny}"] = {k: r[k] for k in ("fuse", "rows", "list_blocks")}
         out["lbm_chunk"] = self.lbm_chunk(sorted(lbm_shapes))["value"] if lbm_shapes else 32
         self.save()
         return out

@@ -340,8 +340,12 @@ int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
 /* Autotuner table: launch-shape choices that leave every result bit unchanged, per (parameter,
  * nx, ny, precision, grids); value -1 clears an entry, and unset entries use built-in defaults.
  * FB_TUNE_DST_ROWS: rows per block of the DST-I kernel (grids = batch size, 0 = any);
- * FB_TUNE_LBM_LIST_BLOCKS: blocks striding over a lattice's flagged cells (grids = 1). */
-enum fb_tune_param { FB_TUNE_DST_ROWS = 0, FB_TUNE_LBM_LIST_BLOCKS = 1 };
+ * FB_TUNE_LBM_LIST_BLOCKS: blocks striding over a lattice's flagged cells (grids = 1);
+ * FB_TUNE_LBM_FUSE: 1 runs eligible lattices (non-periodic, foil clear of the edges) two steps
+ * per launch; otherwise (the default) one step per launch (grids = 1);
+ * FB_TUNE_LBM_ROWS: interior rows per thread of the lattice step, 1 or 2; unset, 2 from 2 Mi
+ * cells (grids = 1). */
+enum fb_tune_param { FB_TUNE_DST_ROWS = 0, FB_TUNE_LBM_LIST_BLOCKS = 1, FB_TUNE_LBM_FUSE = 2, FB_TUNE_LBM_ROWS = 3 };
 int fb_tune_set(int param, int nx, int ny, int precision, int grids, int value);
 int fb_tune_get(int param, int nx, int ny, int precision, int grids, int* value);
 /* Average milliseconds of one DST-I preconditioner application (FP32 or FP64) to `grids` stacked

@@ -34,6 +34,11 @@ CASES = {
     # static foil, open boundaries: Zou-He rows and inlet, outlet, corners, sponge, Bouzidi
     "static": dict(resolution=(64, 32), controls=((0.0, -8.0),), bounds=((-1.5, 2.5), (-1.0, 1.0)),
                    output_dt=0.02, reynolds=300.0),
+    # the same with the foil more than a chord plus eight cells from every edge, run with fused
+    # pairs switched on (two steps per launch, the first held in LDS); 96 = 64 + 32 columns leave
+    # a partial last tile
+    "static-wide": dict(resolution=(96, 64), controls=((0.0, -8.0),), bounds=((-2.5, 3.5), (-2.0, 2.0)),
+                        output_dt=0.05, reynolds=300.0),
     # rotating foil: per-substep geometry, uncovered-cell reset, moving-wall term
     "moving": dict(resolution=(64, 32), controls=((0.0, 4.0), (0.2, 30.0)), bounds=((-1.5, 2.5), (-1.0, 1.0)),
                    output_dt=0.02, reynolds=300.0),
@@ -48,8 +53,19 @@ CASES = {
 }
 
 
+def _fuse(nx, ny, precision, value):
+    """Sets the tuning table's fusion choice for a lattice shape (1 on, 0 off, -1 default)."""
+    from foilbench_amd import _native as N
+
+    ctx = N.Context.default()
+    prec = N.FB_F64 if precision == "float64" else N.FB_F32
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, value))
+
+
 def _run_pair(case, precision, steps, share_geometry):
     sc = make_scenario(precision=precision, **CASES[case])
+    if case == "static-wide":
+        _fuse(*CASES[case]["resolution"], precision, 1)
     gpu = _solver()
     gpu.initialize(sc, _foil(sc), 0)
     orc = LBMOracle()
@@ -74,9 +90,13 @@ def _run_pair(case, precision, steps, share_geometry):
 
 
 @pytest.mark.parametrize("precision", ["float64", "float32"])
-@pytest.mark.parametrize("case", ["static", "taylor-green", "poiseuille"])
+@pytest.mark.parametrize("case", ["static", "static-wide", "taylor-green", "poiseuille"])
 def test_lbm_plain_bit_exact_with_shared_geometry(case, precision):
-    gpu, orc, reports = _run_pair(case, precision, steps=4, share_geometry=True)
+    try:
+        gpu, orc, reports = _run_pair(case, precision, steps=4, share_geometry=True)
+    finally:
+        if case == "static-wide":
+            _fuse(*CASES[case]["resolution"], precision, -1)
     f, outlet = gpu.populations()
     np.testing.assert_array_equal(f, orc.f)
     np.testing.assert_array_equal(outlet, orc.outlet)
@@ -169,3 +189,49 @@ def test_lbm_mixed_forms_track_fp64(form, tol):
     fl = ~solid
     err = np.linalg.norm((ua[fl] - ub[fl]).ravel()) / np.linalg.norm(ub[fl].ravel())
     assert err < tol, err
+
+
+def _lattice_run(form, nx, ny, channel, fuse, chunks):
+    """A harness lattice stepped in advances of the given lengths, with fused pairs switched on, or
+    off with two interior rows per thread in the single-step kernel, through the tuning table;
+    returns the state and the statistics of each advance."""
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import Lattice, LatticeCase
+    from foilbench_amd.tuning import _PREC
+
+    ctx = N.Context.default()
+    prec = _PREC[form]
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, 1 if fuse else 0))
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, 1 if fuse else 2))
+    lat = Lattice(LatticeCase(nx=nx, ny=ny, channel=channel), prec, ctx)
+    try:
+        stats = []
+        for n in chunks:
+            st = lat.advance(n)
+            stats.append((st.min_population, st.max_speed, st.density_excursion, st.nonfinite_populations,
+                          st.bad_fluid_density, st.nonfinite_velocity))
+        f, outlet = lat.state()
+    finally:
+        lat.close()
+        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, -1))
+        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, -1))
+    return f, outlet, stats
+
+
+@pytest.mark.parametrize("form", ["f64", "f32", "f32-mixed", "f16-mixed"])
+@pytest.mark.parametrize("nx,ny,channel", [(256, 128, 0), (200, 100, 0), (192, 98, 1)])
+def test_lbm_fused_pairs_equal_single_steps(form, nx, ny, channel):
+    """Two steps per launch (odd steps with one interior row per thread) against one step per
+    launch with two rows per thread: the same bits for the populations, the outlet history and the
+    in-kernel postcondition reductions. The shapes leave partial tiles
+    (200 x 100, 192 x 98) and include channel walls; advances of odd length end on a single step.
+    The harness foil (unit chord at the origin in [-1.6, 4.8] x [-1.6, 1.6]) keeps more than a chord
+    plus eight cells from every edge, so every pair of steps is fused when allowed."""
+    chunks = (7, 4, 1)
+    f1, o1, s1 = _lattice_run(form, nx, ny, channel, True, chunks)
+    f0, o0, s0 = _lattice_run(form, nx, ny, channel, False, chunks)
+    assert np.isfinite(f1).all()
+    np.testing.assert_array_equal(f1, f0)
+    np.testing.assert_array_equal(o1, o0)
+    assert s1 == s0
+

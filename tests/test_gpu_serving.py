@@ -233,11 +233,13 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
     N.check(ctx.lib.fb_lbm_run_fixed(a._handle, C.byref(p), 12))
     ctx.synchronize()
     N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_LIST_BLOCKS, 160, 80, N.FB_F32_MIXED, 1, 8))
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_ROWS, 160, 80, N.FB_F32_MIXED, 1, 2))
     try:
         N.check(ctx.lib.fb_lbm_run_fixed(b._handle, C.byref(p), 12))
         ctx.synchronize()
     finally:
         N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_LIST_BLOCKS, 160, 80, N.FB_F32_MIXED, 1, -1))
+        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_LBM_ROWS, 160, 80, N.FB_F32_MIXED, 1, -1))
     assert np.array_equal(a.populations()[0], b.populations()[0])
 
     tuner = Autotuner(ctx, reps=5)
@@ -247,3 +249,10 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
     N.check(ctx.lib.fb_tune_get(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, C.byref(v)))
     assert v.value == rec["value"]
     N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, -1))
+
+    rec = tuner.lbm(160, 80, "f32-mixed", steps=10)
+    assert rec["fuse"] in (0, 1) and rec["rows"] in (1, 2) and all(t > 0 for t in rec["ms"].values())
+    for param, nx, ny, prec, grids, value in rec["set"]:
+        N.check(ctx.lib.fb_tune_get(param, nx, ny, prec, grids, C.byref(v)))
+        assert v.value == value
+        N.check(ctx.lib.fb_tune_set(param, nx, ny, prec, grids, -1))
