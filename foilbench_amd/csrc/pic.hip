@@ -441,24 +441,35 @@ __global__ void __launch_bounds__(256) p2g_kernel(P2GArgs<T> a) {
   if ((threadIdx.x & 63) == 0 && uns) atomicAdd(a.unsupported, (unsigned long long)uns);
 }
 
-// The same gather for a non-periodic box, staged through LDS: a block owns a tile of
-// kTileW x kTileH faces, loads once the particles of the (kTileW + 3) x (kTileH + 3) cells that can
-// reach it (each cell row's slice of the sorted arrays is contiguous), with their grid coordinates
-// computed once per particle, and every face then scans its 4 x 4 cells in LDS. Candidate order,
-// weights and sums are those of p2g_kernel, so the results are the same bits. A tile whose cells
-// hold more particles than the LDS capacity gathers from global memory instead.
-constexpr int kTileW = 32, kTileH = 4;
-constexpr int kTileCap = 3200;  // staged particles per tile (24 B each)
+// The same gather for a non-periodic box, staged through LDS. A block owns a tile of
+// kTileW x kTileH faces and loads once the particles of the (kTileW + 3) x (kTileH + 3) cells that
+// can reach it (each cell row's slice of the sorted arrays is contiguous), with their grid
+// coordinates computed once per particle. Four threads share a face, one per row of its 4 x 4
+// candidate cells; each sums its row's contributions in FP64 in cell and particle order, and the
+// four partial sums are added in row order, so the result is deterministic. A face's weight and
+// momentum are rounded to the storage type once (the reference rounds after every particle, in
+// particle order). A tile whose cells hold more particles than the LDS capacity gathers from
+// global memory instead.
+constexpr int kTileW = 32, kTileH = 4, kTileRows = 4;
+constexpr int kTileThreads = kTileW * kTileH * kTileRows;
+template <class T>
+constexpr int tile_cap() {
+  return sizeof(T) == 4 ? 3800 : 3200;  // staged particles per tile (20 B or 24 B each, ~76 KB)
+}
 
 template <class T>
-__global__ void __launch_bounds__(kTileW* kTileH) p2g_tile_kernel(P2GArgs<T> a) {
-  __shared__ double sgx[kTileCap], sgy[kTileCap], sv[kTileCap];
+__global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
+  constexpr int CAP = tile_cap<T>();
+  __shared__ double sgx[CAP], sgy[CAP];
+  __shared__ T sv[CAP];
   __shared__ int cell_off[kTileH + 3][kTileW + 4];
   __shared__ int row_base[kTileH + 4];
   __shared__ int row_first[kTileH + 3];
+  static_assert(2 * kTileThreads <= CAP, "the partial sums reuse the staging arrays");
   const Comp& c = a.c;
   const PicGeom& g = a.g;
-  const int tid = threadIdx.y * kTileW + threadIdx.x;
+  const int face = threadIdx.y * kTileW + threadIdx.x;
+  const int tid = threadIdx.z * (kTileW * kTileH) + face;
   const int X0 = blockIdx.x * kTileW, Y0 = blockIdx.y * kTileH;
   constexpr int NR = kTileH + 3, NC = kTileW + 3;
   // per cell row: the particle slice of cells X0-2 .. X0+kTileW (clipped to the box)
@@ -482,9 +493,9 @@ __global__ void __launch_bounds__(kTileW* kTileH) p2g_tile_kernel(P2GArgs<T> a) 
   }
   __syncthreads();
   const int total = row_base[NR];
-  const bool staged = total <= kTileCap;
+  const bool staged = total <= CAP;
   // local cell offsets: cell (r, lc) spans [cell_off[r][lc], cell_off[r][lc + 1])
-  for (int k = tid; k < NR * (NC + 1); k += kTileW * kTileH) {
+  for (int k = tid; k < NR * (NC + 1); k += kTileThreads) {
     const int r = k / (NC + 1), lc = k % (NC + 1);
     const int cy = Y0 - 2 + r;
     int off = row_base[r];
@@ -498,7 +509,7 @@ __global__ void __launch_bounds__(kTileW* kTileH) p2g_tile_kernel(P2GArgs<T> a) 
   if (staged) {
     for (int r = 0; r < NR; ++r) {
       const int n = row_base[r + 1] - row_base[r];
-      for (int k = tid; k < n; k += kTileW * kTileH) {
+      for (int k = tid; k < n; k += kTileThreads) {
         const size_t p = (size_t)row_first[r] + k;
         T x, y, vx, vy;
         ld2(a.spos, p, x, y);
@@ -506,61 +517,73 @@ __global__ void __launch_bounds__(kTileW* kTileH) p2g_tile_kernel(P2GArgs<T> a) 
         const int o = row_base[r] + k;
         sgx[o] = ((double)x - g.x0) / g.dx + c.gxo;
         sgy[o] = ((double)y - g.y0) / g.dy + c.gyo;
-        sv[o] = (double)(a.comp ? vy : vx);
+        sv[o] = a.comp ? vy : vx;
       }
     }
   }
   __syncthreads();
-  const int fx = X0 + threadIdx.x, fy = Y0 + threadIdx.y;
-  long long uns = 0;
-  if (fx < c.W && fy < c.H) {
-    T wsum = (T)0, msum = (T)0;
-    for (int wy = -2; wy <= 1; ++wy) {
-      const int cy = fy + wy;
-      if (cy < 0 || cy >= g.ny) continue;
-      const int r = threadIdx.y + wy + 2;
-      for (int wx = -2; wx <= 1; ++wx) {
-        const int cx = fx + wx;
-        if (cx < 0 || cx >= g.nx) continue;
-        const int lc = threadIdx.x + wx + 2;
-        int k0, k1;
+  const int fx = X0 + (int)threadIdx.x, fy = Y0 + (int)threadIdx.y;
+  const int wy = (int)threadIdx.z - 2;
+  double wsum = 0.0, msum = 0.0;
+  const int cy = fy + wy;
+  if (fx < c.W && fy < c.H && cy >= 0 && cy < g.ny) {
+    const int r = (int)threadIdx.y + wy + 2;
+    for (int wx = -2; wx <= 1; ++wx) {
+      const int cx = fx + wx;
+      if (cx < 0 || cx >= g.nx) continue;
+      const int lc = (int)threadIdx.x + wx + 2;
+      int k0, k1;
+      if (staged) {
+        k0 = cell_off[r][lc];
+        k1 = cell_off[r][lc + 1];
+      } else {
+        k0 = a.start[cy * g.nx + cx];
+        k1 = a.start[cy * g.nx + cx + 1];
+      }
+      for (int k = k0; k < k1; ++k) {
+        double gx, gy, val;
         if (staged) {
-          k0 = cell_off[r][lc];
-          k1 = cell_off[r][lc + 1];
+          gx = sgx[k];
+          gy = sgy[k];
         } else {
-          k0 = a.start[cy * g.nx + cx];
-          k1 = a.start[cy * g.nx + cx + 1];
+          T x, y;
+          ld2(a.spos, k, x, y);
+          gx = ((double)x - g.x0) / g.dx + c.gxo;
+          gy = ((double)y - g.y0) / g.dy + c.gyo;
         }
-        for (int k = k0; k < k1; ++k) {
-          double gx, gy, val;
-          if (staged) {
-            gx = sgx[k];
-            gy = sgy[k];
-          } else {
-            T x, y;
-            ld2(a.spos, k, x, y);
-            gx = ((double)x - g.x0) / g.dx + c.gxo;
-            gy = ((double)y - g.y0) / g.dy + c.gyo;
-          }
-          const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
-          if (fx < bx || fx > bx + 2) continue;
-          if (fy < by || fy > by + 2) continue;
-          const double w = bspline(gy - (double)fy) * bspline(gx - (double)fx);
-          if (staged) {
-            val = sv[k];
-          } else {
-            T vx, vy;
-            ld2(a.sval, k, vx, vy);
-            val = (double)(a.comp ? vy : vx);
-          }
-          wsum = (T)((double)wsum + w);
-          msum = (T)((double)msum + w * val);
+        const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+        if (fx < bx || fx > bx + 2) continue;
+        if (fy < by || fy > by + 2) continue;
+        const double w = bspline(gy - (double)fy) * bspline(gx - (double)fx);
+        if (staged) {
+          val = (double)sv[k];
+        } else {
+          T vx, vy;
+          ld2(a.sval, k, vx, vy);
+          val = (double)(a.comp ? vy : vx);
         }
+        wsum = wsum + w;
+        msum = msum + w * val;
       }
     }
+  }
+  // the four rows' partial sums, added in row order (the staging arrays are free now)
+  __syncthreads();
+  sgx[threadIdx.z * (kTileW * kTileH) + face] = wsum;
+  sgy[threadIdx.z * (kTileW * kTileH) + face] = msum;
+  __syncthreads();
+  long long uns = 0;
+  if (threadIdx.z == 0 && fx < c.W && fy < c.H) {
+    double w = sgx[face], m = sgy[face];
+#pragma unroll
+    for (int q = 1; q < kTileRows; ++q) {
+      w = w + sgx[q * (kTileW * kTileH) + face];
+      m = m + sgy[q * (kTileW * kTileH) + face];
+    }
+    const T wt = (T)w, mt = (T)m;
     T outv;
-    const bool supported = (double)wsum > 1.0e-12;
-    if (supported) outv = msum / wsum;
+    const bool supported = (double)wt > 1.0e-12;
+    if (supported) outv = mt / wt;
     else outv = a.fallback[(size_t)fy * c.W + fx];
     uns = supported ? 0 : 1;
     a.out[(size_t)fy * c.W + fx] = outv;
@@ -1019,7 +1042,7 @@ static int launch_p2g(hipStream_t s, const PicGeom& g, const SortBufs& b, const 
     a.unsupported = unsupported;
     if (!g.px && !g.py) {
       dim3 grid((a.c.W + kTileW - 1) / kTileW, (a.c.H + kTileH - 1) / kTileH);
-      hipLaunchKernelGGL(p2g_tile_kernel<T>, grid, dim3(kTileW, kTileH), 0, s, a);
+      hipLaunchKernelGGL(p2g_tile_kernel<T>, grid, dim3(kTileW, kTileH, kTileRows), 0, s, a);
     } else {
       dim3 grid((a.c.W + 255) / 256, a.c.H);
       hipLaunchKernelGGL(p2g_kernel<T>, grid, dim3(256), 0, s, a);
