@@ -137,6 +137,7 @@ FB_TUNE_DST_ROWS = 0
 FB_TUNE_LBM_LIST_BLOCKS = 1
 FB_TUNE_LBM_FUSE = 2
 FB_TUNE_LBM_ROWS = 3
+FB_TUNE_DST_PPT = 4
 
 
 class fb_server_result(C.Structure):

@@ -58,7 +58,14 @@ inline hipError_t stream_free(hipStream_t s) {
 // Launch-shape choices of the autotuner, per (parameter, nx, ny, precision, grids) key; -1 when
 // unset (the caller then uses its own default). Only parameters that leave every result bit
 // unchanged are tuned.
-enum TuneParam { TUNE_DST_ROWS = 0, TUNE_LBM_LIST_BLOCKS = 1, TUNE_LBM_FUSE = 2, TUNE_LBM_ROWS = 3, TUNE_COUNT };
+enum TuneParam {
+  TUNE_DST_ROWS = 0,
+  TUNE_LBM_LIST_BLOCKS = 1,
+  TUNE_LBM_FUSE = 2,
+  TUNE_LBM_ROWS = 3,
+  TUNE_DST_PPT = 4,
+  TUNE_COUNT
+};
 int tune_get(int param, int nx, int ny, int precision, int grids);
 void tune_set(int param, int nx, int ny, int precision, int grids, int value);
 

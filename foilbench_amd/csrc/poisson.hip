@@ -410,10 +410,10 @@ __device__ __forceinline__ void dft_r(cplx<T>* v) {
 // One radix-R Stockham pass (sub-transforms of length L -> L*R) over a row of M points in
 // LDS, in place: each of the row's M/16 threads reads its 16 points, the block synchronises,
 // then writes them to their autosorted places.
-template <class T, int R>
+template <class T, int R, int PPT>
 __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int L, int t,
                                               int nthr) {
-  constexpr int NB = 16 / R;
+  constexpr int NB = PPT / R;
   const int MR = M / R;
   cplx<T> v[NB][R];
 #pragma unroll
@@ -444,21 +444,24 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __res
   __syncthreads();
 }
 
-// Forward FFT of each row's M = 2^logM points (radix-8 passes, then one radix-2 or -4 pass).
-template <class T>
+// Forward FFT of each row's M = 2^logM points (radix-8 passes, then one radix-2 or -4 pass), with
+// M / PPT threads per row, each taking PPT points of every pass.
+template <class T, int PPT>
 __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int logM, int t,
                                          int nthr) {
   int L = 1;
-  for (int p = 0; p < logM / 3; ++p, L *= 8) stockham_pass<T, 8>(buf, roots, M, L, t, nthr);
-  if (logM % 3 == 2) stockham_pass<T, 4>(buf, roots, M, L, t, nthr);
-  else if (logM % 3 == 1) stockham_pass<T, 2>(buf, roots, M, L, t, nthr);
+  for (int p = 0; p < logM / 3; ++p, L *= 8) stockham_pass<T, 8, PPT>(buf, roots, M, L, t, nthr);
+  if (logM % 3 == 2) stockham_pass<T, 4, PPT>(buf, roots, M, L, t, nthr);
+  else if (logM % 3 == 1) stockham_pass<T, 2, PPT>(buf, roots, M, L, t, nthr);
 }
 
 // Orthonormal DST-I along x of every row, Bluestein chirp-z with Stockham FFTs in LDS. A block
-// holds `rows` rows of M points, M/16 threads each.
+// holds `rows` rows of M points, M/PPT threads each (PPT 16 or 8: more threads per row for short
+// grids; the results are the same bits either way).
 // mode 0: input r masked to fluid cells, output to `out`.
-// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row.
-template <class T>
+// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
+// fixed by M alone: M/16 strided sums, then a pairwise tree.
+template <class T, int PPT>
 __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* __restrict__ in,
                                                        const uint8_t* __restrict__ fl, const T* __restrict__ rsrc,
                                                        T* __restrict__ out, const cplx<T>* __restrict__ chirp,
@@ -477,13 +480,14 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
     if (scal) scal += (size_t)blockIdx.z * kScalStride;
   }
   if (scal && CG_DONE(scal)) return;
-  const int nthr = M >> 4;
+  const int nthr = M / PPT;
+  const int nred = M >> 4;
   const int lr = threadIdx.x / nthr;  // row within the block
   const int t = threadIdx.x - lr * nthr;
   const int row = blockIdx.x * rows + lr;
   const bool live = row < ny;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw) + (size_t)lr * M;
-  double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * M * sizeof(cplx<T>));
+  double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * M * sizeof(cplx<T>)) + (size_t)lr * nred;
   const size_t base = (size_t)(live ? row : 0) * nx;
   for (int j = t; j < M; j += nthr) {
     cplx<T> a{(T)0, (T)0};
@@ -496,42 +500,63 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
     buf[j] = a;
   }
   __syncthreads();
-  fft_rows(buf, roots, M, logM, t, nthr);
+  fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
   // convolution with the chirp: inverse FFT as conj(FFT(conj(.)))
   for (int k = t; k < M; k += nthr) {
     const cplx<T> v = cmul(buf[k], fbt[k]);
     buf[k] = {v.x, -v.y};
   }
   __syncthreads();
-  fft_rows(buf, roots, M, logM, t, nthr);
+  fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
   const T invM = (T)1 / (T)M;
-  double acc = 0.0;
-  if (live) {
-    for (int k = t; k < nx; k += nthr) {
+  // this thread's outputs k = t + j nthr < nx (nx <= M / 2, so at most PPT / 2 of them)
+  constexpr int KMAX = PPT / 2;
+  T dv[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    const int k = t + j * nthr;
+    dv[j] = (T)0;
+    if (live && k < nx) {
       const cplx<T> w = buf[k];
       const cplx<T> conv{w.x * invM, -w.y * invM};
       const cplx<T> y = cmul(chirp[k], conv);
-      const T d = -y.y * scale;
-      if (mode == 0) {
-        out[base + k] = d;
-      } else {
-        const T rv = rsrc[base + k];
-        const T zv = (fl[base + k] & PF_SOLID) ? rv : d;
-        out[base + k] = zv;
-        acc += (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
-      }
+      dv[j] = -y.y * scale;
     }
   }
-  if (mode == 1) {
-    // deterministic per-row partial: fixed tree over the row's threads
-    red[threadIdx.x] = acc;
+  if (mode == 0) {
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+      const int k = t + j * nthr;
+      if (live && k < nx) out[base + k] = dv[j];
+    }
+    return;
+  }
+  // the products r.z go to the row's (now free) LDS buffer, then are summed in the fixed order
+  __syncthreads();
+  double* prod = reinterpret_cast<double*>(buf);
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    const int k = t + j * nthr;
+    if (live && k < nx) {
+      const T rv = rsrc[base + k];
+      const T zv = (fl[base + k] & PF_SOLID) ? rv : dv[j];
+      out[base + k] = zv;
+      prod[k] = (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+    }
+  }
+  __syncthreads();
+  for (int v = t; v < nred; v += nthr) {
+    double acc = 0.0;
+    if (live)
+      for (int k = v; k < nx; k += nred) acc += prod[k];
+    red[v] = acc;
+  }
+  __syncthreads();
+  for (int s2 = nred >> 1; s2 > 0; s2 >>= 1) {
+    for (int v = t; v < s2; v += nthr) red[v] += red[v + s2];
     __syncthreads();
-    for (int s2 = nthr >> 1; s2 > 0; s2 >>= 1) {
-      if (t < s2) red[threadIdx.x] += red[threadIdx.x + s2];
-      __syncthreads();
-    }
-    if (t == 0 && live) partial[row] = red[threadIdx.x];
   }
+  if (t == 0 && live) partial[row] = red[0];
 }
 
 // ---------------------------------------------------------------- tridiagonal solves along y
@@ -1107,16 +1132,26 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
     return reduce(partial, nblocks, 1);
   }
   const DstPlan& P = *v.plan;
-  const int nthr = P.M / 16;
-  // several short rows per block, but never fewer blocks than about two per CU (unless tuned)
   const int tp = sizeof(T) == 8 ? FB_F64 : FB_F32;
-  int rows = tune_get(TUNE_DST_ROWS, nx, ny, tp, nb_);
-  if (rows < 1) rows = tune_get(TUNE_DST_ROWS, nx, ny, tp, 0);  // tuned for any batch size
+  // points per thread (16, or 8 for twice the threads per row) and rows per block, as tuned for
+  // this shape and batch size (or any batch size). By default 8 points per thread for transform
+  // lengths 512 to 4096 (measured faster there, the latency-bound passes get twice the waves),
+  // and several short rows per block, but never fewer blocks than about two per CU.
+  auto tuned = [&](int param) {
+    const int t = tune_get(param, nx, ny, tp, nb_);
+    return t >= 1 ? t : tune_get(param, nx, ny, tp, 0);
+  };
+  int ppt = tuned(TUNE_DST_PPT);
+  if (ppt < 1) ppt = (P.M >= 512 && P.M <= 4096) ? 8 : 16;
+  if (ppt != 8 || P.M / 8 > 512) ppt = 16;
+  const int nthr = P.M / ppt;
+  int rows = tuned(TUNE_DST_ROWS);
   if (rows < 1 || rows * nthr > 512 || (size_t)rows * P.M * sizeof(cplx<T>) > 96 * 1024)
     rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
-  const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * nthr * sizeof(double);
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dg, dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin, v.work,
+  const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
+  auto* kern = ppt == 8 ? dst_rows_kernel<T, 8> : dst_rows_kernel<T, 16>;
+  hipLaunchKernelGGL(kern, dg, dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin, v.work,
                      (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
                      (T)P.scale, 0, partial, scal, ps);
   FB_HIP(hipGetLastError());
@@ -1129,7 +1164,7 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
   hipLaunchKernelGGL(tri_spike_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P,
                      (const T*)v.tri_yF, (const T*)v.tri_yE, scal);
   FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(dst_rows_kernel<T>, dg, dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin, zout,
+  hipLaunchKernelGGL(kern, dg, dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin, zout,
                      (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
                      (T)P.scale, 1, partial, scal, ps);
   FB_HIP(hipGetLastError());

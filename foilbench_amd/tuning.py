@@ -2,8 +2,9 @@
 
 Tuned here, per grid shape and precision, and stored in the library's tuning table
 (`fb_tune_set`), which the launches consult:
-* rows per block of the DST-I kernel (the tile of the preconditioner's x-direction transform),
-  timed as whole preconditioner applications on stacked grids (`fb_poisson_time_precond`);
+* points per thread and rows per block of the DST-I kernel (the tile of the preconditioner's
+  x-direction transform), timed as whole preconditioner applications on stacked grids
+  (`fb_poisson_time_precond`);
 * for a lattice, whether two steps run per launch (`lbm_pair_kernel`), the interior rows per
   thread of the single-step kernel and the blocks striding over its flagged cells, timed as
   lattice steps (`fb_lbm_time_fixed`).
@@ -54,8 +55,8 @@ class Autotuner:
     # ------------------------------------------------------------------ Poisson preconditioner
     def poisson(self, nx: int, ny: int, dx: float, dy: float, precision: int = N.FB_F32, grids: int = 1,
                 key_grids: int | None = None) -> dict:
-        """Rows per DST block for `grids` stacked nx x ny grids; stored under batch size `key_grids`
-        (default `grids`; 0 = any batch size)."""
+        """Points per thread and rows per DST block for `grids` stacked nx x ny grids; stored under
+        batch size `key_grids` (default `grids`; 0 = any batch size). "value" is the rows per block."""
         kg = grids if key_grids is None else key_grids
         key = f"dst_rows:{nx}x{ny}:p{precision}:g{kg}"
         if key in self.choices:
@@ -63,21 +64,29 @@ class Autotuner:
         m = 16
         while m < 2 * nx - 1:
             m *= 2
-        nthr = m // 16
         esize = 8 if precision == N.FB_F32 else 16  # bytes of one complex value in LDS
-        cands = [r for r in (1, 2, 4, 8, 16, 32) if r * nthr <= 512 and r * m * esize <= 96 * 1024 and r <= ny * grids]
         times = {}
-        for r in cands:
-            self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, r)
-            ms = C.c_double()
-            N.check(self.lib.fb_poisson_time_precond(self.ctx.device, nx, ny, dx, dy, precision, grids, self.reps,
-                                                     C.byref(ms)))
-            times[r] = ms.value
+        for ppt in (16, 8):
+            nthr = m // ppt
+            if nthr > 512:
+                continue
+            cands = [r for r in (1, 2, 4, 8, 16, 32)
+                     if r * nthr <= 512 and r * m * esize <= 96 * 1024 and r <= ny * grids]
+            self._set(N.FB_TUNE_DST_PPT, nx, ny, precision, grids, ppt)
+            for r in cands:
+                self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, r)
+                ms = C.c_double()
+                N.check(self.lib.fb_poisson_time_precond(self.ctx.device, nx, ny, dx, dy, precision, grids, self.reps,
+                                                         C.byref(ms)))
+                times[f"ppt{ppt}:rows{r}"] = ms.value
         self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, -1)
+        self._set(N.FB_TUNE_DST_PPT, nx, ny, precision, grids, -1)
         best = min(times, key=times.get)
-        self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, kg, best)
-        rec = {"param": N.FB_TUNE_DST_ROWS, "nx": nx, "ny": ny, "precision": precision, "grids": kg, "value": best,
-               "ms": times}
+        ppt, rows = int(best.split(":")[0][3:]), int(best.split(":")[1][4:])
+        self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, kg, rows)
+        self._set(N.FB_TUNE_DST_PPT, nx, ny, precision, kg, ppt)
+        rec = {"param": N.FB_TUNE_DST_ROWS, "nx": nx, "ny": ny, "precision": precision, "grids": kg, "value": rows,
+               "ppt": ppt, "ms": times, "set": [[N.FB_TUNE_DST_PPT, nx, ny, precision, kg, ppt]]}
         self.choices[key] = rec
         return rec
 

@@ -344,8 +344,16 @@ int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
  * FB_TUNE_LBM_FUSE: 1 runs eligible lattices (non-periodic, foil clear of the edges) two steps
  * per launch; otherwise (the default) one step per launch (grids = 1);
  * FB_TUNE_LBM_ROWS: interior rows per thread of the lattice step, 1 or 2; unset, 2 from 2 Mi
- * cells (grids = 1). */
-enum fb_tune_param { FB_TUNE_DST_ROWS = 0, FB_TUNE_LBM_LIST_BLOCKS = 1, FB_TUNE_LBM_FUSE = 2, FB_TUNE_LBM_ROWS = 3 };
+ * cells (grids = 1);
+ * FB_TUNE_DST_PPT: points per thread of the DST-I kernel's FFT passes, 16 or 8; unset, 8 for
+ * FFT lengths 512 to 4096 (grids as for FB_TUNE_DST_ROWS). */
+enum fb_tune_param {
+  FB_TUNE_DST_ROWS = 0,
+  FB_TUNE_LBM_LIST_BLOCKS = 1,
+  FB_TUNE_LBM_FUSE = 2,
+  FB_TUNE_LBM_ROWS = 3,
+  FB_TUNE_DST_PPT = 4
+};
 int fb_tune_set(int param, int nx, int ny, int precision, int grids, int value);
 int fb_tune_get(int param, int nx, int ny, int precision, int grids, int* value);
 /* Average milliseconds of one DST-I preconditioner application (FP32 or FP64) to `grids` stacked

@@ -218,14 +218,17 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
         return u, v
 
     base = run_sf()
-    for prec in (N.FB_F32, N.FB_F64):
-        N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, 4))
-    try:
-        tuned = run_sf()
-    finally:
+    for rows, ppt in ((4, 16), (2, 8)):
         for prec in (N.FB_F32, N.FB_F64):
-            N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, -1))
-    assert np.array_equal(base[0], tuned[0]) and np.array_equal(base[1], tuned[1])
+            N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, rows))
+            N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_PPT, 128, 64, prec, 0, ppt))
+        try:
+            tuned = run_sf()
+        finally:
+            for prec in (N.FB_F32, N.FB_F64):
+                N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, -1))
+                N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_PPT, 128, 64, prec, 0, -1))
+        assert np.array_equal(base[0], tuned[0]) and np.array_equal(base[1], tuned[1])
 
     a = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
     b = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
@@ -244,11 +247,14 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
 
     tuner = Autotuner(ctx, reps=5)
     rec = tuner.poisson(128, 64, 4.0 / 128, 4.0 / 128, N.FB_F32, grids=4, key_grids=0)
-    assert rec["value"] in rec["ms"] and all(t > 0 for t in rec["ms"].values())
+    assert f"ppt{rec['ppt']}:rows{rec['value']}" in rec["ms"] and all(t > 0 for t in rec["ms"].values())
     v = C.c_int()
     N.check(ctx.lib.fb_tune_get(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, C.byref(v)))
     assert v.value == rec["value"]
+    N.check(ctx.lib.fb_tune_get(N.FB_TUNE_DST_PPT, 128, 64, N.FB_F32, 0, C.byref(v)))
+    assert v.value == rec["ppt"]
     N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, N.FB_F32, 0, -1))
+    N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_PPT, 128, 64, N.FB_F32, 0, -1))
 
     rec = tuner.lbm(160, 80, "f32-mixed", steps=10)
     assert rec["fuse"] in (0, 1) and rec["rows"] in (1, 2) and all(t > 0 for t in rec["ms"].values())
