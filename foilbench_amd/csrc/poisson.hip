@@ -28,7 +28,9 @@ enum : uint8_t { PF_SOLID = 1, PF_RIGHT = 2, PF_UP = 4, PF_WRAPX = 8, PF_WRAPY =
 enum { S_RN2 = 0, S_RHO, S_RHOPREV, S_PQ, S_ALPHA, S_BETA, S_ATOL, S_DONE, S_ITER, S_FAIL, S_INFO, S_MAXIT, S_DOT,
        S_COUNT = 16,
        // after the two right-hand-side status words at S_COUNT
-       S_BSUM = S_COUNT + 2, S_BCNT, S_RES2, S_B2 };
+       S_BSUM = S_COUNT + 2, S_BCNT, S_RES2, S_B2,
+       // block ticket of the fused folds (an unsigned int in this double's storage)
+       S_TICKET = S_COUNT + 6 };
 
 template <class T>
 struct cplx {
@@ -159,17 +161,15 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 // `gs` values apart, its partials `ps` apart and its scalars kScalStride apart.
 constexpr int kScalStride = S_COUNT + 8;
 
-// Sum `n` partials in fixed order into scal[slot]; optional post-processing by `mode`.
+// Sum `n` partials in fixed order into scal[slot]; optional post-processing by `mode`. Run by a
+// block of 256 threads (a launch of its own, or the last block of the kernel that wrote them).
 // mode 0: plain dot -> scal[S_DOT]
 // mode 1: rho = sum; beta = rho / rho_prev (iteration > 0)
 // mode 2: pq = sum; alpha = rho / pq
 // mode 3: rn2 = sum; rho_prev = rho; iteration += 1; then the top-of-loop check
 // mode 4: rn2 = sum (initial) and the top-of-loop check
-__global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restrict__ partial, int n, double* scal,
-                                                          int mode, int ps) {
-  __shared__ double lds[4];
-  partial += (size_t)blockIdx.z * ps;
-  scal += (size_t)blockIdx.z * kScalStride;
+__device__ __forceinline__ void finalize_body(const double* __restrict__ partial, int n, double* scal, int mode,
+                                              double* lds) {
   if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
@@ -228,6 +228,34 @@ __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restri
   }
 }
 
+__global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restrict__ partial, int n, double* scal,
+                                                          int mode, int ps) {
+  __shared__ double lds[4];
+  finalize_body(partial + (size_t)blockIdx.z * ps, n, scal + (size_t)blockIdx.z * kScalStride, mode, lds);
+}
+
+// After each block of a 256-thread kernel has written its partial (thread 0), the block that
+// finishes last folds all gridDim.x partials with finalize_body: the same sums in the same order
+// as a cg_finalize_kernel launch, one launch fewer. The grid's ticket (zeroed by cg_init_kernel)
+// counts the finished blocks and is reset by the folding block. Every block takes the ticket with
+// one atomic on one address, which serialises; it is used only for grids of at most
+// kFoldBlocks blocks, where the saved launch gap outweighs it.
+constexpr int kFoldBlocks = 256;
+__device__ __forceinline__ void fold_if_last(const double* __restrict__ partial, double* scal, int mode,
+                                             double* lds) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    unsigned* ticket = reinterpret_cast<unsigned*>(scal + S_TICKET);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this block's partial before its ticket
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (last) *ticket = 0u;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every block's partial, not a stale copy
+  finalize_body(partial, (int)gridDim.x, scal, mode, lds);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, size_t n,
                                                           double* __restrict__ partial, int ps) {
@@ -273,7 +301,7 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
                                                     const double* __restrict__ diag, const T* __restrict__ z,
                                                     const T* __restrict__ pold, T* __restrict__ pnew,
                                                     T* __restrict__ q, double* __restrict__ partial,
-                                                    const double* __restrict__ scal, T ix2, T iy2, int ps) {
+                                                    double* scal, T ix2, T iy2, int ps, int fold) {
   __shared__ double lds[4];
   const size_t N = (size_t)nx * ny;
   {
@@ -306,13 +334,14 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
   }
   v = block_sum<256>(v, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
+  if (fold) fold_if_last(partial, scal, 2, lds);  // pq and alpha
 }
 
 // x += alpha p; r -= alpha q; partial ||r||^2; non-finite iterate flag (the reference's callback).
 template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t N, T* __restrict__ x, T* __restrict__ r,
                                                         const T* __restrict__ p, const T* __restrict__ q,
-                                                        double* __restrict__ partial, double* scal, int ps) {
+                                                        double* __restrict__ partial, double* scal, int ps, int fold) {
   __shared__ double lds[4];
   {
     const size_t go = (size_t)blockIdx.z * N;
@@ -339,6 +368,7 @@ __global__ void __launch_bounds__(256) cg_update_kernel(size_t N, T* __restrict_
   if (__any(bad) && (threadIdx.x & 63) == 0) scal[S_FAIL] = 3.0;
   v = block_sum<256>(v, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
+  if (fold) fold_if_last(partial, scal, 3, lds);  // ||r||^2 and the top-of-loop check
 }
 
 // Jacobi preconditioner (periodic domains): z = (1/max(diag, eps)) r; partial r.z.
@@ -925,8 +955,9 @@ __global__ void __launch_bounds__(256) add_f32_scaled_g(size_t N, double* __rest
 // CG scalars of grid g = blockIdx.z: atol <- rtol[g] (scaled by ||b|| later), maxiter <-
 // maxit[g] (par = [maxit x nb, rtol x nb]); a grid with maxit 0 ends before its first iteration.
 __global__ void cg_init_kernel(double* scal, const double* __restrict__ par, int nb) {
-  if (threadIdx.x >= S_COUNT) return;
   double* sc = scal + (size_t)blockIdx.z * kScalStride;
+  if (threadIdx.x == S_COUNT) *reinterpret_cast<unsigned*>(sc + S_TICKET) = 0u;
+  if (threadIdx.x >= S_COUNT) return;
   const int k = threadIdx.x;
   sc[k] = k == S_ATOL ? par[nb + blockIdx.z] : (k == S_MAXIT ? par[blockIdx.z] : 0.0);
 }
@@ -1279,13 +1310,16 @@ int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
   int rc;
   for (int c = 0; c < n; ++c) {
     if ((rc = precond(v, v.r, v.z))) return rc;
+    // on small grids each of these folds its partials in its last block (pq and alpha; ||r||^2
+    // and the check); otherwise a fold launch follows each
+    const int fold = nblocks <= kFoldBlocks ? 1 : 0;
     hipLaunchKernelGGL(cg_pq_kernel<T>, grid, dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q, partial,
-                       scal, ix2, iy2, ps);
+                       scal, ix2, iy2, ps, fold);
     FB_HIP(hipGetLastError());
-    if ((rc = reduce(partial, nblocks, 2))) return rc;
-    hipLaunchKernelGGL(cg_update_kernel<T>, grid, dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal, ps);
+    if (!fold && (rc = reduce(partial, nblocks, 2))) return rc;
+    hipLaunchKernelGGL(cg_update_kernel<T>, grid, dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal, ps, fold);
     FB_HIP(hipGetLastError());
-    if ((rc = reduce(partial, nblocks, 3))) return rc;
+    if (!fold && (rc = reduce(partial, nblocks, 3))) return rc;
     std::swap(pa, pb);
   }
   return FB_OK;
