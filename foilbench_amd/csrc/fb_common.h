@@ -9,14 +9,17 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
 
 #include "../../include/foilbench_amd.h"
 
 namespace fb {
 
-// Graph capture on one thread and a call that may synchronise the device on another (allocation,
-// free, pinned-host allocation) do not mix: the library holds this lock across every capture and
-// every such call, so that handles can be created and freed while the request loop captures.
+// Calls that may synchronise the device (allocation, free, pinned-host allocation, graph
+// instantiation and destruction, stream creation) are serialised by this lock, so that handles
+// can be created and freed from several threads while the request loop runs.
 std::recursive_mutex& hip_lock();
 
 // Device memory through a block cache: a freed block goes back to a per-size free list instead of
@@ -54,6 +57,47 @@ inline hipError_t stream_free(hipStream_t s) {
   std::lock_guard<std::recursive_mutex> g(hip_lock());
   return hipStreamDestroy(s);
 }
+
+// Launches kernels on a stream, or records them as a chain of kernel nodes of a graph under
+// construction (when `graph` is set). Graphs are built node by node instead of by stream capture:
+// a capture is invalidated by synchronising calls that other threads make meanwhile (stream and
+// event synchronisation, copies from pageable memory), and the request loop's threads make such
+// calls all the time.
+struct Launcher {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+
+  template <class... P, class... A>
+  void operator()(void (*f)(P...), dim3 grid, dim3 block, size_t shm, A&&... a) {
+    if (!graph) {
+      hipLaunchKernelGGL(f, grid, block, shm, stream, std::forward<A>(a)...);
+      return;
+    }
+    if (err != hipSuccess) return;
+    std::tuple<std::decay_t<P>...> values(std::forward<A>(a)...);
+    void* params[sizeof...(P) + 1];
+    std::apply(
+        [&](auto&... e) {
+          int k = 0;
+          ((params[k++] = (void*)&e), ...);
+        },
+        values);
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(f);
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.sharedMemBytes = (unsigned)shm;
+    kp.kernelParams = params;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &kp);
+    if (err == hipSuccess) last = node;
+  }
+  // the first failure: a node that could not be added, or (on a stream) the last launch error
+  hipError_t status() { return graph ? err : hipGetLastError(); }
+};
 
 // Launch-shape choices of the autotuner, per (parameter, nx, ny, precision, grids) key; -1 when
 // unset (the caller then uses its own default). Only parameters that leave every result bit

@@ -1992,7 +1992,7 @@ struct LbmBatch {
   int chunk = 32;
   bool use_graph = true;
   std::vector<LbmMember> m;
-  bool changed = true;       // membership changed since the graph was captured
+  bool changed = true;       // membership changed since the graph was built
   void* d_args = nullptr;    // chunk x members LbmArgs
   size_t args_bytes = 0;
   int* d_start = nullptr;
@@ -2076,35 +2076,35 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
     // the copy reads pageable memory: the host buffer may be reused once the call returns
     B.h_prev = B.h_args;
   }
-  auto launch_all = [&]() {
+  auto launch_all = [&](Launcher& L) {
     for (int j = 0; j < K; ++j)
-      hipLaunchKernelGGL(lbm_batch_kernel<P>, dim3(B.total_blocks), dim3(256), 0, s,
-                         (const LbmArgs<S>*)B.d_args + (size_t)j * n, (const int*)B.d_start, n);
+      L(lbm_batch_kernel<P>, dim3(B.total_blocks), dim3(256), 0, (const LbmArgs<S>*)B.d_args + (size_t)j * n,
+        (const int*)B.d_start, n);
   };
   if (B.use_graph) {
     if (!B.exec) {
+      // the chunk's graph is built node by node (no stream capture)
       hipGraph_t graph = nullptr;
-      std::lock_guard<std::recursive_mutex> guard(hip_lock());
-      FB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_all();
-      hipError_t e = hipStreamEndCapture(s, &graph);
-      if (e != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        set_error("lattice batch capture failed: %s", hipGetErrorString(e));
-        return FB_ERR_HIP;
+      FB_HIP(hipGraphCreate(&graph, 0));
+      Launcher L{s, graph};
+      launch_all(L);
+      hipError_t e = L.status();
+      if (e == hipSuccess) {
+        std::lock_guard<std::recursive_mutex> guard(hip_lock());
+        e = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
       }
-      e = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) {
         B.exec = nullptr;
-        set_error("lattice batch graph instantiation failed: %s", hipGetErrorString(e));
+        set_error("lattice batch graph failed: %s", hipGetErrorString(e));
         return FB_ERR_HIP;
       }
     }
     FB_HIP(hipGraphLaunch(B.exec, s));
   } else {
-    launch_all();
-    FB_HIP(hipGetLastError());
+    Launcher L{s};
+    launch_all(L);
+    FB_HIP(L.status());
   }
   B.changed = false;
   FB_HIP(hipStreamSynchronize(s));
@@ -2268,7 +2268,7 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   if ((e = dev_alloc((void**)&L->sponge_x, dom->nx * sizeof(double)))) return fail(e);
   if ((e = dev_alloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
   if ((e = host_alloc((void**)&L->red_host, 8 * sizeof(unsigned long long)))) return fail(e);
-  // stream-ordered copies: a legacy-stream copy would conflict with graph capture on other threads
+  // stream-ordered copies: a legacy-stream copy would wait on every other thread's stream
   if ((e = hipMemcpyAsync(L->sponge_y, sponge_y, dom->ny * sizeof(double), hipMemcpyHostToDevice, c->stream)))
     return fail(e);
   if ((e = hipMemcpyAsync(L->sponge_x, sponge_x, dom->nx * sizeof(double), hipMemcpyHostToDevice, c->stream)))

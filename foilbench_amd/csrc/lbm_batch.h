@@ -1,4 +1,4 @@
-// Batched lattice stepping: many D2Q9 lattices advanced by one launch per step, captured in a
+// Batched lattice stepping: many D2Q9 lattices advanced by one launch per step, recorded in a
 // HIP graph per chunk of steps, with lattices joining and leaving at chunk boundaries.
 #pragma once
 

@@ -102,7 +102,7 @@ int mac_substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_re
 // Moves M onto context c: its kernels, copies and reduction slots then use c's stream. The
 // caller makes sure no work of M is pending on the previous stream.
 void mac_bind(MacGrid& M, Ctx* c);
-// Whether the pressure solve replays captured graphs for its CG iteration chunks.
+// Whether the pressure solve replays graphs of its CG iteration chunks.
 void mac_use_graphs(MacGrid& M, bool on);
 
 }  // namespace fb

@@ -105,6 +105,8 @@ class Poisson {
   template <class T>
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>
+  int precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout);
+  template <class T>
   int pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs, int first_chunk,
           int* iterations, int* info, int* failure);
   int run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps);
@@ -112,8 +114,9 @@ class Poisson {
   template <class T>
   int iterations_chunk(CgVecs<T>& v, T* x, int n, int parity);
   template <class T>
-  int enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity);
+  int enqueue_iterations(Launcher& L, CgVecs<T>& v, T* x, int n, int parity);
   int reduce(const double* partial_in, int n, int mode);
+  int reduce(Launcher& L, const double* partial_in, int n, int mode);
   int sum_to(const double* partial_in, int n, int slot);
   struct GraphKey {
     size_t elem;

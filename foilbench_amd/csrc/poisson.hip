@@ -120,7 +120,7 @@ static int make_plan(DstPlan*& plan, int n, hipStream_t stream) {
   FB_HIP(dev_alloc(&plan->chirp, n * sizeof(cplx<T>)));
   FB_HIP(dev_alloc(&plan->fb, M * sizeof(cplx<T>)));
   FB_HIP(dev_alloc(&plan->tw, M * sizeof(cplx<T>)));
-  // stream-ordered copies (a legacy-stream copy would conflict with graph capture on other threads)
+  // stream-ordered copies (a legacy-stream copy would wait on every other thread's stream)
   FB_HIP(hipMemcpyAsync(plan->chirp, chirp.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
   FB_HIP(hipMemcpyAsync(plan->fb, fbv.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
   FB_HIP(hipMemcpyAsync(plan->tw, tw.data(), M * sizeof(cplx<T>), hipMemcpyHostToDevice, stream));
@@ -1133,8 +1133,12 @@ int Poisson::load_slot(int k, const Poisson& src) {
 }
 
 int Poisson::reduce(const double* pin, int n, int mode) {
-  hipLaunchKernelGGL(cg_finalize_kernel, dim3(1, 1, nb_), dim3(256), 0, stream, pin, n, scal, mode, ps);
-  FB_HIP(hipGetLastError());
+  Launcher L{stream};
+  return reduce(L, pin, n, mode);
+}
+int Poisson::reduce(Launcher& L, const double* pin, int n, int mode) {
+  L(cg_finalize_kernel, dim3(1, 1, nb_), dim3(256), 0, pin, n, scal, mode, ps);
+  FB_HIP(L.status());
   return FB_OK;
 }
 
@@ -1156,11 +1160,17 @@ int Poisson::put_par(const double* v, int count) {
 
 template <class T>
 int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
+  Launcher L{stream};
+  return precond(L, v, rin, zout);
+}
+
+template <class T>
+int Poisson::precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout) {
   if (periodic_x || periodic_y) {
-    hipLaunchKernelGGL(jacobi_kernel<T>, dim3(nblocks), dim3(256), 0, stream, N, diag, rin, zout, partial, scal,
-                       (double)std::numeric_limits<double>::epsilon());
-    FB_HIP(hipGetLastError());
-    return reduce(partial, nblocks, 1);
+    L(jacobi_kernel<T>, dim3(nblocks), dim3(256), 0, N, diag, rin, zout, partial, scal,
+      (double)std::numeric_limits<double>::epsilon());
+    FB_HIP(L.status());
+    return reduce(L, partial, nblocks, 1);
   }
   const DstPlan& P = *v.plan;
   const int tp = sizeof(T) == 8 ? FB_F64 : FB_F32;
@@ -1182,24 +1192,22 @@ int Poisson::precond(CgVecs<T>& v, const T* rin, T* zout) {
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
   auto* kern = ppt == 8 ? dst_rows_kernel<T, 8> : dst_rows_kernel<T, 16>;
-  hipLaunchKernelGGL(kern, dg, dim3(rows * nthr), lds, stream, nx, ny, rin, flags, rin, v.work,
-                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
-                     (T)P.scale, 0, partial, scal, ps);
-  FB_HIP(hipGetLastError());
+  L(kern, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, rin, v.work, (const cplx<T>*)P.chirp,
+    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 0, partial, scal, ps);
+  FB_HIP(L.status());
   const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
   const T a = (T)(-(1.0 / (dy * dy)));
-  hipLaunchKernelGGL(tri_chunk_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a,
-                     v.tri_yF, v.tri_yE, scal);
-  hipLaunchKernelGGL(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, stream, nx, v.tri_P,
-                     (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
-  hipLaunchKernelGGL(tri_spike_kernel<T>, tg, tb, 0, stream, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P,
-                     (const T*)v.tri_yF, (const T*)v.tri_yE, scal);
-  FB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(kern, dg, dim3(rows * nthr), lds, stream, nx, ny, v.work, flags, rin, zout,
-                     (const cplx<T>*)P.chirp, (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows,
-                     (T)P.scale, 1, partial, scal, ps);
-  FB_HIP(hipGetLastError());
-  return reduce(partial, ny, 1);
+  L(tri_chunk_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a, v.tri_yF, v.tri_yE,
+    scal);
+  L(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf,
+    v.tri_yF, v.tri_yE, scal);
+  L(tri_spike_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, (const T*)v.tri_yF,
+    (const T*)v.tri_yE, scal);
+  FB_HIP(L.status());
+  L(kern, dg, dim3(rows * nthr), lds, nx, ny, v.work, flags, rin, zout, (const cplx<T>*)P.chirp,
+    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 1, partial, scal, ps);
+  FB_HIP(L.status());
+  return reduce(L, partial, ny, 1);
 }
 
 template <class T>
@@ -1253,7 +1261,7 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* 
 }
 
 // A chunk of n CG iterations (with the preconditioner). Between host polls the
-// iterations only read and write device memory, so a chunk is captured once as a HIP graph
+// iterations only read and write device memory, so a chunk is built once as a HIP graph
 // per (precision, solution buffer, length, parity, grids) and replayed; every kernel returns at
 // once when its grid's device-side done flag is set.
 template <class T>
@@ -1265,31 +1273,31 @@ int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
       FB_HIP(hipGraphLaunch(g.exec, stream));
       return FB_OK;
     }
-  if (!use_graphs_) return enqueue_iterations(v, x, n, parity);
-  // capture a chunk shape the second time it comes up: a one-off solve (such as a set-up
-  // projection) does not pay for capture and instantiation
+  Launcher direct{stream};
+  if (!use_graphs_) return enqueue_iterations(direct, v, x, n, parity);
+  // a graph is built for a chunk shape the second time it comes up: a one-off solve (such as a
+  // set-up projection) does not pay for building and instantiation
   bool seen = false;
   for (auto& k : seen_)
     if (k == key) seen = true;
   if (!seen) {
     seen_.push_back(key);
-    return enqueue_iterations(v, x, n, parity);
+    return enqueue_iterations(direct, v, x, n, parity);
   }
   hipGraph_t graph = nullptr;
-  std::lock_guard<std::recursive_mutex> guard(hip_lock());
-  FB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_iterations(v, x, n, parity);
-  hipError_t e = hipStreamEndCapture(stream, &graph);
+  FB_HIP(hipGraphCreate(&graph, 0));
+  Launcher L{stream, graph};
+  int rc = enqueue_iterations(L, v, x, n, parity);
   if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     return rc;
   }
-  if (e != hipSuccess) {
-    set_error("CG graph capture failed: %s", hipGetErrorString(e));
-    return FB_ERR_HIP;
-  }
   hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipError_t e;
+  {
+    std::lock_guard<std::recursive_mutex> guard(hip_lock());
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  }
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
     set_error("CG graph instantiation failed: %s", hipGetErrorString(e));
@@ -1301,7 +1309,7 @@ int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
 }
 
 template <class T>
-int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
+int Poisson::enqueue_iterations(Launcher& L, CgVecs<T>& v, T* x, int n, int parity) {
   const T ix2 = (T)(1.0 / (dx * dx)), iy2 = (T)(1.0 / (dy * dy));
   const size_t pn = (size_t)cap * N;
   T* pa = parity ? v.p + pn : v.p;
@@ -1309,17 +1317,16 @@ int Poisson::enqueue_iterations(CgVecs<T>& v, T* x, int n, int parity) {
   const dim3 grid(nblocks, 1, nb_);
   int rc;
   for (int c = 0; c < n; ++c) {
-    if ((rc = precond(v, v.r, v.z))) return rc;
+    if ((rc = precond(L, v, v.r, v.z))) return rc;
     // on small grids each of these folds its partials in its last block (pq and alpha; ||r||^2
     // and the check); otherwise a fold launch follows each
     const int fold = nblocks <= kFoldBlocks ? 1 : 0;
-    hipLaunchKernelGGL(cg_pq_kernel<T>, grid, dim3(256), 0, stream, nx, ny, flags, diag, v.z, pa, pb, v.q, partial,
-                       scal, ix2, iy2, ps, fold);
-    FB_HIP(hipGetLastError());
-    if (!fold && (rc = reduce(partial, nblocks, 2))) return rc;
-    hipLaunchKernelGGL(cg_update_kernel<T>, grid, dim3(256), 0, stream, N, x, v.r, pb, v.q, partial, scal, ps, fold);
-    FB_HIP(hipGetLastError());
-    if (!fold && (rc = reduce(partial, nblocks, 3))) return rc;
+    L(cg_pq_kernel<T>, grid, dim3(256), 0, nx, ny, flags, diag, v.z, pa, pb, v.q, partial, scal, ix2, iy2, ps, fold);
+    FB_HIP(L.status());
+    if (!fold && (rc = reduce(L, partial, nblocks, 2))) return rc;
+    L(cg_update_kernel<T>, grid, dim3(256), 0, N, x, v.r, pb, v.q, partial, scal, ps, fold);
+    FB_HIP(L.status());
+    if (!fold && (rc = reduce(L, partial, nblocks, 3))) return rc;
     std::swap(pa, pb);
   }
   return FB_OK;

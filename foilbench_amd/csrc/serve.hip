@@ -3,7 +3,7 @@
 // Stable Fluids requests run on a pool of worker threads, each with its own context (stream and
 // reduction slots); a request's handle is moved onto the worker's context for its run, so the
 // CG polling of one request never stalls another. Lattice requests form one continuous batch per
-// precision: every lattice step of every member is one launch, replayed from a captured graph in
+// precision: every lattice step of every member is one launch, replayed from a graph in
 // chunks, and lattices join and leave at chunk boundaries.
 #include <algorithm>
 #include <chrono>

@@ -59,6 +59,7 @@ class Outcome:
     finished_ms: float
     pressure_iterations: int = 0
     fields: dict = field(default_factory=dict)
+    error: str = ""  # why a request failed before reaching the server (its build raised)
 
     @property
     def latency_ms(self) -> float:
@@ -224,7 +225,7 @@ class Server:
                         used -= need
                         t = self.clock_ms()
                         outcomes[rid] = Outcome(r.id, r.solver, r.cells, 0, -1, 0, arrival[rid], t_adm,
-                                                t, t)
+                                                t, t, error=repr(fut.exception()))
                         progressed = True
                 for res in self.poll():
                     r = by_id[res.id]

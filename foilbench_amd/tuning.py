@@ -8,7 +8,7 @@ Tuned here, per grid shape and precision, and stored in the library's tuning tab
 * for a lattice, whether two steps run per launch (`lbm_pair_kernel`), the interior rows per
   thread of the single-step kernel and the blocks striding over its flagged cells, timed as
   lattice steps (`fb_lbm_time_fixed`).
-For the request loop it also picks the lattice-batch chunk (steps per captured graph) by timing
+For the request loop it also picks the lattice-batch chunk (steps per graph) by timing
 batched steps of the mix's lattice shapes (`fb_lbm_run_batch`).
 
 Every tuned choice leaves each result bit unchanged (rows are independent transforms; a lattice

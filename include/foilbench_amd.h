@@ -366,7 +366,7 @@ int fb_mem_info(int device, int64_t* free_bytes, int64_t* total_bytes);
 
 /* `steps` lattice steps of each of n lattices (one precision) in batched launches: one
  * launch per step covers every lattice, and chunks of `chunk` steps (even) are replayed
- * from one captured HIP graph. Same arithmetic per cell as fb_lbm_advance; stats[k] (may
+ * from one HIP graph. Same arithmetic per cell as fb_lbm_advance; stats[k] (may
  * be NULL) receives lattice k's postcondition reductions. Synchronous. */
 int fb_lbm_run_batch(fb_lbm* const* lattices, const fb_lbm_params* params, int n, int steps, int chunk,
                      fb_lbm_stats* stats);
@@ -375,8 +375,8 @@ typedef struct fb_server fb_server;
 
 typedef struct fb_server_options {
   int sf_workers;  /* Stable Fluids requests in flight at once, one stream each (>= 1) */
-  int lbm_chunk;   /* lattice steps per captured graph launch (even, >= 2) */
-  int use_graphs;  /* 1: replay captured graphs (lattice batches, CG iteration chunks) */
+  int lbm_chunk;   /* lattice steps per graph launch (even, >= 2) */
+  int use_graphs;  /* 1: replay HIP graphs (lattice batches, CG iteration chunks) */
   int sf_batch;    /* > 1: pressure solves of up to this many same-shape requests share launches */
   double sf_gather_us; /* how long a batched solve waits for more same-shape requests to join */
 } fb_server_options;
