@@ -13,6 +13,8 @@ roofline and an oracle baseline on the host:
   sf_plain   Stable Fluids FP64, NACA 0012 AoA 8, 256 x 128, Re 500, CG to 1e-6;
   sf_mixed   Stable Fluids mixed (FP32 CG sweeps, FP64 refinement), NACA 0012 AoA 15,
              2048 x 1024, Re 2000, CG to 1e-8, with its error against the FP64 plain form;
+  sf_precond32  the sf_mixed workload with FP64 CG and the DST-I preconditioner sweeps in FP32
+             (the serving form's pressure solve), with its error against the FP64 plain form;
   pic        PIC/FLIP FP32, NACA 4412 AoA 10, 1024 x 512, 8 particles per fluid cell.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-legs]
@@ -140,6 +142,8 @@ def _legs(with_cpu: bool) -> dict:
         "sf_mixed": (lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=4, warmup=1,
                                                  reference_form="plain"),
                      lambda: L.sf_cpu_baseline(512, 256, -15.0, 2000.0, 1e-8, 1)),
+        "sf_precond32": (lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "precond32", 1e-8, steps=4,
+                                                     warmup=1, reference_form="plain"), None),
         "pic": (lambda: L.pic_leg(1024, 512, 8, steps=4),
                 lambda: L.pic_cpu_baseline(256, 128, 8, 1)),
         "serving": (lambda: serving_leg(), None),

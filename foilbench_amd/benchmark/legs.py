@@ -99,6 +99,10 @@ def stable_fluids_leg(nx: int, ny: int, angle: float, reynolds: float, form: str
     if form == "mixed":
         refinements = float(np.mean([r.evidence.get("pressure_refinements", 0) for r in reps]))
         bpc = sf_bytes_per_cell(8, k_d, 0.0) + k_p * 17 * 4 + refinements * 3 * 8
+    elif form == "precond32":
+        # FP64 CG vectors (11 words per iteration); the DST pair reads r in FP64 and writes z in FP64,
+        # its tridiagonal solve and intermediate planes are FP32 (2 * 8 + 4 * 4 bytes)
+        bpc = sf_bytes_per_cell(8, k_d, 0.0) + k_p * (11 * 8 + 2 * 8 + 4 * 4)
     else:
         bpc = sf_bytes_per_cell(8, k_d, k_p)
     achieved = bpc * cells * subs / total / 1e9
@@ -209,7 +213,7 @@ def pic_cpu_baseline(nx: int, ny: int, per_cell: int, steps: int) -> dict:
 
 
 def main() -> None:
-    """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|pic|serving} [steps]``: one leg, one JSON
+    """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|sf_precond32|pic|serving} [steps]``: one leg, one JSON
     line (for ``serving`` the second argument is the number of requests)."""
     import json
     import sys
@@ -218,8 +222,8 @@ def main() -> None:
     steps = int(sys.argv[2]) if len(sys.argv) > 2 else 4
     if which == "sf_plain":
         out = stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=steps)
-    elif which == "sf_mixed":
-        out = stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=steps, warmup=1)
+    elif which in ("sf_mixed", "sf_precond32"):
+        out = stable_fluids_leg(2048, 1024, -15.0, 2000.0, which[3:], 1e-8, steps=steps, warmup=1)
     elif which == "pic":
         out = pic_leg(1024, 512, 8, steps=steps)
     elif which == "serving":

@@ -42,7 +42,7 @@ struct MacGrid;
 // Replacement for the grid's own pressure solve (the request loop batches solves of several grids):
 // solves the grid's rhs into its pres, with the report complete on return.
 struct SolveHook {
-  int (*fn)(void* user, MacGrid& M, double tolerance, int max_iterations, bool mixed, PoissonReport* rep) = nullptr;
+  int (*fn)(void* user, MacGrid& M, double tolerance, int max_iterations, int mode, PoissonReport* rep) = nullptr;
   void* user = nullptr;
 };
 

@@ -717,8 +717,8 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   FB_HIP(hipGetLastError());
   PoissonReport pr;
   const bool hooked = M.hook.fn != nullptr;
-  if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr);
-  else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson != 0, &pr);
+  if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, p.mixed_poisson, &pr);
+  else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson, &pr);
   if (rc) return rc;
   hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
   hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
@@ -1084,6 +1084,8 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
   FB_REQUIRE(mac && params && pose && report, FB_ERR_ARG, "NULL argument");
   auto* M = reinterpret_cast<MacGrid*>(mac);
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
+  FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
+             (int)params->mixed_poisson);
   FB_HIP(hipSetDevice(M->ctx->device));
   *report = fb_sf_report{};
   return MAC_DISPATCH(M, fb::substep, *M, *params, *pose, report);

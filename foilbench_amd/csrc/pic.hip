@@ -1569,6 +1569,8 @@ int fb_pic_substep(fb_pic* pic, const fb_pic_params* params, const fb_pose* pose
                    fb_pic_report* report) {
   FB_REQUIRE(pic && params && pose && sweep_table && report, FB_ERR_ARG, "NULL argument");
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
+  FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
+             (int)params->mixed_poisson);
   auto* P = reinterpret_cast<Pic*>(pic);
   FB_REQUIRE(P->grid_valid, FB_ERR_STATE, "no grid velocity has been set");
   FB_HIP(hipSetDevice(P->ctx->device));

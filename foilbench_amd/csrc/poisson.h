@@ -8,6 +8,9 @@
 
 namespace fb {
 
+// Precision of a pressure solve.
+enum PoissonMode { kPlain = 0, kRefine = 1, kPrecond32 = 2 };
+
 // Result of one solve (mirrors the reference's (info, iterations, relative residual)).
 struct PoissonReport {
   int info = 0;                    // 0 converged; maxiter when exhausted (SciPy convention)
@@ -54,14 +57,15 @@ class Poisson {
             const uint8_t* d_solid, int capacity = 1);
   // Copies the mask-derived operator of a single-grid solver of the same shape into slot k.
   int load_slot(int k, const Poisson& src);
-  // Solve A x = b (b FP64 on device, ny*nx). `mixed`: FP32 PCG sweeps inside an FP64
-  // residual-refinement loop to the same relative tolerance.
-  int solve(const double* d_rhs, double* d_x, double tolerance, int max_iterations, bool mixed,
+  // Solve A x = b (b FP64 on device, ny*nx) to the relative tolerance. `mode`: kPlain (FP64 PCG),
+  // kRefine (FP32 PCG sweeps inside an FP64 residual-refinement loop) or kPrecond32 (FP64 PCG
+  // whose DST-I preconditioner runs in FP32).
+  int solve(const double* d_rhs, double* d_x, double tolerance, int max_iterations, int mode,
             PoissonReport* rep);
   // nb independent solves (grid g uses slot g) sharing every launch; each grid follows the
   // same sequence as solve() and stops on its own. Synchronous; reports are complete.
   int solve_batch(int nb, const double* const* rhs, double* const* x, double tolerance, int max_iterations,
-                  bool mixed, PoissonReport* reps);
+                  int mode, PoissonReport* reps);
   // Completes `rep` (the relative residual) once the stream has been synchronised after solve():
   // solve() leaves its last scalars in flight to host memory.
   void finish(PoissonReport* rep, int g = 0) const;
@@ -106,10 +110,14 @@ class Poisson {
   int precond(CgVecs<T>& v, const T* rin, T* zout);
   template <class T>
   int precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout);
+  // DST-I preconditioner with tables, work array and arithmetic of type T (from `v`) on a
+  // residual and output of type TIO
+  template <class T, class TIO>
+  int dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout);
   template <class T>
   int pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs, int first_chunk,
           int* iterations, int* info, int* failure);
-  int run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps);
+  int run(const double* d_rhs, double* d_x, double tolerance, int maxit, int mode, PoissonReport* reps);
   int put_par(const double* v, int count);
   template <class T>
   int iterations_chunk(CgVecs<T>& v, T* x, int n, int parity);
@@ -121,9 +129,10 @@ class Poisson {
   struct GraphKey {
     size_t elem;
     const void* x;
-    int n, parity, grids;
+    int n, parity, grids, precond32;
     bool operator==(const GraphKey& o) const {
-      return elem == o.elem && x == o.x && n == o.n && parity == o.parity && grids == o.grids;
+      return elem == o.elem && x == o.x && n == o.n && parity == o.parity && grids == o.grids &&
+             precond32 == o.precond32;
     }
   };
   struct CachedGraph {
@@ -133,6 +142,7 @@ class Poisson {
   std::vector<CachedGraph> graphs_;
   std::vector<GraphKey> seen_;  // chunk shapes run once without a graph
   bool use_graphs_ = true;
+  bool precond32_ = false;  // FP64 PCG with the FP32 preconditioner (solve mode kPrecond32)
   int last_iterations_ = 0, last_inner_ = 0, pending_failure_ = 0;
   int read_scalars();
 };

@@ -488,13 +488,15 @@ __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict
 // Orthonormal DST-I along x of every row, Bluestein chirp-z with Stockham FFTs in LDS. A block
 // holds `rows` rows of M points, M/PPT threads each (PPT 16 or 8: more threads per row for short
 // grids; the results are the same bits either way).
+// The transform runs in T; `in` (TI) and `out`, `rsrc` (TO) may be wider (an FP64 residual through
+// the FP32 transform).
 // mode 0: input r masked to fluid cells, output to `out`.
 // mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
 // fixed by M alone: M/16 strided sums, then a pairwise tree.
-template <class T, int PPT>
-__global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* __restrict__ in,
-                                                       const uint8_t* __restrict__ fl, const T* __restrict__ rsrc,
-                                                       T* __restrict__ out, const cplx<T>* __restrict__ chirp,
+template <class T, class TI, class TO, int PPT>
+__global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI* __restrict__ in,
+                                                       const uint8_t* __restrict__ fl, const TO* __restrict__ rsrc,
+                                                       TO* __restrict__ out, const cplx<T>* __restrict__ chirp,
                                                        const cplx<T>* __restrict__ fbt,
                                                        const cplx<T>* __restrict__ roots, int M, int logM, int rows,
                                                        T scale, int mode, double* __restrict__ partial,
@@ -522,7 +524,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
   for (int j = t; j < M; j += nthr) {
     cplx<T> a{(T)0, (T)0};
     if (live && j < nx) {
-      T v = in[base + j];
+      T v = (T)in[base + j];
       if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
       const cplx<T> c = chirp[j];
       a = {v * c.x, v * c.y};
@@ -557,7 +559,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
       const int k = t + j * nthr;
-      if (live && k < nx) out[base + k] = dv[j];
+      if (live && k < nx) out[base + k] = (TO)dv[j];
     }
     return;
   }
@@ -568,10 +570,10 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const T* 
   for (int j = 0; j < KMAX; ++j) {
     const int k = t + j * nthr;
     if (live && k < nx) {
-      const T rv = rsrc[base + k];
-      const T zv = (fl[base + k] & PF_SOLID) ? rv : dv[j];
+      const TO rv = rsrc[base + k];
+      const TO zv = (fl[base + k] & PF_SOLID) ? rv : (TO)dv[j];
       out[base + k] = zv;
-      prod[k] = (sizeof(T) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+      prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
     }
   }
   __syncthreads();
@@ -1172,6 +1174,14 @@ int Poisson::precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout) {
     FB_HIP(L.status());
     return reduce(L, partial, nblocks, 1);
   }
+  if constexpr (sizeof(T) == 8) {
+    if (precond32_) return dst_precond<float, double>(L, v32, rin, zout);
+  }
+  return dst_precond<T, T>(L, v, rin, zout);
+}
+
+template <class T, class TIO>
+int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
   const DstPlan& P = *v.plan;
   const int tp = sizeof(T) == 8 ? FB_F64 : FB_F32;
   // points per thread (16, or 8 for twice the threads per row) and rows per block, as tuned for
@@ -1191,8 +1201,9 @@ int Poisson::precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout) {
     rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
-  auto* kern = ppt == 8 ? dst_rows_kernel<T, 8> : dst_rows_kernel<T, 16>;
-  L(kern, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, rin, v.work, (const cplx<T>*)P.chirp,
+  auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8> : dst_rows_kernel<T, TIO, T, 16>;
+  auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8> : dst_rows_kernel<T, T, TIO, 16>;
+  L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,
     (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 0, partial, scal, ps);
   FB_HIP(L.status());
   const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
@@ -1204,7 +1215,7 @@ int Poisson::precond(Launcher& L, CgVecs<T>& v, const T* rin, T* zout) {
   L(tri_spike_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, (const T*)v.tri_yF,
     (const T*)v.tri_yE, scal);
   FB_HIP(L.status());
-  L(kern, dg, dim3(rows * nthr), lds, nx, ny, v.work, flags, rin, zout, (const cplx<T>*)P.chirp,
+  L(kern_out, dg, dim3(rows * nthr), lds, nx, ny, (const T*)v.work, flags, rin, zout, (const cplx<T>*)P.chirp,
     (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 1, partial, scal, ps);
   FB_HIP(L.status());
   return reduce(L, partial, ny, 1);
@@ -1267,7 +1278,7 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* 
 template <class T>
 int Poisson::iterations_chunk(CgVecs<T>& v, T* x, int n, int parity) {
   if (n <= 0) return FB_OK;
-  const GraphKey key{sizeof(T), (const void*)x, n, parity, nb_};
+  const GraphKey key{sizeof(T), (const void*)x, n, parity, nb_, precond32_ ? 1 : 0};
   for (auto& g : graphs_)
     if (g.key == key) {
       FB_HIP(hipGraphLaunch(g.exec, stream));
@@ -1338,17 +1349,17 @@ int Poisson::sum_to(const double* pin, int n, int slot) {
   return FB_OK;
 }
 
-int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* rep) {
+int Poisson::solve(const double* d_rhs, double* d_x, double tolerance, int maxit, int mode, PoissonReport* rep) {
   nb_ = 1;
-  return run(d_rhs, d_x, tolerance, maxit, mixed, rep);
+  return run(d_rhs, d_x, tolerance, maxit, mode, rep);
 }
 
-int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, double tolerance, int maxit, bool mixed,
+int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, double tolerance, int maxit, int mode,
                          PoissonReport* reps) {
   FB_REQUIRE(nb >= 1 && nb <= cap && nb <= kMaxBatch, FB_ERR_ARG, "batch of %d grids exceeds the capacity %d", nb,
              cap);
   if (nb == 1 && cap == 1) {
-    int rc = solve(rhs[0], x[0], tolerance, maxit, mixed, reps);
+    int rc = solve(rhs[0], x[0], tolerance, maxit, mode, reps);
     if (rc) return rc;
     FB_HIP(hipStreamSynchronize(stream));
     finish(reps);
@@ -1357,7 +1368,7 @@ int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, dou
   nb_ = nb;
   for (int g = 0; g < nb; ++g)
     FB_HIP(hipMemcpyAsync(bstage + (size_t)g * N, rhs[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  int rc = run(bstage, xstage, tolerance, maxit, mixed, reps);
+  int rc = run(bstage, xstage, tolerance, maxit, mode, reps);
   if (rc) return rc;
   for (int g = 0; g < nb; ++g)
     FB_HIP(hipMemcpyAsync(x[g], xstage + (size_t)g * N, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
@@ -1367,7 +1378,7 @@ int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, dou
 }
 
 // The solve of nb_ grids whose right-hand sides sit N apart at d_rhs; solutions go N apart to d_x.
-int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, bool mixed, PoissonReport* reps) {
+int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, int mode, PoissonReport* reps) {
   const int G = nb_;
   for (int g = 0; g < G; ++g) reps[g] = PoissonReport{};
   const int nb = nblocks;
@@ -1390,12 +1401,17 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
   pending_failure_ = 0;
   int its[kMaxBatch], info[kMaxBatch], fail[kMaxBatch], mi[kMaxBatch];
   double rt[kMaxBatch];
-  if (!mixed) {
+  if (mode != kRefine) {
     for (int g = 0; g < G; ++g) {
       mi[g] = maxit;
       rt[g] = rtol;
     }
-    if ((rc = pcg<double>(v64, b64, d_x, rt, mi, true, last_iterations_ + 1, its, info, fail))) return rc;
+    // kPrecond32: the same FP64 PCG, its preconditioner applied with the FP32 tables
+    if (mode == kPrecond32 && (rc = alloc(v32))) return rc;
+    precond32_ = mode == kPrecond32;
+    rc = pcg<double>(v64, b64, d_x, rt, mi, true, last_iterations_ + 1, its, info, fail);
+    precond32_ = false;
+    if (rc) return rc;
     int mx = 0;
     for (int g = 0; g < G; ++g) {
       reps[g].iterations = its[g];

@@ -165,7 +165,7 @@ static void batcher_loop(Batcher* B) {
       x[k] = jobs[k]->M->pres;
     }
     if (rc == FB_OK)
-      rc = B->P.solve_batch(nb, rhs.data(), x.data(), B->key.tol, B->key.maxit, B->key.mixed != 0, reps.data());
+      rc = B->P.solve_batch(nb, rhs.data(), x.data(), B->key.tol, B->key.maxit, B->key.mixed, reps.data());
     if (rc) {
       // the slots may hold partial copies: reload them next time
       for (auto& v : B->slot_ver) v = 0;
@@ -205,9 +205,9 @@ static Batcher* batcher_for(Server* S, const SolveKey& key, int* rc) {
 }
 
 // SolveHook of a served grid: hand the solve to the grid's batcher and wait for it.
-static int batched_solve(void* user, MacGrid& M, double tol, int maxit, bool mixed, PoissonReport* rep) {
+static int batched_solve(void* user, MacGrid& M, double tol, int maxit, int mixed, PoissonReport* rep) {
   auto* B = static_cast<Batcher*>(user);
-  if (B->key.tol != tol || B->key.maxit != maxit || (B->key.mixed != 0) != mixed) {
+  if (B->key.tol != tol || B->key.maxit != maxit || B->key.mixed != mixed) {
     // not the batch's solve: the grid's own solver, completed here as the hook requires
     int rc = M.poisson.solve(M.rhs, M.pres, tol, maxit, mixed, rep);
     if (rc) return rc;
@@ -252,7 +252,7 @@ static void sf_worker(Server* S, int w) {
     int rc = FB_OK;
     Batcher* B = nullptr;
     if (S->opt.sf_batch > 1 && !M.dom.periodic_x && !M.dom.periodic_y) {
-      const SolveKey key{M.dom.nx, M.dom.ny, job.p.mixed_poisson != 0 ? 1 : 0, job.p.max_iterations, M.g.dx, M.g.dy,
+      const SolveKey key{M.dom.nx, M.dom.ny, job.p.mixed_poisson, job.p.max_iterations, M.g.dx, M.g.dy,
                          job.p.tolerance};
       B = batcher_for(S, key, &rc);
     }
@@ -488,6 +488,8 @@ int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_par
   FB_REQUIRE(srv && mac && params && pose, FB_ERR_ARG, "NULL argument");
   FB_REQUIRE(steps >= 1, FB_ERR_ARG, "steps must be positive");
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
+  FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
+             (int)params->mixed_poisson);
   auto* S = reinterpret_cast<Server*>(srv);
   auto* M = reinterpret_cast<fb::MacGrid*>(mac);
   FB_REQUIRE(M->ctx->device == S->device, FB_ERR_ARG, "handle lives on another device");

@@ -100,6 +100,10 @@ def raise_sf_status(rep: N.fb_sf_report, evidence: dict | None = None) -> None:
     raise NumericalFailure(reason, f"Stable Fluids {msg}", stage, ev)
 
 
+
+# Pressure-solve precision of each form (fb_sf_params.mixed_poisson).
+_POISSON_MODE = {"plain": 0, "mixed": 1, "precond32": 2}
+
 class StableFluidsSolver:
     info = SolverInfo(
         id="stable-fluids",
@@ -111,8 +115,9 @@ class StableFluidsSolver:
     )
 
     def __init__(self, form: str = "plain", ctx: N.Context | None = None) -> None:
-        if form not in ("plain", "mixed"):
-            raise ValueError("Stable Fluids forms: 'plain' (FP64 pressure) or 'mixed' (FP32 CG + FP64 refinement)")
+        if form not in _POISSON_MODE:
+            raise ValueError("Stable Fluids forms: 'plain' (FP64 pressure), 'mixed' (FP32 CG + FP64 refinement) or "
+                             "'precond32' (FP64 CG with the DST-I preconditioner in FP32)")
         self._form = form
         self._ctx = ctx
         self._handle = None
@@ -288,7 +293,7 @@ class StableFluidsSolver:
         p.wall_omega_rad = float(np.deg2rad(control.angular_velocity_degrees))
         p.cfl_limit = max(_MIN_PROJECTION_CFL_LIMIT, 2.0 * cfl)
         p.channel_walls = 1 if str(sc.solver_options.get("initial_condition", "")) == "poiseuille" else 0
-        p.mixed_poisson = 1 if self._form == "mixed" else 0
+        p.mixed_poisson = _POISSON_MODE[self._form]
         return p
 
     def _record(self, rep: N.fb_sf_report, projection: bool = True, viscosity: bool = False) -> None:

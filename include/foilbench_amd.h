@@ -195,7 +195,8 @@ typedef struct fb_sf_params {
   double wall_omega_rad;   /* foil angular velocity in rad/s */
   double cfl_limit;        /* projection CFL limit max(1, 2*stable_cfl) */
   int32_t channel_walls;   /* Poiseuille channel walls */
-  int32_t mixed_poisson;   /* 1: FP32 CG sweeps inside FP64 residual refinement */
+  int32_t mixed_poisson;   /* 0: FP64 PCG; 1: FP32 CG sweeps inside FP64 residual refinement;
+                              2: FP64 PCG with its DST-I preconditioner in FP32 */
 } fb_sf_params;
 
 typedef struct fb_sf_report {
@@ -267,7 +268,8 @@ typedef struct fb_pic_params {
   double blend;            /* FLIP fraction of this substep (0 while settling) */
   double wall_travel;      /* |deg2rad(angle - start angle)| * maximum_radius */
   double maximum_radius;   /* NacaFoil.maximum_radius (fb/core/geometry.py:28-33) */
-  int32_t mixed_poisson;   /* 1: FP32 CG sweeps inside FP64 residual refinement */
+  int32_t mixed_poisson;   /* 0: FP64 PCG; 1: FP32 CG sweeps inside FP64 residual refinement;
+                              2: FP64 PCG with its DST-I preconditioner in FP32 */
   int32_t pad_;
 } fb_pic_params;
 

@@ -204,10 +204,10 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
 
     N, ctx = _lib()
 
-    def run_sf():
+    def run_sf(form):
         req = Request(0, 0.0, SF, 128, 64, "2412", 6.0, 700.0, 3)
         sc = req.scenario("float32")
-        s = StableFluidsSolver("mixed", ctx=ctx)
+        s = StableFluidsSolver(form, ctx=ctx)
         s.initialize(sc, NacaFoil(sc.foil), 0)
         t = 0.0
         for _ in range(3):
@@ -217,18 +217,20 @@ def test_autotuned_launch_shapes_leave_results_unchanged():
         s.close()
         return u, v
 
-    base = run_sf()
+    forms = ("mixed", "precond32")
+    base = {f: run_sf(f) for f in forms}
     for rows, ppt in ((4, 16), (2, 8)):
         for prec in (N.FB_F32, N.FB_F64):
             N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, rows))
             N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_PPT, 128, 64, prec, 0, ppt))
         try:
-            tuned = run_sf()
+            tuned = {f: run_sf(f) for f in forms}
         finally:
             for prec in (N.FB_F32, N.FB_F64):
                 N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_ROWS, 128, 64, prec, 0, -1))
                 N.check(ctx.lib.fb_tune_set(N.FB_TUNE_DST_PPT, 128, 64, prec, 0, -1))
-        assert np.array_equal(base[0], tuned[0]) and np.array_equal(base[1], tuned[1])
+        for f in forms:
+            assert np.array_equal(base[f][0], tuned[f][0]) and np.array_equal(base[f][1], tuned[f][1]), f
 
     a = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")
     b = _lattice(160, 80, "4412", 10.0, 1000.0, "f32-mixed")

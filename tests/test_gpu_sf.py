@@ -3,8 +3,9 @@
 Bar: FP64 fields agree with the oracle to 1e-9 relative per advance (the CG dot products
 and the DST are computed in a different order than BLAS/pocketfft, so iterates differ at
 rounding level) and the CG and Jacobi iteration counts agree within one; FP32 scenarios to
-2e-5 relative. The mixed form (FP32 CG sweeps, FP64 refinement) meets the same pressure
-tolerance and tracks the plain form within 1e-6 relative.
+2e-5 relative. The mixed form (FP32 CG sweeps, FP64 refinement) and the precond32 form (FP64
+CG, DST-I preconditioner sweeps in FP32) meet the same pressure tolerance and track the plain
+form within 1e-6 relative.
 """
 
 from __future__ import annotations
@@ -120,9 +121,10 @@ def test_projection_alone_matches_oracle_and_dst_is_exact_without_solids():
     del ControlState
 
 
-def test_mixed_poisson_form_tracks_plain():
+@pytest.mark.parametrize("form", ["mixed", "precond32"])
+def test_reduced_precision_poisson_forms_track_plain(form):
     sc = make_scenario(precision="float64", **CASES["static"])
-    a = _solver("mixed")
+    a = _solver(form)
     b = _solver("plain")
     a.initialize(sc, _foil(sc), 0)
     b.initialize(sc, _foil(sc), 0)
@@ -133,9 +135,31 @@ def test_mixed_poisson_form_tracks_plain():
         rb = b.advance(c, sc.output_dt)
         t += sc.output_dt
         assert ra.evidence["pressure_relative_residual"] <= 1e-6 * 1.000001
+        if form == "precond32":
+            # same FP64 CG recurrence; only the preconditioner's sweeps are rounded to FP32
+            assert ra.evidence.get("pressure_refinements", 0) == 0
+            assert abs(ra.evidence["pressure_iterations"] - rb.evidence["pressure_iterations"]) <= 1
     ua, va = a.faces()
     ub, vb = b.faces()
     assert _rel(ua, ub) < 1e-6 and _rel(va, vb) < 1e-6
+    assert _rel(a.pressure(), b.pressure()) < 1e-5
+
+
+def test_precond32_reaches_tolerances_below_fp32_rounding():
+    """The CG recurrence of the precond32 form stays in FP64, so a tolerance far below FP32's
+    unit roundoff (6e-8) is still met, and the pressure matches the plain form's to about
+    that tolerance."""
+    sc = make_scenario(precision="float64", **dict(CASES["static"], pressure_tolerance=1e-11))
+    a = _solver("precond32")
+    b = _solver("plain")
+    a.initialize(sc, _foil(sc), 0)
+    b.initialize(sc, _foil(sc), 0)
+    c = sc.control_at(sc.output_dt)
+    ra = a.advance(c, sc.output_dt)
+    rb = b.advance(c, sc.output_dt)
+    assert ra.evidence["pressure_relative_residual"] <= 1e-11 * 1.000001
+    assert ra.evidence["pressure_iterations"] <= rb.evidence["pressure_iterations"] + 3
+    assert _rel(a.pressure(), b.pressure()) < 1e-8
 
 
 def test_sf_rollback_restores_faces_bitwise():
