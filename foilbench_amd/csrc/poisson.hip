@@ -32,8 +32,10 @@ enum { S_RN2 = 0, S_RHO, S_RHOPREV, S_PQ, S_ALPHA, S_BETA, S_ATOL, S_DONE, S_ITE
        // block ticket of the fused folds (an unsigned int in this double's storage)
        S_TICKET = S_COUNT + 6 };
 
+// Aligned to its full size so that LDS and global accesses move a complex value in one 64-bit
+// (FP32) or 128-bit (FP64) instruction.
 template <class T>
-struct cplx {
+struct alignas(2 * sizeof(T)) cplx {
   T x, y;
 };
 template <class T>
@@ -437,8 +439,14 @@ __device__ __forceinline__ void dft_r(cplx<T>* v) {
   else dft2(v);
 }
 
+// LDS slot of point i of a row: one pad slot after every 8 points, so that the writes of the
+// first Stockham passes (lanes 8 or 64 points apart) land in different banks. A row of M points
+// takes lds_row(M) slots.
+__host__ __device__ constexpr int lds_slot(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int lds_row(int M) { return M + (M >> 3); }
+
 // One radix-R Stockham pass (sub-transforms of length L -> L*R) over a row of M points in
-// LDS, in place: each of the row's M/16 threads reads its 16 points, the block synchronises,
+// LDS, in place: each of the row's M/PPT threads reads its PPT points, the block synchronises,
 // then writes them to their autosorted places.
 template <class T, int R, int PPT>
 __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int L, int t,
@@ -450,7 +458,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __res
   for (int u = 0; u < NB; ++u) {
     const int b = t + u * nthr;
 #pragma unroll
-    for (int s = 0; s < R; ++s) v[u][s] = buf[b + s * MR];
+    for (int s = 0; s < R; ++s) v[u][s] = buf[lds_slot(b + s * MR)];
   }
   __syncthreads();
   const int stride = M / (L * R);
@@ -469,7 +477,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __res
     dft_r<T, R>(v[u]);
     const int base = (b - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[base + q * L] = v[u][q];
+    for (int q = 0; q < R; ++q) buf[lds_slot(base + q * L)] = v[u][q];
   }
   __syncthreads();
 }
@@ -518,8 +526,8 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
   const int t = threadIdx.x - lr * nthr;
   const int row = blockIdx.x * rows + lr;
   const bool live = row < ny;
-  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw) + (size_t)lr * M;
-  double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * M * sizeof(cplx<T>)) + (size_t)lr * nred;
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw) + (size_t)lr * lds_row(M);
+  double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * lds_row(M) * sizeof(cplx<T>)) + (size_t)lr * nred;
   const size_t base = (size_t)(live ? row : 0) * nx;
   for (int j = t; j < M; j += nthr) {
     cplx<T> a{(T)0, (T)0};
@@ -529,14 +537,14 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
       const cplx<T> c = chirp[j];
       a = {v * c.x, v * c.y};
     }
-    buf[j] = a;
+    buf[lds_slot(j)] = a;
   }
   __syncthreads();
   fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
   // convolution with the chirp: inverse FFT as conj(FFT(conj(.)))
   for (int k = t; k < M; k += nthr) {
-    const cplx<T> v = cmul(buf[k], fbt[k]);
-    buf[k] = {v.x, -v.y};
+    const cplx<T> v = cmul(buf[lds_slot(k)], fbt[k]);
+    buf[lds_slot(k)] = {v.x, -v.y};
   }
   __syncthreads();
   fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
@@ -549,7 +557,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
     const int k = t + j * nthr;
     dv[j] = (T)0;
     if (live && k < nx) {
-      const cplx<T> w = buf[k];
+      const cplx<T> w = buf[lds_slot(k)];
       const cplx<T> conv{w.x * invM, -w.y * invM};
       const cplx<T> y = cmul(chirp[k], conv);
       dv[j] = -y.y * scale;
@@ -1197,10 +1205,10 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
   if (ppt != 8 || P.M / 8 > 512) ppt = 16;
   const int nthr = P.M / ppt;
   int rows = tuned(TUNE_DST_ROWS);
-  if (rows < 1 || rows * nthr > 512 || (size_t)rows * P.M * sizeof(cplx<T>) > 96 * 1024)
+  if (rows < 1 || rows * nthr > 512 || (size_t)rows * lds_row(P.M) * sizeof(cplx<T>) > 96 * 1024)
     rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
-  const size_t lds = (size_t)rows * P.M * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
+  const size_t lds = (size_t)rows * lds_row(P.M) * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
   auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8> : dst_rows_kernel<T, TIO, T, 16>;
   auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8> : dst_rows_kernel<T, T, TIO, 16>;
   L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,

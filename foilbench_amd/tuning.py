@@ -65,13 +65,14 @@ class Autotuner:
         while m < 2 * nx - 1:
             m *= 2
         esize = 8 if precision == N.FB_F32 else 16  # bytes of one complex value in LDS
+        slots = m + m // 8  # LDS slots of one row: a pad slot after every 8 points
         times = {}
         for ppt in (16, 8):
             nthr = m // ppt
             if nthr > 512:
                 continue
             cands = [r for r in (1, 2, 4, 8, 16, 32)
-                     if r * nthr <= 512 and r * m * esize <= 96 * 1024 and r <= ny * grids]
+                     if r * nthr <= 512 and r * slots * esize <= 96 * 1024 and r <= ny * grids]
             self._set(N.FB_TUNE_DST_PPT, nx, ny, precision, grids, ppt)
             for r in cands:
                 self._set(N.FB_TUNE_DST_ROWS, nx, ny, precision, grids, r)
