@@ -70,6 +70,7 @@ class fb_sf_params(C.Structure):
         ("max_iterations", C.c_int32), ("maccormack", C.c_int32),
         ("freestream_x", C.c_double), ("freestream_y", C.c_double), ("wall_omega_rad", C.c_double),
         ("cfl_limit", C.c_double), ("channel_walls", C.c_int32), ("mixed_poisson", C.c_int32),
+        ("warm_pressure", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

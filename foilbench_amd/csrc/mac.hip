@@ -717,8 +717,9 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   FB_HIP(hipGetLastError());
   PoissonReport pr;
   const bool hooked = M.hook.fn != nullptr;
-  if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, p.mixed_poisson, &pr);
-  else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, p.mixed_poisson, &pr);
+  const int mode = p.mixed_poisson | (p.warm_pressure ? kWarmStart : 0);
+  if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, mode, &pr);
+  else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, mode, &pr);
   if (rc) return rc;
   hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
   hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
@@ -937,6 +938,8 @@ int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision,
   A((void**)&M->ck_solid, M->N);
   A((void**)&M->rhs, M->N * sizeof(double));
   A((void**)&M->pres, M->N * sizeof(double));
+  // the pressure is the initial guess of a warm-started solve: zero before the first one
+  if (e == hipSuccess) e = hipMemsetAsync(M->pres, 0, M->N * sizeof(double), c->stream);
   A((void**)&M->keys, 16 * sizeof(unsigned long long));
   if (e == hipSuccess) e = host_alloc((void**)&M->host_keys, 16 * sizeof(unsigned long long));
   if (e != hipSuccess) {
@@ -1086,6 +1089,9 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
   FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
              (int)params->mixed_poisson);
+  FB_REQUIRE((params->warm_pressure == 0 || params->warm_pressure == 1) &&
+                 !(params->warm_pressure && params->mixed_poisson == 1),
+             FB_ERR_ARG, "warm_pressure must be 0 or 1, and 0 with mixed_poisson 1");
   FB_HIP(hipSetDevice(M->ctx->device));
   *report = fb_sf_report{};
   return MAC_DISPATCH(M, fb::substep, *M, *params, *pose, report);

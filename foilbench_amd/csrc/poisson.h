@@ -8,8 +8,9 @@
 
 namespace fb {
 
-// Precision of a pressure solve.
-enum PoissonMode { kPlain = 0, kRefine = 1, kPrecond32 = 2 };
+// Precision of a pressure solve (low two bits), and the warm-start flag: CG from the solution
+// buffer's current contents instead of zero (plain and kPrecond32 only).
+enum PoissonMode { kPlain = 0, kRefine = 1, kPrecond32 = 2, kPrecisionMask = 3, kWarmStart = 4 };
 
 // Result of one solve (mirrors the reference's (info, iterations, relative residual)).
 struct PoissonReport {
@@ -116,7 +117,7 @@ class Poisson {
   int dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout);
   template <class T>
   int pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs, int first_chunk,
-          int* iterations, int* info, int* failure);
+          int* iterations, int* info, int* failure, bool warm = false);
   int run(const double* d_rhs, double* d_x, double tolerance, int maxit, int mode, PoissonReport* reps);
   int put_par(const double* v, int count);
   template <class T>

@@ -204,6 +204,7 @@ __device__ __forceinline__ void finalize_body(const double* __restrict__ partial
     scal[S_RHOPREV] = scal[S_RHO];
     scal[S_ITER] += 1.0;
   }
+  // mode 5 (after mode 4 on ||b||^2, warm start): s = ||b - A x0||^2, the residual tests only
   if (mode == 4) {
     // scipy: atol = max(atol, rtol*||b||); ||b|| == 0 returns b with info 0 before the loop
     scal[S_ATOL] = fmax(0.0, scal[S_ATOL] * sqrt(s));
@@ -943,6 +944,14 @@ __global__ void __launch_bounds__(256) add_f32_scaled(size_t N, double* __restri
   if (i < N) x[i] += (double)e[i] * s;
 }
 
+// Warm start: a grid already done before its first iteration (||b|| == 0, or a rejected
+// right-hand side) returns x = 0, as SciPy returns b for b == 0.
+__global__ void __launch_bounds__(256) zero_if_done_kernel(size_t N, double* __restrict__ x, const double* scal) {
+  if (!CG_DONE(scal + (size_t)blockIdx.z * kScalStride)) return;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) x[(size_t)blockIdx.z * N + i] = 0.0;
+}
+
 __global__ void __launch_bounds__(256) fill_zero(size_t N, double* __restrict__ a) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < N) a[i] = 0.0;
@@ -1231,10 +1240,13 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
 
 template <class T>
 int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* maxit, bool check_rhs,
-                 int first_chunk, int* iterations, int* info, int* failure) {
+                 int first_chunk, int* iterations, int* info, int* failure, bool warm) {
   const size_t n = (size_t)nb_ * N;
-  FB_HIP(hipMemsetAsync(x, 0, n * sizeof(T), stream));
-  FB_HIP(hipMemcpyAsync(v.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+  if constexpr (sizeof(T) != 8) warm = false;
+  if (!warm) {
+    FB_HIP(hipMemsetAsync(x, 0, n * sizeof(T), stream));
+    FB_HIP(hipMemcpyAsync(v.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+  }
   int maxall = 0;
   {
     double par[2 * kMaxBatch];
@@ -1247,7 +1259,9 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* 
     if (rc) return rc;
   }
   hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, (const double*)dpar, nb_);
-  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks, 1, nb_), dim3(256), 0, stream, v.r, v.r, N, partial, ps);
+  // ||b||^2 (r = b when starting from zero) sets the absolute tolerance
+  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(nblocks, 1, nb_), dim3(256), 0, stream, warm ? b : (const T*)v.r,
+                     warm ? b : (const T*)v.r, N, partial, ps);
   FB_HIP(hipGetLastError());
   int rc = reduce(partial, nblocks, 4);
   if (rc) return rc;
@@ -1256,6 +1270,17 @@ int Poisson::pcg(CgVecs<T>& v, const T* b, T* x, const double* rtol, const int* 
     const double safe = 0.25 * std::sqrt(std::numeric_limits<double>::max() / (double)std::max<size_t>(N, 1));
     hipLaunchKernelGGL(rhs_check_kernel, dim3(1, 1, nb_), dim3(64), 0, stream, scal, safe);
     FB_HIP(hipGetLastError());
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (warm) {
+      // r0 = b - A x0 and the residual tests on it; grids done already return x = 0
+      hipLaunchKernelGGL(zero_if_done_kernel, dim3(nblocks, 1, nb_), dim3(256), 0, stream, N, x,
+                         (const double*)scal);
+      hipLaunchKernelGGL(residual_kernel, dim3(nblocks, 1, nb_), dim3(256), 0, stream, nx, ny, flags, diag, b, x,
+                         v.r, partial, 1.0 / (dx * dx), 1.0 / (dy * dy), ps);
+      FB_HIP(hipGetLastError());
+      if ((rc = reduce(partial, nblocks, 5))) return rc;
+    }
   }
   int launched = 0;
   // the first poll comes after about as many iterations as the previous solve needed
@@ -1374,8 +1399,11 @@ int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, dou
     return FB_OK;
   }
   nb_ = nb;
-  for (int g = 0; g < nb; ++g)
+  for (int g = 0; g < nb; ++g) {
     FB_HIP(hipMemcpyAsync(bstage + (size_t)g * N, rhs[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (mode & kWarmStart)
+      FB_HIP(hipMemcpyAsync(xstage + (size_t)g * N, x[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  }
   int rc = run(bstage, xstage, tolerance, maxit, mode, reps);
   if (rc) return rc;
   for (int g = 0; g < nb; ++g)
@@ -1406,6 +1434,9 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
   }
   const double rtol = 0.9 * tolerance;
   const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
+  const bool warm = (mode & kWarmStart) != 0;
+  mode &= kPrecisionMask;
+  FB_REQUIRE(!(warm && mode == kRefine), FB_ERR_ARG, "a warm-started pressure solve needs FP64 CG (mode 0 or 2)");
   pending_failure_ = 0;
   int its[kMaxBatch], info[kMaxBatch], fail[kMaxBatch], mi[kMaxBatch];
   double rt[kMaxBatch];
@@ -1417,7 +1448,7 @@ int Poisson::run(const double* d_rhs, double* d_x, double tolerance, int maxit, 
     // kPrecond32: the same FP64 PCG, its preconditioner applied with the FP32 tables
     if (mode == kPrecond32 && (rc = alloc(v32))) return rc;
     precond32_ = mode == kPrecond32;
-    rc = pcg<double>(v64, b64, d_x, rt, mi, true, last_iterations_ + 1, its, info, fail);
+    rc = pcg<double>(v64, b64, d_x, rt, mi, true, last_iterations_ + 1, its, info, fail, warm);
     precond32_ = false;
     if (rc) return rc;
     int mx = 0;

@@ -252,7 +252,8 @@ static void sf_worker(Server* S, int w) {
     int rc = FB_OK;
     Batcher* B = nullptr;
     if (S->opt.sf_batch > 1 && !M.dom.periodic_x && !M.dom.periodic_y) {
-      const SolveKey key{M.dom.nx, M.dom.ny, job.p.mixed_poisson, job.p.max_iterations, M.g.dx, M.g.dy,
+      const int mode = job.p.mixed_poisson | (job.p.warm_pressure ? kWarmStart : 0);
+      const SolveKey key{M.dom.nx, M.dom.ny, mode, job.p.max_iterations, M.g.dx, M.g.dy,
                          job.p.tolerance};
       B = batcher_for(S, key, &rc);
     }
@@ -490,6 +491,9 @@ int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_par
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
   FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
              (int)params->mixed_poisson);
+  FB_REQUIRE((params->warm_pressure == 0 || params->warm_pressure == 1) &&
+                 !(params->warm_pressure && params->mixed_poisson == 1),
+             FB_ERR_ARG, "warm_pressure must be 0 or 1, and 0 with mixed_poisson 1");
   auto* S = reinterpret_cast<Server*>(srv);
   auto* M = reinterpret_cast<fb::MacGrid*>(mac);
   FB_REQUIRE(M->ctx->device == S->device, FB_ERR_ARG, "handle lives on another device");

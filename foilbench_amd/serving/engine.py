@@ -71,8 +71,12 @@ def _build(req: Request, form: str, ctx: N.Context):
     if req.solver == SF:
         sc = req.scenario("float64" if form == "plain" else "float32")
         # serving form: FP64 CG with the DST-I preconditioner sweeps in FP32 (fewer, cheaper
-        # iterations than FP32 CG with FP64 refinement restarts, and a smaller error)
-        s = StableFluidsSolver("plain" if form == "plain" else "precond32", ctx=ctx)
+        # iterations than FP32 CG with FP64 refinement restarts, and a smaller error), started
+        # from the previous substep's pressure
+        if form == "plain":
+            s = StableFluidsSolver("plain", ctx=ctx)
+        else:
+            s = StableFluidsSolver("precond32", ctx=ctx, warm_pressure=True)
         s.initialize(sc, NacaFoil(sc.foil), 0)
         params = s._params(sc.output_dt, sc.control_at(0.0))
         return s, s._handle, params, pose_struct(-req.aoa_degrees)

@@ -114,11 +114,16 @@ class StableFluidsSolver:
         acceleration="gfx950 HIP MAC grid + DST-preconditioned CG (foilbench_amd)",
     )
 
-    def __init__(self, form: str = "plain", ctx: N.Context | None = None) -> None:
+    def __init__(self, form: str = "plain", ctx: N.Context | None = None, warm_pressure: bool = False) -> None:
+        """`warm_pressure`: each pressure CG starts from the previous substep's pressure instead of
+        zero (same stopping rule; fewer iterations). Not with the 'mixed' form."""
         if form not in _POISSON_MODE:
             raise ValueError("Stable Fluids forms: 'plain' (FP64 pressure), 'mixed' (FP32 CG + FP64 refinement) or "
                              "'precond32' (FP64 CG with the DST-I preconditioner in FP32)")
+        if warm_pressure and form == "mixed":
+            raise ValueError("warm_pressure needs FP64 CG: form 'plain' or 'precond32'")
         self._form = form
+        self._warm = bool(warm_pressure)
         self._ctx = ctx
         self._handle = None
         self._scenario: Scenario | None = None
@@ -257,6 +262,9 @@ class StableFluidsSolver:
         N.check(self._lib().fb_mac_get_pressure(self._handle, N.ptr(p)))
         return p
 
+    def _set_pressure(self, p: np.ndarray) -> None:
+        N.check(self._lib().fb_mac_set_pressure(self._handle, N.ptr(np.ascontiguousarray(p, dtype=np.float64))))
+
     def restart(self, scenario: Scenario, geometry: NacaFoil, seed: int, start: RestartState) -> None:
         # fb/solvers/stable_fluids.py:218-236
         validate_restart_state(start)
@@ -294,6 +302,7 @@ class StableFluidsSolver:
         p.cfl_limit = max(_MIN_PROJECTION_CFL_LIMIT, 2.0 * cfl)
         p.channel_walls = 1 if str(sc.solver_options.get("initial_condition", "")) == "poiseuille" else 0
         p.mixed_poisson = _POISSON_MODE[self._form]
+        p.warm_pressure = 1 if self._warm else 0
         return p
 
     def _record(self, rep: N.fb_sf_report, projection: bool = True, viscosity: bool = False) -> None:

@@ -197,6 +197,10 @@ typedef struct fb_sf_params {
   int32_t channel_walls;   /* Poiseuille channel walls */
   int32_t mixed_poisson;   /* 0: FP64 PCG; 1: FP32 CG sweeps inside FP64 residual refinement;
                               2: FP64 PCG with its DST-I preconditioner in FP32 */
+  int32_t warm_pressure;   /* 0: CG starts from zero (the reference); 1: from the grid's previous
+                              pressure (SciPy cg with x0: r0 = b - A x0, same stopping rule);
+                              not with mixed_poisson 1 */
+  int32_t pad_;
 } fb_sf_params;
 
 typedef struct fb_sf_report {
