@@ -67,6 +67,8 @@ struct MacGrid {
   bool geom_valid = false, ck_valid = false;
   double* rhs = nullptr;
   double* pres = nullptr;
+  double* pres_prev = nullptr;  // extrapolated warm starts: the pressure of the solve before the last
+  int pres_solves = 0;          // solves whose results sit in pres and pres_prev since the last reset
   unsigned long long* keys = nullptr;
   unsigned long long* host_keys = nullptr;
   unsigned long long* resid = nullptr;  // 2 x (maxit + 2)

@@ -696,6 +696,26 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   return FB_OK;
 }
 
+// Extrapolated warm start: p <- 2 p - prev (when `extrapolate`), prev <- the old p.
+__global__ void __launch_bounds__(256) extrapolate_kernel(size_t N, double* __restrict__ p, double* __restrict__ prev,
+                                                          int extrapolate) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const double t = p[i];
+  if (extrapolate) p[i] = 2.0 * t - prev[i];
+  prev[i] = t;
+}
+
+// A warm-started projection that ends with any non-zero status leaves no usable initial guess:
+// the pressure is zeroed and the extrapolation history dropped.
+static int warm_reset_on_failure(MacGrid& M, const fb_sf_params& p, const fb_sf_report* rep) {
+  if (p.warm_pressure && rep->status != 0) {
+    FB_HIP(hipMemsetAsync(M.pres, 0, M.N * sizeof(double), M.ctx->stream));
+    M.pres_solves = 0;
+  }
+  return FB_OK;
+}
+
 template <class T>
 int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   // The reference checks the CFL guard and finiteness before solving (fb/solvers/stable_fluids.py:
@@ -718,9 +738,20 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   PoissonReport pr;
   const bool hooked = M.hook.fn != nullptr;
   const int mode = p.mixed_poisson | (p.warm_pressure ? kWarmStart : 0);
+  if (p.warm_pressure == 2) {
+    // initial guess 2 p_n - p_(n-1) once two solves are on record, else p_n
+    if (!M.pres_prev) {
+      FB_HIP(dev_alloc((void**)&M.pres_prev, M.N * sizeof(double)));
+      M.pres_solves = 0;
+    }
+    hipLaunchKernelGGL(extrapolate_kernel, dim3((unsigned)((M.N + 255) / 256)), dim3(256), 0, s, M.N, M.pres,
+                       M.pres_prev, M.pres_solves >= 2 ? 1 : 0);
+    FB_HIP(hipGetLastError());
+  }
   if (hooked) rc = M.hook.fn(M.hook.user, M, p.tolerance, p.max_iterations, mode, &pr);
   else rc = M.poisson.solve(M.rhs, M.pres, p.tolerance, p.max_iterations, mode, &pr);
   if (rc) return rc;
+  M.pres_solves++;
   hipLaunchKernelGGL(grad_u_kernel<T>, gu, dim3(256), 0, s, M.g, (const T*)M.U[o], M.pres, dt, (T*)M.U[a]);
   hipLaunchKernelGGL(grad_v_kernel<T>, gv, dim3(256), 0, s, M.g, (const T*)M.V[o], M.pres, dt, (T*)M.V[a]);
   hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)M.U[a], (T*)M.U[o]);
@@ -736,13 +767,13 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   stats_from_host(M, 8, &post);
   if (pre.nonfinite_faces || pre.nonfinite_wall) {
     rep->status = FB_SF_NONFINITE_PROJECTION_INPUT;
-    return FB_OK;
+    return warm_reset_on_failure(M, p, rep);
   }
   const double pc = std::max(pre.max_face_speed, pre.max_wall_speed) * dt / std::min(M.g.dx, M.g.dy);
   rep->projection_cfl = pc;
   if (pc > p.cfl_limit) {
     rep->status = FB_SF_PROJECTION_CFL;
-    return FB_OK;
+    return warm_reset_on_failure(M, p, rep);
   }
   rep->pressure_iterations = pr.iterations;
   rep->pressure_residual = pr.relative_residual;
@@ -753,20 +784,20 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
                   : pr.failure == 2 ? FB_SF_RHS_RANGE
                   : pr.failure == 3 ? FB_SF_NONFINITE_ITERATE
                                     : FB_SF_CG_BREAKDOWN;
-    return FB_OK;
+    return warm_reset_on_failure(M, p, rep);
   }
   rep->pressure_converged =
       (pr.info == 0 && pr.relative_residual <= p.tolerance * (1.0 + 1.0e-6)) ? 1 : 0;
   if (!rep->pressure_converged) {
     rep->status = FB_SF_PROJECTION_NOT_CONVERGED;
-    return FB_OK;
+    return warm_reset_on_failure(M, p, rep);
   }
   if (post.nonfinite_faces) rep->status = FB_SF_NONFINITE_PROJECTION_OUTPUT;
   // the statistics of the projected faces, for the caller's postconditions
   M.last_stats = post;
   M.last_stats_omega = p.wall_omega_rad;
   M.last_stats_valid = true;
-  return FB_OK;
+  return warm_reset_on_failure(M, p, rep);
 }
 
 template <class T>
@@ -855,7 +886,7 @@ void free_mac(MacGrid* M) {
   for (void* ptr : {M->U[0], M->U[1], M->V[0], M->V[1], M->CU[0], M->CU[1], M->CU[2], M->CU[3], M->CV[0], M->CV[1],
                     M->CV[2], M->CV[3], M->DU[0], M->DU[1], M->DU[2], M->DV[0], M->DV[1], M->DV[2], M->CKU, M->CKV,
                     (void*)M->sdf, (void*)M->solid, (void*)M->ck_solid, (void*)M->rhs, (void*)M->pres,
-                    (void*)M->keys, (void*)M->resid})
+                    (void*)M->pres_prev, (void*)M->keys, (void*)M->resid})
     if (ptr) (void)dev_free(ptr);
   if (M->host_keys) (void)host_free(M->host_keys);
   if (M->host_resid) (void)host_free(M->host_resid);
@@ -1018,6 +1049,7 @@ int fb_mac_set_pressure(fb_mac* mac, const double* p_host) {
   FB_HIP(hipSetDevice(M->ctx->device));
   FB_HIP(hipMemcpyAsync(M->pres, p_host, M->N * sizeof(double), hipMemcpyHostToDevice, M->ctx->stream));
   FB_HIP(hipStreamSynchronize(M->ctx->stream));
+  M->pres_solves = 1;  // an imported pressure starts a new extrapolation history
   return FB_OK;
 }
 
@@ -1089,9 +1121,9 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
   FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
              (int)params->mixed_poisson);
-  FB_REQUIRE((params->warm_pressure == 0 || params->warm_pressure == 1) &&
+  FB_REQUIRE(params->warm_pressure >= 0 && params->warm_pressure <= 2 &&
                  !(params->warm_pressure && params->mixed_poisson == 1),
-             FB_ERR_ARG, "warm_pressure must be 0 or 1, and 0 with mixed_poisson 1");
+             FB_ERR_ARG, "warm_pressure must be 0, 1 or 2, and 0 with mixed_poisson 1");
   FB_HIP(hipSetDevice(M->ctx->device));
   *report = fb_sf_report{};
   return MAC_DISPATCH(M, fb::substep, *M, *params, *pose, report);

@@ -491,9 +491,9 @@ int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_par
   FB_REQUIRE(params->max_iterations >= 1, FB_ERR_ARG, "max_iterations must be positive");
   FB_REQUIRE(params->mixed_poisson >= 0 && params->mixed_poisson <= 2, FB_ERR_ARG, "unknown pressure precision %d",
              (int)params->mixed_poisson);
-  FB_REQUIRE((params->warm_pressure == 0 || params->warm_pressure == 1) &&
+  FB_REQUIRE(params->warm_pressure >= 0 && params->warm_pressure <= 2 &&
                  !(params->warm_pressure && params->mixed_poisson == 1),
-             FB_ERR_ARG, "warm_pressure must be 0 or 1, and 0 with mixed_poisson 1");
+             FB_ERR_ARG, "warm_pressure must be 0, 1 or 2, and 0 with mixed_poisson 1");
   auto* S = reinterpret_cast<Server*>(srv);
   auto* M = reinterpret_cast<fb::MacGrid*>(mac);
   FB_REQUIRE(M->ctx->device == S->device, FB_ERR_ARG, "handle lives on another device");

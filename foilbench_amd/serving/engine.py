@@ -33,9 +33,12 @@ from .mix import LBM, SF, Request
 
 # Device bytes per cell of a request's state, from the allocations of fb_mac_create/Poisson set-up
 # and fb_lbm_create, with a margin; plus a fixed allowance per request.
-_SF_BYTES_PER_CELL = 420
+_SF_BYTES_PER_CELL = 430
 _LBM_BYTES_PER_CELL = {"f64": 750, "f32-mixed": 500, "f16-mixed": 400, "f32": 500}
 _FIXED_BYTES = 8 << 20
+# Initial guess of the serving form's pressure solves (fb_sf_params.warm_pressure): 1, the
+# previous substep's pressure; 2, its linear extrapolation from the two previous ones.
+SF_WARM_PRESSURE = 2
 
 
 def estimate_bytes(req: Request, form: str) -> int:
@@ -76,7 +79,7 @@ def _build(req: Request, form: str, ctx: N.Context):
         if form == "plain":
             s = StableFluidsSolver("plain", ctx=ctx)
         else:
-            s = StableFluidsSolver("precond32", ctx=ctx, warm_pressure=True)
+            s = StableFluidsSolver("precond32", ctx=ctx, warm_pressure=SF_WARM_PRESSURE)
         s.initialize(sc, NacaFoil(sc.foil), 0)
         params = s._params(sc.output_dt, sc.control_at(0.0))
         return s, s._handle, params, pose_struct(-req.aoa_degrees)

@@ -114,16 +114,19 @@ class StableFluidsSolver:
         acceleration="gfx950 HIP MAC grid + DST-preconditioned CG (foilbench_amd)",
     )
 
-    def __init__(self, form: str = "plain", ctx: N.Context | None = None, warm_pressure: bool = False) -> None:
-        """`warm_pressure`: each pressure CG starts from the previous substep's pressure instead of
-        zero (same stopping rule; fewer iterations). Not with the 'mixed' form."""
+    def __init__(self, form: str = "plain", ctx: N.Context | None = None, warm_pressure: int = 0) -> None:
+        """`warm_pressure`: 0, each pressure CG starts from zero (the reference); 1, from the previous
+        substep's pressure; 2, from its linear extrapolation 2 p_n - p_(n-1). Same stopping rule,
+        fewer iterations. Not with the 'mixed' form."""
+        if int(warm_pressure) not in (0, 1, 2):
+            raise ValueError("warm_pressure must be 0, 1 or 2")
         if form not in _POISSON_MODE:
             raise ValueError("Stable Fluids forms: 'plain' (FP64 pressure), 'mixed' (FP32 CG + FP64 refinement) or "
                              "'precond32' (FP64 CG with the DST-I preconditioner in FP32)")
         if warm_pressure and form == "mixed":
             raise ValueError("warm_pressure needs FP64 CG: form 'plain' or 'precond32'")
         self._form = form
-        self._warm = bool(warm_pressure)
+        self._warm = int(warm_pressure)
         self._ctx = ctx
         self._handle = None
         self._scenario: Scenario | None = None
@@ -302,7 +305,7 @@ class StableFluidsSolver:
         p.cfl_limit = max(_MIN_PROJECTION_CFL_LIMIT, 2.0 * cfl)
         p.channel_walls = 1 if str(sc.solver_options.get("initial_condition", "")) == "poiseuille" else 0
         p.mixed_poisson = _POISSON_MODE[self._form]
-        p.warm_pressure = 1 if self._warm else 0
+        p.warm_pressure = self._warm
         return p
 
     def _record(self, rep: N.fb_sf_report, projection: bool = True, viscosity: bool = False) -> None:

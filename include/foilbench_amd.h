@@ -198,8 +198,9 @@ typedef struct fb_sf_params {
   int32_t mixed_poisson;   /* 0: FP64 PCG; 1: FP32 CG sweeps inside FP64 residual refinement;
                               2: FP64 PCG with its DST-I preconditioner in FP32 */
   int32_t warm_pressure;   /* 0: CG starts from zero (the reference); 1: from the grid's previous
-                              pressure (SciPy cg with x0: r0 = b - A x0, same stopping rule);
-                              not with mixed_poisson 1 */
+                              pressure p_n; 2: from 2 p_n - p_(n-1) once two solves are on record
+                              (SciPy cg with x0: r0 = b - A x0, same stopping rule); a projection
+                              that fails zeroes the pressure; not with mixed_poisson 1 */
   int32_t pad_;
 } fb_sf_params;
 

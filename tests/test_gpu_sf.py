@@ -26,10 +26,10 @@ def _solver(form="plain"):
     return StableFluidsSolver(form)
 
 
-def _solver_warm(form):
+def _solver_warm(form, warm=1):
     from foilbench_amd.solvers.stable_fluids import StableFluidsSolver
 
-    return StableFluidsSolver(form, warm_pressure=True)
+    return StableFluidsSolver(form, warm_pressure=warm)
 
 
 def _foil(sc):
@@ -151,13 +151,13 @@ def test_reduced_precision_poisson_forms_track_plain(form):
     assert _rel(a.pressure(), b.pressure()) < 1e-5
 
 
-@pytest.mark.parametrize("form", ["plain", "precond32"])
-def test_warm_started_pressure_meets_tolerance_with_fewer_iterations(form):
-    """CG from the previous substep's pressure (SciPy cg with x0): the same relative-residual
-    test on ||b - A x||, fewer iterations after the first substep, fields within the tolerance's
-    reach of the zero-start solve."""
+@pytest.mark.parametrize("form,warm", [("plain", 1), ("precond32", 1), ("precond32", 2)])
+def test_warm_started_pressure_meets_tolerance_with_fewer_iterations(form, warm):
+    """CG from the previous substep's pressure or its extrapolation (SciPy cg with x0): the same
+    relative-residual test on ||b - A x||, fewer iterations after the first substep, fields within
+    the tolerance's reach of the zero-start solve."""
     sc = make_scenario(precision="float64", **CASES["static"])
-    a = _solver_warm(form)
+    a = _solver_warm(form, warm)
     b = _solver(form)
     a.initialize(sc, _foil(sc), 0)
     b.initialize(sc, _foil(sc), 0)
@@ -188,6 +188,27 @@ def test_warm_start_with_zero_rhs_returns_zero_pressure():
     r = gpu.advance(sc.control_at(sc.output_dt), sc.output_dt)
     assert r.evidence["pressure_iterations"] == 0
     np.testing.assert_array_equal(gpu.pressure(), np.zeros((24, 40)))
+
+
+def test_failed_warm_started_projection_zeroes_the_pressure():
+    """A projection that fails leaves no initial guess for the next solve: the pressure is zero."""
+    from foilbench_amd.core.models import NumericalFailure
+
+    sc = make_scenario(precision="float64", **CASES["static"])
+    gpu = _solver_warm("precond32", 2)
+    gpu.initialize(sc, _foil(sc), 0)
+    t = 0.0
+    for _ in range(3):
+        t += sc.output_dt
+        gpu.advance(sc.control_at(t), sc.output_dt)
+    assert np.abs(gpu.pressure()).max() > 0.0
+    # a projection alone (no viscosity sweeps) with one CG iteration allowed
+    gpu._scenario = make_scenario(precision="float64", **dict(CASES["static"], pressure_max_iterations=1,
+                                                              pressure_tolerance=1e-12))
+    with pytest.raises(NumericalFailure) as err:
+        gpu._apply_projection(sc.output_dt)
+    assert err.value.reason == "projection_failure"
+    np.testing.assert_array_equal(gpu.pressure(), np.zeros_like(gpu.pressure()))
 
 
 def test_precond32_reaches_tolerances_below_fp32_rounding():
