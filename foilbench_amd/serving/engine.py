@@ -112,7 +112,7 @@ class Server:
 
     def __init__(self, sf_workers: int = 8, lbm_chunk: int = 32, use_graphs: bool = True,
                  memory_budget: int | None = None, max_inflight: int = 64, sf_batch: int | None = None,
-                 sf_gather_us: float = 300.0, builders: int = 4, ctx: N.Context | None = None) -> None:
+                 sf_gather_us: float = 2000.0, builders: int = 4, ctx: N.Context | None = None) -> None:
         self.ctx = ctx or N.Context.default()
         self.lib = self.ctx.lib
         self.builders = max(1, int(builders))
