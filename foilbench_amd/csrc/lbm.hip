@@ -1948,12 +1948,31 @@ __global__ void __launch_bounds__(256) lbm_force_kernel(int nx, int ny, int px, 
   }
 }
 
-__global__ void __launch_bounds__(64) lbm_sum_pairs_kernel(const double* __restrict__ part, int n,
-                                                           double* __restrict__ out) {
-  if (threadIdx.x >= 2) return;
-  double s = 0.0;
-  for (int k = 0; k < n; ++k) s += part[2 * k + threadIdx.x];
-  out[threadIdx.x] = s;
+// Fixed-order sum of n pairs of block partials into out[0..1]: thread t sums pairs t, t + 256, ...
+// in order, then a fixed tree combines the 256 thread sums.
+__global__ void __launch_bounds__(256) lbm_sum_pairs_kernel(const double* __restrict__ part, int n,
+                                                            double* __restrict__ out) {
+  __shared__ double lds[2][256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int k = t; k < n; k += 256) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  lds[0][t] = a;
+  lds[1][t] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      lds[0][t] += lds[0][t + s];
+      lds[1][t] += lds[1][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = lds[0][0];
+    out[1] = lds[1][0];
+  }
 }
 
 template <class P>
@@ -1964,7 +1983,7 @@ static int forces_impl(Lattice& L, double* force) {
   double* out = part + 2 * (size_t)kPartialBlocks;
   hipLaunchKernelGGL(lbm_force_kernel<P>, dim3(nb), dim3(256), 0, s, L.dom.nx, L.dom.ny, L.dom.periodic_x,
                      L.dom.periodic_y, (const typename P::S*)L.PRE[L.pre], (const uint8_t*)L.G[L.g].flags, part);
-  hipLaunchKernelGGL(lbm_sum_pairs_kernel, dim3(1), dim3(64), 0, s, part, (int)nb, out);
+  hipLaunchKernelGGL(lbm_sum_pairs_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, out);
   FB_HIP(hipGetLastError());
   double h[2];
   FB_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -518,13 +518,31 @@ __global__ void __launch_bounds__(256) mac_force_kernel(MacGeom g, double nu, do
   }
 }
 
-// Fixed-order sum of n pairs of block partials into out[0..1].
-__global__ void __launch_bounds__(64) sum_pairs_kernel(const double* __restrict__ part, int n,
-                                                       double* __restrict__ out) {
-  if (threadIdx.x >= 2) return;
-  double s = 0.0;
-  for (int k = 0; k < n; ++k) s += part[2 * k + threadIdx.x];
-  out[threadIdx.x] = s;
+// Fixed-order sum of n pairs of block partials into out[0..1]: thread t sums pairs t, t + 256, ...
+// in order, then a fixed tree combines the 256 thread sums.
+__global__ void __launch_bounds__(256) sum_pairs_kernel(const double* __restrict__ part, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double lds[2][256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int k = t; k < n; k += 256) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  lds[0][t] = a;
+  lds[1][t] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      lds[0][t] += lds[0][t + s];
+      lds[1][t] += lds[1][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = lds[0][0];
+    out[1] = lds[1][0];
+  }
 }
 
 // ---------------------------------------------------------------- host handle
@@ -1070,7 +1088,7 @@ int fb_mac_forces(fb_mac* mac, double viscosity, double wall_omega_rad, double* 
     hipLaunchKernelGGL(fb::mac_force_kernel<float>, dim3(nb), dim3(256), 0, s, M->g, viscosity, wall_omega_rad,
                        M->foil.pivot_x, M->foil.pivot_y, M->solid, M->pres, (const float*)M->U[M->cur],
                        (const float*)M->V[M->cur], part);
-  hipLaunchKernelGGL(fb::sum_pairs_kernel, dim3(1), dim3(64), 0, s, part, (int)nb, out);
+  hipLaunchKernelGGL(fb::sum_pairs_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, out);
   FB_HIP(hipGetLastError());
   FB_HIP(hipMemcpyAsync(M->host_keys, out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   FB_HIP(hipStreamSynchronize(s));
