@@ -173,8 +173,18 @@ constexpr int kScalStride = S_COUNT + 8;
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partial, int n, double* scal, int mode,
                                               double* lds) {
   if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  // eight independent strided sums per thread (loads in flight together), combined pairwise: an
+  // order fixed by n alone
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < n; i += 8 * 256) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += partial[i + u * 256];
+  }
+#pragma unroll
+  for (int u = 0; u < 7; ++u)
+    if (i + u * 256 < n) a[u] += partial[i + u * 256];
+  double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   s = block_sum<256>(s, lds);
   if (threadIdx.x != 0) return;
   if (mode == 0) {
