@@ -456,12 +456,14 @@ __device__ __forceinline__ void dft_r(cplx<T>* v) {
 __host__ __device__ constexpr int lds_slot(int i) { return i + (i >> 3); }
 __host__ __device__ constexpr int lds_row(int M) { return M + (M >> 3); }
 
-// One radix-R Stockham pass (sub-transforms of length L -> L*R) over a row of M points in
-// LDS, in place: each of the row's M/PPT threads reads its PPT points, the block synchronises,
-// then writes them to their autosorted places.
-template <class T, int R, int PPT>
-__device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int L, int t,
-                                              int nthr) {
+// One radix-R Stockham pass (sub-transforms of length L -> L*R) over a row of M points: each of
+// the row's M/PPT threads takes its PPT points through `ld` (the row in LDS, or the pass's input
+// straight from global memory), the block synchronises when the pass reads and writes the same
+// LDS row (SYNC_IN), and the results go through `st` to their autosorted places (the LDS row, or
+// the transform's consumer).
+template <class T, int R, int PPT, bool SYNC_IN, class Ld, class St>
+__device__ __forceinline__ void stockham_pass(Ld&& ld, St&& st, const cplx<T>* __restrict__ roots, int M, int L,
+                                              int t, int nthr) {
   constexpr int NB = PPT / R;
   const int MR = M / R;
   cplx<T> v[NB][R];
@@ -469,9 +471,9 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __res
   for (int u = 0; u < NB; ++u) {
     const int b = t + u * nthr;
 #pragma unroll
-    for (int s = 0; s < R; ++s) v[u][s] = buf[lds_slot(b + s * MR)];
+    for (int s = 0; s < R; ++s) v[u][s] = ld(b + s * MR);
   }
-  __syncthreads();
+  if constexpr (SYNC_IN) __syncthreads();
   const int stride = M / (L * R);
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -488,31 +490,44 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, const cplx<T>* __res
     dft_r<T, R>(v[u]);
     const int base = (b - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[lds_slot(base + q * L)] = v[u][q];
+    for (int q = 0; q < R; ++q) st(base + q * L, v[u][q]);
   }
   __syncthreads();
 }
 
-// Forward FFT of each row's M = 2^logM points (radix-8 passes, then one radix-2 or -4 pass), with
-// M / PPT threads per row, each taking PPT points of every pass.
-template <class T, int PPT>
+// Forward FFT of each row's M = 2^logM points (M >= 16: radix-8 passes, then one radix-2 or -4
+// pass), with M / PPT threads per row, each taking PPT points of every pass. The first pass reads
+// through `first_ld` (SYNC_FIRST when that reads the LDS row), the last pass writes through
+// `last_st`; the passes between keep the row in LDS.
+template <class T, int PPT, bool SYNC_FIRST, class Ld, class St>
 __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int logM, int t,
-                                         int nthr) {
+                                         int nthr, Ld&& first_ld, St&& last_st) {
+  auto lds_ld = [buf](int i) { return buf[lds_slot(i)]; };
+  auto lds_st = [buf](int i, cplx<T> v) { buf[lds_slot(i)] = v; };
+  const int n8 = logM / 3, rem = logM % 3;
+  const int last8 = rem ? -1 : n8 - 1;  // the last radix-8 pass is the transform's last pass
   int L = 1;
-  for (int p = 0; p < logM / 3; ++p, L *= 8) stockham_pass<T, 8, PPT>(buf, roots, M, L, t, nthr);
-  if (logM % 3 == 2) stockham_pass<T, 4, PPT>(buf, roots, M, L, t, nthr);
-  else if (logM % 3 == 1) stockham_pass<T, 2, PPT>(buf, roots, M, L, t, nthr);
+  for (int p = 0; p < n8; ++p, L *= 8) {
+    if (p == 0) stockham_pass<T, 8, PPT, SYNC_FIRST>(first_ld, lds_st, roots, M, L, t, nthr);
+    else if (p == last8) stockham_pass<T, 8, PPT, true>(lds_ld, last_st, roots, M, L, t, nthr);
+    else stockham_pass<T, 8, PPT, true>(lds_ld, lds_st, roots, M, L, t, nthr);
+  }
+  if (rem == 2) stockham_pass<T, 4, PPT, true>(lds_ld, last_st, roots, M, L, t, nthr);
+  else if (rem == 1) stockham_pass<T, 2, PPT, true>(lds_ld, last_st, roots, M, L, t, nthr);
 }
 
 // Orthonormal DST-I along x of every row, Bluestein chirp-z with Stockham FFTs in LDS. A block
 // holds `rows` rows of M points, M/PPT threads each (PPT 16 or 8: more threads per row for short
 // grids; the results are the same bits either way).
 // The transform runs in T; `in` (TI) and `out`, `rsrc` (TO) may be wider (an FP64 residual through
-// the FP32 transform).
+// the FP32 transform). The product with the transformed chirp kernel is applied as the first
+// FFT's last pass stores. With FUSE_IO the chirp product also feeds the first FFT pass straight
+// from global memory and the second FFT's last pass hands its points to the output; without it
+// (fewer registers) both go through the LDS row. The arithmetic is the same either way.
 // mode 0: input r masked to fluid cells, output to `out`.
 // mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
 // fixed by M alone: M/16 strided sums, then a pairwise tree.
-template <class T, class TI, class TO, int PPT>
+template <class T, class TI, class TO, int PPT, bool FUSE_IO>
 __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI* __restrict__ in,
                                                        const uint8_t* __restrict__ fl, const TO* __restrict__ rsrc,
                                                        TO* __restrict__ out, const cplx<T>* __restrict__ chirp,
@@ -539,63 +554,66 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
   const bool live = row < ny;
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw) + (size_t)lr * lds_row(M);
   double* red = reinterpret_cast<double*>(smem_raw + (size_t)rows * lds_row(M) * sizeof(cplx<T>)) + (size_t)lr * nred;
+  double* prod = reinterpret_cast<double*>(buf);  // the row's r.z products, once the transform is done
   const size_t base = (size_t)(live ? row : 0) * nx;
-  for (int j = t; j < M; j += nthr) {
-    cplx<T> a{(T)0, (T)0};
-    if (live && j < nx) {
-      T v = (T)in[base + j];
-      if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
-      const cplx<T> c = chirp[j];
-      a = {v * c.x, v * c.y};
-    }
-    buf[lds_slot(j)] = a;
-  }
-  __syncthreads();
-  fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
-  // convolution with the chirp: inverse FFT as conj(FFT(conj(.)))
-  for (int k = t; k < M; k += nthr) {
-    const cplx<T> v = cmul(buf[lds_slot(k)], fbt[k]);
+  // first FFT: the row times the chirp (zero past nx), then the convolution kernel's spectrum,
+  // conjugated for the inverse FFT taken as conj(FFT(conj(.)))
+  auto chirped = [&](int j) -> cplx<T> {
+    if (!(live && j < nx)) return {(T)0, (T)0};
+    T v = (T)in[base + j];
+    if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
+    const cplx<T> c = chirp[j];
+    return {v * c.x, v * c.y};
+  };
+  auto spectrum = [&](int k, cplx<T> w) {
+    const cplx<T> v = cmul(w, fbt[k]);
     buf[lds_slot(k)] = {v.x, -v.y};
+  };
+  auto lds_ld = [buf](int i) { return buf[lds_slot(i)]; };
+  if constexpr (FUSE_IO) {
+    fft_rows<T, PPT, false>(buf, roots, M, logM, t, nthr, chirped, spectrum);
+  } else {
+    for (int j = t; j < M; j += nthr) buf[lds_slot(j)] = chirped(j);
+    __syncthreads();
+    fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, spectrum);
   }
-  __syncthreads();
-  fft_rows<T, PPT>(buf, roots, M, logM, t, nthr);
+  // second FFT: its outputs k < nx times the chirp, scaled, are the transform
   const T invM = (T)1 / (T)M;
-  // this thread's outputs k = t + j nthr < nx (nx <= M / 2, so at most PPT / 2 of them)
-  constexpr int KMAX = PPT / 2;
-  T dv[KMAX];
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const int k = t + j * nthr;
-    dv[j] = (T)0;
-    if (live && k < nx) {
-      const cplx<T> w = buf[lds_slot(k)];
-      const cplx<T> conv{w.x * invM, -w.y * invM};
-      const cplx<T> y = cmul(chirp[k], conv);
-      dv[j] = -y.y * scale;
+  auto output = [&](int k, cplx<T> w) {
+    if (!(live && k < nx)) return;
+    const cplx<T> conv{w.x * invM, -w.y * invM};
+    const cplx<T> y = cmul(chirp[k], conv);
+    const T dv = -y.y * scale;
+    if (mode == 0) {
+      out[base + k] = (TO)dv;
+      return;
     }
-  }
-  if (mode == 0) {
+    const TO rv = rsrc[base + k];
+    const TO zv = (fl[base + k] & PF_SOLID) ? rv : (TO)dv;
+    out[base + k] = zv;
+    prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+  };
+  if constexpr (FUSE_IO) {
+    fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, output);
+  } else {
+    auto lds_st = [buf](int i, cplx<T> v) { buf[lds_slot(i)] = v; };
+    fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, lds_st);
+    // this thread's outputs k = t + j nthr < nx (nx <= M / 2, so at most PPT / 2 of them), read
+    // before any product overwrites the row
+    constexpr int KMAX = PPT / 2;
+    cplx<T> w[KMAX];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
       const int k = t + j * nthr;
-      if (live && k < nx) out[base + k] = (TO)dv[j];
+      w[j] = k < M ? buf[lds_slot(k)] : cplx<T>{(T)0, (T)0};
     }
-    return;
-  }
-  // the products r.z go to the row's (now free) LDS buffer, then are summed in the fixed order
-  __syncthreads();
-  double* prod = reinterpret_cast<double*>(buf);
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const int k = t + j * nthr;
-    if (live && k < nx) {
-      const TO rv = rsrc[base + k];
-      const TO zv = (fl[base + k] & PF_SOLID) ? rv : (TO)dv[j];
-      out[base + k] = zv;
-      prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
-    }
+    for (int j = 0; j < KMAX; ++j) output(t + j * nthr, w[j]);
+    __syncthreads();
   }
-  __syncthreads();
+  if (mode == 0) return;
+  // the products are summed in the fixed order (the last writes are behind a barrier)
   for (int v = t; v < nred; v += nthr) {
     double acc = 0.0;
     if (live)
@@ -1228,8 +1246,11 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
     rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * lds_row(P.M) * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
-  auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8> : dst_rows_kernel<T, TIO, T, 16>;
-  auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8> : dst_rows_kernel<T, T, TIO, 16>;
+  // the FP64 transform gains from the fused input and output passes; the FP32 one keeps its
+  // occupancy without them (measured)
+  constexpr bool fuse = sizeof(T) == 8;
+  auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8, fuse> : dst_rows_kernel<T, TIO, T, 16, fuse>;
+  auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8, fuse> : dst_rows_kernel<T, T, TIO, 16, fuse>;
   L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,
     (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 0, partial, scal, ps);
   FB_HIP(L.status());
