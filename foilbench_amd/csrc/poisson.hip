@@ -521,13 +521,13 @@ __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict
 // grids; the results are the same bits either way).
 // The transform runs in T; `in` (TI) and `out`, `rsrc` (TO) may be wider (an FP64 residual through
 // the FP32 transform). The product with the transformed chirp kernel is applied as the first
-// FFT's last pass stores. With FUSE_IO the chirp product also feeds the first FFT pass straight
-// from global memory and the second FFT's last pass hands its points to the output; without it
-// (fewer registers) both go through the LDS row. The arithmetic is the same either way.
+// FFT's last pass stores. FUSE bit 1: the chirp product feeds the first FFT pass straight from
+// global memory; bit 2: the second FFT's last pass hands its points to the output. Without them
+// (fewer registers) the input and output go through the LDS row. The arithmetic is the same.
 // mode 0: input r masked to fluid cells, output to `out`.
 // mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
 // fixed by M alone: M/16 strided sums, then a pairwise tree.
-template <class T, class TI, class TO, int PPT, bool FUSE_IO>
+template <class T, class TI, class TO, int PPT, int FUSE>
 __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI* __restrict__ in,
                                                        const uint8_t* __restrict__ fl, const TO* __restrict__ rsrc,
                                                        TO* __restrict__ out, const cplx<T>* __restrict__ chirp,
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
     buf[lds_slot(k)] = {v.x, -v.y};
   };
   auto lds_ld = [buf](int i) { return buf[lds_slot(i)]; };
-  if constexpr (FUSE_IO) {
+  if constexpr ((FUSE & 1) != 0) {
     fft_rows<T, PPT, false>(buf, roots, M, logM, t, nthr, chirped, spectrum);
   } else {
     for (int j = t; j < M; j += nthr) buf[lds_slot(j)] = chirped(j);
@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
     out[base + k] = zv;
     prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
   };
-  if constexpr (FUSE_IO) {
+  if constexpr ((FUSE & 2) != 0) {
     fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, output);
   } else {
     auto lds_st = [buf](int i, cplx<T> v) { buf[lds_slot(i)] = v; };
@@ -1246,9 +1246,9 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
     rows = std::max(1, std::min(256 / nthr, ny * nb_ / 512));
   const dim3 dg((unsigned)((ny + rows - 1) / rows), 1, nb_);
   const size_t lds = (size_t)rows * lds_row(P.M) * sizeof(cplx<T>) + (size_t)rows * (P.M / 16) * sizeof(double);
-  // the FP64 transform gains from the fused input and output passes; the FP32 one keeps its
-  // occupancy without them (measured)
-  constexpr bool fuse = sizeof(T) == 8;
+  // the FP64 transform gains from the fused input and output passes; the FP32 one from the fused
+  // output only (a fused input takes it from 59 to 94 registers, and was measured slower)
+  constexpr int fuse = sizeof(T) == 8 ? 3 : 2;
   auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8, fuse> : dst_rows_kernel<T, TIO, T, 16, fuse>;
   auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8, fuse> : dst_rows_kernel<T, T, TIO, 16, fuse>;
   L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,
