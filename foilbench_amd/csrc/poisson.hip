@@ -479,13 +479,16 @@ __device__ __forceinline__ void stockham_pass(Ld&& ld, St&& st, const cplx<T>* _
   for (int u = 0; u < NB; ++u) {
     const int b = t + u * nthr;
     const int k = b & (L - 1);
-    // one table load per butterfly; the higher twiddles are its powers
-    const cplx<T> w1 = roots[k * stride];
-    cplx<T> ws = w1;
+    // one table load per butterfly; the higher twiddles are its powers (table loads of every
+    // power were measured slower: the lanes' addresses scatter). The first pass (L = 1) has none.
+    if (L > 1) {
+      const cplx<T> w1 = roots[k * stride];
+      cplx<T> ws = w1;
 #pragma unroll
-    for (int s = 1; s < R; ++s) {
-      v[u][s] = cmul(v[u][s], ws);
-      if (s + 1 < R) ws = cmul(ws, w1);
+      for (int s = 1; s < R; ++s) {
+        v[u][s] = cmul(v[u][s], ws);
+        if (s + 1 < R) ws = cmul(ws, w1);
+      }
     }
     dft_r<T, R>(v[u]);
     const int base = (b - k) * R + k;

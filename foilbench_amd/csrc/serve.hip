@@ -193,6 +193,7 @@ static Batcher* batcher_for(Server* S, const SolveKey& key, int* rc) {
   *rc = make_ctx(S->device, &B->ctx);
   if (*rc == FB_OK)
     *rc = B->P.setup(B->ctx->stream, key.nx, key.ny, key.dx, key.dy, 0, 0, nullptr, B->capacity);
+  B->P.set_use_graphs(S->opt.use_graphs != 0);
   if (*rc) {
     free_ctx(B->ctx);
     delete B;
