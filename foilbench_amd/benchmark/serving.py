@@ -2,9 +2,10 @@
 
 Three runs over the same mix (PCG32 seed 42, stream 1001; step counts scaled so that the leg
 finishes in about a minute):
-* capacity: every request queued at t = 0, served with continuous batching (FP32 storage, FP32
-  CG sweeps in FP64 refinement, shifted FP32 populations, batched lattice launches and batched
-  same-shape pressure solves) -> aggregate cell-steps/s;
+* capacity: every request queued at t = 0, served with continuous batching (FP32 storage, FP64
+  CG with the FP32 DST-I preconditioner from the extrapolated previous pressure, shifted FP32
+  populations, batched lattice launches and batched same-shape pressure solves) -> aggregate
+  cell-steps/s;
 * plain: the drop-in FP64 form, one request after another -> aggregate cell-steps/s;
 * latency: Poisson arrivals at the mix's rate on the wall clock -> p50/p99 completion latency.
 Launch shapes and the lattice chunk are picked by the autotuner (`foilbench_amd.tuning`) for

@@ -9,8 +9,9 @@ polled as they finish, their fields read back if asked for, and their memory fre
 the next waiting requests.
 
 Forms:
-* ``serving``: Stable Fluids in FP32 storage with FP32 CG sweeps inside FP64 residual
-  refinement; lattices in FP32 storage shifted by the lattice weights, batched.
+* ``serving``: Stable Fluids in FP32 storage, pressure by FP64 CG with the FP32 DST-I
+  preconditioner started from the extrapolated previous pressure; lattices in FP32 storage
+  shifted by the lattice weights, batched.
 * ``plain``: the drop-in form, FP64, one scenario at a time on the default stream with no server
   (substep calls for Stable Fluids, one launch per lattice step).
 """
