@@ -643,6 +643,7 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
 //     neighbouring interface values.
 // Tables are [chunk type][quantity][row][mode] so that a wave's 64 modes read 64 neighbours.
 constexpr int kTriL = 8;  // rows per chunk
+constexpr int kTriFusedMaxP = 16;  // chunks per column up to which the three stages run as one launch
 
 template <class T>
 __global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __restrict__ w,
@@ -777,6 +778,145 @@ __global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const
       Fn = F;
     }
   }
+}
+
+// The three stages in one launch for short columns: 16 modes x 16 chunk segments per block, each
+// thread eliminating, then composing and replaying, then correcting its segment's chunks (the same
+// arithmetic per chunk and mode as the three kernels, so the same bits). Used when P <= 16, where
+// the launches cost more than the work.
+template <class T>
+__global__ void __launch_bounds__(256) tri_fused_kernel(int nx, int ny, T* __restrict__ w, const T* __restrict__ fac,
+                                                        int Llast, int P, T a, const T* __restrict__ itf,
+                                                        T* __restrict__ yF, T* __restrict__ yE, const double* scal) {
+  w += (size_t)blockIdx.z * nx * ny;
+  yF += (size_t)blockIdx.z * P * nx;
+  yE += (size_t)blockIdx.z * P * nx;
+  if (scal) scal += (size_t)blockIdx.z * kScalStride;
+  if (scal && CG_DONE(scal)) return;
+  __shared__ T sA[16][16], sB[16][16], sS[16][16];
+  const int m = threadIdx.x, gseg = threadIdx.y;
+  const int i = blockIdx.x * 16 + m;
+  const bool on = i < nx;
+  const int S = (P + 15) / 16;
+  const int k0 = min(P, gseg * S), k1 = min(P, k0 + S);
+  // 1. chunk elimination (tri_chunk_kernel's arithmetic)
+  if (on)
+    for (int c = k0; c < k1; ++c) {
+      const int last = c == P - 1;
+      const int Lc = last ? Llast : kTriL;
+      const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
+      const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
+      T* col = w + (size_t)c * kTriL * nx + i;
+      T v[kTriL];
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) v[j] = col[(size_t)j * nx];
+      T y = (T)0;
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) {
+          y = (v[j] - a * y) * invm[(size_t)j * nx];
+          v[j] = y;
+        }
+      T yn = y;
+#pragma unroll
+      for (int j = kTriL - 2; j >= 0; --j)
+        if (j < Lc - 1) {
+          yn = v[j] - cp[(size_t)j * nx] * yn;
+          v[j] = yn;
+        }
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) col[(size_t)j * nx] = v[j];
+      yF[(size_t)c * nx + i] = v[0];
+      yE[(size_t)c * nx + i] = y;
+    }
+  // 2. interface recurrences (tri_interface_kernel's arithmetic); this thread's own chunks
+  const size_t PQ = (size_t)P * nx;
+  const T* cg = itf;
+  const T* ch = itf + PQ;
+  const T* cA = itf + 2 * PQ;
+  const T* cmg = itf + 3 * PQ;
+  const T* csv = itf + 4 * PQ;
+  const T* cqq = itf + 5 * PQ;
+  T Ac = (T)1, Bc = (T)0;
+  if (on)
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T A = cA[o];
+      Ac = A * Ac;
+      Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
+    }
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = 0; q < 16; ++q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
+    }
+  }
+  __syncthreads();
+  if (on) {
+    T pp = sS[gseg][m];
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T f = yF[o];
+      const T r = cg[o] * f + ch[o] * pp;
+      pp = cA[o] * pp + (yE[o] + cmg[o] * f);
+      yF[o] = r;
+      yE[o] = pp;
+    }
+  }
+  Ac = (T)1;
+  Bc = (T)0;
+  if (on)
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T sv = csv[o];
+      Ac = sv * Ac;
+      Bc = sv * Bc + yF[o];
+    }
+  __syncthreads();
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = 15; q >= 0; --q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
+    }
+  }
+  __syncthreads();
+  if (on) {
+    T Fn = sS[gseg][m];
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T F = yF[o] + csv[o] * Fn;
+      const T E = yE[o] + cqq[o] * Fn;
+      yF[o] = F;
+      yE[o] = E;
+      Fn = F;
+    }
+  }
+  // the neighbouring segments' interface values, written by other threads of this block
+  __syncthreads();
+  // 3. spikes (tri_spike_kernel's arithmetic)
+  if (on)
+    for (int c = k0; c < k1; ++c) {
+      const int last = c == P - 1;
+      const int Lc = last ? Llast : kTriL;
+      const T* al = fac + (size_t)(last * 4 + 2) * kTriL * nx + i;
+      const T* be = fac + (size_t)(last * 4 + 3) * kTriL * nx + i;
+      const T El = c > 0 ? yE[(size_t)(c - 1) * nx + i] : (T)0;
+      const T Fr = c < P - 1 ? yF[(size_t)(c + 1) * nx + i] : (T)0;
+      T* col = w + (size_t)c * kTriL * nx + i;
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
+    }
 }
 
 template <class T>
@@ -1259,12 +1399,17 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
   FB_HIP(L.status());
   const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
   const T a = (T)(-(1.0 / (dy * dy)));
-  L(tri_chunk_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a, v.tri_yF, v.tri_yE,
-    scal);
-  L(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf,
-    v.tri_yF, v.tri_yE, scal);
-  L(tri_spike_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, (const T*)v.tri_yF,
-    (const T*)v.tri_yE, scal);
+  if (v.tri_P <= kTriFusedMaxP) {
+    L(tri_fused_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, ny, v.work, (const T*)v.tri,
+      v.tri_Llast, v.tri_P, a, (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
+  } else {
+    L(tri_chunk_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a, v.tri_yF, v.tri_yE,
+      scal);
+    L(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf,
+      v.tri_yF, v.tri_yE, scal);
+    L(tri_spike_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, (const T*)v.tri_yF,
+      (const T*)v.tri_yE, scal);
+  }
   FB_HIP(L.status());
   L(kern_out, dg, dim3(rows * nthr), lds, nx, ny, (const T*)v.work, flags, rin, zout, (const cplx<T>*)P.chirp,
     (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 1, partial, scal, ps);
