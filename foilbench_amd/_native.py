@@ -70,7 +70,7 @@ class fb_sf_params(C.Structure):
         ("max_iterations", C.c_int32), ("maccormack", C.c_int32),
         ("freestream_x", C.c_double), ("freestream_y", C.c_double), ("wall_omega_rad", C.c_double),
         ("cfl_limit", C.c_double), ("channel_walls", C.c_int32), ("mixed_poisson", C.c_int32),
-        ("warm_pressure", C.c_int32), ("pad_", C.c_int32),
+        ("warm_pressure", C.c_int32), ("face_transport", C.c_int32),
     ]
 
 
@@ -180,6 +180,7 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_lbm_time_fixed": (C.c_int, [_P, C.POINTER(fb_lbm_params), C.c_int, C.POINTER(C.c_double)]),
     "fb_lbm_forces": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "fb_mac_forces": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    "fb_mac_skew_rate": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "fb_mac_set_pressure": (C.c_int, [_P, _P]),
     "fb_lbm_run_batch": (C.c_int, [C.POINTER(_P), C.POINTER(fb_lbm_params), C.c_int, C.c_int, C.c_int,
                                    C.POINTER(fb_lbm_stats)]),

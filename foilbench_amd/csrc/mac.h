@@ -11,6 +11,7 @@ namespace fb {
 struct MacGeom {
   int nx, ny, px, py;
   double x0, y0, dx, dy;
+  double x1, y1;  // far bounds (the last face coordinates)
   // cell-centre linspace coefficients
   double sx, stx, ex, sy, sty, ey;
 };
@@ -25,6 +26,8 @@ inline MacGeom make_geom(const fb_domain& d) {
   g.y0 = d.y0;
   g.dx = (d.x1 - d.x0) / d.nx;
   g.dy = (d.y1 - d.y0) / d.ny;
+  g.x1 = d.x1;
+  g.y1 = d.y1;
   CellGrid c = make_cell_grid(d);
   g.sx = c.sx;
   g.stx = c.stx;
@@ -68,6 +71,7 @@ struct MacGrid {
   double* rhs = nullptr;
   double* pres = nullptr;
   double* pres_prev = nullptr;  // extrapolated warm starts: the pressure of the solve before the last
+  unsigned long long* skew_key = nullptr;  // skew-RK2 advection rate (order-preserving key)
   int pres_solves = 0;          // solves whose results sit in pres and pres_prev since the last reset
   unsigned long long* keys = nullptr;
   unsigned long long* host_keys = nullptr;

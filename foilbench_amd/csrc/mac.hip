@@ -337,6 +337,171 @@ __global__ void __launch_bounds__(256) bc_v_kernel(BcArgs b, const T* __restrict
 }
 
 // rhs = fluid ? -div / max(dt, 1e-12) : 0 (fb/core/grid.py:148-150), dtype arithmetic, then FP64.
+// ---------------------------------------------------------------- face-native transports
+
+// sample_staggered_scalar (fb/core/interpolation.py:9-52) on a W x H field (row pitch W) whose
+// point (0, 0) sits at cell-unit offset (ox, oy): FP64 weights, the result rounded to T. Periodic
+// axes wrap with the cell count (a duplicated endpoint is never read).
+template <class T>
+__device__ __forceinline__ T sample_field(const MacGeom& g, const T* __restrict__ f, int W, int H, double ox,
+                                          double oy, double px, double py) {
+  double gx = (px - g.x0) / g.dx - ox;
+  double gy = (py - g.y0) / g.dy - oy;
+  gx = g.px ? pymod(gx, (double)g.nx) : fmin(fmax(gx, 0.0), (double)W - 1.0);
+  gy = g.py ? pymod(gy, (double)g.ny) : fmin(fmax(gy, 0.0), (double)H - 1.0);
+  const double fx = floor(gx), fy = floor(gy);
+  int ix0 = (int)fx, iy0 = (int)fy;
+  const double tx = gx - fx, ty = gy - fy;
+  // np.mod can round a tiny negative argument up to the period itself: NumPy then reads index nx
+  // (ny), which on a face field is the duplicated endpoint, and wraps the second index to 1; only
+  // an index past the field's extent is clamped
+  ix0 = min(max(ix0, 0), W - 1);
+  iy0 = min(max(iy0, 0), H - 1);
+  const int ix1 = g.px ? (ix0 + 1) % g.nx : min(ix0 + 1, W - 1);
+  const int iy1 = g.py ? (iy0 + 1) % g.ny : min(iy0 + 1, H - 1);
+  const double a = (double)f[(size_t)iy0 * W + ix0], b = (double)f[(size_t)iy0 * W + ix1];
+  const double c = (double)f[(size_t)iy1 * W + ix0], d = (double)f[(size_t)iy1 * W + ix1];
+  return (T)((1.0 - tx) * (1.0 - ty) * a + tx * (1.0 - ty) * b + (1.0 - tx) * ty * c + tx * ty * d);
+}
+
+// Face coordinates (fb/core/grid.py:306-320): u faces at (linspace(x0, x1, nx+1), cell-centre y),
+// v faces at (cell-centre x, linspace(y0, y1, ny+1)).
+__device__ __forceinline__ double ufx(const MacGeom& g, int i) { return linspace_at(i, g.nx + 1, g.x0, g.dx, g.x1); }
+__device__ __forceinline__ double vfy(const MacGeom& g, int j) { return linspace_at(j, g.ny + 1, g.y0, g.dy, g.y1); }
+
+// Face advection (fb/core/grid.py:323-376), both components in one launch over a
+// (nx + 1) x (ny + 1) grid of threads. Without `orig`: each face takes fu/fv at its RK2 departure
+// point through the cell velocity (cu, cv). With `orig` (the MacCormack correction, dt negated by
+// the caller): the face's back-traced value b of fu/fv gives first + 0.5 (orig - b), clipped to
+// orig's edge-replicated 3 x 3 bounds.
+template <class T>
+__global__ void __launch_bounds__(256) face_advect_kernel(MacGeom g, const T* __restrict__ cu, const T* __restrict__ cv,
+                                                          const T* __restrict__ fu, const T* __restrict__ fv,
+                                                          const T* __restrict__ ou, const T* __restrict__ ov,
+                                                          double dt, T* __restrict__ out_u, T* __restrict__ out_v) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  auto bounded = [&](const T* orig, int W, int H, T c) {
+    T lo = orig[(size_t)j * W + i], hi = lo;
+    for (int oy = -1; oy <= 1; ++oy)
+      for (int ox = -1; ox <= 1; ++ox) {
+        const int jj = min(max(j + oy, 0), H - 1), ii = min(max(i + ox, 0), W - 1);
+        const T q = orig[(size_t)jj * W + ii];
+        lo = q < lo ? q : lo;
+        hi = q > hi ? q : hi;
+      }
+    c = c > lo ? c : lo;
+    return c < hi ? c : hi;
+  };
+  if (j < g.ny && i <= g.nx) {
+    const int W = g.nx + 1;
+    double qx, qy;
+    backtrace(g, cu, cv, ufx(g, i), cy_at(g, j), dt, qx, qy);
+    const T b = sample_field(g, fu, W, g.ny, 0.0, 0.5, qx, qy);
+    const size_t k = (size_t)j * W + i;
+    out_u[k] = ou ? bounded(ou, W, g.ny, fu[k] + (T)0.5 * (ou[k] - b)) : b;
+  }
+  if (j <= g.ny && i < g.nx) {
+    const int W = g.nx;
+    double qx, qy;
+    backtrace(g, cu, cv, cx_at(g, i), vfy(g, j), dt, qx, qy);
+    const T b = sample_field(g, fv, W, g.ny + 1, 0.5, 0.0, qx, qy);
+    const size_t k = (size_t)j * W + i;
+    out_v[k] = ov ? bounded(ov, W, g.ny + 1, fv[k] + (T)0.5 * (ov[k] - b)) : b;
+  }
+}
+
+// centered_derivative (fb/core/grid.py:379-405) at point k of a line of n points spaced h, the
+// values given by f(k): central differences with wrap (over n - 1 logical points when the last
+// duplicates the first, whose derivative it takes), else numpy.gradient with second-order
+// one-sided edges. Arithmetic in T with the spacing terms rounded to T, as NumPy does.
+template <class T, class F>
+__device__ __forceinline__ T cdiff(F&& f, int k, int n, double h, bool periodic, bool dup) {
+  if (periodic) {
+    const int L = dup ? n - 1 : n;
+    if (k >= L) k = 0;
+    const int kp = k + 1 == L ? 0 : k + 1, km = k == 0 ? L - 1 : k - 1;
+    return (f(kp) - f(km)) / (T)(2.0 * h);
+  }
+  if (k == 0) return (T)(-1.5 / h) * f(0) + (T)(2.0 / h) * f(1) + (T)(-0.5 / h) * f(2);
+  if (k == n - 1) return (T)(0.5 / h) * f(n - 3) + (T)(-2.0 / h) * f(n - 2) + (T)(1.5 / h) * f(n - 1);
+  return (f(k + 1) - f(k - 1)) / (T)(2.0 * h);
+}
+
+// _cross_velocity_on_faces (fb/core/grid.py:408-431): v averaged onto u face (j, i) through the
+// cells on either side, u onto v face (j, i).
+template <class T>
+__device__ __forceinline__ T v_on_u(const MacGeom& g, const T* __restrict__ v, int j, int i) {
+  auto cell_v = [&](int ii) { return (T)0.5 * (v[(size_t)j * g.nx + ii] + v[(size_t)(j + 1) * g.nx + ii]); };
+  if (i > 0 && i < g.nx) return (T)0.5 * (cell_v(i - 1) + cell_v(i));
+  if (g.px) return (T)0.5 * (cell_v(g.nx - 1) + cell_v(0));
+  return i == 0 ? cell_v(0) : cell_v(g.nx - 1);
+}
+template <class T>
+__device__ __forceinline__ T u_on_v(const MacGeom& g, const T* __restrict__ u, int j, int i) {
+  const size_t W = g.nx + 1;
+  auto cell_u = [&](int jj) { return (T)0.5 * (u[(size_t)jj * W + i] + u[(size_t)jj * W + i + 1]); };
+  if (j > 0 && j < g.ny) return (T)0.5 * (cell_u(j - 1) + cell_u(j));
+  if (g.py) return (T)0.5 * (cell_u(g.ny - 1) + cell_u(0));
+  return j == 0 ? cell_u(0) : cell_u(g.ny - 1);
+}
+
+// One skew-RK2 stage (fb/core/grid.py:446-503): out = base - c * S(su, sv), S the skew-symmetric
+// convection (half the advective plus half the conservative form) of the faces (su, sv), c the
+// stage's 0.5 dt or dt rounded to T. Boundaries are imposed by the caller.
+template <class T>
+__global__ void __launch_bounds__(256) skew_stage_kernel(MacGeom g, const T* __restrict__ su, const T* __restrict__ sv,
+                                                         const T* __restrict__ bu, const T* __restrict__ bv, double c,
+                                                         T* __restrict__ out_u, T* __restrict__ out_v) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  const bool px = g.px != 0, py = g.py != 0;
+  const T ct = (T)c;
+  if (j < g.ny && i <= g.nx) {
+    const size_t W = g.nx + 1;
+    auto U = [&](int jj, int ii) { return su[(size_t)jj * W + ii]; };
+    const T uc = U(j, i), vu = v_on_u(g, sv, j, i);
+    const T dudx = cdiff<T>([&](int k) { return U(j, k); }, i, g.nx + 1, g.dx, px, px);
+    const T dudy = cdiff<T>([&](int k) { return U(k, i); }, j, g.ny, g.dy, py, false);
+    const T adv = uc * dudx + vu * dudy;
+    const T con = cdiff<T>([&](int k) { const T q = U(j, k); return q * q; }, i, g.nx + 1, g.dx, px, px) +
+                  cdiff<T>([&](int k) { return v_on_u(g, sv, k, i) * U(k, i); }, j, g.ny, g.dy, py, false);
+    const size_t k = (size_t)j * W + i;
+    out_u[k] = bu[k] - ct * ((T)0.5 * (adv + con));
+  }
+  if (j <= g.ny && i < g.nx) {
+    const size_t W = g.nx;
+    auto V = [&](int jj, int ii) { return sv[(size_t)jj * W + ii]; };
+    const T vc = V(j, i), uv = u_on_v(g, su, j, i);
+    const T dvdx = cdiff<T>([&](int k) { return V(j, k); }, i, g.nx, g.dx, px, false);
+    const T dvdy = cdiff<T>([&](int k) { return V(k, i); }, j, g.ny + 1, g.dy, py, py);
+    const T adv = uv * dvdx + vc * dvdy;
+    const T con = cdiff<T>([&](int k) { return u_on_v(g, su, j, k) * V(j, k); }, i, g.nx, g.dx, px, false) +
+                  cdiff<T>([&](int k) { const T q = V(k, i); return q * q; }, j, g.ny + 1, g.dy, py, py);
+    const size_t k = (size_t)j * W + i;
+    out_v[k] = bv[k] - ct * ((T)0.5 * (adv + con));
+  }
+}
+
+// skew_face_advection_rate (fb/core/grid.py:434-443): per-wave maxima of the face rates, in T,
+// into one order-preserving key.
+template <class T>
+__global__ void __launch_bounds__(256) skew_rate_kernel(MacGeom g, const T* __restrict__ u, const T* __restrict__ v,
+                                                        unsigned long long* key) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  double r = 0.0;
+  const T rdx = (T)g.dx, rdy = (T)g.dy;
+  if (j < g.ny && i <= g.nx) {
+    const T a = (T)fabs((double)u[(size_t)j * (g.nx + 1) + i]), b = (T)fabs((double)v_on_u(g, v, j, i));
+    r = fmax(r, (double)(a / rdx + b / rdy));
+  }
+  if (j <= g.ny && i < g.nx) {
+    const T a = (T)fabs((double)u_on_v(g, u, j, i)), b = (T)fabs((double)v[(size_t)j * g.nx + i]);
+    r = fmax(r, (double)(a / rdx + b / rdy));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+  if ((threadIdx.x & 63) == 0) key_max(key, dkey(r));
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) rhs_kernel(MacGeom g, const T* __restrict__ u, const T* __restrict__ v,
                                                   const uint8_t* __restrict__ solid, double dt,
@@ -818,12 +983,81 @@ int project(MacGrid& M, const fb_sf_params& p, double dt, fb_sf_report* rep) {
   return warm_reset_on_failure(M, p, rep);
 }
 
+// Face-native transports with face diffusion (fb/solvers/stable_fluids.py:431-460): skew-RK2
+// (face_transport 2) or semi-Lagrangian / MacCormack face advection (1), then implicit
+// viscosity on each face component; the faces end in U[cur], V[cur].
+template <class T>
+static int face_transport_step(MacGrid& M, const fb_sf_params& p, fb_sf_report* rep) {
+  hipStream_t s = M.ctx->stream;
+  int rc;
+  const int nx = M.dom.nx, ny = M.dom.ny, o = 1 - M.cur;
+  T* u = (T*)M.U[M.cur];
+  T* v = (T*)M.V[M.cur];
+  T* ou = (T*)M.U[o];
+  T* ov = (T*)M.V[o];
+  T* tu = (T*)M.DU[0];
+  T* tv = (T*)M.DV[0];
+  const dim3 gf((nx + 1 + 255) / 256, ny + 1);
+  if (p.face_transport == 2) {
+    // boundaries without channel walls (grid.py:495, 501) and the mask of the step's start
+    BcArgs b{M.g, p.freestream_x, p.freestream_y, 0, p.wall_omega_rad, M.foil.pivot_x, M.foil.pivot_y, M.solid};
+    const dim3 gu((nx + 1 + 255) / 256, ny), gv((nx + 255) / 256, ny + 1);
+    hipLaunchKernelGGL(skew_stage_kernel<T>, gf, dim3(256), 0, s, M.g, (const T*)u, (const T*)v, (const T*)u,
+                       (const T*)v, 0.5 * p.dt, tu, tv);
+    hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)tu, ou);
+    hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)tv, ov);
+    hipLaunchKernelGGL(skew_stage_kernel<T>, gf, dim3(256), 0, s, M.g, (const T*)ou, (const T*)ov, (const T*)u,
+                       (const T*)v, p.dt, tu, tv);
+    hipLaunchKernelGGL(bc_u_kernel<T>, gu, dim3(256), 0, s, b, (const T*)tu, ou);
+    hipLaunchKernelGGL(bc_v_kernel<T>, gv, dim3(256), 0, s, b, (const T*)tv, ov);
+  } else {
+    T* cu = (T*)M.CU[0];
+    T* cv = (T*)M.CV[0];
+    if ((rc = launch_faces_to_cell<T>(M, u, v, cu, cv))) return rc;
+    hipLaunchKernelGGL(face_advect_kernel<T>, gf, dim3(256), 0, s, M.g, (const T*)cu, (const T*)cv, (const T*)u,
+                       (const T*)v, (const T*)nullptr, (const T*)nullptr, p.dt, p.maccormack ? tu : ou,
+                       p.maccormack ? tv : ov);
+    if (p.maccormack) {
+      T* fcu = (T*)M.CU[1];
+      T* fcv = (T*)M.CV[1];
+      if ((rc = launch_faces_to_cell<T>(M, tu, tv, fcu, fcv))) return rc;
+      hipLaunchKernelGGL(face_advect_kernel<T>, gf, dim3(256), 0, s, M.g, (const T*)fcu, (const T*)fcv,
+                         (const T*)tu, (const T*)tv, (const T*)u, (const T*)v, -p.dt, ou, ov);
+    }
+  }
+  FB_HIP(hipGetLastError());
+  // implicit viscosity on the faces (grid.py:278-303): Jacobi over the logical faces, the
+  // duplicated periodic endpoint copied from the first
+  T* res[2];
+  if ((rc = diffuse<T>(M, ou, ov, nx + 1 - M.g.px, ny, nx + 1, nx, ny + 1 - M.g.py, nx, p.viscosity, p.dt,
+                       p.tolerance, p.max_iterations, res, rep)))
+    return rc;
+  if (p.viscosity > 0.0) {
+    if ((rc = launch_dup_copy<T>(M, res[0], ny, nx + 1, nx + 1, M.g.px, 0))) return rc;
+    if ((rc = launch_dup_copy<T>(M, res[1], ny + 1, nx, nx, 0, M.g.py))) return rc;
+  }
+  FB_HIP(hipMemcpyAsync(u, res[0], M.NU * M.sz, hipMemcpyDeviceToDevice, s));
+  FB_HIP(hipMemcpyAsync(v, res[1], M.NV * M.sz, hipMemcpyDeviceToDevice, s));
+  return FB_OK;
+}
+
 template <class T>
 static int substep(MacGrid& M, const fb_sf_params& p, const fb_pose& pose, fb_sf_report* rep) {
   hipStream_t s = M.ctx->stream;
   M.last_stats_valid = false;
   int rc;
   const int nx = M.dom.nx, ny = M.dom.ny;
+  if (p.face_transport) {
+    if ((rc = face_transport_step<T>(M, p, rep))) return rc;
+    if (!rep->viscosity_converged) {
+      rep->status = FB_SF_VISCOSITY_NOT_CONVERGED;
+      return FB_OK;
+    }
+    if (!M.geom_valid || pose.angle_degrees != M.angle) {
+      if ((rc = mac_geometry(M, pose))) return rc;
+    }
+    return project<T>(M, p, p.dt, rep);
+  }
   dim3 gc((nx + 255) / 256, ny);
   T* cu = (T*)M.CU[0];
   T* cv = (T*)M.CV[0];
@@ -904,7 +1138,7 @@ void free_mac(MacGrid* M) {
   for (void* ptr : {M->U[0], M->U[1], M->V[0], M->V[1], M->CU[0], M->CU[1], M->CU[2], M->CU[3], M->CV[0], M->CV[1],
                     M->CV[2], M->CV[3], M->DU[0], M->DU[1], M->DU[2], M->DV[0], M->DV[1], M->DV[2], M->CKU, M->CKV,
                     (void*)M->sdf, (void*)M->solid, (void*)M->ck_solid, (void*)M->rhs, (void*)M->pres,
-                    (void*)M->pres_prev, (void*)M->keys, (void*)M->resid})
+                    (void*)M->pres_prev, (void*)M->keys, (void*)M->skew_key, (void*)M->resid})
     if (ptr) (void)dev_free(ptr);
   if (M->host_keys) (void)host_free(M->host_keys);
   if (M->host_resid) (void)host_free(M->host_resid);
@@ -990,6 +1224,7 @@ int create_mac(Ctx* c, const fb_domain* dom, const fb_foil* foil, int precision,
   // the pressure is the initial guess of a warm-started solve: zero before the first one
   if (e == hipSuccess) e = hipMemsetAsync(M->pres, 0, M->N * sizeof(double), c->stream);
   A((void**)&M->keys, 16 * sizeof(unsigned long long));
+  A((void**)&M->skew_key, sizeof(unsigned long long));
   if (e == hipSuccess) e = host_alloc((void**)&M->host_keys, 16 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     set_error("device allocation failed: %s", hipGetErrorString(e));
@@ -1071,6 +1306,27 @@ int fb_mac_set_pressure(fb_mac* mac, const double* p_host) {
   return FB_OK;
 }
 
+int fb_mac_skew_rate(fb_mac* mac, double* rate) {
+  FB_REQUIRE(mac && rate, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  hipStream_t s = M->ctx->stream;
+  const dim3 gf((M->dom.nx + 1 + 255) / 256, M->dom.ny + 1);
+  FB_HIP(hipMemsetAsync(M->skew_key, 0, sizeof(unsigned long long), s));
+  if (M->prec == FB_F64)
+    hipLaunchKernelGGL(fb::skew_rate_kernel<double>, gf, dim3(256), 0, s, M->g, (const double*)M->U[M->cur],
+                       (const double*)M->V[M->cur], M->skew_key);
+  else
+    hipLaunchKernelGGL(fb::skew_rate_kernel<float>, gf, dim3(256), 0, s, M->g, (const float*)M->U[M->cur],
+                       (const float*)M->V[M->cur], M->skew_key);
+  FB_HIP(hipGetLastError());
+  unsigned long long k = 0;
+  FB_HIP(hipMemcpyAsync(&k, M->skew_key, sizeof(k), hipMemcpyDeviceToHost, s));
+  FB_HIP(hipStreamSynchronize(s));
+  *rate = k ? fb::dkey_inv(k) : 0.0;
+  return FB_OK;
+}
+
 int fb_mac_forces(fb_mac* mac, double viscosity, double wall_omega_rad, double* force) {
   FB_REQUIRE(mac && force, FB_ERR_ARG, "NULL argument");
   auto* M = reinterpret_cast<MacGrid*>(mac);
@@ -1142,6 +1398,8 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
   FB_REQUIRE(params->warm_pressure >= 0 && params->warm_pressure <= 2 &&
                  !(params->warm_pressure && params->mixed_poisson == 1),
              FB_ERR_ARG, "warm_pressure must be 0, 1 or 2, and 0 with mixed_poisson 1");
+  FB_REQUIRE(params->face_transport >= 0 && params->face_transport <= 2, FB_ERR_ARG,
+             "face_transport must be 0 (cell), 1 (face advection) or 2 (skew-RK2)");
   FB_HIP(hipSetDevice(M->ctx->device));
   *report = fb_sf_report{};
   return MAC_DISPATCH(M, fb::substep, *M, *params, *pose, report);

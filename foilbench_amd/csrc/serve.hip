@@ -495,6 +495,8 @@ int fb_server_submit_sf(fb_server* srv, int64_t id, fb_mac* mac, const fb_sf_par
   FB_REQUIRE(params->warm_pressure >= 0 && params->warm_pressure <= 2 &&
                  !(params->warm_pressure && params->mixed_poisson == 1),
              FB_ERR_ARG, "warm_pressure must be 0, 1 or 2, and 0 with mixed_poisson 1");
+  FB_REQUIRE(params->face_transport >= 0 && params->face_transport <= 2, FB_ERR_ARG,
+             "face_transport must be 0 (cell), 1 (face advection) or 2 (skew-RK2)");
   auto* S = reinterpret_cast<Server*>(srv);
   auto* M = reinterpret_cast<fb::MacGrid*>(mac);
   FB_REQUIRE(M->ctx->device == S->device, FB_ERR_ARG, "handle lives on another device");

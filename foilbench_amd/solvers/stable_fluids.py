@@ -167,13 +167,12 @@ class StableFluidsSolver:
         return "maccormack" if self._maccormack else "semi-lagrangian"
 
     def set_transport_mode(self, mode: str) -> None:
+        # fb/solvers/stable_fluids.py:169-174
         if mode not in _MODES:
             raise ValueError(f"unsupported Stable Fluids advection: {mode}")
-        if mode == "skew-rk2":
-            raise NotImplementedError("skew-rk2 transport is not available on the GPU path yet")
         changed = mode != self.transport_mode
         self._maccormack = mode == "maccormack"
-        self._skew_rk2 = False
+        self._skew_rk2 = mode == "skew-rk2"
         if changed and self._scenario is not None:
             self._revision += 1
 
@@ -182,7 +181,7 @@ class StableFluidsSolver:
         return InteractiveTuning("stable-advection", "adv", m, m, m != "maccormack", m != "skew-rk2")
 
     def adjust_interactive_tuning(self, direction: int) -> InteractiveTuning:
-        self.set_transport_mode("maccormack" if direction < 0 else "semi-lagrangian")
+        self.set_transport_mode("maccormack" if direction < 0 else "skew-rk2")
         return self.interactive_tuning()
 
     def apply_interactive_tuning(self, value) -> InteractiveTuning:
@@ -219,9 +218,9 @@ class StableFluidsSolver:
         mode = scenario.solver_options.get("stable_advection", "maccormack")
         if mode not in _MODES:
             raise ValueError(f"unsupported Stable Fluids advection: {mode}")
-        if mode == "skew-rk2" or bool(scenario.solver_options.get("stable_face_advection", False)):
-            raise NotImplementedError("skew-rk2 and face-native advection are not available on the GPU path yet")
         self._maccormack = mode == "maccormack"
+        self._skew_rk2 = mode == "skew-rk2"
+        self._face_advection = bool(scenario.solver_options.get("stable_face_advection", False))
         self._create_handle()
         u, v = cell_to_faces(initial_velocity(scenario))
         self._set_faces(u, v)
@@ -302,10 +301,13 @@ class StableFluidsSolver:
         p.freestream_x = float(sc.freestream[0])
         p.freestream_y = float(sc.freestream[1])
         p.wall_omega_rad = float(np.deg2rad(control.angular_velocity_degrees))
+        if self._skew_rk2:
+            cfl = min(cfl, 0.4)  # fb/solvers/stable_fluids.py:265-266
         p.cfl_limit = max(_MIN_PROJECTION_CFL_LIMIT, 2.0 * cfl)
         p.channel_walls = 1 if str(sc.solver_options.get("initial_condition", "")) == "poiseuille" else 0
         p.mixed_poisson = _POISSON_MODE[self._form]
         p.warm_pressure = self._warm
+        p.face_transport = 2 if self._skew_rk2 else (1 if self._face_advection else 0)
         return p
 
     def _record(self, rep: N.fb_sf_report, projection: bool = True, viscosity: bool = False) -> None:
@@ -350,6 +352,12 @@ class StableFluidsSolver:
                                              C.byref(st)))
         return st
 
+    def _skew_rate(self) -> float:
+        """skew_face_advection_rate of the current faces (fb/core/grid.py:434-443), on the device."""
+        r = C.c_double()
+        N.check(self._lib().fb_mac_skew_rate(self._handle, C.byref(r)))
+        return float(r.value)
+
     def advance(self, control: ControlState, target_dt: float) -> StepReport:
         # fb/solvers/stable_fluids.py:332-603
         sc, geometry = self._require()
@@ -360,16 +368,19 @@ class StableFluidsSolver:
         if not isinstance(cfl_option, (int, float)):
             raise TypeError("stable_cfl must be numeric")
         cfl = float(cfl_option)
+        if self._skew_rk2:
+            cfl = min(cfl, 0.4)
         spacing = min(sc.domain.dx, sc.domain.dy)
         wall_speed = abs(np.deg2rad(control.angular_velocity_degrees)) * geometry.maximum_radius
         sweep_cells = abs(np.deg2rad(control.angle_degrees - self._control.angle_degrees)) * geometry.maximum_radius / spacing
-        advective_rate = max_speed / spacing
+        advective_rate = self._skew_rate() if self._skew_rk2 else max_speed / spacing
         fluid_measure = target_dt * advective_rate
         required = max(1.05 * fluid_measure / cfl, target_dt * wall_speed / (cfl * spacing), sweep_cells / cfl)
         substeps = max(1, int(np.ceil(required)))
         if substeps > 512:
             raise NumericalFailure("stability_limit", "Stable Fluids motion requires too many internal substeps",
-                                   "boundary", {"required_substeps": substeps, "maximum_substeps": 512,
+                                   "advection" if self._skew_rk2 else "boundary",
+                                   {"required_substeps": substeps, "maximum_substeps": 512,
                                                 "maximum_fluid_speed": max_speed, "maximum_wall_speed": wall_speed,
                                                 "boundary_sweep_cells": sweep_cells})
         dt = target_dt / substeps
@@ -414,7 +425,7 @@ class StableFluidsSolver:
                     raise NumericalFailure("nonfinite_state", "Stable Fluids produced non-finite velocity",
                                            "postcondition")
                 final_speed = st.max_cell_speed
-                advective_rate = final_speed / spacing
+                advective_rate = self._skew_rate() if self._skew_rk2 else final_speed / spacing
                 accepted = dt * advective_rate
             except Exception:
                 restore()

@@ -201,7 +201,10 @@ typedef struct fb_sf_params {
                               pressure p_n; 2: from 2 p_n - p_(n-1) once two solves are on record
                               (SciPy cg with x0: r0 = b - A x0, same stopping rule); a projection
                               that fails zeroes the pressure; not with mixed_poisson 1 */
-  int32_t pad_;
+  int32_t face_transport;  /* 0: cell-centred advection (maccormack selects the limiter);
+                              1: face-native advection, stable_face_advection (fb/core/grid.py:350-376);
+                              2: skew-symmetric RK2, stable_advection = "skew-rk2" (grid.py:481-503);
+                              1 and 2 diffuse on the faces (grid.py:278-303) */
 } fb_sf_params;
 
 typedef struct fb_sf_report {
@@ -244,6 +247,9 @@ int fb_sf_substep(fb_mac* mac, const fb_sf_params* params, const fb_pose* pose, 
 /* Projection only (fb/solvers/stable_fluids.py:256-330; initialize and import). */
 int fb_mac_project(fb_mac* mac, const fb_sf_params* params, fb_sf_report* report);
 int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats);
+/* Skew-RK2 advection rate of the current faces (fb/core/grid.py:434-443): the largest
+ * |u|/dx + |v on u|/dy over u faces and |u on v|/dx + |v|/dy over v faces, in the storage dtype. */
+int fb_mac_skew_rate(fb_mac* mac, double* rate);
 /* Pressure (ny, nx) FP64 of the last projection, replaced (import, tests). */
 int fb_mac_set_pressure(fb_mac* mac, const double* pressure_host);
 /* Force on the foil per unit depth at unit density, in FP64 (build-defined: the reference computes

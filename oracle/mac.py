@@ -393,6 +393,138 @@ def advect_velocity(vel, dt, dom, maccormack=True):
     return np.asarray(np.clip(corr, lo, hi), dtype=vel.dtype)
 
 
+# ----------------------------------------------------------------------------- face-native transports
+
+def implicit_diffuse_faces(u, v, nu, dt, dom, tol=1.0e-5, maxit=640):
+    """fb/core/grid.py:278-303: the scalar Jacobi solve on each face component."""
+    ou, iu, ru, cu = implicit_diffuse_scalar(u, nu, dt, dom, tol, maxit)
+    ov, iv, rv, cv = implicit_diffuse_scalar(v, nu, dt, dom, tol, maxit)
+    return ou, ov, (max(iu, iv), max(ru, rv), cu and cv)
+
+
+def face_positions(dom):
+    """fb/core/grid.py:306-320: u-face points (ny*(nx+1), 2) and v-face points ((ny+1)*nx, 2)."""
+    (x0, x1), (y0, y1) = dom.bounds
+    ux = np.linspace(x0, x1, dom.nx + 1)
+    uy = np.linspace(y0 + 0.5 * dom.dy, y1 - 0.5 * dom.dy, dom.ny)
+    vx = np.linspace(x0 + 0.5 * dom.dx, x1 - 0.5 * dom.dx, dom.nx)
+    vy = np.linspace(y0, y1, dom.ny + 1)
+    pu = np.stack(np.meshgrid(ux, uy), axis=2).reshape(-1, 2)
+    pv = np.stack(np.meshgrid(vx, vy), axis=2).reshape(-1, 2)
+    return pu, pv
+
+
+def face_local_bounds(f):
+    """fb/core/grid.py:335-347: min and max over each point's 3x3 neighbourhood, edges replicated."""
+    h, w = f.shape
+    padded = np.pad(f, 1, mode="edge")
+    around = [padded[1 + oy: 1 + oy + h, 1 + ox: 1 + ox + w] for oy in (-1, 0, 1) for ox in (-1, 0, 1)]
+    return np.minimum.reduce(around), np.maximum.reduce(around)
+
+
+def _face_departure_sample(f, vel, points, offset, dt, dom):
+    """fb/core/grid.py:323-332: the component sampled at its faces' RK2 departure points."""
+    return sample_staggered_scalar(f, rk2_backtrace(vel, points, dt, dom), dom, offset).reshape(f.shape)
+
+
+def advect_faces(u, v, dt, dom, maccormack=True):
+    """fb/core/grid.py:350-376: semi-Lagrangian advection of each face component through the
+    cell velocity; with MacCormack, the backward-forward correction clipped to the component's
+    own 3x3 bounds."""
+    vel = faces_to_cell(u, v)
+    pu, pv = face_positions(dom)
+    first_u = _face_departure_sample(u, vel, pu, (0.0, 0.5), dt, dom)
+    first_v = _face_departure_sample(v, vel, pv, (0.5, 0.0), dt, dom)
+    if not maccormack:
+        return first_u, first_v
+    fvel = faces_to_cell(first_u, first_v)
+    back_u = _face_departure_sample(first_u, fvel, pu, (0.0, 0.5), -dt, dom)
+    back_v = _face_departure_sample(first_v, fvel, pv, (0.5, 0.0), -dt, dom)
+    corr_u = first_u + 0.5 * (u - back_u)
+    corr_v = first_v + 0.5 * (v - back_v)
+    lo, hi = face_local_bounds(u)
+    corr_u = np.clip(corr_u, lo, hi)
+    lo, hi = face_local_bounds(v)
+    corr_v = np.clip(corr_v, lo, hi)
+    return np.asarray(corr_u, dtype=u.dtype), np.asarray(corr_v, dtype=v.dtype)
+
+
+def centered_derivative(f, h, axis, periodic, duplicate_endpoint=False):
+    """fb/core/grid.py:379-405: central differences with wrap (over the logical points when the
+    last one duplicates the first), else numpy.gradient with second-order one-sided edges."""
+    if periodic:
+        if duplicate_endpoint:
+            n = f.shape[axis] - 1
+            core = np.take(f, np.arange(n), axis=axis)
+            d = (np.roll(core, -1, axis=axis) - np.roll(core, 1, axis=axis)) / (2.0 * h)
+            return np.asarray(np.concatenate((d, np.take(d, [0], axis=axis)), axis=axis), dtype=f.dtype)
+        return np.asarray((np.roll(f, -1, axis=axis) - np.roll(f, 1, axis=axis)) / (2.0 * h), dtype=f.dtype)
+    return np.asarray(np.gradient(f, h, axis=axis, edge_order=2), dtype=f.dtype)
+
+
+def cross_velocity_on_faces(u, v, dom):
+    """fb/core/grid.py:408-431: v averaged onto u faces and u onto v faces through the cells."""
+    cu = 0.5 * (u[:, :-1] + u[:, 1:])
+    cv = 0.5 * (v[:-1, :] + v[1:, :])
+    v_on_u = np.empty_like(u)
+    v_on_u[:, 1:-1] = 0.5 * (cv[:, :-1] + cv[:, 1:])
+    if "x" in dom.periodic_axes:
+        edge = 0.5 * (cv[:, -1] + cv[:, 0])
+        v_on_u[:, 0] = edge
+        v_on_u[:, -1] = edge
+    else:
+        v_on_u[:, 0] = cv[:, 0]
+        v_on_u[:, -1] = cv[:, -1]
+    u_on_v = np.empty_like(v)
+    u_on_v[1:-1, :] = 0.5 * (cu[:-1, :] + cu[1:, :])
+    if "y" in dom.periodic_axes:
+        edge = 0.5 * (cu[-1, :] + cu[0, :])
+        u_on_v[0, :] = edge
+        u_on_v[-1, :] = edge
+    else:
+        u_on_v[0, :] = cu[0, :]
+        u_on_v[-1, :] = cu[-1, :]
+    return v_on_u, u_on_v
+
+
+def skew_face_advection_rate(u, v, dom):
+    """fb/core/grid.py:434-443: max over faces of |u_n|/dx + |u_t|/dy (u faces) and the v-face
+    counterpart."""
+    v_on_u, u_on_v = cross_velocity_on_faces(u, v, dom)
+    ru = np.abs(u) / dom.dx + np.abs(v_on_u) / dom.dy
+    rv = np.abs(u_on_v) / dom.dx + np.abs(v) / dom.dy
+    return max(float(np.max(ru)), float(np.max(rv)))
+
+
+def skew_symmetric_convection(u, v, dom):
+    """fb/core/grid.py:446-478: half the advective plus half the conservative convection."""
+    v_on_u, u_on_v = cross_velocity_on_faces(u, v, dom)
+    px = "x" in dom.periodic_axes
+    py = "y" in dom.periodic_axes
+    d = centered_derivative
+    adv_u = u * d(u, dom.dx, 1, px, px) + v_on_u * d(u, dom.dy, 0, py)
+    adv_v = u_on_v * d(v, dom.dx, 1, px) + v * d(v, dom.dy, 0, py, py)
+    con_u = d(u * u, dom.dx, 1, px, px) + d(v_on_u * u, dom.dy, 0, py)
+    con_v = d(u_on_v * v, dom.dx, 1, px) + d(v * v, dom.dy, 0, py, py)
+    return np.asarray(0.5 * (adv_u + con_u), dtype=u.dtype), np.asarray(0.5 * (adv_v + con_v), dtype=v.dtype)
+
+
+def advect_faces_skew_rk2(u, v, dt, dom, solid, wall, freestream):
+    """fb/core/grid.py:481-503: explicit midpoint step; boundaries (without channel walls) and
+    solid faces are imposed on the midpoint and on the result."""
+    ku, kv = skew_symmetric_convection(u, v, dom)
+    mid_u = np.asarray(u - 0.5 * dt * ku, dtype=u.dtype)
+    mid_v = np.asarray(v - 0.5 * dt * kv, dtype=v.dtype)
+    apply_domain_boundaries(mid_u, mid_v, dom, freestream)
+    enforce_solid_faces(mid_u, mid_v, solid, wall)
+    ku, kv = skew_symmetric_convection(mid_u, mid_v, dom)
+    out_u = np.asarray(u - dt * ku, dtype=u.dtype)
+    out_v = np.asarray(v - dt * kv, dtype=v.dtype)
+    apply_domain_boundaries(out_u, out_v, dom, freestream)
+    enforce_solid_faces(out_u, out_v, solid, wall)
+    return out_u, out_v
+
+
 # ----------------------------------------------------------------------------- solver
 
 def _opt(sc, name, default):
@@ -419,8 +551,9 @@ def initial_velocity(sc):
 
 
 class StableFluidsOracle:
-    """Restates ``StableFluidsSolver`` (fb/solvers/stable_fluids.py:119-603), default
-    MacCormack transport without face advection."""
+    """Restates ``StableFluidsSolver`` (fb/solvers/stable_fluids.py:119-603): cell-centred
+    MacCormack or semi-Lagrangian transport, face-native transport (``stable_face_advection``)
+    and skew-symmetric RK2 (``stable_advection="skew-rk2"``)."""
 
     def initialize(self, sc, seed=0, solid_fn=None):
         # fb/solvers/stable_fluids.py:195-216
@@ -432,7 +565,10 @@ class StableFluidsOracle:
         self.time = 0.0
         self.revision = 0
         self.reynolds = float(sc.reynolds)
-        self.maccormack = str(_opt(sc, "stable_advection", "maccormack")) == "maccormack"
+        mode = str(_opt(sc, "stable_advection", "maccormack"))
+        self.maccormack = mode == "maccormack"
+        self.skew = mode == "skew-rk2"
+        self.face_advection = bool(_opt(sc, "stable_face_advection", False))
         self.u, self.v = cell_to_faces(initial_velocity(sc))
         self.solid = self.solid_fn(self.control.angle_degrees)
         self.last_projection = (0, 0.0, True, 0)
@@ -452,6 +588,8 @@ class StableFluidsOracle:
         tol = float(_opt(sc, "pressure_tolerance", 1.0e-5))
         maxit = int(_opt(sc, "pressure_max_iterations", 640))
         cfl = float(_opt(sc, "stable_cfl", 0.7))
+        if self.skew:
+            cfl = min(cfl, 0.4)
         limit = max(1.0, 2.0 * cfl)
         wall = self.wall_grid(self.control.angular_velocity_degrees)
         if not (np.isfinite(self.u).all() and np.isfinite(self.v).all()):
@@ -473,16 +611,18 @@ class StableFluidsOracle:
         return faces_to_cell(self.u, self.v)
 
     def advance(self, control, target_dt):
-        # fb/solvers/stable_fluids.py:332-603 (MacCormack path)
+        # fb/solvers/stable_fluids.py:332-603
         sc = self.sc
         dom = self.dom
         cv = self.cell_velocity()
         max_speed = max(float(np.max(np.linalg.norm(cv, axis=2))), abs(sc.freestream[0]), 1.0e-6)
         cfl = float(_opt(sc, "stable_cfl", 0.7))
+        if self.skew:
+            cfl = min(cfl, 0.4)
         spacing = min(dom.dx, dom.dy)
         wall_speed = abs(np.deg2rad(control.angular_velocity_degrees)) * self.foil.maximum_radius
         sweep = abs(np.deg2rad(control.angle_degrees - self.control.angle_degrees)) * self.foil.maximum_radius / spacing
-        rate = max_speed / spacing
+        rate = skew_face_advection_rate(self.u, self.v, dom) if self.skew else max_speed / spacing
         measure = target_dt * rate
         required = max(1.05 * measure / cfl, target_dt * wall_speed / (cfl * spacing), sweep / cfl)
         subs = max(1, int(np.ceil(required)))
@@ -509,10 +649,18 @@ class StableFluidsOracle:
                     sub = type(control)(start_time + frac * target_dt,
                                         start_angle + frac * (control.angle_degrees - start_angle),
                                         control.angular_velocity_degrees)
-                    vel = faces_to_cell(self.u, self.v)
-                    vel = advect_velocity(vel, dt, dom, self.maccormack)
-                    vel, self.last_viscosity = implicit_diffuse(vel, nu, dt, dom, tol, maxit)
-                    self.u, self.v = cell_to_faces(vel)
+                    if self.skew:
+                        wall = self.wall_grid(sub.angular_velocity_degrees)
+                        au, av = advect_faces_skew_rk2(self.u, self.v, dt, dom, self.solid, wall, sc.freestream)
+                        self.u, self.v, self.last_viscosity = implicit_diffuse_faces(au, av, nu, dt, dom, tol, maxit)
+                    elif self.face_advection:
+                        au, av = advect_faces(self.u, self.v, dt, dom, self.maccormack)
+                        self.u, self.v, self.last_viscosity = implicit_diffuse_faces(au, av, nu, dt, dom, tol, maxit)
+                    else:
+                        vel = faces_to_cell(self.u, self.v)
+                        vel = advect_velocity(vel, dt, dom, self.maccormack)
+                        vel, self.last_viscosity = implicit_diffuse(vel, nu, dt, dom, tol, maxit)
+                        self.u, self.v = cell_to_faces(vel)
                     if not self.last_viscosity[2]:
                         raise OracleFailure("convergence_failure", "implicit viscosity did not converge", "viscosity")
                     self.control = sub
@@ -520,7 +668,7 @@ class StableFluidsOracle:
                     self.apply_projection(dt)
                 fv = self.cell_velocity()
                 final_speed = float(np.max(np.linalg.norm(fv, axis=2)))
-                accepted = dt * (final_speed / spacing)
+                accepted = dt * (skew_face_advection_rate(self.u, self.v, dom) if self.skew else final_speed / spacing)
             except Exception:
                 restore()
                 raise

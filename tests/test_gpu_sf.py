@@ -55,6 +55,29 @@ CASES = {
                        controls=((0.0, 0.0),)),
     "semi-lagrangian": dict(resolution=(48, 24), controls=((0.0, -4.0),), bounds=((-1.5, 2.5), (-1.0, 1.0)),
                             output_dt=0.02, stable_advection="semi-lagrangian"),
+    # face-native transports (fb/core/grid.py:350-503)
+    "face-maccormack": dict(resolution=(64, 32), controls=((0.0, -8.0),), bounds=((-1.5, 2.5), (-1.0, 1.0)),
+                            output_dt=1.0 / 64.0, reynolds=500.0, naca="0012", pressure_tolerance=1e-6,
+                            stable_cfl=1.0, stable_face_advection=True),
+    "face-semi-lagrangian": dict(resolution=(48, 24), controls=((0.0, 4.0), (0.3, 12.0)),
+                                 bounds=((-1.5, 2.5), (-1.0, 1.0)), output_dt=0.02, stable_face_advection=True,
+                                 stable_advection="semi-lagrangian"),
+    "face-periodic": dict(resolution=(32, 32), initial_condition="taylor-green", foil_in_domain=False,
+                          periodic_axes=("x", "y"), bounds=((0.0, 6.283185307179586), (0.0, 6.283185307179586)),
+                          output_dt=0.05, controls=((0.0, 0.0),), pressure_tolerance=1e-8,
+                          stable_face_advection=True),
+    "skew-rk2": dict(resolution=(64, 32), controls=((0.0, -8.0),), bounds=((-1.5, 2.5), (-1.0, 1.0)),
+                     output_dt=1.0 / 64.0, reynolds=500.0, naca="0012", pressure_tolerance=1e-6, stable_cfl=1.0,
+                     stable_advection="skew-rk2"),
+    "skew-rk2-moving": dict(resolution=(48, 24), controls=((0.0, 4.0), (0.3, 16.0)), bounds=((-1.5, 2.5), (-1.0, 1.0)),
+                            output_dt=0.02, stable_advection="skew-rk2"),
+    "skew-rk2-periodic": dict(resolution=(32, 32), initial_condition="taylor-green", foil_in_domain=False,
+                              periodic_axes=("x", "y"), bounds=((0.0, 6.283185307179586), (0.0, 6.283185307179586)),
+                              output_dt=0.05, controls=((0.0, 0.0),), pressure_tolerance=1e-8,
+                              stable_advection="skew-rk2"),
+    "skew-rk2-channel": dict(resolution=(32, 16), initial_condition="poiseuille", foil_in_domain=False,
+                             periodic_axes=("x",), bounds=((0.0, 2.0), (-0.5, 0.5)), output_dt=0.02,
+                             controls=((0.0, 0.0),), stable_advection="skew-rk2"),
 }
 
 
@@ -90,6 +113,33 @@ def test_sf_advance_matches_oracle_fp64(case):
         np.testing.assert_allclose(rg.evidence["divergence_linf"], ro["divergence_linf"], rtol=1e-6, atol=1e-9)
     p = gpu.pressure()
     assert _rel(p, orc.pressure) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["face-maccormack", "skew-rk2"])
+def test_face_transports_match_oracle_fp32(case):
+    gpu, orc, reports = _pair(case, "float32", steps=3)
+    u, v = gpu.faces()
+    assert u.dtype == np.float32
+    assert _rel(u, orc.u) < 2e-5 and _rel(v, orc.v) < 2e-5, (_rel(u, orc.u), _rel(v, orc.v))
+    for rg, ro in reports:
+        assert rg.substeps == ro["substeps"]
+
+
+def test_skew_rate_matches_oracle():
+    """fb_mac_skew_rate against skew_face_advection_rate on the same faces (FP64 and FP32)."""
+    from oracle import mac as OM
+
+    for precision, tol in (("float64", 1e-14), ("float32", 1e-6)):
+        sc = make_scenario(precision=precision, **CASES["skew-rk2-moving"])
+        gpu = _solver()
+        gpu.initialize(sc, _foil(sc), 0)
+        t = 0.0
+        for _ in range(2):
+            t += sc.output_dt
+            gpu.advance(sc.control_at(t), sc.output_dt)
+        u, v = gpu.faces()
+        want = OM.skew_face_advection_rate(u, v, OM.Dom.of(sc.domain))
+        assert gpu._skew_rate() == pytest.approx(want, rel=tol)
 
 
 def test_sf_advance_matches_oracle_fp32():
