@@ -186,7 +186,8 @@ struct JacobiArgs {
   const unsigned long long* scale_keys;  // keys of max |original| per component
   double tol;
   int k;
-  const unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
+  unsigned long long* resid;  // [comp][maxit+1] keys of max |upd - res|
+  int direct;                 // blocks fold into resid themselves (else per-block partials and a fold launch)
   int stride;
   unsigned long long* part;  // per-block maxima of this sweep, slot = component
 };
@@ -206,7 +207,11 @@ __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
     if (key != 0ull && dkey_inv(key) / jacobi_scale(a.scale_keys[c]) <= a.tol) {
       // converged: carry the residual forward so every later sweep of this solve skips too
       // (a later sweep would otherwise read a ping-pong buffer this solve never wrote)
-      if (threadIdx.x == 0) a.part[(size_t)c * kPartialBlocks + blockIdx.x] = key;
+      if (a.direct) {
+        if (threadIdx.x == 0 && blockIdx.x == 0) key_max(&a.resid[c * a.stride + a.k], key);
+      } else if (threadIdx.x == 0) {
+        a.part[(size_t)c * kPartialBlocks + blockIdx.x] = key;
+      }
       return;
     }
   }
@@ -230,7 +235,14 @@ __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
     const unsigned long long k = mag_key((double)(upd - at(x, y)));
     m = k > m ? k : m;
   }
-  put_max(a.part, c, m, lds);
+  if (a.direct) {
+    // the sweep's max |upd - res| straight into its slot (a maximum does not depend on the order in
+    // which the blocks arrive); for small grids, where this saves the fold's launch
+    m = block_max_u64(m, lds);
+    if (threadIdx.x == 0) key_max(&a.resid[c * a.stride + a.k], m);
+  } else {
+    put_max(a.part, c, m, lds);
+  }
 }
 
 // Periodic duplicate endpoint copy after face diffusion (fb/core/grid.py:271-274).
@@ -828,6 +840,8 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
   a.part = M.ctx->partials;
   T* buf[2][2] = {{(T*)M.DU[1], (T*)M.DU[2]}, {(T*)M.DV[1], (T*)M.DV[2]}};
   dim3 grid(reduction_blocks(std::max((size_t)W0 * H0, (size_t)W1 * H1)), 1, 2);
+  // same-address maxima serialise: only small grids fold inside the sweep
+  a.direct = grid.x <= 256 ? 1 : 0;
   int done[2] = {0, 0};
   double fres[2] = {INFINITY, INFINITY};
   int k = 1;
@@ -843,10 +857,12 @@ int diffuse(MacGrid& M, const T* in0, const T* in1, int W0, int H0, int P0, int 
       a.k = kk;
       hipLaunchKernelGGL(jacobi_sweep_kernel<T>, grid, dim3(256), 0, s, a);
       FB_HIP(hipGetLastError());
-      FinishSpec f;
-      f.add(kk, false);
-      f.add(M.resid_cap + kk, false);
-      if ((rc = finish_partials(M.ctx, grid.x, f, M.resid))) return rc;
+      if (!a.direct) {
+        FinishSpec f;
+        f.add(kk, false);
+        f.add(M.resid_cap + kk, false);
+        if ((rc = finish_partials(M.ctx, grid.x, f, M.resid))) return rc;
+      }
     }
     FB_HIP(hipMemcpyAsync(M.host_resid, M.resid, 2 * M.resid_cap * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
