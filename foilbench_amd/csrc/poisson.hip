@@ -1123,6 +1123,26 @@ __global__ void __launch_bounds__(256) zero_if_done_kernel(size_t N, double* __r
   if (i < N) x[(size_t)blockIdx.z * N + i] = 0.0;
 }
 
+// Batched-solve staging in one launch per direction: grid z copies its N values between its own
+// buffer (a pointer table in dpar: the right-hand sides first, then the solutions) and its slice
+// of the stacked arrays. gather: rhs -> bflat and, with `with_x`, x -> xflat; else xflat -> x.
+__global__ void __launch_bounds__(256) batch_stage_kernel(size_t N, int G, const double* __restrict__ tbl,
+                                                          double* __restrict__ bflat, double* __restrict__ xflat,
+                                                          int gather, int with_x) {
+  const int g = blockIdx.z;
+  const double* rhs = reinterpret_cast<const double* const*>(tbl)[g];
+  double* x = reinterpret_cast<double* const*>(tbl)[G + g];
+  const size_t o = (size_t)g * N;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t)gridDim.x * 256) {
+    if (gather) {
+      bflat[o + i] = rhs[i];
+      if (with_x) xflat[o + i] = x[i];
+    } else {
+      x[i] = xflat[o + i];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) fill_zero(size_t N, double* __restrict__ a) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < N) a[i] = 0.0;
@@ -1578,15 +1598,24 @@ int Poisson::solve_batch(int nb, const double* const* rhs, double* const* x, dou
     return FB_OK;
   }
   nb_ = nb;
+  // one staging launch each way for the whole batch (its pointer table staged like the launch
+  // parameters): the grids' right-hand sides (and warm-start solutions) in, the solutions out
+  double tbl[2 * kMaxBatch];
   for (int g = 0; g < nb; ++g) {
-    FB_HIP(hipMemcpyAsync(bstage + (size_t)g * N, rhs[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
-    if (mode & kWarmStart)
-      FB_HIP(hipMemcpyAsync(xstage + (size_t)g * N, x[g], N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    memcpy(&tbl[g], &rhs[g], sizeof(double));
+    memcpy(&tbl[nb + g], &x[g], sizeof(double));
   }
-  int rc = run(bstage, xstage, tolerance, maxit, mode, reps);
+  const dim3 sg((unsigned)std::min<size_t>((N + 255) / 256, 2048), 1, (unsigned)nb);
+  int rc = put_par(tbl, 2 * nb);
   if (rc) return rc;
-  for (int g = 0; g < nb; ++g)
-    FB_HIP(hipMemcpyAsync(x[g], xstage + (size_t)g * N, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  hipLaunchKernelGGL(batch_stage_kernel, sg, dim3(256), 0, stream, N, nb, (const double*)dpar, bstage, xstage, 1,
+                     (mode & kWarmStart) ? 1 : 0);
+  FB_HIP(hipGetLastError());
+  rc = run(bstage, xstage, tolerance, maxit, mode, reps);
+  if (rc) return rc;
+  if ((rc = put_par(tbl, 2 * nb))) return rc;
+  hipLaunchKernelGGL(batch_stage_kernel, sg, dim3(256), 0, stream, N, nb, (const double*)dpar, bstage, xstage, 0, 0);
+  FB_HIP(hipGetLastError());
   FB_HIP(hipStreamSynchronize(stream));
   for (int g = 0; g < nb; ++g) finish(reps + g, g);
   return FB_OK;
