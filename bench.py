@@ -33,6 +33,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+ROOFLINE_LAUNCHES = 4000  # launches in the event-timed sample of the step kernel
 
 
 def _gpu_forms(steps: int, warmup: int):
@@ -63,9 +64,9 @@ def _gpu_forms(steps: int, warmup: int):
         lat.run(steps)
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        # average launch duration: one HIP event pair around back-to-back launches on the kernel's stream
-        ev_steps = max(1, min(steps, 5000))
-        avg_ms = lat.time(ev_steps) / ev_steps
+        # average launch duration: one HIP event pair around ROOFLINE_LAUNCHES back-to-back launches on the
+        # kernel's stream, whatever --steps is
+        avg_ms = lat.time(ROOFLINE_LAUNCHES) / ROOFLINE_LAUNCHES
         bytes_launch = BYTES_PER_CELL[prec] * lat.cells
         per_launch = 2 if tuned["fuse"] else 1  # lattice steps per launch of the step kernel
         out[name] = {

@@ -5,7 +5,7 @@ from __future__ import annotations
 
 _SOLVERS = ("stable-fluids", "lbm-d2q9", "pic-flip")
 _FORMS = {
-    "stable-fluids": ("plain", "mixed"),
+    "stable-fluids": ("plain", "mixed", "precond32"),
     "lbm-d2q9": ("plain", "f64", "f32", "f32-mixed", "f16-mixed"),
     "pic-flip": ("plain", "mixed"),
 }

@@ -245,6 +245,16 @@ class LBMOracle:
             dom.nx, dom.ny, "x" in dom.periodic_axes, "y" in dom.periodic_axes, self.channel
         )
 
+    def restart(self, scenario, time: float, angle_degrees: float, reynolds: float) -> None:
+        # fb/solvers/lbm.py:296-309: initialise, then take the restart's Reynolds number, time and pose
+        self.initialize(scenario)
+        self.reynolds = float(reynolds)
+        self.configure_relaxation()
+        self.time = float(time)
+        self.control = type(self.control)(float(time), float(angle_degrees), 0.0)
+        self.update_solid(float(angle_degrees))
+        self.revision = 0
+
     def configure_relaxation(self) -> None:
         # fb/solvers/lbm.py:311-338
         sc = self.sc

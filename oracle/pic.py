@@ -338,8 +338,48 @@ class PicFlipOracle:
             self.velocities[esc] = np.asarray(self.sc.freestream[:2], dtype=self.sc.dtype)
         self.resolve_collisions(control)
 
+    def _checkpoint(self):
+        # fb/solvers/pic_flip.py:737-755
+        return (self.positions.copy(), self.velocities.copy(), self.grid.copy(), self.solid.copy(), self.solid_angle,
+                self.control, self.time, (self.rng.state, self.rng.inc), self.advance_count, self.settling,
+                getattr(self, "unsupported", 0.0))
+
+    def _restore(self, c):
+        (self.positions, self.velocities, self.grid, self.solid, self.solid_angle, self.control, self.time, words,
+         self.advance_count, self.settling, self.unsupported) = c
+        self.rng.state, self.rng.inc = words
+
     def advance(self, control, target_dt):
-        # fb/solvers/pic_flip.py:679-1008 (no CFL retry wrapper: callers pick safe steps)
+        # fb/solvers/pic_flip.py:637-677: retry with more substeps when the post-step particle CFL exceeds pic_cfl
+        minimum = 1
+        retries = 0
+        while True:
+            try:
+                return self.advance_once(control, target_dt, minimum, retries)
+            except OracleFailure as err:
+                acc = err.evidence.get("accepted_cfl")
+                mx = err.evidence.get("maximum_cfl")
+                att = err.evidence.get("substeps")
+                if not (err.reason == "stability_limit" and isinstance(acc, float) and isinstance(mx, float)
+                        and isinstance(att, int) and acc > mx > 0.0):
+                    raise
+                nxt = max(att + 1, int(np.ceil(1.05 * att * acc / mx)))
+                if nxt > 512:
+                    raise OracleFailure("stability_limit", "retry requires too many substeps", "particle-advection",
+                                        {**err.evidence, "required_substeps": nxt, "stability_retries": retries})
+                retries += 1
+                minimum = nxt
+
+    def advance_once(self, control, target_dt, minimum_substeps=1, retries=0):
+        # fb/solvers/pic_flip.py:679-1008
+        saved = self._checkpoint()
+        try:
+            return self._advance_body(control, target_dt, minimum_substeps, retries)
+        except Exception:
+            self._restore(saved)
+            raise
+
+    def _advance_body(self, control, target_dt, minimum_substeps, retries):
         sc = self.sc
         dom = self.dom
         max_speed = max(float(np.max(np.linalg.norm(self.grid, axis=2))),
@@ -349,7 +389,10 @@ class PicFlipOracle:
         wall_speed = self.foil.maximum_radius * max(abs(np.deg2rad(bav)), abs(np.deg2rad(sweep_av)))
         transport = max(max_speed, wall_speed)
         stable_dt = self.cfl * min(dom.dx, dom.dy) / max(transport, 1.0e-6)
-        subs = max(1, int(np.ceil(target_dt / stable_dt)))
+        subs = max(minimum_substeps, int(np.ceil(target_dt / stable_dt)))
+        if subs > 512:
+            raise OracleFailure("stability_limit", "PIC/FLIP motion requires too many internal substeps",
+                                "particle-advection", {"required_substeps": subs})
         dt = target_dt / subs
         start_time = self.time
         start_angle = self.control.angle_degrees
@@ -392,11 +435,28 @@ class PicFlipOracle:
                 self.velocities[moved] = self._f2p(fu, fv, self.positions[moved])
                 self.resolve_collisions(control)
         self.resolve_collisions(control)
+        if not (np.isfinite(self.grid).all() and np.isfinite(self.positions).all()
+                and np.isfinite(self.velocities).all()):
+            raise OracleFailure("nonfinite_state", "PIC/FLIP produced non-finite state", "postcondition")
+        inside = self.foil.contains(self.positions, control.angle_degrees)
+        if np.any(inside):
+            raise OracleFailure("postcondition_failure", "particles inside the moving solid", "postcondition",
+                                {"unresolved_solid_particles": int(np.count_nonzero(inside))})
+        # fb/solvers/pic_flip.py:872-893: the accepted particle CFL of the step
+        report_speed = max(transport, float(np.max(np.linalg.norm(self.grid, axis=2))),
+                           float(np.max(np.linalg.norm(self.velocities, axis=1))))
+        accepted = dt * report_speed / min(dom.dx, dom.dy)
+        if accepted > self.cfl * (1.0 + 1.0e-6):
+            raise OracleFailure("stability_limit", "post-step motion exceeded its swept envelope", "particle-advection",
+                                {"accepted_cfl": accepted, "maximum_cfl": self.cfl, "substeps": subs})
         self.time = start_time + target_dt
         self.control = type(control)(self.time, control.angle_degrees, control.angular_velocity_degrees)
         wallf = self._wall(control.angular_velocity_degrees)
         return {"substeps": subs, "pressure_iterations": self.last_projection[0],
+                "pressure_relative_residual": self.last_projection[1],
                 "viscosity_iterations": self.last_viscosity[0],
+                "viscosity_final_residual": self.last_viscosity[1],
                 "divergence_linf": native_divergence_linf(qu, qv, dom, self.solid),
                 "solid_leakage": solid_face_leakage(qu, qv, self.solid, wallf),
-                "unsupported_face_fraction": self.unsupported}
+                "unsupported_face_fraction": self.unsupported, "stability_retries": retries,
+                "maximum_particle_cfl": accepted, "particle_count": int(self.positions.shape[0])}

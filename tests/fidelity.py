@@ -1,0 +1,403 @@
+"""The reference's fidelity and validity fixtures as runnable checks, for the oracle and the GPU solvers.
+
+Fixtures (copied data, tests/golden/): ``fidelity-cases.json`` (spec/conformance/fidelity-cases.json) and
+``solver-validity.json`` (spec/conformance/solver-validity.json), with the scenario documents they name under
+``tests/golden/scenarios/``. The checks restate the reference's tests:
+
+- tests/fidelity/test_canonical_cases.py:52-188 (uniform drift, Taylor-Green decay exp(-2 nu t), Poiseuille
+  profile, NACA 0012 symmetry at zero incidence, finite declared metrics of the dynamic NACA 2412 case);
+- tests/conformance/test_solvers.py:347-468 (accepted evidence keys, limits, revisions, invalid completion time,
+  extreme motion);
+- tests/fidelity/test_chaotic_experiment.py:27-143 (the planning retry cases).
+
+A ``Run`` drives one solver, oracle or GPU, through the same calls and exposes what the checks read: the
+exported cell velocity (solid cells zero), the exported density (LBM), the kinetic energy and solid leakage of
+the solver's diagnostics, and the evidence of the last advance.
+"""
+
+from __future__ import annotations
+
+import json
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+
+from foilbench_amd.core.geometry import NacaFoil
+from foilbench_amd.core.metrics import kinetic_energy, vorticity
+from foilbench_amd.core.models import ControlState
+from foilbench_amd.core.scenario import load_scenario
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIDELITY = json.loads((GOLDEN / "fidelity-cases.json").read_text(encoding="utf-8"))
+VALIDITY = json.loads((GOLDEN / "solver-validity.json").read_text(encoding="utf-8"))
+SOLVER_IDS = ("stable-fluids", "lbm-d2q9", "pic-flip")
+
+# scenario paths named by the fixtures -> the copies under tests/golden/scenarios
+_SCENARIO_FILES = {
+    "scenarios/validation/uniform.json": "uniform.json",
+    "scenarios/validation/taylor-green.json": "taylor-green.json",
+    "scenarios/validation/poiseuille.json": "poiseuille.json",
+    "scenarios/validation/naca0012-zero.json": "naca0012-zero.json",
+    "scenarios/airfoil/default.json": "naca2412-dynamic.json",
+    "scenarios/airfoil/chaotic-experimental.json": "chaotic-experimental.json",
+}
+
+
+def scenario_file(name: str):
+    return load_scenario(GOLDEN / "scenarios" / _SCENARIO_FILES[name])
+
+
+def case(case_id: str) -> dict:
+    for c in FIDELITY["cases"]:
+        if c["id"] == case_id:
+            return c
+    raise KeyError(case_id)
+
+
+def fidelity_scenario(case_id: str):
+    """tests/fidelity/test_canonical_cases.py:36-44: the case's scenario at its resolution and duration."""
+    c = case(case_id)
+    sc = scenario_file(c["scenario"])
+    return replace(sc, domain=replace(sc.domain, resolution=tuple(c["resolution"])), duration=float(c["duration"]))
+
+
+def threshold(case_id: str, metric: str) -> float:
+    return float(case(case_id)["metrics"][metric]["threshold"])
+
+
+def steps_of(sc) -> int:
+    return round(sc.duration / sc.output_dt)
+
+
+# ----------------------------------------------------------------------------- runs
+
+class OracleRun:
+    """The oracle (oracle/mac.py, oracle/lbm.py, oracle/pic.py) behind the calls the checks make."""
+
+    def __init__(self, solver_id: str):
+        self.solver_id = solver_id
+
+    def initialize(self, sc, seed=0):
+        from oracle.lbm import LBMOracle
+        from oracle.mac import StableFluidsOracle
+        from oracle.pic import PicFlipOracle
+
+        self.sc = sc
+        self.o = {"stable-fluids": StableFluidsOracle, "lbm-d2q9": LBMOracle, "pic-flip": PicFlipOracle}[
+            self.solver_id]()
+        self.o.initialize(sc, seed)
+        self.evidence = {}
+
+    def advance(self, control, dt):
+        self.evidence = self.o.advance(control, dt)
+        return self.evidence
+
+    @property
+    def solid(self):
+        return np.asarray(self.o.solid, dtype=bool)
+
+    def velocity(self):
+        if self.solver_id == "lbm-d2q9":
+            vel, _ = self.o.export_velocity_density()
+            return vel
+        vel = (self.o.cell_velocity() if self.solver_id == "stable-fluids" else self.o.grid).copy()
+        vel[self.solid] = 0.0
+        return vel
+
+    def density(self):
+        if self.solver_id != "lbm-d2q9":
+            return None
+        _, rho = self.o.export_velocity_density()
+        return rho
+
+    def kinetic_energy(self):
+        # the diagnostics' kinetic energy: cell velocity (SF), physical lattice velocity (LBM), grid (PIC)
+        if self.solver_id == "stable-fluids":
+            return kinetic_energy(self.o.cell_velocity())
+        if self.solver_id == "lbm-d2q9":
+            return kinetic_energy(self.o.physical_velocity())
+        return kinetic_energy(self.o.grid)
+
+    def solid_leakage(self):
+        if self.solver_id == "lbm-d2q9":
+            return 0.0
+        if self.solver_id == "stable-fluids":
+            from oracle.mac import solid_face_leakage
+
+            return float(solid_face_leakage(self.o.u, self.o.v, self.o.solid,
+                                            self.o.wall_grid(self.o.control.angular_velocity_degrees)))
+        return float(self.evidence.get("solid_leakage", 0.0))
+
+
+class GpuRun:
+    """A GPU solver class of the drop-in (``create_solver``) behind the same calls."""
+
+    def __init__(self, solver_id: str, form: str = "plain"):
+        self.solver_id = solver_id
+        self.form = form
+
+    def initialize(self, sc, seed=0):
+        from foilbench_amd.solvers.factory import create_solver
+
+        self.sc = sc
+        self.s = create_solver(self.solver_id, self.form)
+        self.s.initialize(sc, NacaFoil(sc.foil), seed)
+        self.evidence = {}
+
+    def advance(self, control, dt):
+        self.report = self.s.advance(control, dt)
+        self.evidence = self.report.evidence
+        return self.evidence
+
+    def velocity(self):
+        return self.s.export_state().velocity[0]
+
+    def density(self):
+        d = self.s.export_state().density
+        return None if d is None else d[0]
+
+    def kinetic_energy(self):
+        return float(self.s.diagnostics().values["kinetic_energy"])
+
+    def solid_leakage(self):
+        return float(self.s.diagnostics().values["solid_leakage"])
+
+
+def run_case(run, sc, seed=0):
+    run.initialize(sc, seed)
+    for step in range(steps_of(sc)):
+        run.advance(sc.control_at((step + 1) * sc.output_dt), sc.output_dt)
+    return run
+
+
+# ----------------------------------------------------------------------------- fidelity checks
+
+def check_uniform(run):
+    """tests/fidelity/test_canonical_cases.py:52-80."""
+    sc = fidelity_scenario("uniform")
+    run.initialize(sc, 0)
+    before_v, before_d = run.velocity().copy(), run.density()
+    before_d = None if before_d is None else before_d.copy()
+    for step in range(steps_of(sc)):
+        run.advance(sc.control_at((step + 1) * sc.output_dt), sc.output_dt)
+    after_v, after_d = run.velocity(), run.density()
+    drift = float(np.sqrt(np.mean((after_v.astype(np.float64) - before_v) ** 2)))
+    spurious = float(np.sqrt(np.mean(vorticity(after_v, sc.domain) ** 2)))
+    ddrift = 0.0 if after_d is None else float(np.max(np.abs(after_d.astype(np.float64) - before_d)))
+    values = {"velocity_rms_drift": drift, "spurious_vorticity_rms": spurious, "density_linf_drift": ddrift}
+    for metric, value in values.items():
+        assert value < threshold("uniform", metric), (run.solver_id, metric, value)
+    return values
+
+
+def check_taylor_green(run):
+    """tests/fidelity/test_canonical_cases.py:83-109: velocity error against the decaying exact solution."""
+    from foilbench_amd.core.geometry import cell_centers
+
+    sc = fidelity_scenario("taylor-green")
+    run.initialize(sc, 0)
+    before = run.kinetic_energy()
+    for step in range(steps_of(sc)):
+        run.advance(sc.control_at((step + 1) * sc.output_dt), sc.output_dt)
+    pos = cell_centers(sc.domain)
+    initial = np.stack((np.sin(pos[:, :, 0]) * np.cos(pos[:, :, 1]),
+                        -np.cos(pos[:, :, 0]) * np.sin(pos[:, :, 1])), axis=2)
+    nu = sc.reference_speed * sc.foil.chord / sc.reynolds
+    expected = initial * np.exp(-2.0 * nu * sc.duration)
+    err = float(np.sqrt(np.mean((run.velocity() - expected) ** 2)))
+    after = run.kinetic_energy()
+    assert err < threshold("taylor-green", "velocity_l2_error"), (run.solver_id, err)
+    assert 0.0 <= after <= before * threshold("taylor-green", "kinetic_energy_ratio"), (run.solver_id, before, after)
+    return {"velocity_l2_error": err, "kinetic_energy_ratio": after / before}
+
+
+def check_poiseuille(run):
+    """tests/fidelity/test_canonical_cases.py:112-146."""
+    from foilbench_amd.core.geometry import cell_centers
+
+    sc = fidelity_scenario("poiseuille")
+    run_case(run, sc)
+    vel = run.velocity()
+    pos = cell_centers(sc.domain)
+    y0, y1 = sc.domain.bounds[1]
+    radius, cy = 0.5 * (y1 - y0), 0.5 * (y0 + y1)
+    expected = 1.5 * (1.0 - ((pos[:, :, 1] - cy) / radius) ** 2)
+    err = float(np.sqrt(np.mean((vel[:, :, 0] - expected) ** 2)))
+    center = float(np.mean(vel[vel.shape[0] // 2, :, 0]))
+    walls = 0.5 * (float(np.mean(vel[0, :, 0])) + float(np.mean(vel[-1, :, 0])))
+    leak = max(float(np.max(np.abs(vel[0, :, 1]))), float(np.max(np.abs(vel[-1, :, 1]))))
+    assert center > walls, (run.solver_id, center, walls)
+    assert err < threshold("poiseuille", "profile_l2_error"), (run.solver_id, err)
+    assert leak < threshold("poiseuille", "normal_wall_leakage"), (run.solver_id, leak)
+    return {"profile_l2_error": err, "normal_wall_leakage": leak}
+
+
+def check_naca0012_zero(run):
+    """tests/fidelity/test_canonical_cases.py:149-170: mirror symmetry about the chord line, no leakage."""
+    sc = fidelity_scenario("naca0012-zero")
+    run_case(run, sc, sc.seed)
+    vel = run.velocity()
+    se = np.mean((vel[:, :, 0] - vel[::-1, :, 0]) ** 2)
+    te = np.mean((vel[:, :, 1] + vel[::-1, :, 1]) ** 2)
+    sym = float(np.sqrt(se + te))
+    leak = run.solid_leakage()
+    assert sym < threshold("naca0012-zero", "symmetry_l2_error"), (run.solver_id, sym)
+    assert leak < threshold("naca0012-zero", "solid_leakage"), (run.solver_id, leak)
+    return {"symmetry_l2_error": sym, "solid_leakage": leak}
+
+
+# ----------------------------------------------------------------------------- validity checks
+
+def validity_scenario():
+    sc = scenario_file(VALIDITY["scenario"])
+    return replace(sc, domain=replace(sc.domain, resolution=tuple(VALIDITY["resolution"])))
+
+
+def check_validity_evidence(solver_id: str, evidence: dict):
+    """tests/conformance/test_solvers.py:371-434: the accepted evidence keys are present and finite,
+    convergence flags true, and each solver's limits hold."""
+    import math
+
+    lim = VALIDITY["limits"]
+    for key in VALIDITY["accepted_evidence"][solver_id]:
+        assert key in evidence, (solver_id, key)
+        value = evidence[key]
+        if isinstance(value, bool):
+            if key.endswith("_converged"):
+                assert value, (solver_id, key)
+        else:
+            assert math.isfinite(float(value)), (solver_id, key, value)
+    if solver_id == "stable-fluids":
+        assert float(evidence["maximum_characteristic_displacement"]) <= lim["stable_maximum_characteristic_displacement"]
+        assert float(evidence["maximum_boundary_sweep"]) <= lim["stable_maximum_boundary_sweep"]
+    elif solver_id == "lbm-d2q9":
+        assert float(evidence["maximum_lattice_mach"]) <= lim["lbm_maximum_mach"] * (1.0 + 1.0e-6)
+        assert float(evidence["density_excursion"]) <= lim["lbm_maximum_density_excursion"]
+        assert float(evidence["minimum_population"]) >= lim["lbm_minimum_population"]
+        assert abs(float(evidence["trt_magic"]) - lim["lbm_trt_magic"]) <= 1e-6 * lim["lbm_trt_magic"]
+    else:
+        assert float(evidence["maximum_particle_cfl"]) <= lim["pic_maximum_particle_cfl"] * (1.0 + 1.0e-6)
+        assert float(evidence["empty_cell_fraction"]) <= lim["pic_maximum_empty_cell_fraction"]
+        assert float(evidence["underfilled_cell_fraction"]) <= lim["pic_maximum_underfilled_cell_fraction"]
+        assert float(evidence["unsupported_face_fraction"]) <= lim["pic_maximum_unsupported_face_fraction"]
+        assert int(evidence["unresolved_solid_particles"]) <= lim["pic_maximum_unresolved_solid_particles"]
+    if solver_id in ("stable-fluids", "pic-flip"):
+        assert float(evidence["pressure_relative_residual"]) <= lim["pressure_maximum_relative_residual"]
+        assert float(evidence["viscosity_final_residual"]) <= lim["viscosity_maximum_final_residual"]
+        assert float(evidence["divergence_linf"]) <= lim["mac_maximum_divergence_linf"]
+        assert float(evidence["solid_leakage"]) <= lim["mac_maximum_solid_leakage"]
+
+
+def retry_case(solver_id: str) -> dict:
+    return VALIDITY["planning_retry_cases"][solver_id]
+
+
+def retry_scenario(solver_id: str):
+    """tests/fidelity/test_chaotic_experiment.py:14-24 (Stable Fluids and PIC/FLIP)."""
+    rc = retry_case(solver_id)
+    sc = scenario_file(rc["scenario"])
+    opts = dict(sc.solver_options)
+    opts.update(rc.get("solver_options", {}))
+    return replace(sc, domain=replace(sc.domain, resolution=tuple(rc["resolution"])), output_dt=float(rc["target_dt"]),
+                   solver_options=opts)
+
+
+def retry_controls(solver_id: str, sc):
+    """The controls of the planning retry case's advances (test_chaotic_experiment.py:44-52, 85-93, 134-139)."""
+    rc = retry_case(solver_id)
+    out = []
+    for step in range(int(rc["expected_steps"])):
+        if solver_id == "lbm-d2q9":
+            out.append((ControlState((step + 1) * sc.output_dt, float(rc["angle_degrees"]), 0.0), sc.output_dt))
+        else:
+            dt = float(rc["target_dt"])
+            out.append((ControlState((step + 1) * dt, float(rc["angle_degrees"]),
+                                     float(rc["angular_velocity_degrees"])), dt))
+    return out
+
+
+# ----------------------------------------------------------------------------- closed forms
+
+def shear_wave_scenario(solver_id: str, nx=64, ny=16, duration=0.25):
+    """A doubly periodic box [0, 2] x [0, 0.5] in a uniform stream u = 1 with no foil, Re 100."""
+    from foilbench_amd.core.models import ControlKeyframe, DomainSpec, FoilSpec, Scenario
+
+    return Scenario(schema_version=1, id="shear-wave", domain=DomainSpec(2, ((0.0, 2.0), (0.0, 0.5)), (nx, ny),
+                                                                          ("x", "y")),
+                    reynolds=100.0, freestream=(1.0, 0.0), foil=FoilSpec("0012", 0.2, (-10.0, -10.0)),
+                    controls=(ControlKeyframe(0.0, 0.0),), duration=duration, output_dt=0.025, precision="float64",
+                    seed=0, solver_options={"pressure_tolerance": 1e-8})
+
+
+def shear_wave_field(sc, eps=0.01):
+    """u = 1, v = eps sin(2 pi x / L): divergence free, and the exact solution of the Navier-Stokes equations
+    is the same wave carried downstream at the stream speed (u . grad v = dv/dx, u . grad u = 0) and decaying
+    as exp(-nu k^2 t)."""
+    from foilbench_amd.core.geometry import cell_centers
+
+    pos = cell_centers(sc.domain)
+    length = sc.domain.bounds[0][1] - sc.domain.bounds[0][0]
+    vel = np.zeros((sc.domain.ny, sc.domain.nx, 2))
+    vel[..., 0] = 1.0
+    vel[..., 1] = eps * np.sin(2.0 * np.pi * pos[..., 0] / length)
+    return vel
+
+
+def wave_phase_shift(sc, vel):
+    """Downstream shift s of v(x) = A sin(k (x - s)) from its first Fourier coefficient, averaged over rows."""
+    from foilbench_amd.core.geometry import cell_centers
+
+    x = cell_centers(sc.domain)[0, :, 0]
+    length = sc.domain.bounds[0][1] - sc.domain.bounds[0][0]
+    k = 2.0 * np.pi / length
+    v = np.asarray(vel[..., 1], dtype=np.float64).mean(axis=0)
+    c = np.sum(v * np.exp(-1j * k * x))  # = A (n / 2) (-i) e^{-i k s}
+    phase = np.angle(c * 1j)
+    s = (-phase / k) % length
+    amp = 2.0 * abs(c) / x.size
+    return s, amp
+
+
+def set_oracle_velocity(run, vel):
+    """Put a cell velocity field into an oracle run (what import_state does for the GPU solvers)."""
+    from oracle.lbm import equilibrium
+    from oracle.mac import cell_to_faces
+
+    o = run.o
+    if run.solver_id == "stable-fluids":
+        o.u, o.v = cell_to_faces(vel.astype(o.u.dtype))
+    elif run.solver_id == "lbm-d2q9":
+        lat = vel * (o.lattice_speed / o.ref_speed)
+        o.f = equilibrium(np.ones(vel.shape[:2]), lat).astype(o.f.dtype)
+        o.outlet = o.f[:, -1:, :].copy()
+    else:
+        o.grid = vel.astype(o.grid.dtype)
+        fu, fv = cell_to_faces(o.grid)
+        o.velocities = o._f2p(fu, fv, o.positions)
+
+
+def run_shear_wave(run, sc, importer=None):
+    """Advance the shear wave to sc.duration and return (measured shift, expected shift, amplitude ratio)."""
+    run.initialize(sc, 0)
+    vel = shear_wave_field(sc)
+    if importer is None:
+        set_oracle_velocity(run, vel)
+    else:
+        importer(run, vel)
+    s0, a0 = wave_phase_shift(sc, vel)
+    t = 0.0
+    for step in range(steps_of(sc)):
+        t = (step + 1) * sc.output_dt
+        run.advance(sc.control_at(t), sc.output_dt)
+    s1, a1 = wave_phase_shift(sc, run.velocity())
+    length = sc.domain.bounds[0][1] - sc.domain.bounds[0][0]
+    return (s1 - s0) % length, t * sc.freestream[0], a1 / a0
+
+
+def gpu_import(run, vel):
+    """Put a cell velocity field into a GPU solver through its canonical import (``import_state``)."""
+    st = run.s.export_state()
+    state = replace(st, velocity=np.asarray(vel, dtype=st.velocity.dtype)[None, ...])
+    outcome = run.s.import_state(state, ControlState(st.time, st.angle_degrees, st.angular_velocity_degrees))
+    assert outcome.accepted, outcome
