@@ -36,63 +36,117 @@ HBM_PEAK_GBS = 8000.0
 ROOFLINE_LAUNCHES = 4000  # launches in the event-timed sample of the step kernel
 
 
-def _gpu_forms(steps: int, warmup: int):
-    import numpy as np
+FORMS = (("f32", "FB_F32"), ("f64", "FB_F64"), ("f32-mixed", "FB_F32_MIXED"), ("f16-mixed", "FB_F16_MIXED"))
+ERROR_STEPS = 20000  # lattice steps of the per-form error against FP64: the LBM configuration's horizon
+SERVING_ERROR_STEPS = 5000  # the serving mix's longest request
+SERVING_SHAPES = ((128, 64), (256, 128), (512, 256), (1024, 512))
+LARGE_SHAPE = (4096, 2048)  # working set above the 256 MB Infinity Cache (604 MB in FP32)
+LARGE_LAUNCHES = 1000
 
+
+def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
+    """Throughput over `steps` back-to-back lattice steps and the step kernel's event-timed average over
+    `launches` launches, with the autotuner's launch shape for this lattice and storage form."""
     from foilbench_amd import _native as N
-    from foilbench_amd.benchmark.lbm_harness import BYTES_PER_CELL, Lattice, LatticeCase
+    from foilbench_amd.benchmark.lbm_harness import BYTES_PER_CELL, Lattice
 
-    from foilbench_amd.tuning import Autotuner
-
-    case = LatticeCase()
-    ctx = N.Context.default()
-    tuner = Autotuner(ctx)
-    out = {}
-    velocities = {}
-    solid = None
-    for name, prec, pname in (("f32", N.FB_F32, "FB_F32"), ("f64", N.FB_F64, "FB_F64"),
-                              ("f32-mixed", N.FB_F32_MIXED, "FB_F32_MIXED"),
-                              ("f16-mixed", N.FB_F16_MIXED, "FB_F16_MIXED")):
-        # fused pairs, interior rows per thread and the special-cell blocks are the autotuner's
-        # picks for this lattice and storage form (none of them changes a result bit)
-        rec = tuner.lbm(case.nx, case.ny, name)
-        tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks")}
-        lat = Lattice(case, prec, ctx)
+    prec = getattr(N, pname)
+    # fused pairs, interior rows per thread and the special-cell blocks are the autotuner's picks for this
+    # lattice and storage form (none of them changes a result bit)
+    rec = tuner.lbm(case.nx, case.ny, name)
+    tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks")}
+    lat = Lattice(case, prec, ctx)
+    try:
         lat.run(warmup)
         ctx.synchronize()
         t0 = time.perf_counter()
         lat.run(steps)
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        # average launch duration: one HIP event pair around ROOFLINE_LAUNCHES back-to-back launches on the
-        # kernel's stream, whatever --steps is
-        avg_ms = lat.time(ROOFLINE_LAUNCHES) / ROOFLINE_LAUNCHES
-        bytes_launch = BYTES_PER_CELL[prec] * lat.cells
-        per_launch = 2 if tuned["fuse"] else 1  # lattice steps per launch of the step kernel
-        out[name] = {
-            "cell_steps_per_s": lat.cells * steps / wall,
-            "ms_per_step": 1e3 * wall / steps,
-            "kernel_ms": avg_ms,
-            "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
-            "bytes_per_cell": BYTES_PER_CELL[prec],
-            "tuned_step_shape": tuned,
-            "kernel": (f"lbm_pair_kernel<Prec<{pname}>, 16>" if tuned["fuse"]
-                       else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}>"),
-            "steps_per_launch": per_launch,
-            "launch_ms": avg_ms * per_launch,
-        }
-        velocities[name] = lat.velocity().astype(np.float64)
-        if solid is None:
-            solid = lat.solid()
+        # average launch duration: one HIP event pair around `launches` back-to-back launches on the kernel's
+        # stream, whatever --steps is
+        avg_ms = lat.time(launches) / launches
+    finally:
         lat.close()
-        print(f"[bench] {name}: {out[name]['cell_steps_per_s']:.4g} cell-steps/s, "
-              f"kernel {avg_ms * 1e3:.2f} us ({out[name]['kernel_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+    bytes_launch = BYTES_PER_CELL[prec] * lat.cells
+    per_launch = 2 if tuned["fuse"] else 1  # lattice steps per launch of the step kernel
+    return {
+        "cell_steps_per_s": lat.cells * steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "kernel_ms": avg_ms,
+        "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
+        "bytes_per_cell": BYTES_PER_CELL[prec],
+        "tuned_step_shape": tuned,
+        "kernel": (f"lbm_pair_kernel<Prec<{pname}>, 16>" if tuned["fuse"]
+                   else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}>"),
+        "steps_per_launch": per_launch,
+        "launch_ms": avg_ms * per_launch,
+        "event_launches": launches,
+    }
+
+
+def _gpu_forms(steps: int, warmup: int):
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import LatticeCase
+    from foilbench_amd.tuning import Autotuner
+
+    case = LatticeCase()
+    ctx = N.Context.default()
+    tuner = Autotuner(ctx)
+    out = {}
+    for name, pname in FORMS:
+        out[name] = _time_form(tuner, case, name, pname, ctx, steps, warmup, ROOFLINE_LAUNCHES)
+        print(f"[bench] {name}: {out[name]['cell_steps_per_s']:.4g} cell-steps/s, kernel "
+              f"{out[name]['kernel_ms'] * 1e3:.2f} us ({out[name]['kernel_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+    return case, out, tuner
+
+
+def _form_errors(nx: int, ny: int, steps: int, forms=("f32", "f32-mixed", "f16-mixed")) -> dict:
+    """Relative L2 velocity error over the fluid cells of each storage form against FP64 after exactly
+    `steps` lattice steps from the same inflow equilibrium (NACA 2412 at AoA 12, u = 0.05, Re 1000)."""
+    import numpy as np
+
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import Lattice, LatticeCase
+
+    ctx = N.Context.default()
+    case = LatticeCase(nx=nx, ny=ny)
+    vel = {}
+    solid = None
+    for name in ("f64",) + tuple(forms):
+        lat = Lattice(case, getattr(N, dict(FORMS)[name]), ctx)
+        try:
+            lat.run(steps)
+            ctx.synchronize()
+            vel[name] = lat.velocity().astype(np.float64)
+            if solid is None:
+                solid = lat.solid()
+        finally:
+            lat.close()
     fl = ~solid
-    ref = velocities["f64"][fl]
-    for name in out:
-        d = velocities[name][fl] - ref
-        out[name]["velocity_rel_err_vs_f64"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
-    return case, out
+    ref = vel["f64"][fl]
+    return {name: float(np.linalg.norm(vel[name][fl] - ref) / np.linalg.norm(ref)) for name in forms}
+
+
+def _large_lattice(tuner) -> dict:
+    """The step kernel on a 4096 x 2048 lattice (chord 640 cells), whose populations (604 MB in FP32) cannot
+    stay in the 256 MB Infinity Cache between steps: the HBM-resident figure."""
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import LatticeCase
+
+    ctx = N.Context.default()
+    case = LatticeCase(nx=LARGE_SHAPE[0], ny=LARGE_SHAPE[1])
+    out = {"workload": case.name, "working_set_mb": {}}
+    for name, pname in FORMS:
+        if name == "f64":
+            continue
+        r = _time_form(tuner, case, name, pname, ctx, 200, 20, LARGE_LAUNCHES)
+        r["frac"] = r["kernel_gbs"] / HBM_PEAK_GBS
+        out[name] = r
+        out["working_set_mb"][name] = r["bytes_per_cell"] * case.nx * case.ny / 1e6  # two population buffers
+        print(f"[bench] large {name}: kernel {r['kernel_ms'] * 1e3:.1f} us ({r['kernel_gbs']:.0f} GB/s)",
+              file=sys.stderr, flush=True)
+    return out
 
 
 def _cpu_baseline(sample_steps: int):
@@ -188,7 +242,16 @@ def main() -> int:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path does not shard "
               "across GPUs in this build)", file=sys.stderr)
         return 2
-    case, forms = _gpu_forms(args.steps, args.warmup)
+    case, forms, tuner = _gpu_forms(args.steps, args.warmup)
+    errors = {"lattice_1024x512_20000_steps": _form_errors(case.nx, case.ny, ERROR_STEPS)}
+    for nx, ny in SERVING_SHAPES:
+        errors[f"lattice_{nx}x{ny}_{SERVING_ERROR_STEPS}_steps"] = _form_errors(nx, ny, SERVING_ERROR_STEPS,
+                                                                                ("f32-mixed", "f16-mixed"))
+    for name in forms:
+        if name != "f64":
+            forms[name]["velocity_rel_err_vs_f64"] = errors["lattice_1024x512_20000_steps"][name]
+    print(f"[bench] errors vs FP64: {errors}", file=sys.stderr, flush=True)
+    large = _large_lattice(tuner)
     main_form = forms["f32"]
     achieved = main_form["kernel_gbs"]
     traffic = _traffic(main_form["kernel"])
@@ -227,6 +290,8 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "forms": forms,
+        "velocity_rel_err_vs_f64": errors,
+        "hbm_resident": large,
         "legs": legs,
     }
     sv = (legs or {}).get("serving") or {}

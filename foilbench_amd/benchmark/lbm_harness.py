@@ -88,7 +88,8 @@ class Lattice:
         p.wall_omega_rad = 0.0
         p.wall_scale = self.case.u_lattice
         p.velocity_scale = 1.0 / self.case.u_lattice
-        f = feq9(t[0], t[1], hd)
+        # shifted storage: the equilibrium in FP64 (the deviation f - w is what the lattice keeps)
+        f = feq9(t[0], t[1], np.float64 if self.precision in (N.FB_F32_MIXED, N.FB_F16_MIXED) else hd)
         for d in range(9):
             p.feq_boundary[d] = float(f[d])
             p.feq_uncovered[d] = float(f[d])

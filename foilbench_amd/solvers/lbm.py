@@ -270,8 +270,11 @@ class LBMSolver:
         p.wall_omega_rad = float(np.deg2rad(angular_velocity_degrees))
         p.wall_scale = self._lattice_speed / self._reference_speed
         p.velocity_scale = self._reference_speed / self._lattice_speed
-        fb_ = feq9(target[0], target[1], dt)
-        fu = feq9(uncovered[0], uncovered[1], dt)
+        # shifted forms store f - w: their equilibria are formed in FP64 (from the same target velocity), so
+        # the deviation from the weights does not carry the rounding of an FP32 value of size w
+        fdt = np.float64 if self.precision_code in (N.FB_F32_MIXED, N.FB_F16_MIXED) else dt
+        fb_ = feq9(target[0], target[1], fdt)
+        fu = feq9(uncovered[0], uncovered[1], fdt)
         for d in range(9):
             p.feq_boundary[d] = float(fb_[d])
             p.feq_uncovered[d] = float(fu[d])
