@@ -230,6 +230,9 @@ _SIGNATURES: dict[str, tuple] = {
     "fb_pic_particle_to_faces": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P, _P,
                                            C.POINTER(C.c_double)]),
+    "fb_pic_particle_to_faces_noalloc": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, C.c_int, C.c_int,
+                                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                                   _P, _P, _P]),
     "fb_pic_faces_to_particle": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P]),
 }

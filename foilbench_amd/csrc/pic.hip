@@ -1705,6 +1705,89 @@ static int standalone_f2p(Ctx* c, int64_t n, const void* u, const void* v, const
   return rc;
 }
 
+// ------------------------------------------------------------------ allocation-free P2G
+// fb_pic_particle_to_faces_noalloc keeps its counting-sort buffers (cell of each particle, per-cell counts,
+// starts and cursors, the scan's block sums, the sorted index and the sorted positions and values) in one
+// file-scope device array of fixed size, laid out for the largest particle and cell counts it accepts. Each
+// GPU holds its own copy until the process ends. Host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side stand-in).
+constexpr int64_t kNoAllocMaxParticles = 1 << 19;  // 524,288 particles
+constexpr int64_t kNoAllocMaxCells = 1 << 18;      // 262,144 cells (512 x 512)
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocScanBlocks =
+    (size_t)((kNoAllocMaxCells > kNoAllocMaxParticles ? kNoAllocMaxCells : kNoAllocMaxParticles) + 1023) / 1024 + 2;
+constexpr size_t kNoAllocBytes = 2 * al256(kNoAllocMaxParticles * 4)         // cellof, sidx
+                                 + 2 * al256((kNoAllocMaxCells + 1) * 4)      // counts, cstart
+                                 + al256(kNoAllocMaxCells * 4)                // cursor
+                                 + al256(kNoAllocScanBlocks * 4) + al256(16)  // bsum, total
+                                 + 2 * al256(kNoAllocMaxParticles * 2 * 8);   // spos, sval (FP64 worst case)
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current on the calling thread), looked up on the first call per
+// device and kept.
+static int scratch_address(int device, void** out) {
+  FB_REQUIRE(device >= 0 && device < kNoAllocMaxDevices, FB_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+      return FB_ERR_HIP;
+    }
+    g_scratch_dev[device] = p;
+  }
+  *out = g_scratch_dev[device];
+  return FB_OK;
+}
+
+// The sort buffers carved out of the scratch array at fixed offsets.
+static SortBufs scratch_sort(void* base, int ncell, int cap) {
+  unsigned char* p = static_cast<unsigned char*>(base);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    void* r = p + off;
+    off += al256(bytes);
+    return r;
+  };
+  SortBufs b;
+  b.ncell = ncell;
+  b.cap = cap;
+  b.cellof = (int*)take(kNoAllocMaxParticles * 4);
+  b.sidx = (int*)take(kNoAllocMaxParticles * 4);
+  b.counts = (int*)take((kNoAllocMaxCells + 1) * 4);
+  b.cstart = (int*)take((kNoAllocMaxCells + 1) * 4);
+  b.cursor = (int*)take(kNoAllocMaxCells * 4);
+  b.bsum = (int*)take(kNoAllocScanBlocks * 4);
+  b.total = (int*)take(16);
+  b.spos = take(kNoAllocMaxParticles * 2 * 8);
+  b.sval = take(kNoAllocMaxParticles * 2 * 8);
+  return b;
+}
+
+template <class T>
+static int noalloc_p2f(Ctx* c, int64_t n, const void* pos, const void* vel, const void* fu, const void* fv,
+                       const PicGeom& g, void* uo, void* vo, uint64_t* unsupported) {
+  void* base = nullptr;
+  int rc = scratch_address(c->device, &base);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  SortBufs b = scratch_sort(base, g.nx * g.ny, (int)std::max<int64_t>(n, 1));
+  // the scratch keeps the previous call's contents: the counts and cursors are cleared by sort_by_cell's
+  // fill kernels, the unsupported-face count here (kernels, not memsets, so the calls can be captured)
+  fill_keys(s, reinterpret_cast<unsigned long long*>(unsupported), 1);
+  rc = sort_by_cell<T>(s, g, (int)n, (const T*)pos, (const T*)vel, b);
+  if (!rc)
+    rc = launch_p2g<T>(s, g, b, (const T*)fu, (const T*)fv, (T*)uo, (T*)vo,
+                       reinterpret_cast<unsigned long long*>(unsupported));
+  return rc;
+}
+
 static PicGeom transfer_geom(int nx, int ny, double x0, double y0, double dx, double dy, int px, int py) {
   PicGeom g;
   g.nx = nx;
@@ -1743,6 +1826,31 @@ int fb_pic_particle_to_faces(fb_ctx* ctx, int precision, int64_t n, const void* 
                                           unsupported_fraction)
              : fb::standalone_p2f<float>(c, n, positions, velocities, fallback_u, fallback_v, g, u_out, v_out,
                                          unsupported_fraction);
+}
+
+int fb_pic_particle_to_faces_noalloc(fb_ctx* ctx, int precision, int64_t n, const void* positions,
+                                     const void* velocities, const void* fallback_u, const void* fallback_v, int nx,
+                                     int ny, double x0, double y0, double dx, double dy, int periodic_x,
+                                     int periodic_y, void* u_out, void* v_out, uint64_t* unsupported_faces) {
+  FB_REQUIRE(ctx && fallback_u && fallback_v && u_out && v_out && unsupported_faces, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(n == 0 || (positions && velocities), FB_ERR_ARG, "NULL particle arrays");
+  FB_REQUIRE(n >= 0, FB_ERR_ARG, "negative particle count");
+  FB_REQUIRE(n <= fb::kNoAllocMaxParticles, FB_ERR_ARG,
+             "%lld particles exceed the allocation-free maximum of %lld", (long long)n,
+             (long long)fb::kNoAllocMaxParticles);
+  FB_REQUIRE(nx >= 4 && ny >= 4, FB_ERR_ARG, "each face grid axis must span at least four cells");
+  FB_REQUIRE((int64_t)nx * ny <= fb::kNoAllocMaxCells, FB_ERR_ARG,
+             "%d x %d cells exceed the allocation-free maximum of %lld", nx, ny, (long long)fb::kNoAllocMaxCells);
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  FB_REQUIRE(dx > 0.0 && dy > 0.0, FB_ERR_ARG, "cell sizes must be positive");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
+  return precision == FB_F64
+             ? fb::noalloc_p2f<double>(c, n, positions, velocities, fallback_u, fallback_v, g, u_out, v_out,
+                                       unsupported_faces)
+             : fb::noalloc_p2f<float>(c, n, positions, velocities, fallback_u, fallback_v, g, u_out, v_out,
+                                      unsupported_faces);
 }
 
 int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* u, const void* v,

@@ -341,6 +341,20 @@ int fb_pic_particle_to_faces(fb_ctx* ctx, int precision, int64_t n, const void* 
                              const void* fallback_u, const void* fallback_v, int nx, int ny, double x0, double y0,
                              double dx, double dy, int periodic_x, int periodic_y, void* u_out, void* v_out,
                              double* unsupported_fraction);
+/* fb_pic_particle_to_faces without any allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Every pointer is a DEVICE pointer: positions/velocities (n, 2), fallback and output faces u (ny, nx+1),
+ * v (ny+1, nx), and unsupported_faces, one uint64 that receives the count of u and v faces whose P2G weight
+ * is <= 1e-12 (the fraction is that count over ny*(nx+1) + (ny+1)*nx). The work is queued on the context's
+ * stream and the call returns without synchronising (fb_ctx_synchronize waits for it). The counting-sort
+ * buffers live in a fixed device array of the library (about 23 MiB per GPU, held until the process ends):
+ * at most 524,288 particles and 262,144 cells; larger inputs are rejected with FB_ERR_ARG. All calls on one
+ * GPU share that array, so calls that overlap on one GPU race: serialise them (one stream, or wait between
+ * calls). Same kernels and results as fb_pic_particle_to_faces. */
+int fb_pic_particle_to_faces_noalloc(fb_ctx* ctx, int precision, int64_t n, const void* positions,
+                                     const void* velocities, const void* fallback_u, const void* fallback_v, int nx,
+                                     int ny, double x0, double y0, double dx, double dy, int periodic_x,
+                                     int periodic_y, void* u_out, void* v_out, uint64_t* unsupported_faces);
 /* faces_to_particle (fb/solvers/_numba_adapter.py:138-160, kern/pic.py:164-205). */
 int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* u, const void* v,
                              const void* positions, int nx, int ny, double x0, double y0, double dx, double dy,
