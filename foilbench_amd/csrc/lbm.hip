@@ -1665,12 +1665,16 @@ static int compute_geometry(Lattice& L, int gi, const fb_pose& pose) {
   return FB_OK;
 }
 
+// Seeds of the postcondition reductions: the minimum at the all-ones key (above every finite key), the
+// maxima at key 0 (below the key of every value), the counts at 0.
+__global__ void red_seed_kernel(unsigned long long* red) {
+  if (threadIdx.x < 6) red[threadIdx.x] = threadIdx.x == 0 ? ~0ull : 0ull;
+}
+
 static int reset_red(Lattice& L) {
-  unsigned long long init[6] = {~0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-  // Encoded: min starts at +inf key (all ones is above every finite key), max at key of -inf.
-  init[1] = 0ull;
-  init[2] = 0ull;
-  FB_HIP(hipMemcpyAsync(L.red, init, sizeof(init), hipMemcpyHostToDevice, L.ctx->stream));
+  // a kernel, not a copy from host memory: queued work never reads a host buffer after this returns
+  hipLaunchKernelGGL(red_seed_kernel, dim3(1), dim3(64), 0, L.ctx->stream, L.red);
+  FB_HIP(hipGetLastError());
   return FB_OK;
 }
 

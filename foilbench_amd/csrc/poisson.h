@@ -91,6 +91,7 @@ class Poisson {
   double* host_par = nullptr;  // kParSlots x 2 cap pinned staging slots
   static constexpr int kParSlots = 16;
   unsigned par_slot_ = 0;
+  hipEvent_t par_ev[kParSlots] = {};  // recorded after each slot's copy; waited on before the slot is rewritten
   uint8_t* flags = nullptr;
   double* diag = nullptr;
   double* ex = nullptr;

@@ -1181,6 +1181,8 @@ Poisson::~Poisson() {
                     (void*)bstage, (void*)xstage, (void*)dpar})
     if (ptr) (void)dev_free(ptr);
   if (host_scal) (void)host_free(host_scal);
+  for (auto& e : par_ev)
+    if (e) (void)hipEventDestroy(e);
   if (host_par) (void)host_free(host_par);
   auto freev = [](auto& v) {
     for (void* ptr : {(void*)v.r, (void*)v.z, (void*)v.p, (void*)v.q, (void*)v.x, (void*)v.b, (void*)v.work,
@@ -1360,11 +1362,16 @@ int Poisson::read_scalars() {
 }
 
 // Per-grid launch parameters to dpar. The copy reads pinned memory when the stream reaches it,
-// so successive calls between synchronisations use successive host slots of a ring.
+// so successive calls use successive host slots of a ring, and a slot is rewritten only once the
+// copy that last read it has run (its event).
 int Poisson::put_par(const double* v, int count) {
-  double* h = host_par + (size_t)(par_slot_++ % kParSlots) * 2 * cap;
+  const unsigned k = par_slot_++ % kParSlots;
+  if (par_ev[k]) FB_HIP(hipEventSynchronize(par_ev[k]));
+  else FB_HIP(hipEventCreateWithFlags(&par_ev[k], hipEventDisableTiming));
+  double* h = host_par + (size_t)k * 2 * cap;
   for (int g = 0; g < count; ++g) h[g] = v[g];
   FB_HIP(hipMemcpyAsync(dpar, h, count * sizeof(double), hipMemcpyHostToDevice, stream));
+  FB_HIP(hipEventRecord(par_ev[k], stream));
   return FB_OK;
 }
 
