@@ -7,7 +7,8 @@ interpolated bounce-back on the foil, FP32 population storage.
 
 Prints one JSON line. Besides the headline FP32 value it reports the FP64 plain form and the
 shifted FP32/FP16 precision forms (throughput and relative velocity error against FP64),
-the roofline of the step kernel from HIP events, and the NumPy oracle timed on the host.
+the roofline of the step kernel from HIP events, and the oracle timed on the host's cores (the C restatement of
+the lattice step on one pinned core and on all cores, with the CPU model).
 Under "legs" it measures the other paths through their solver lifecycles, each with a model
 roofline and an oracle baseline on the host:
   sf_plain   Stable Fluids FP64, NACA 0012 AoA 8, 256 x 128, Re 500, CG to 1e-6;
@@ -17,7 +18,7 @@ roofline and an oracle baseline on the host:
              (the serving form's pressure solve), with its error against the FP64 plain form;
   pic        PIC/FLIP FP32, NACA 4412 AoA 10, 1024 x 512, 8 particles per fluid cell.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-legs]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-scale S] [--no-legs]
 """
 
 from __future__ import annotations
@@ -128,6 +129,41 @@ def _form_errors(nx: int, ny: int, steps: int, forms=("f32", "f32-mixed", "f16-m
     return {name: float(np.linalg.norm(vel[name][fl] - ref) / np.linalg.norm(ref)) for name in forms}
 
 
+def _serving_lattice_errors() -> dict:
+    """The serving form's lattices (shifted FP32 storage, FP64 collision, one batched launch per step) against
+    the plain FP64 form after SERVING_ERROR_STEPS steps, for one request of each grid shape of the serving mix
+    (NACA 2412 at AoA 12, Re 1000; the lattice viscosity clamp of the solver lifecycle applies at the small
+    shapes): relative L2 velocity error over the fluid cells."""
+    import ctypes as C
+
+    import numpy as np
+
+    from foilbench_amd import _native as N
+    from foilbench_amd.serving.engine import _build, _close
+    from foilbench_amd.serving.mix import LBM, Request
+
+    ctx = N.Context.default()
+    out = {}
+    for nx, ny in SERVING_SHAPES:
+        req = Request(0, 0.0, LBM, nx, ny, "2412", 12.0, 1000.0, SERVING_ERROR_STEPS)
+        vel = {}
+        for form in ("plain", "serving"):
+            s, handle, params, _ = _build(req, form, ctx)
+            try:
+                N.check(ctx.lib.fb_lbm_run_batch((C.c_void_p * 1)(handle), C.byref(params), 1, req.steps, 32, None))
+                ctx.synchronize()
+                _, v = s._macroscopic()
+                vel[form] = np.asarray(v, np.float64)
+                solid = s.geometry_fields()[1]
+            finally:
+                _close(s)
+        fl = ~solid
+        ref = vel["plain"][fl]
+        out[f"served_lattice_{nx}x{ny}_{SERVING_ERROR_STEPS}_steps"] = float(
+            np.linalg.norm(vel["serving"][fl] - ref) / np.linalg.norm(ref))
+    return out
+
+
 def _large_lattice(tuner) -> dict:
     """The step kernel on a 4096 x 2048 lattice (chord 640 cells), whose populations (604 MB in FP32) cannot
     stay in the 256 MB Infinity Cache between steps: the HBM-resident figure."""
@@ -149,67 +185,82 @@ def _large_lattice(tuner) -> dict:
     return out
 
 
-def _cpu_baseline(sample_steps: int):
-    import numpy as np
+CPU_STEPS = {"lbm": 80, "sf_plain": 50, "sf_mixed": 1, "pic": 1}
 
-    from oracle import lbm as L
-    from oracle.geometry import NacaFoil, cell_centers
-    from foilbench_amd.benchmark.lbm_harness import LatticeCase
 
-    case = LatticeCase()
-    nx, ny = case.nx, case.ny
-    dx = (case.bounds[0][1] - case.bounds[0][0]) / nx
-    dy = (case.bounds[1][1] - case.bounds[1][0]) / ny
-    foil = NacaFoil(case.naca, case.chord, case.pivot)
-    sdf = foil.signed_distance(cell_centers(case.bounds, nx, ny).reshape(-1, 2), case.angle_degrees).reshape(ny, nx)
-    solid = sdf <= 0.0
-    u = case.u_lattice
-    vel = np.zeros((ny, nx, 2), np.float32)
-    vel[..., 0] = u
-    f = L.equilibrium(np.ones((ny, nx), np.float32), vel)
-    feqb = f[0, 0].copy()
-    outlet = f[:, -1:].copy()
-    sp = L.sponge_strength_profile(nx, ny, False, False, False)
-    _, op, om = case.relaxation()
-    args = (solid, sdf, op, om, u, (np.float32(u), np.float32(0.0)), feqb, sp)
-    L.kernel_steps(f, *args, outlet, case.bounds, dx, dy, case.pivot, 0.0, u, False, False, False, 1)
-    t0 = time.perf_counter()
-    L.kernel_steps(f, *args, outlet, case.bounds, dx, dy, case.pivot, 0.0, u, False, False, False, sample_steps)
-    el = time.perf_counter() - t0
-    return {
-        "value": nx * ny * sample_steps / el,
-        "unit": "cell-steps/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"NumPy oracle (oracle/lbm.py kernel_steps), same 1024x512 FP32 lattice, {sample_steps} "
-                  f"lattice steps after 1 warm-up step, {el:.1f} s, single-threaded NumPy",
+def _cpu_run(leg_args: list[str], threads: int, pin: int | None) -> dict:
+    """One oracle baseline (oracle/baseline.py) in a child process whose OpenMP and BLAS thread counts are set
+    before NumPy loads, pinned to one core when `pin` is given."""
+    import subprocess
+
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = str(threads)
+    cmd = [sys.executable, "-m", "oracle.baseline", *leg_args, "--threads", str(threads)]
+    if pin is not None:
+        cmd += ["--pin", str(pin)]
+    res = subprocess.run(cmd, cwd=str(ROOT), env=env, capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        return {"error": res.stderr[-2000:]}
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    out["kind"] = "port"
+    return out
+
+
+def _cpu_baselines(scale: float) -> tuple[dict, dict]:
+    """The headline lattice's baseline (the C restatement of the lattice step on one pinned core, and on all
+    the cores this process may use) and the legs' baselines (the NumPy/SciPy oracle on one pinned core)."""
+    from oracle.baseline import cpu_model
+
+    cpus = sorted(os.sched_getaffinity(0))
+    pin = cpus[0]
+    all_cores = min(len(cpus), 16)
+    host = {"nproc": os.cpu_count(), "affinity_cpus": len(cpus), "cpu_model": cpu_model()}
+    n = lambda k: max(1, int(round(CPU_STEPS[k] * scale)))  # noqa: E731
+    lbm1 = _cpu_run(["lbm", "--steps", str(n("lbm"))], 1, pin)
+    lbmn = _cpu_run(["lbm", "--steps", str(4 * n("lbm"))], all_cores, None)
+    head = dict(lbm1)
+    head["host"] = host
+    head["all_cores"] = {k: lbmn.get(k) for k in ("value", "unit", "cores", "sample", "median_advance_ms")}
+    legs = {
+        "sf_plain": _cpu_run(["sf", "--nx", "256", "--ny", "128", "--angle", "-8", "--re", "500", "--tol", "1e-6",
+                              "--steps", str(n("sf_plain"))], 1, pin),
+        # the 2048 x 1024 workload at a quarter of its cells (one oracle advance at full size takes ~40 s):
+        # the rate per cell-substep of the NumPy/SciPy oracle is nearly size-independent (CG iterations grow
+        # from 20 to 22 between the two sizes)
+        "sf_mixed": _cpu_run(["sf", "--nx", "1024", "--ny", "512", "--angle", "-15", "--re", "2000", "--tol", "1e-8",
+                              "--steps", str(n("sf_mixed"))], 1, pin),
+        # the 1024 x 512 workload at 256 x 128 (one oracle advance at full size takes about a minute)
+        "pic": _cpu_run(["pic", "--nx", "256", "--ny", "128", "--per-cell", "8", "--steps", str(n("pic"))], 1, pin),
     }
+    for v in legs.values():
+        v["host"] = host
+    return head, legs
 
 
-def _legs(with_cpu: bool) -> dict:
-    """Stable Fluids and PIC/FLIP workloads; a failing leg is reported, not fatal."""
+def _legs(cpu_legs: dict | None) -> dict:
+    """Stable Fluids and PIC/FLIP workloads; a failing leg is reported, not fatal. `cpu_legs`: the oracle
+    baselines of the legs (sf_precond32 shares sf_mixed's FP64 oracle workload)."""
     from foilbench_amd.benchmark import legs as L
     from foilbench_amd.benchmark.serving import serving_leg
 
     plan = {
-        "sf_plain": (lambda: L.stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=30),
-                     lambda: L.sf_cpu_baseline(256, 128, -8.0, 500.0, 1e-6, 4)),
-        "sf_mixed": (lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=4, warmup=1,
-                                                 reference_form="plain"),
-                     lambda: L.sf_cpu_baseline(512, 256, -15.0, 2000.0, 1e-8, 1)),
-        "sf_precond32": (lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "precond32", 1e-8, steps=4,
-                                                     warmup=1, reference_form="plain"), None),
-        "pic": (lambda: L.pic_leg(1024, 512, 8, steps=4),
-                lambda: L.pic_cpu_baseline(256, 128, 8, 1)),
-        "serving": (lambda: serving_leg(), None),
+        "sf_plain": lambda: L.stable_fluids_leg(256, 128, -8.0, 500.0, "plain", 1e-6, steps=30),
+        "sf_mixed": lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "mixed", 1e-8, steps=4, warmup=1,
+                                                reference_form="plain"),
+        "sf_precond32": lambda: L.stable_fluids_leg(2048, 1024, -15.0, 2000.0, "precond32", 1e-8, steps=4,
+                                                    warmup=1, reference_form="plain"),
+        "pic": lambda: L.pic_leg(1024, 512, 8, steps=4),
+        "serving": lambda: serving_leg(),
     }
+    cpu_of = {"sf_plain": "sf_plain", "sf_mixed": "sf_mixed", "sf_precond32": "sf_mixed", "pic": "pic"}
     out = {}
-    for name, (gpu, cpu) in plan.items():
+    for name, gpu in plan.items():
         t0 = time.perf_counter()
         try:
             out[name] = gpu()
-            if with_cpu and cpu is not None:
-                out[name]["cpu_baseline"] = cpu()
+            if cpu_legs and name in cpu_of:
+                out[name]["cpu_baseline"] = cpu_legs[cpu_of[name]]
         except Exception as err:  # reported in the JSON line instead of aborting the headline
             out[name] = {"error": f"{type(err).__name__}: {err}"}
         print(f"[bench] leg {name}: {time.perf_counter() - t0:.1f} s "
@@ -235,7 +286,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-scale", type=float, default=1.0,
+                    help="scale of the CPU baselines' step counts (0: no CPU baseline)")
     ap.add_argument("--no-legs", action="store_true", help="measure only the LBM headline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -244,9 +296,7 @@ def main() -> int:
         return 2
     case, forms, tuner = _gpu_forms(args.steps, args.warmup)
     errors = {"lattice_1024x512_20000_steps": _form_errors(case.nx, case.ny, ERROR_STEPS)}
-    for nx, ny in SERVING_SHAPES:
-        errors[f"lattice_{nx}x{ny}_{SERVING_ERROR_STEPS}_steps"] = _form_errors(nx, ny, SERVING_ERROR_STEPS,
-                                                                                ("f32-mixed", "f16-mixed"))
+    errors.update(_serving_lattice_errors())
     for name in forms:
         if name != "f64":
             forms[name]["velocity_rel_err_vs_f64"] = errors["lattice_1024x512_20000_steps"][name]
@@ -255,8 +305,8 @@ def main() -> int:
     main_form = forms["f32"]
     achieved = main_form["kernel_gbs"]
     traffic = _traffic(main_form["kernel"])
-    cpu = _cpu_baseline(args.cpu_steps) if args.cpu_steps > 0 else None
-    legs = None if args.no_legs else _legs(args.cpu_steps > 0)
+    cpu, cpu_legs = _cpu_baselines(args.cpu_scale) if args.cpu_scale > 0 else (None, None)
+    legs = None if args.no_legs else _legs(cpu_legs)
     line = {
         "metric": "cell-steps/s/GPU",
         "value": main_form["cell_steps_per_s"],
@@ -296,8 +346,9 @@ def main() -> int:
     }
     sv = (legs or {}).get("serving") or {}
     if "value" in sv:
-        line["serving"] = {k: sv[k] for k in ("metric", "value", "unit", "plain_value", "ratio_vs_plain", "p50_ms",
-                                              "p99_ms", "max_rel_field_error", "workload")}
+        line["serving"] = {k: sv[k] for k in ("metric", "value", "unit", "plain_value", "ratio_vs_plain",
+                                              "ratio_under_arrivals", "p50_ms", "p99_ms", "max_rel_field_error",
+                                              "workload")}
     print(json.dumps(line), flush=True)
     return 0
 

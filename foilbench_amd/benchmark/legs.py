@@ -172,46 +172,6 @@ def pic_leg(nx: int, ny: int, per_cell: int, steps: int, warmup: int = 1, precis
     }
 
 
-def sf_cpu_baseline(nx: int, ny: int, angle: float, reynolds: float, tol: float, steps: int) -> dict:
-    """The NumPy/SciPy oracle on the same workload, `steps` advances after one warm-up."""
-    from oracle.mac import StableFluidsOracle
-
-    sc = sf_scenario(nx, ny, angle, reynolds, "float64", tol)
-    o = StableFluidsOracle()
-    o.initialize(sc)
-    t = sc.output_dt
-    o.advance(sc.control_at(t), sc.output_dt)
-    subs = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        t += sc.output_dt
-        subs += o.advance(sc.control_at(t), sc.output_dt)["substeps"]
-    el = time.perf_counter() - t0
-    return {"value": nx * ny * subs / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/mac.py StableFluidsOracle, {nx}x{ny}, {steps} advances after 1 warm-up, "
-                      f"{el:.1f} s (NumPy/SciPy)"}
-
-
-def pic_cpu_baseline(nx: int, ny: int, per_cell: int, steps: int) -> dict:
-    from oracle.pic import PicFlipOracle
-
-    sc = pic_scenario(nx, ny, "float32")
-    o = PicFlipOracle()
-    o.initialize(sc, seed=42, per_cell=per_cell)
-    t = sc.output_dt
-    o.advance(sc.control_at(t), sc.output_dt)
-    subs = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        t += sc.output_dt
-        subs += o.advance(sc.control_at(t), sc.output_dt)["substeps"]
-    el = time.perf_counter() - t0
-    n = o.positions.shape[0]
-    return {"value": nx * ny * subs / el, "unit": "cell-steps/s", "particle_steps_per_s": n * subs / el, "cores": 1,
-            "kind": "port", "sample": f"oracle/pic.py PicFlipOracle, {nx}x{ny}, {per_cell}/cell ({n} particles), "
-                                      f"{steps} advances after 1 warm-up, {el:.1f} s (NumPy)"}
-
-
 def main() -> None:
     """``python -m foilbench_amd.benchmark.legs {sf_plain|sf_mixed|sf_precond32|pic|serving} [steps]``: one leg, one JSON
     line (for ``serving`` the second argument is the number of requests)."""
