@@ -436,6 +436,50 @@ class LBMOracle:
                     raise
         raise RuntimeError("unreachable")
 
+    def import_state(self, velocity, density, time, control):
+        # fb/solvers/lbm.py:917-1052 with the import scaling of :412-426: the lattice speed chosen from the
+        # imported maximum speed (independent of the output interval), the density and Mach envelopes checked,
+        # then the populations rebuilt as the local equilibrium (solid cells at unit density and the wall
+        # velocity of the import pose); rolled back on failure
+        saved = self._checkpoint()
+        try:
+            sc = self.sc
+            dom = sc.domain
+            physical = np.asarray(velocity, dtype=sc.dtype).copy()
+            maximum = float(np.max(np.linalg.norm(physical, axis=2)))
+            wall_speed = abs(np.deg2rad(control.angular_velocity_degrees)) * self.foil.maximum_radius
+            sel = max(max(maximum, wall_speed), self.ref_speed)
+            self.lattice_speed = MAX_LATTICE_SPEED * self.ref_speed / sel
+            self.lattice_dt = self.lattice_speed * dom.dx / self.ref_speed
+            self.configure_relaxation()
+            self.rebuild_boundary_equilibrium()
+            rho = (np.ones((dom.ny, dom.nx), dtype=sc.dtype) if density is None
+                   else np.asarray(density, dtype=sc.dtype).copy())
+            sdf = self.foil.signed_distance(self.centers.reshape(-1, 2), control.angle_degrees).reshape(dom.ny, dom.nx)
+            solid = sdf <= 0.0
+            fr = rho[~solid]
+            if not np.isfinite(rho).all() or np.any(fr <= 0.0) or (fr.size and float(np.max(np.abs(fr - 1.0))) > 0.75):
+                raise OracleFailure("invalid_density", "warm import density outside the envelope", "canonical-import")
+            mach = max(maximum, wall_speed, self.ref_speed) * self.lattice_speed / (self.ref_speed * SOUND)
+            if mach > MAX_MACH * (1.0 + 1.0e-6):
+                raise OracleFailure("excessive_velocity", "warm import exceeds the Mach limit", "canonical-import",
+                                    {"maximum_lattice_mach": mach})
+            self.time = float(time)
+            self.control = control
+            wall = self.foil.wall_velocity(self.centers.reshape(-1, 2),
+                                           control.angular_velocity_degrees).reshape(dom.ny, dom.nx, 2)
+            physical[solid] = wall[solid]
+            rho[solid] = 1.0
+            self.f = equilibrium(rho, physical * (self.lattice_speed / self.ref_speed))
+            self.outlet = self.f[:, -1:, :].copy()
+            self.sdf = sdf
+            self.solid = solid
+            self.solid_angle = control.angle_degrees
+        except Exception:
+            self._restore(saved)
+            raise
+        self.revision += 1
+
     def export_velocity_density(self):
         # fb/solvers/lbm.py:892-915
         rho, _ = macroscopic(self.f)

@@ -610,6 +610,26 @@ class StableFluidsOracle:
     def cell_velocity(self):
         return faces_to_cell(self.u, self.v)
 
+    def import_state(self, velocity, time, control):
+        # fb/solvers/stable_fluids.py:636-706: cell velocity with the solid cells at the wall velocity of the
+        # import pose, to faces, projected with a 1e-4 pseudo-step; rolled back on failure
+        saved = (self.u.copy(), self.v.copy(), self.solid.copy(), self.control, self.time, self.revision,
+                 self.last_projection)
+        try:
+            vel = np.asarray(velocity, dtype=self.sc.dtype).copy()
+            self.time = float(time)
+            self.control = control
+            self.solid = self.solid_fn(control.angle_degrees)
+            wall = self.wall_grid(control.angular_velocity_degrees)
+            vel[self.solid] = wall[self.solid]
+            self.u, self.v = cell_to_faces(vel)
+            self.apply_projection(1.0e-4)
+        except Exception:
+            (u, v, sol, self.control, self.time, self.revision, self.last_projection) = saved
+            self.u, self.v, self.solid = u, v, sol
+            raise
+        self.revision += 1
+
     def advance(self, control, target_dt):
         # fb/solvers/stable_fluids.py:332-603
         sc = self.sc

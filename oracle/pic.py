@@ -222,11 +222,34 @@ class PicFlipOracle:
         from .mac import initial_velocity
         vel = initial_velocity(sc)
         self.grid = vel
+        self.per_cell = per_cell
         self.positions = seed_particles(self.solid, self.dom, self.rng, sc.dtype, per_cell)
         u, v = cell_to_faces(vel)
         self.velocities = self._f2p(u, v, self.positions)
         # fb/solvers/pic_flip.py:287: the P2G result is discarded; only the unsupported fraction is kept
         self._p2g_cell()
+
+    def import_state(self, velocity, time, control):
+        # fb/solvers/pic_flip.py:1035-1113: the grid velocity with the solid cells at the wall velocity of the
+        # import pose, particles reseeded from the stream where it stands (fb/solvers/pic_flip.py:317-333) with
+        # velocities gathered from that grid, and one settling advance (blend 0) to follow
+        saved = self._checkpoint()
+        try:
+            vel = np.asarray(velocity, dtype=self.sc.dtype).copy()
+            self.time = float(time)
+            self.control = control
+            self.solid = self.foil.mask(self.dom.bounds, self.dom.nx, self.dom.ny, control.angle_degrees)
+            self.solid_angle = control.angle_degrees
+            wall = self._wall(control.angular_velocity_degrees)
+            vel[self.solid] = wall[self.solid]
+            self.grid = vel
+            self.positions = seed_particles(self.solid, self.dom, self.rng, self.sc.dtype, self.per_cell)
+            u, v = cell_to_faces(self.grid)
+            self.velocities = self._f2p(u, v, self.positions)
+            self.settling = 1
+        except Exception:
+            self._restore(saved)
+            raise
 
     def _px(self):
         return "x" in self.dom.periodic_axes, "y" in self.dom.periodic_axes
