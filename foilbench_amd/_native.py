@@ -235,6 +235,11 @@ _SIGNATURES: dict[str, tuple] = {
                                                    _P, _P, _P]),
     "fb_pic_faces_to_particle": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P]),
+    "fb_pic_particle_to_cells": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, C.c_int, C.c_int, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                           C.c_int, _P]),
+    "fb_pic_cells_to_particle": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, C.c_int, C.c_int, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P]),
 }
 
 _lock = threading.Lock()

@@ -167,6 +167,24 @@ __device__ __forceinline__ int bin_cell(const PicGeom& g, double x, double y) {
   return cy * g.nx + cx;
 }
 
+// Cell-centred transfers (kern/pic.py:22-102).
+// Stencil base of a coordinate: grid coordinate g = (x - x0) / d - 0.5, base floor(g - 0.5).
+__device__ __forceinline__ double cell_coord(double x, double x0, double d) { return (x - x0) / d - 0.5; }
+
+// Target of stencil index s along an axis of n cells.
+__device__ __forceinline__ int cell_target(int s, int n, int periodic) {
+  return periodic ? pmod(s, n) : min(max(s, 0), n - 1);
+}
+
+// Bin of a stencil base along an axis: wrapped on a periodic axis (n bins); otherwise the base
+// clamped to [-1, n - 1] and shifted by one (n + 1 bins). A base below -1 reaches only cell 0 and
+// one above n - 2 only cell n - 1, so each end bin holds every particle beyond it and the cells
+// that can receive from a bin are those of its bases.
+__device__ __forceinline__ int base_bin(int b, int n, int periodic) {
+  return periodic ? pmod(b, n) : min(max(b, -1), n - 1) + 1;
+}
+__host__ __device__ inline int base_bins(int n, int periodic) { return periodic ? n : n + 1; }
+
 __device__ __forceinline__ long long wave_sum_i(long long v) { return wave_sum_ll(v); }
 
 // ------------------------------------------------------------------ PCG32 (fb/core/rng.py)
@@ -301,7 +319,8 @@ __global__ void __launch_bounds__(256) fill_int_kernel(int* __restrict__ a, int 
 
 // ------------------------------------------------------------------ sort by cell
 
-template <class T>
+// BASE: bin by cell-centred stencil base (the bins of p2c_kernel) instead of the home cell.
+template <class T, bool BASE>
 __global__ void __launch_bounds__(256) bin_count_kernel(PicGeom g, int n, const T* __restrict__ pos,
                                                         int* __restrict__ cellof, int* __restrict__ count) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -310,7 +329,13 @@ __global__ void __launch_bounds__(256) bin_count_kernel(PicGeom g, int n, const 
   if (act) {
     T x, y;
     ld2(pos, i, x, y);
-    c = bin_cell(g, (double)x, (double)y);
+    if constexpr (BASE) {
+      const int bx = ifloor(cell_coord((double)x, g.x0, g.dx) - 0.5);
+      const int by = ifloor(cell_coord((double)y, g.y0, g.dy) - 0.5);
+      c = base_bin(by, g.ny, g.py) * base_bins(g.nx, g.px) + base_bin(bx, g.nx, g.px);
+    } else {
+      c = bin_cell(g, (double)x, (double)y);
+    }
     cellof[i] = c;
   }
   wave_run_add(count, c, act);
@@ -607,6 +632,135 @@ __global__ void __launch_bounds__(256) g2p_kernel(PicGeom g, int n, const int* _
   gather<T, false>(g, comp_u(g), u, nullptr, (double)x, (double)y, a, dummy);
   gather<T, false>(g, comp_v(g), v, nullptr, (double)x, (double)y, b, dummy);
   st2(vel, i, (T)a, (T)b);
+}
+
+// ------------------------------------------------------------------ cell-centred transfers
+// particle_to_grid_kernel and grid_to_particle_kernel (kern/pic.py:22-102): quadratic B-spline
+// transfers between particles and a (ny, nx, 2) cell-centred grid, the 3 x 3 stencil wrapped on a
+// periodic axis and clamped to the edge cells otherwise, so an edge cell can take several of one
+// particle's nine weights.
+
+// P2G onto cells as a gather in the reference's order. Cell (tx, ty) receives only from particles
+// whose stencil base lies in ty-2..ty and tx-2..tx (after wrapping or clamping): at most 3 x 3 bins,
+// each listing its particles by index. The gather merges those lists by particle index and adds each
+// particle's weights for this cell in its 3 x 3 offset order, so every cell sees the sequence of
+// additions of the reference's particle-major loop: the FP64 contribution added to the dtype
+// accumulator in FP64 and rounded back (kern/pic.py:49-51). Results are bit-identical to it.
+template <class T>
+__global__ void __launch_bounds__(256) p2c_kernel(PicGeom g, const int* __restrict__ start,
+                                                  const int* __restrict__ sidx, const T* __restrict__ spos,
+                                                  const T* __restrict__ sval, double fsx, double fsy,
+                                                  T* __restrict__ grid) {
+  const int tx = blockIdx.x * 256 + threadIdx.x;
+  const int ty = blockIdx.y;
+  if (tx >= g.nx) return;
+  const int bw = base_bins(g.nx, g.px);
+  // the bins of bases ty-2..ty and tx-2..tx, each taken once (they repeat at a clamped edge and on
+  // a periodic axis of fewer than three cells)
+  int rows[3], cols[3];
+  bool rnew[3], cnew[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    rows[k] = base_bin(ty - 2 + k, g.ny, g.py);
+    cols[k] = base_bin(tx - 2 + k, g.nx, g.px);
+    rnew[k] = cnew[k] = true;
+#pragma unroll
+    for (int j = 0; j < k; ++j) {
+      rnew[k] = rnew[k] && rows[j] != rows[k];
+      cnew[k] = cnew[k] && cols[j] != cols[k];
+    }
+  }
+  // cursor, end and head particle index of each candidate bin (INT_MAX once drained)
+  int cur[9], end[9], head[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int r = j / 3, c = j % 3;
+    cur[j] = end[j] = 0;
+    head[j] = INT_MAX;
+    if (rnew[r] && cnew[c]) {
+      const int bin = rows[r] * bw + cols[c];
+      cur[j] = start[bin];
+      end[j] = start[bin + 1];
+      if (cur[j] < end[j]) head[j] = sidx[cur[j]];
+    }
+  }
+  T w = (T)0, mx = (T)0, my = (T)0;
+  for (;;) {
+    int best = -1, bi = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+      if (head[j] < bi) {
+        bi = head[j];
+        best = j;
+      }
+    if (best < 0) break;
+    int k = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+      if (j == best) {
+        k = cur[j]++;
+        head[j] = cur[j] < end[j] ? sidx[cur[j]] : INT_MAX;
+      }
+    T x, y, vx, vy;
+    ld2(spos, k, x, y);
+    ld2(sval, k, vx, vy);
+    const double gx = cell_coord((double)x, g.x0, g.dx), gy = cell_coord((double)y, g.y0, g.dy);
+    const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+    for (int oy = 0; oy < 3; ++oy) {
+      const int sy = by + oy;
+      if (cell_target(sy, g.ny, g.py) != ty) continue;
+      const double wy = bspline(gy - (double)sy);
+      for (int ox = 0; ox < 3; ++ox) {
+        const int sx = bx + ox;
+        if (cell_target(sx, g.nx, g.px) != tx) continue;
+        const double wt = wy * bspline(gx - (double)sx);
+        w = (T)((double)w + wt);
+        mx = (T)((double)mx + wt * (double)vx);
+        my = (T)((double)my + wt * (double)vy);
+      }
+    }
+  }
+  T ox, oy;
+  if ((double)w > 1.0e-12) {
+    // the dtype quotient: an FP64 quotient of FP32 operands rounds to the correctly rounded FP32 one
+    ox = (T)((double)mx / (double)w);
+    oy = (T)((double)my / (double)w);
+  } else {
+    ox = (T)fsx;
+    oy = (T)fsy;
+  }
+  st2(grid, (size_t)ty * g.nx + tx, ox, oy);
+}
+
+// grid_to_particle_kernel (kern/pic.py:67-102): FP64 sums in the 3 x 3 order, scaled by
+// 1 / max(weight sum, 1e-12), rounded to the dtype.
+template <class T>
+__global__ void __launch_bounds__(256) c2p_kernel(PicGeom g, int n, const T* __restrict__ grid,
+                                                  const T* __restrict__ pos, T* __restrict__ vel) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  T x, y;
+  ld2(pos, k, x, y);
+  const double gx = cell_coord((double)x, g.x0, g.dx), gy = cell_coord((double)y, g.y0, g.dy);
+  const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+  double vx = 0.0, vy = 0.0, ws = 0.0;
+  for (int oy = 0; oy < 3; ++oy) {
+    const int sy = by + oy;
+    const int ty = cell_target(sy, g.ny, g.py);
+    const double wy = bspline(gy - (double)sy);
+    for (int ox = 0; ox < 3; ++ox) {
+      const int sx = bx + ox;
+      const int tx = cell_target(sx, g.nx, g.px);
+      const double w = wy * bspline(gx - (double)sx);
+      T a, b;
+      ld2(grid, (size_t)ty * g.nx + tx, a, b);
+      vx += w * (double)a;
+      vy += w * (double)b;
+      ws += w;
+    }
+  }
+  const double inv = 1.0 / fmax(ws, 1.0e-12);
+  st2(vel, k, (T)(vx * inv), (T)(vy * inv));
 }
 
 
@@ -1006,13 +1160,16 @@ static int scan_ints(hipStream_t s, const int* in, int* out, int n, int* bsum, i
 
 static inline unsigned blocks(long long n) { return (unsigned)std::max<long long>(1, (n + 255) / 256); }
 
-template <class T>
+// Counting sort of the particles by bin (b.ncell bins: home cells, or with BASE the cell-centred stencil
+// bases), each bin's list ordered by particle index, with the positions and values copied in that order.
+template <class T, bool BASE = false>
 static int sort_by_cell(hipStream_t s, const PicGeom& g, int n, const T* pos, const T* vel, SortBufs& b) {
   const int nc = b.ncell;
   hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc + 1)), dim3(256), 0, s, b.counts, nc + 1, 0);
   hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(nc)), dim3(256), 0, s, b.cursor, nc, 0);
   if (n > 0)
-    hipLaunchKernelGGL(bin_count_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, g, n, pos, b.cellof, b.counts);
+    hipLaunchKernelGGL((bin_count_kernel<T, BASE>), dim3(blocks(n)), dim3(256), 0, s, g, n, pos, b.cellof,
+                       b.counts);
   FB_HIP(hipGetLastError());
   int rc = scan_ints(s, b.counts, b.cstart, nc, b.bsum, b.total);
   if (rc) return rc;
@@ -1705,6 +1862,71 @@ static int standalone_f2p(Ctx* c, int64_t n, const void* u, const void* v, const
   return rc;
 }
 
+template <class T>
+static int standalone_p2c(Ctx* c, int64_t n, const void* pos, const void* vel, const PicGeom& g, double fsx,
+                          double fsy, void* out) {
+  hipStream_t s = c->stream;
+  const size_t gb = (size_t)g.nx * g.ny * 2 * sizeof(T);
+  const int nbin = base_bins(g.nx, g.px) * base_bins(g.ny, g.py);
+  SortBufs b;
+  int rc = alloc_sort(b, nbin, (int)std::max<int64_t>(n, 1), sizeof(T), s, true);
+  if (rc) return rc;
+  void *dp = nullptr, *dv = nullptr, *dg = nullptr;
+  const size_t pb = (size_t)std::max<int64_t>(n, 1) * 2 * sizeof(T);
+  hipError_t e = hipMallocAsync(&dp, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dv, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dg, gb, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dp, pos, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dv, vel, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    rc = sort_by_cell<T, true>(s, g, (int)n, (const T*)dp, (const T*)dv, b);
+    if (!rc) {
+      hipLaunchKernelGGL(p2c_kernel<T>, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, g, b.cstart, b.sidx,
+                         (const T*)b.spos, (const T*)b.sval, fsx, fsy, (T*)dg);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(out, dg, gb, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+  }
+  for (void* p : {dp, dv, dg})
+    if (p) (void)hipFreeAsync(p, s);
+  free_sort(b, s, true);
+  if (e != hipSuccess) {
+    set_error("particle_to_cells failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  FB_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+template <class T>
+static int standalone_c2p(Ctx* c, int64_t n, const void* grid, const void* pos, const PicGeom& g, void* out) {
+  hipStream_t s = c->stream;
+  const size_t gb = (size_t)g.nx * g.ny * 2 * sizeof(T);
+  void *dp = nullptr, *dg = nullptr, *dout = nullptr;
+  const size_t pb = (size_t)std::max<int64_t>(n, 1) * 2 * sizeof(T);
+  hipError_t e = hipMallocAsync(&dp, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dout, pb, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dg, gb, s);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(dp, pos, (size_t)n * 2 * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dg, grid, gb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && n > 0) {
+    hipLaunchKernelGGL(c2p_kernel<T>, dim3(blocks(n)), dim3(256), 0, s, g, (int)n, (const T*)dg, (const T*)dp,
+                       (T*)dout);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)n * 2 * sizeof(T), hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (void* p : {dp, dg, dout})
+    if (p) (void)hipFreeAsync(p, s);
+  if (e != hipSuccess) {
+    set_error("cells_to_particle failed: %s", hipGetErrorString(e));
+    return FB_ERR_HIP;
+  }
+  FB_HIP(hipStreamSynchronize(s));
+  return FB_OK;
+}
+
 // ------------------------------------------------------------------ allocation-free P2G
 // fb_pic_particle_to_faces_noalloc keeps its counting-sort buffers (cell of each particle, per-cell counts,
 // starts and cursors, the scan's block sums, the sorted index and the sorted positions and values) in one
@@ -1867,6 +2089,41 @@ int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
   fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
   return precision == FB_F64 ? fb::standalone_f2p<double>(c, n, u, v, positions, g, out)
                              : fb::standalone_f2p<float>(c, n, u, v, positions, g, out);
+}
+
+int fb_pic_particle_to_cells(fb_ctx* ctx, int precision, int64_t n, const void* positions, const void* velocities,
+                             int nx, int ny, double x0, double y0, double dx, double dy, double freestream_x,
+                             double freestream_y, int periodic_x, int periodic_y, void* grid_out) {
+  FB_REQUIRE(ctx && grid_out, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(n == 0 || (positions && velocities), FB_ERR_ARG, "NULL particle arrays");
+  FB_REQUIRE(n >= 0 && n < (1ll << 30), FB_ERR_ARG, "particle count out of range");
+  FB_REQUIRE(nx > 0 && ny > 0 && dx > 0.0 && dy > 0.0, FB_ERR_ARG, "grid spacing and resolution must be positive");
+  FB_REQUIRE(ny < 65536 && (int64_t)nx * ny <= (1ll << 28), FB_ERR_ARG, "grid of %d x %d cells is too large", nx,
+             ny);
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
+  return precision == FB_F64
+             ? fb::standalone_p2c<double>(c, n, positions, velocities, g, freestream_x, freestream_y, grid_out)
+             : fb::standalone_p2c<float>(c, n, positions, velocities, g, freestream_x, freestream_y, grid_out);
+}
+
+int fb_pic_cells_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* grid, const void* positions, int nx,
+                             int ny, double x0, double y0, double dx, double dy, int periodic_x, int periodic_y,
+                             void* velocities_out) {
+  FB_REQUIRE(ctx && grid, FB_ERR_ARG, "NULL argument");
+  FB_REQUIRE(n == 0 || (positions && velocities_out), FB_ERR_ARG, "NULL particle arrays");
+  FB_REQUIRE(n >= 0 && n < (1ll << 30), FB_ERR_ARG, "particle count out of range");
+  FB_REQUIRE(nx > 0 && ny > 0, FB_ERR_ARG, "grid axes must be nonempty");
+  FB_REQUIRE((int64_t)nx * ny <= (1ll << 28), FB_ERR_ARG, "grid of %d x %d cells is too large", nx, ny);
+  FB_REQUIRE(dx > 0.0 && dy > 0.0, FB_ERR_ARG, "grid spacing must be positive");
+  FB_REQUIRE(precision == FB_F64 || precision == FB_F32, FB_ERR_ARG, "precision must be FB_F64 or FB_F32");
+  auto* c = reinterpret_cast<fb::Ctx*>(ctx);
+  FB_HIP(hipSetDevice(c->device));
+  fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
+  return precision == FB_F64 ? fb::standalone_c2p<double>(c, n, grid, positions, g, velocities_out)
+                             : fb::standalone_c2p<float>(c, n, grid, positions, g, velocities_out);
 }
 
 }  // extern "C"

@@ -37,7 +37,7 @@ enum fb_status {
  * F64: FP64 storage and arithmetic (the plain drop-in form).
  * F32: FP32 storage with FP64 kernel arithmetic, like the reference's float32 scenarios.
  * F32_MIXED / F16_MIXED: populations stored shifted by their lattice weights (f - w) in
- *   FP32 / FP16 with FP32 arithmetic (the precision form). */
+ *   FP32 / FP16, collided in FP64 / FP32 arithmetic (the precision forms). */
 enum fb_precision { FB_F64 = 0, FB_F32 = 1, FB_F32_MIXED = 2, FB_F16_MIXED = 3 };
 
 typedef struct fb_ctx fb_ctx;
@@ -359,6 +359,18 @@ int fb_pic_particle_to_faces_noalloc(fb_ctx* ctx, int precision, int64_t n, cons
 int fb_pic_faces_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* u, const void* v,
                              const void* positions, int nx, int ny, double x0, double y0, double dx, double dy,
                              int periodic_x, int periodic_y, void* out);
+/* particle_to_grid (fb/solvers/_numba_adapter.py:35-71, kern/pic.py:22-65) on host arrays: positions and
+ * velocities (n, 2), grid_out (ny, nx, 2) cell-centred; the stencil is wrapped on a periodic axis and clamped
+ * to the edge cells otherwise; a cell of weight <= 1e-12 takes the freestream. Bit-identical to the
+ * reference's particle-order accumulation. */
+int fb_pic_particle_to_cells(fb_ctx* ctx, int precision, int64_t n, const void* positions, const void* velocities,
+                             int nx, int ny, double x0, double y0, double dx, double dy, double freestream_x,
+                             double freestream_y, int periodic_x, int periodic_y, void* grid_out);
+/* grid_to_particle (fb/solvers/_numba_adapter.py:74-105, kern/pic.py:67-102): grid (ny, nx, 2), positions and
+ * velocities_out (n, 2). */
+int fb_pic_cells_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* grid, const void* positions, int nx,
+                             int ny, double x0, double y0, double dx, double dy, int periodic_x, int periodic_y,
+                             void* velocities_out);
 
 /* ------------------------------------------------------------------- Serving ------ */
 /* No counterpart in the reference, which advances one scenario at a time in-process. These

@@ -1,9 +1,9 @@
 """Blended PIC/FLIP restated from kern/pic.py, fb/solvers/_numba_adapter.py:108-160,
 fb/solvers/pic_flip.py and fb/core/particle_population.py (test infrastructure only).
 
-Transfers are vectorised but keep the reference's accumulation order (particle-major, then
-the 3x3 offsets, via unbuffered np.add.at into accumulators of the positions' dtype) and
-its FP64 weight arithmetic.
+Transfers (face-centred, and the cell-centred pair of kern/pic.py:22-102) are vectorised but
+keep the reference's accumulation order (particle-major, then the 3x3 offsets, via unbuffered
+np.add.at into accumulators of the positions' dtype) and its FP64 weight arithmetic.
 
 Pinned by (tests/test_oracle_pic.py): a per-particle loop restating the numba kernels on
 tiny inputs, constant preservation and periodic wrap of the transfers
@@ -115,6 +115,62 @@ def faces_to_particle(u, v, positions, x0, y0, dx, dy, px=False, py=False):
     out[:, 1] = gather_face_component(v, positions, x0, y0, dx, dy, -0.5, 0.0, px, py, False, py)
     return out
 
+
+
+def _cell_stencil(positions, x0, y0, dx, dy, nx, ny, px, py):
+    """Targets and FP64 weights of the cell-centred 3x3 stencil, particle-major then (oy, ox) (kern/pic.py:40-53,
+    84-94): the stencil index wrapped on a periodic axis, clamped to [0, n - 1] otherwise."""
+    gx, gy, bx, by = _support(positions, x0, y0, dx, dy, -0.5, -0.5)
+    tgt, wt = [], []
+    for oy in range(3):
+        sy = by + oy
+        ty = np.mod(sy, ny) if py else np.clip(sy, 0, ny - 1)
+        wy = bspline(gy - sy)
+        for ox in range(3):
+            sx = bx + ox
+            tx = np.mod(sx, nx) if px else np.clip(sx, 0, nx - 1)
+            tgt.append(ty * nx + tx)
+            wt.append(wy * bspline(gx - sx))
+    return np.stack(tgt, axis=1), np.stack(wt, axis=1)
+
+
+def particle_to_grid(positions, velocities, x0, y0, dx, dy, nx, ny, freestream, px=False, py=False):
+    """kern/pic.py:22-65 behind fb/solvers/_numba_adapter.py:35-71: weights and momenta accumulated in the
+    positions' dtype in particle order (each FP64 contribution added in FP64 and rounded back), the cell velocity
+    their dtype quotient where the weight exceeds 1e-12 and the freestream elsewhere."""
+    dt = positions.dtype
+    tgt, wt = _cell_stencil(positions, x0, y0, dx, dy, nx, ny, px, py)
+    idx = tgt.ravel()
+    wt = wt.ravel()
+    w = np.zeros(ny * nx, dtype=dt)
+    mx = np.zeros(ny * nx, dtype=dt)
+    my = np.zeros(ny * nx, dtype=dt)
+    np.add.at(w, idx, wt)
+    np.add.at(mx, idx, wt * np.repeat(velocities[:, 0].astype(np.float64), 9))
+    np.add.at(my, idx, wt * np.repeat(velocities[:, 1].astype(np.float64), 9))
+    ok = w > 1.0e-12
+    den = np.where(ok, w, dt.type(1))
+    grid = np.empty((ny, nx, 2), dtype=dt)
+    grid[..., 0] = np.where(ok, mx / den, dt.type(freestream[0])).reshape(ny, nx)
+    grid[..., 1] = np.where(ok, my / den, dt.type(freestream[1])).reshape(ny, nx)
+    return grid
+
+
+def grid_to_particle(grid, positions, x0, y0, dx, dy, px=False, py=False):
+    """kern/pic.py:67-102 behind fb/solvers/_numba_adapter.py:74-105: FP64 sums in the fixed 3x3 order, times
+    1 / max(weight sum, 1e-12), rounded to the dtype."""
+    ny, nx, _ = grid.shape
+    tgt, wt = _cell_stencil(positions, x0, y0, dx, dy, nx, ny, px, py)
+    g = grid.reshape(ny * nx, 2).astype(np.float64)
+    vx = np.zeros(positions.shape[0])
+    vy = np.zeros(positions.shape[0])
+    ws = np.zeros(positions.shape[0])
+    for k in range(9):
+        vx = vx + wt[:, k] * g[tgt[:, k], 0]
+        vy = vy + wt[:, k] * g[tgt[:, k], 1]
+        ws = ws + wt[:, k]
+    inv = 1.0 / np.maximum(ws, 1.0e-12)
+    return np.stack((vx * inv, vy * inv), axis=1).astype(positions.dtype)
 
 # ---------------------------------------------------------------- population maintenance
 
