@@ -127,6 +127,26 @@ class fb_pic_stats(C.Structure):
     ]
 
 
+FB_DIAG_HISTOGRAM = 1024
+
+
+class fb_diag_params(C.Structure):
+    _fields_ = [
+        ("freestream_u", C.c_double), ("wake_threshold", C.c_double), ("wake_col0", C.c_int32),
+        ("down_col0", C.c_int32), ("angular_velocity_rad", C.c_double), ("diagonal_link", C.c_double),
+        ("centers_x", C.POINTER(C.c_double)), ("centers_y", C.POINTER(C.c_double)),
+    ]
+
+
+class fb_diag(C.Structure):
+    _fields_ = [
+        ("kinetic_energy", C.c_double), ("enstrophy", C.c_double), ("divergence_l2", C.c_double),
+        ("wake_width", C.c_double), ("recirculation_area", C.c_double), ("density_mean", C.c_double),
+        ("cut_link_adjacent_normal_speed", C.c_double), ("nonfinite", C.c_int64), ("fluid_cells", C.c_int64),
+        ("empty_cells", C.c_int64), ("underfilled_cells", C.c_int64), ("max_particles", C.c_int64),
+    ]
+
+
 class fb_server_options(C.Structure):
     _fields_ = [("sf_workers", C.c_int32), ("lbm_chunk", C.c_int32), ("use_graphs", C.c_int32),
                 ("sf_batch", C.c_int32), ("sf_gather_us", C.c_double)]
@@ -235,6 +255,12 @@ _SIGNATURES: dict[str, tuple] = {
                                                    _P, _P, _P]),
     "fb_pic_faces_to_particle": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _P]),
+    "fb_mac_diagnostics": (C.c_int, [_P, C.POINTER(fb_diag_params), C.POINTER(fb_diag)]),
+    "fb_lbm_diagnostics": (C.c_int, [_P, C.c_double, C.c_int, C.POINTER(fb_diag_params), C.POINTER(fb_diag)]),
+    "fb_pic_diagnostics": (C.c_int, [_P, C.POINTER(fb_diag_params), C.POINTER(fb_diag), _P, _P]),
+    "fb_mac_sample_velocity": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "fb_lbm_sample_velocity": (C.c_int, [_P, C.c_double, C.c_int, C.c_int64, _P, _P]),
+    "fb_pic_sample_velocity": (C.c_int, [_P, C.c_int64, _P, _P]),
     "fb_pic_particle_to_cells": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                            C.c_int, _P]),

@@ -15,6 +15,7 @@
 #include "lbm_batch.h"
 #include <vector>
 
+#include "diag.h"
 #include "geometry.h"
 
 namespace fb {
@@ -1923,6 +1924,52 @@ static int uncover_impl(Lattice& L, const fb_pose& pose, const double* feq_u) {
   return FB_OK;
 }
 
+// Diagnostics and velocity samples of the physical velocity (lattice velocity times vscale) and density,
+// formed by the macroscopic kernel into device buffers and reduced there (diag.h).
+template <class P>
+static int diag_impl(Lattice& L, double vscale, int export_f64, const fb_diag_params* prm, fb_diag* out,
+                     int64_t n, const double* points, void* samples) {
+  using S = typename P::S;
+  using B = typename P::B;
+  B* d_rho = nullptr;
+  B* d_vel = nullptr;
+  FB_HIP(dev_alloc((void**)&d_rho, L.N * sizeof(B)));
+  FB_HIP(dev_alloc((void**)&d_vel, 2 * L.N * sizeof(B)));
+  hipLaunchKernelGGL(lbm_macro_kernel<P>, dim3((L.dom.nx + 255) / 256, L.dom.ny), dim3(256), 0, L.ctx->stream,
+                     L.dom.nx, L.dom.ny, (const S*)L.PRE[L.pre], vscale, d_rho, d_vel);
+  int rc = hipGetLastError() == hipSuccess ? FB_OK : FB_ERR_HIP;
+  if (rc) set_error("macroscopic kernel launch failed");
+  DiagField f;
+  f.nx = L.dom.nx;
+  f.ny = L.dom.ny;
+  f.px = L.dom.periodic_x;
+  f.py = L.dom.periodic_y;
+  f.x0 = L.dom.x0;
+  f.y0 = L.dom.y0;
+  f.dx = (L.dom.x1 - L.dom.x0) / L.dom.nx;
+  f.dy = (L.dom.y1 - L.dom.y0) / L.dom.ny;
+  f.kind = kDiagCells;
+  f.storage_f64 = sizeof(B) == sizeof(double);
+  f.arith_f64 = export_f64 || f.storage_f64;
+  f.u = d_vel;
+  f.v = d_vel + 1;
+  f.stride = 2;
+  f.solid = L.G[L.g].solid;
+  if (!rc && prm) {
+    f.rho = d_rho;
+    f.sdf = L.G[L.g].sdf;
+    f.pivot_x = L.foil.pivot_x;
+    f.pivot_y = L.foil.pivot_y;
+    rc = field_diagnostics(L.ctx, f, *prm, out);
+  } else if (!rc) {
+    rc = sample_cell_velocity(L.ctx, f, n, points, samples);
+  }
+  (void)hipStreamSynchronize(L.ctx->stream);
+  (void)dev_free(d_rho);
+  (void)dev_free(d_vel);
+  return rc;
+}
+
 #define FB_DISPATCH(prec, fn, ...)                        \
   switch (prec) {                                         \
     case FB_F64: return fn<Prec<FB_F64>>(__VA_ARGS__);     \
@@ -1947,6 +1994,10 @@ static int equilibrium_dispatch(Lattice& L, const void* r, const void* v) {
   FB_DISPATCH(L.prec, equilibrium_impl, L, r, v);
 }
 static int macro_dispatch(Lattice& L, double v, void* a, void* b) { FB_DISPATCH(L.prec, macro_impl, L, v, a, b); }
+static int diag_dispatch(Lattice& L, double v, int e, const fb_diag_params* p, fb_diag* o, int64_t n,
+                         const double* pts, void* smp) {
+  FB_DISPATCH(L.prec, diag_impl, L, v, e, p, o, n, pts, smp);
+}
 static int set_state_dispatch(Lattice& L, const void* a, const void* b) { FB_DISPATCH(L.prec, set_state_impl, L, a, b); }
 static int get_state_dispatch(Lattice& L, void* a, void* b) { FB_DISPATCH(L.prec, get_state_impl, L, a, b); }
 
@@ -2560,6 +2611,25 @@ int fb_lbm_run_fixed(fb_lbm* lbm, const fb_lbm_params* params, int steps) {
   FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
   FB_HIP(hipSetDevice(L->ctx->device));
   return fb::advance_dispatch(*L, *params, steps, nullptr, nullptr, false);
+}
+
+int fb_lbm_diagnostics(fb_lbm* lbm, double velocity_scale, int export_f64, const fb_diag_params* params,
+                       fb_diag* out) {
+  FB_REQUIRE(lbm && params && out, FB_ERR_ARG, "NULL argument");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  *out = fb_diag{};
+  return fb::diag_dispatch(*L, velocity_scale, export_f64, params, out, 0, nullptr, nullptr);
+}
+
+int fb_lbm_sample_velocity(fb_lbm* lbm, double velocity_scale, int export_f64, int64_t n, const double* points,
+                           void* out) {
+  FB_REQUIRE(lbm, FB_ERR_ARG, "NULL handle");
+  auto* L = reinterpret_cast<Lattice*>(lbm);
+  FB_REQUIRE(L->G[L->g].valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(L->ctx->device));
+  return fb::diag_dispatch(*L, velocity_scale, export_f64, nullptr, nullptr, n, points, out);
 }
 
 }  // extern "C"

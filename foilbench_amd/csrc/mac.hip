@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "diag.h"
 #include "mac.h"
 
 namespace fb {
@@ -1441,6 +1442,42 @@ int fb_mac_get_stats(fb_mac* mac, double wall_omega_rad, fb_mac_stats* stats) {
   }
   FB_HIP(hipSetDevice(M->ctx->device));
   return MAC_DISPATCH(M, fb::mac_stats, *M, wall_omega_rad, stats);
+}
+
+// The exported cell velocity of the grid: face averages in the face dtype (fb/solvers/stable_fluids.py
+// cell_velocity), with the grid's mask.
+static fb::DiagField mac_diag_field(const fb::MacGrid& M) {
+  fb::DiagField f;
+  f.nx = M.g.nx;
+  f.ny = M.g.ny;
+  f.px = M.g.px;
+  f.py = M.g.py;
+  f.x0 = M.g.x0;
+  f.y0 = M.g.y0;
+  f.dx = M.g.dx;
+  f.dy = M.g.dy;
+  f.kind = fb::kDiagFaces;
+  f.storage_f64 = f.arith_f64 = M.sz == sizeof(double);
+  f.u = M.U[M.cur];
+  f.v = M.V[M.cur];
+  f.solid = M.solid;
+  return f;
+}
+
+int fb_mac_diagnostics(fb_mac* mac, const fb_diag_params* params, fb_diag* out) {
+  FB_REQUIRE(mac && params && out, FB_ERR_ARG, "NULL argument");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_REQUIRE(M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(M->ctx->device));
+  *out = fb_diag{};
+  return fb::field_diagnostics(M->ctx, mac_diag_field(*M), *params, out);
+}
+
+int fb_mac_sample_velocity(fb_mac* mac, int64_t n, const double* points, void* out) {
+  FB_REQUIRE(mac, FB_ERR_ARG, "NULL handle");
+  auto* M = reinterpret_cast<MacGrid*>(mac);
+  FB_HIP(hipSetDevice(M->ctx->device));
+  return fb::sample_cell_velocity(M->ctx, mac_diag_field(*M), n, points, out);
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <vector>
 
+#include "diag.h"
 #include "mac.h"
 
 namespace fb {
@@ -2124,6 +2125,46 @@ int fb_pic_cells_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
   fb::PicGeom g = fb::transfer_geom(nx, ny, x0, y0, dx, dy, periodic_x != 0, periodic_y != 0);
   return precision == FB_F64 ? fb::standalone_c2p<double>(c, n, grid, positions, g, velocities_out)
                              : fb::standalone_c2p<float>(c, n, grid, positions, g, velocities_out);
+}
+
+// The exported grid velocity of the handle (the cell planes of the last grid update) and its mask.
+static fb::DiagField pic_diag_field(const fb::Pic& P) {
+  fb::DiagField f;
+  f.nx = P.g.nx;
+  f.ny = P.g.ny;
+  f.px = P.g.px;
+  f.py = P.g.py;
+  f.x0 = P.g.x0;
+  f.y0 = P.g.y0;
+  f.dx = P.g.dx;
+  f.dy = P.g.dy;
+  f.kind = fb::kDiagCells;
+  f.storage_f64 = f.arith_f64 = P.sz == sizeof(double);
+  f.u = P.GU;
+  f.v = P.GV;
+  f.stride = 1;
+  f.solid = P.M->solid;
+  return f;
+}
+
+int fb_pic_diagnostics(fb_pic* pic, const fb_diag_params* params, fb_diag* out, int64_t* histogram,
+                       int32_t* counts) {
+  FB_REQUIRE(pic && params && out && histogram, FB_ERR_ARG, "NULL argument");
+  auto* P = reinterpret_cast<fb::Pic*>(pic);
+  FB_REQUIRE(P->M->geom_valid, FB_ERR_STATE, "no geometry has been set");
+  FB_HIP(hipSetDevice(P->ctx->device));
+  *out = fb_diag{};
+  const fb::DiagField f = pic_diag_field(*P);
+  int rc = fb::field_diagnostics(P->ctx, f, *params, out);
+  if (!rc) rc = fb::particle_census(P->ctx, f, P->n, P->pos, out, histogram, counts);
+  return rc;
+}
+
+int fb_pic_sample_velocity(fb_pic* pic, int64_t n, const double* points, void* out) {
+  FB_REQUIRE(pic, FB_ERR_ARG, "NULL handle");
+  auto* P = reinterpret_cast<fb::Pic*>(pic);
+  FB_HIP(hipSetDevice(P->ctx->device));
+  return fb::sample_cell_velocity(P->ctx, pic_diag_field(*P), n, points, out);
 }
 
 }  // extern "C"

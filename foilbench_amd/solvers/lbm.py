@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ._diagnostics import DiagParams, sample_points
 from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
-from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
                            NumericalFailure, RestartState, ReynoldsOutcome, Scenario, SolverInfo, StepReport)
 from ..core.validation import validate_advance_request, validate_canonical_import, validate_restart_state
@@ -440,9 +440,14 @@ class LBMSolver:
         return rho.astype(sc.dtype, copy=False), vel.astype(sc.dtype, copy=False)
 
     def sample_velocity(self, points: np.ndarray) -> np.ndarray:
+        # fb/solvers/lbm.py:888-890: the physical velocity sampled bilinearly, on the device
         sc, _ = self._require()
-        _, vel = self._macroscopic()
-        return sample_vector(vel, points, sc.domain)
+        pts = sample_points(points)
+        out = np.empty(pts.shape, dtype=sc.dtype)
+        N.check(self._lib().fb_lbm_sample_velocity(self._handle, self._reference_speed / self._lattice_speed,
+                                                   int(sc.dtype == np.float64), pts.shape[0], N.ptr(pts),
+                                                   N.ptr(out)))
+        return out
 
     def export_state(self) -> CanonicalFlowState:
         # fb/solvers/lbm.py:892-915
@@ -517,20 +522,25 @@ class LBMSolver:
     def diagnostics(self) -> Diagnostics:
         # fb/solvers/lbm.py:1103-1146 (run-level metrics on the exported fields)
         sc, _ = self._require()
-        rho, vel = self._macroscopic()
-        _, solid = self.geometry_fields()
+        d = N.fb_diag()
+        prm = DiagParams(sc, self._control.angular_velocity_degrees)
+        N.check(self._lib().fb_lbm_diagnostics(self._handle, self._reference_speed / self._lattice_speed,
+                                               int(sc.dtype == np.float64), prm.ref(), C.byref(d)))
         values = {
             "time": self._time,
             "requested_reynolds": self._reynolds,
-            "kinetic_energy": kinetic_energy(vel),
-            "enstrophy": enstrophy(vel, sc.domain),
-            "divergence_l2": divergence_l2(vel, sc.domain),
+            "kinetic_energy": d.kinetic_energy,
+            "enstrophy": d.enstrophy,
+            "divergence_l2": d.divergence_l2,
+            # interpolated bounce-back reflects every population aimed through a cut link, so the wall-normal
+            # through-flux is zero by construction; the adjacent-cell speed is reported beside it
             "solid_leakage": 0.0,
-            "density_mean": float(np.mean(rho)),
-            "density_drift": float(np.mean(rho) - self._density_initial),
+            "cut_link_adjacent_normal_speed": d.cut_link_adjacent_normal_speed,
+            "density_mean": d.density_mean,
+            "density_drift": d.density_mean - self._density_initial,
             "effective_reynolds": self._effective_reynolds,
-            "wake_width": wake_width(vel, sc.domain, sc.foil.pivot[0], sc.foil.chord, sc.freestream[0], solid),
-            "recirculation_area": recirculation_area(vel, sc.domain, sc.foil.pivot[0], solid),
+            "wake_width": d.wake_width,
+            "recirculation_area": d.recirculation_area,
         }
         if not all(np.isfinite(v) for v in values.values()):
             raise NumericalFailure("nonfinite_state", "LBM produced non-finite diagnostics")

@@ -16,13 +16,13 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ._diagnostics import DiagParams, fluid_counts_from_histogram, sample_points
 from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
-from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
                            InteractiveTuning, NumericalFailure, RestartState, ReynoldsOutcome, Scenario, SolverInfo,
                            StepReport)
-from ..core.population import maintain_particle_population, particle_cell_counts
+from ..core.population import maintain_particle_population
 from ..core.rng import PCG32
 from ..core.validation import validate_advance_request, validate_canonical_import, validate_restart_state
 from .stable_fluids import initial_velocity
@@ -522,8 +522,12 @@ class PicFlipSolver:
 
     # ------------------------------------------------------------------ state exchange
     def sample_velocity(self, points: np.ndarray) -> np.ndarray:
-        sc, _ = self._require()
-        return sample_vector(self.grid_velocity(), points, sc.domain)
+        # fb/solvers/pic_flip.py:1010-1012: the grid velocity sampled bilinearly, on the device
+        self._require()
+        pts = sample_points(points)
+        out = np.empty(pts.shape, dtype=self._dtype())
+        N.check(self._lib().fb_pic_sample_velocity(self._handle, pts.shape[0], N.ptr(pts), N.ptr(out)))
+        return out
 
     def export_state(self) -> CanonicalFlowState:
         # fb/solvers/pic_flip.py:1014-1033
@@ -587,30 +591,38 @@ class PicFlipSolver:
     def diagnostics(self) -> Diagnostics:
         # fb/solvers/pic_flip.py:1115-1169
         sc, geometry = self._require()
-        pos, _ = self.particles()
-        grid = self.grid_velocity()
-        solid = self.solid_mask()
-        counts = particle_cell_counts(pos, sc.domain).reshape(sc.domain.ny, sc.domain.nx)
-        fc = counts[~solid]
+        d = N.fb_diag()
+        hist = np.zeros(N.FB_DIAG_HISTOGRAM, np.int64)
+        N.check(self._lib().fb_pic_diagnostics(self._handle, DiagParams(sc).ref(), C.byref(d), N.ptr(hist), None))
+
+        def counts():
+            c = np.empty((sc.domain.ny, sc.domain.nx), np.int32)
+            N.check(self._lib().fb_pic_diagnostics(self._handle, DiagParams(sc).ref(), C.byref(N.fb_diag()),
+                                                   N.ptr(np.zeros_like(hist)), N.ptr(c)))
+            return c
+
+        # the fluid cells' particle counts (fb/core/particle_population.py:34-41 at the mask), for NumPy's
+        # percentiles of the same values
+        fc = fluid_counts_from_histogram(hist, counts, self.solid_mask)
         st = self._stats(self._control.angle_degrees)
         values = {
             "time": self._time,
             "requested_reynolds": self._reynolds,
-            "kinetic_energy": kinetic_energy(grid),
-            "enstrophy": enstrophy(grid, sc.domain),
-            "divergence_l2": divergence_l2(grid, sc.domain),
-            "particle_count": float(pos.shape[0]),
+            "kinetic_energy": d.kinetic_energy,
+            "enstrophy": d.enstrophy,
+            "divergence_l2": d.divergence_l2,
+            "particle_count": float(st.particle_count),
             "unsupported_face_fraction": self._unsupported_face_fraction,
-            "empty_fluid_cell_fraction": float(np.count_nonzero(fc == 0)) / fc.size,
-            "underfilled_fluid_cell_fraction": float(np.count_nonzero(fc < 4)) / fc.size,
+            "empty_fluid_cell_fraction": float(d.empty_cells) / d.fluid_cells,
+            "underfilled_fluid_cell_fraction": float(d.underfilled_cells) / d.fluid_cells,
             "p05_particles_per_fluid_cell": float(np.percentile(fc, 5.0)),
             "p95_particles_per_fluid_cell": float(np.percentile(fc, 95.0)),
-            "max_particles_per_fluid_cell": float(np.max(fc)),
+            "max_particles_per_fluid_cell": float(d.max_particles),
             "reseeded_last_step": float(self._reseeded_last_step),
             "swept_collisions_last_step": float(self._swept_collisions_last_step),
             "particles_inside_solid": float(st.particles_inside),
-            "wake_width": wake_width(grid, sc.domain, sc.foil.pivot[0], sc.foil.chord, sc.freestream[0], solid),
-            "recirculation_area": recirculation_area(grid, sc.domain, sc.foil.pivot[0], solid),
+            "wake_width": d.wake_width,
+            "recirculation_area": d.recirculation_area,
         }
         if self._native_divergence_linf is not None:
             values["divergence_linf"] = self._native_divergence_linf

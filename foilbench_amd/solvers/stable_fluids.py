@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from .. import _native as N
+from ._diagnostics import DiagParams, sample_points
 from ..core.forces import lift_drag
 from ..core.geometry import NacaFoil, cell_centers, domain_struct, foil_struct, pose_struct
-from ..core.metrics import divergence_l2, enstrophy, kinetic_energy, recirculation_area, sample_vector, wake_width
 from ..core.models import (CanonicalFlowState, ControlState, Diagnostics, ImportOutcome, ImportReport,
                            InteractiveTuning, NumericalFailure, RestartState, ReynoldsOutcome, Scenario, SolverInfo,
                            StepReport)
@@ -480,8 +480,12 @@ class StableFluidsSolver:
         })
 
     def sample_velocity(self, points: np.ndarray) -> np.ndarray:
+        # fb/solvers/stable_fluids.py:609-611: the cell velocity sampled bilinearly, on the device
         sc, _ = self._require()
-        return sample_vector(self.cell_velocity(), points, sc.domain)
+        pts = sample_points(points)
+        out = np.empty(pts.shape, dtype=sc.dtype)
+        N.check(self._lib().fb_mac_sample_velocity(self._handle, pts.shape[0], N.ptr(pts), N.ptr(out)))
+        return out
 
     def export_state(self) -> CanonicalFlowState:
         # fb/solvers/stable_fluids.py:613-634
@@ -535,19 +539,19 @@ class StableFluidsSolver:
     def diagnostics(self) -> Diagnostics:
         # fb/solvers/stable_fluids.py:708-738
         sc, _ = self._require()
-        vel = self.cell_velocity()
-        solid = self.solid_mask()
         st = self._stats(self._control)
+        d = N.fb_diag()
+        N.check(self._lib().fb_mac_diagnostics(self._handle, DiagParams(sc).ref(), C.byref(d)))
         values = {
             "time": self._time,
             "requested_reynolds": self._reynolds,
-            "kinetic_energy": kinetic_energy(vel),
-            "enstrophy": enstrophy(vel, sc.domain),
-            "divergence_l2": divergence_l2(vel, sc.domain),
+            "kinetic_energy": d.kinetic_energy,
+            "enstrophy": d.enstrophy,
+            "divergence_l2": d.divergence_l2,
             "divergence_linf": st.divergence_linf,
             "solid_leakage": st.solid_leakage,
-            "wake_width": wake_width(vel, sc.domain, sc.foil.pivot[0], sc.foil.chord, sc.freestream[0], solid),
-            "recirculation_area": recirculation_area(vel, sc.domain, sc.foil.pivot[0], solid),
+            "wake_width": d.wake_width,
+            "recirculation_area": d.recirculation_area,
         }
         if not all(np.isfinite(v) for v in values.values()):
             raise NumericalFailure("nonfinite_state", "Stable Fluids produced non-finite diagnostics")

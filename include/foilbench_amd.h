@@ -372,6 +372,56 @@ int fb_pic_cells_to_particle(fb_ctx* ctx, int precision, int64_t n, const void* 
                              int ny, double x0, double y0, double dx, double dy, int periodic_x, int periodic_y,
                              void* velocities_out);
 
+/* ----------------------------------------------------------------- Diagnostics ---- */
+/* Run-level diagnostics computed on the device from the solver's fields (fb/core/metrics.py:24-104,
+ * fb/solvers/stable_fluids.py:708-738, fb/solvers/lbm.py:1054-1146, fb/solvers/pic_flip.py:1115-1169), and
+ * sample_velocity (fb/core/interpolation.py:9-79) of each solver. Per-cell terms are formed in the dtype of
+ * the field the reference exports; counts and maxima equal the reference's on that field, means are FP64
+ * sums in a fixed order. */
+#define FB_DIAG_HISTOGRAM 1024
+
+typedef struct fb_diag_params {
+  double freestream_u;          /* scenario.freestream[0] */
+  double wake_threshold;        /* 0.1 * |freestream_u| (fb/core/metrics.py:64-87) */
+  int32_t wake_col0;            /* first column whose centre lies beyond pivot_x + chord (nx: none) */
+  int32_t down_col0;            /* first column whose centre lies beyond pivot_x (nx: none) */
+  double angular_velocity_rad;  /* lattice cut links: deg2rad of the control's angular velocity */
+  double diagonal_link;         /* lattice cut links: hypot(dx, dy) */
+  const double* centers_x;      /* lattice cut links: host, nx cell-centre x (linspace of the domain) */
+  const double* centers_y;      /* lattice cut links: host, ny cell-centre y */
+} fb_diag_params;
+
+typedef struct fb_diag {
+  double kinetic_energy;     /* 0.5 mean |u|^2 */
+  double enstrophy;          /* 0.5 mean (dv/dx - du/dy)^2, np.gradient differences */
+  double divergence_l2;      /* rms of du/dx + dv/dy */
+  double wake_width;         /* rows with a deficit above the threshold in a wake column (fluid), times dy */
+  double recirculation_area; /* fluid cells downstream of the pivot with u < 0, times dx dy */
+  double density_mean;       /* lattice: mean density */
+  double cut_link_adjacent_normal_speed; /* lattice: fb/solvers/lbm.py:1054-1101 */
+  int64_t nonfinite;         /* non-finite cell velocity components */
+  int64_t fluid_cells;       /* PIC/FLIP census over fluid cells (particle_cell_counts at the mask): */
+  int64_t empty_cells;       /*   cells without particles */
+  int64_t underfilled_cells; /*   cells with fewer than 4 */
+  int64_t max_particles;     /*   largest count */
+} fb_diag;
+
+/* Stable Fluids cell velocity (face averages) and mask. */
+int fb_mac_diagnostics(fb_mac* mac, const fb_diag_params* params, fb_diag* out);
+/* Lattice physical velocity (lattice velocity times velocity_scale) and density, with cut-link speeds;
+ * export_f64: the exported field is float64 (per-cell terms in FP64 on the lattice's FP32 values). */
+int fb_lbm_diagnostics(fb_lbm* lbm, double velocity_scale, int export_f64, const fb_diag_params* params,
+                       fb_diag* out);
+/* PIC/FLIP grid velocity and particle census: histogram (FB_DIAG_HISTOGRAM int64, the last entry counting
+ * every fluid cell at or above it) and, when counts is non-NULL, the (ny, nx) particle counts. */
+int fb_pic_diagnostics(fb_pic* pic, const fb_diag_params* params, fb_diag* out, int64_t* histogram,
+                       int32_t* counts);
+/* Bilinear sample of the cell velocity at n points (n, 2) float64 into out (n, 2) of the exported dtype. */
+int fb_mac_sample_velocity(fb_mac* mac, int64_t n, const double* points, void* out);
+int fb_lbm_sample_velocity(fb_lbm* lbm, double velocity_scale, int export_f64, int64_t n, const double* points,
+                           void* out);
+int fb_pic_sample_velocity(fb_pic* pic, int64_t n, const double* points, void* out);
+
 /* ------------------------------------------------------------------- Serving ------ */
 /* No counterpart in the reference, which advances one scenario at a time in-process. These
  * entry points run many scenarios of uneven size concurrently on one GPU. */

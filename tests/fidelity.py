@@ -24,7 +24,7 @@ from pathlib import Path
 import numpy as np
 
 from foilbench_amd.core.geometry import NacaFoil
-from foilbench_amd.core.metrics import kinetic_energy, vorticity
+from oracle.metrics import kinetic_energy, vorticity
 from foilbench_amd.core.models import ControlState
 from foilbench_amd.core.scenario import load_scenario
 
