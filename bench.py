@@ -130,10 +130,11 @@ def _form_errors(nx: int, ny: int, steps: int, forms=("f32", "f32-mixed", "f16-m
 
 
 def _serving_lattice_errors() -> dict:
-    """The serving form's lattices (shifted FP32 storage, FP64 collision, one batched launch per step) against
-    the plain FP64 form after SERVING_ERROR_STEPS steps, for one request of each grid shape of the serving mix
-    (NACA 2412 at AoA 12, Re 1000; the lattice viscosity clamp of the solver lifecycle applies at the small
-    shapes): relative L2 velocity error over the fluid cells."""
+    """The serving form's lattices (shifted FP32 storage, FP64 collision) against the plain FP64 form after
+    SERVING_ERROR_STEPS steps, for one request of each grid shape of the serving mix (NACA 2412 at AoA 12,
+    Re 1000; the lattice viscosity clamp of the solver lifecycle applies at the small shapes): relative L2
+    velocity error over the fluid cells. Each lattice is stepped on its own stream; the request loop's batched
+    launches compute the same bits (tests/test_gpu_serving.py)."""
     import ctypes as C
 
     import numpy as np
@@ -150,7 +151,7 @@ def _serving_lattice_errors() -> dict:
         for form in ("plain", "serving"):
             s, handle, params, _ = _build(req, form, ctx)
             try:
-                N.check(ctx.lib.fb_lbm_run_batch((C.c_void_p * 1)(handle), C.byref(params), 1, req.steps, 32, None))
+                N.check(ctx.lib.fb_lbm_run_fixed(handle, C.byref(params), req.steps))
                 ctx.synchronize()
                 _, v = s._macroscopic()
                 vel[form] = np.asarray(v, np.float64)

@@ -168,8 +168,8 @@ def test_lbm_rollback_restores_state_bitwise():
 
 @pytest.mark.parametrize("form,tol", [("f32-mixed", 2e-6), ("f16-mixed", 3e-3)])
 def test_lbm_mixed_forms_track_fp64(form, tol):
-    """Shifted FP32 / FP16 storage with FP32 arithmetic against the FP64 run: relative L2
-    error of the lattice velocity over fluid cells after 4 advances."""
+    """Shifted FP32 storage (FP64 collision) and shifted FP16 storage (FP32 collision) against the FP64 run:
+    relative L2 error of the lattice velocity over fluid cells after 4 advances."""
     sc = make_scenario(precision="float32", **CASES["static"])
     sc64 = make_scenario(precision="float64", **CASES["static"])
     a = _solver(form)
