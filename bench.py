@@ -55,7 +55,7 @@ def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
     # fused pairs, interior rows per thread and the special-cell blocks are the autotuner's picks for this
     # lattice and storage form (none of them changes a result bit)
     rec = tuner.lbm(case.nx, case.ny, name)
-    tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks")}
+    tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks", "nt")}
     lat = Lattice(case, prec, ctx)
     try:
         lat.run(warmup)
@@ -79,7 +79,7 @@ def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
         "bytes_per_cell": BYTES_PER_CELL[prec],
         "tuned_step_shape": tuned,
         "kernel": (f"lbm_pair_kernel<Prec<{pname}>, 16>" if tuned["fuse"]
-                   else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}>"),
+                   else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}, {'true' if tuned['nt'] == 1 else 'false'}>"),
         "steps_per_launch": per_launch,
         "launch_ms": avg_ms * per_launch,
         "event_launches": launches,

@@ -159,6 +159,7 @@ FB_TUNE_LBM_LIST_BLOCKS = 1
 FB_TUNE_LBM_FUSE = 2
 FB_TUNE_LBM_ROWS = 3
 FB_TUNE_DST_PPT = 4
+FB_TUNE_LBM_NT = 5
 
 
 class fb_server_result(C.Structure):

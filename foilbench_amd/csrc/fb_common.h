@@ -108,6 +108,7 @@ enum TuneParam {
   TUNE_LBM_FUSE = 2,
   TUNE_LBM_ROWS = 3,
   TUNE_DST_PPT = 4,
+  TUNE_LBM_NT = 5,
   TUNE_COUNT
 };
 int tune_get(int param, int nx, int ny, int precision, int grids);

@@ -83,6 +83,27 @@ __device__ __forceinline__ void st(S* p, C v) { *p = (S)v; }
 template <>
 __device__ __forceinline__ void st<__half, float>(__half* p, float v) { *p = __float2half_rn(v); }
 
+// Streaming forms of the population loads and stores (NT): nontemporal, so a lattice larger than the
+// caches streams through them without evicting what the step reuses (flags, near-wall data).
+template <bool NT, class S>
+__device__ __forceinline__ S ld_pop(const S* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <>
+__device__ __forceinline__ __half ld_pop<true, __half>(const __half* p) {
+  return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
+}
+template <bool NT, class S, class C>
+__device__ __forceinline__ void st_pop(S* p, C v) {
+  if constexpr (NT) __builtin_nontemporal_store((S)v, p);
+  else *p = (S)v;
+}
+template <>
+__device__ __forceinline__ void st_pop<true, __half, float>(__half* p, float v) {
+  __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), reinterpret_cast<unsigned short*>(p));
+}
+
 // Round a compute value to the storage grid (only for the reference-faithful forms).
 template <class P>
 __device__ __forceinline__ typename P::C rnd(typename P::C v) {
@@ -796,7 +817,7 @@ __device__ __forceinline__ void put_step_stats(unsigned long long* red, double m
 // straight pull of every cell, its flags, next-step flags and sponge weights are loaded before any
 // cell is computed, so the cells share one round of loads; flagged cells are left to the list
 // blocks. step_cell's arithmetic for straight cells (whose own flags are clear, so no reset).
-template <class P, int ROWS>
+template <class P, int ROWS, bool NT>
 __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, int x, int y0, double& mn,
                                                double& vmax, double& exc, long long& nf, long long& bad,
                                                long long& nfu) {
@@ -816,7 +837,7 @@ __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, 
       const size_t i = (size_t)y * a.nx + x;
 #pragma unroll
       for (int d = 0; d < 9; ++d)
-        v[r][d] = a.fin[(size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d])];
+        v[r][d] = ld_pop<NT>(&a.fin[(size_t)d * N + (size_t)(y - D2Q9::cy[d]) * a.nx + (x - D2Q9::cx[d])]);
       fl[r] = a.flags[i];
       fnext[r] = a.flags_next[i];
       spy[r] = a.sponge_y[y];
@@ -841,7 +862,7 @@ __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, 
     if constexpr (P::shift) rho = collide_shifted<P>(a, g, post);
     else rho = collide<P>(a, g, post);
 #pragma unroll
-    for (int d = 0; d < 9; ++d) st<S, C>(a.fout + (size_t)d * N + i, post[d]);
+    for (int d = 0; d < 9; ++d) st_pop<NT, S, C>(a.fout + (size_t)d * N + i, post[d]);
     if (fnext[r] & FL_NEAR) st<S, C>(a.rho_out + i, rho);
   }
 }
@@ -850,7 +871,7 @@ __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, 
 // run in the first blocks of the grid, so their longer dependent chains start early and do not
 // diverge the interior waves; interior blocks take the straight pull.
 // Block `blk` of the `nblk` blocks that cover lattice `a` for one step.
-template <class P, int ROWS>
+template <class P, int ROWS, bool NT>
 __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsigned blk, unsigned nblk) {
   double mn = INFINITY, vmax = 0.0, exc = 0.0;
   long long nf = 0, bad = 0, nfu = 0;
@@ -877,7 +898,7 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
     const unsigned gx = (unsigned)a.int_gx;
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
     const int y0 = 1 + ROWS * (int)(bi / gx);
-    if (ROWS == 1 || a.prime) {
+    if ((ROWS == 1 && !NT) || a.prime) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         const int y = y0 + r;
@@ -886,17 +907,17 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
                        mn, vmax, exc, nf, bad, nfu);
       }
     } else {
-      interior_cells<P, ROWS>(a, x, y0, mn, vmax, exc, nf, bad, nfu);
+      interior_cells<P, ROWS, NT>(a, x, y0, mn, vmax, exc, nf, bad, nfu);
     }
   }
   if (a.write_pre) put_step_stats(a.red, mn, vmax, exc, nf, bad, nfu);
 }
 
 // One lattice step with ROWS interior rows per block (a kernel of its own per ROWS, so that each
-// gets its own register allocation).
-template <class P, int ROWS>
+// gets its own register allocation); NT streams the interior populations past the caches.
+template <class P, int ROWS, bool NT>
 __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
-  lbm_block<P, ROWS>(a, blockIdx.x, gridDim.x);
+  lbm_block<P, ROWS, NT>(a, blockIdx.x, gridDim.x);
 }
 
 // Blocks of one lattice step.
@@ -922,7 +943,7 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const LbmArgs<typename P
   const int nblk = lbm_step_blocks(a);
   const int local = b - start[lo];
   if (a.skip || local >= nblk) return;
-  lbm_block<P, 1>(a, (unsigned)local, (unsigned)nblk);
+  lbm_block<P, 1, false>(a, (unsigned)local, (unsigned)nblk);
 }
 
 // Postcondition statistics of the cells a thread has stepped.
@@ -1537,6 +1558,15 @@ static int int_rows(const Lattice& L) {
   return L.N >= ((size_t)1 << 21) ? 2 : 1;
 }
 
+// Nontemporal population traffic in the single-step kernel: on by default once the two population
+// copies outgrow the 256 MB Infinity Cache (measured at 4096 x 2048 FP32, 140 -> 126 us per step), off
+// below it, where the next step finds the populations in the cache; tuned per shape, no result changes.
+static bool stream_pops(const Lattice& L) {
+  const int t = tune_get(TUNE_LBM_NT, L.dom.nx, L.dom.ny, L.prec, 1);
+  if (t == 0 || t == 1) return t == 1;
+  return 2 * 9 * L.N * L.ssize > ((size_t)192 << 20);
+}
+
 static int other3(int a, int b) {
   for (int k = 0; k < 3; ++k)
     if (k != a && k != b) return k;
@@ -1605,10 +1635,14 @@ static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool pri
   using S = typename P::S;
   const LbmArgs<S> a =
       step_args<S>(L, p, fin_idx, prime, fout_idx, o_in, o_out, g_cur, g_next, write_pre, pre_out, do_reset);
-  if (a.int_rows == 2)
-    hipLaunchKernelGGL((lbm_step_kernel<P, 2>), dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
-  else
-    hipLaunchKernelGGL((lbm_step_kernel<P, 1>), dim3(lbm_step_blocks(a)), dim3(256), 0, L.ctx->stream, a);
+  const dim3 grid(lbm_step_blocks(a));
+  if (stream_pops(L)) {
+    if (a.int_rows == 2) hipLaunchKernelGGL((lbm_step_kernel<P, 2, true>), grid, dim3(256), 0, L.ctx->stream, a);
+    else hipLaunchKernelGGL((lbm_step_kernel<P, 1, true>), grid, dim3(256), 0, L.ctx->stream, a);
+  } else {
+    if (a.int_rows == 2) hipLaunchKernelGGL((lbm_step_kernel<P, 2, false>), grid, dim3(256), 0, L.ctx->stream, a);
+    else hipLaunchKernelGGL((lbm_step_kernel<P, 1, false>), grid, dim3(256), 0, L.ctx->stream, a);
+  }
   FB_HIP(hipGetLastError());
   return FB_OK;
 }

@@ -93,8 +93,9 @@ class Autotuner:
 
     # ------------------------------------------------------------------ lattice step shape
     def lbm(self, nx: int, ny: int, form: str = "f32-mixed", steps: int = 50) -> dict:
-        """Fused pairs on or off and interior rows per thread, then (single steps) the blocks
-        striding over the flagged cells, for one lattice shape and storage form."""
+        """Fused pairs on or off and interior rows per thread, then (single steps) nontemporal population
+        traffic on or off and the blocks striding over the flagged cells, for one lattice shape and storage
+        form."""
         from .benchmark.lbm_harness import Lattice, LatticeCase
 
         prec = _PREC[form]
@@ -119,7 +120,15 @@ class Autotuner:
             self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
             self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
             list_blocks = -1
+            nt = -1
             if not fuse:
+                nts = {}
+                for v in (0, 1):
+                    self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, v)
+                    nts[v] = timed()
+                nt = min(nts, key=nts.get)
+                times.update({f"nt{v}": t for v, t in nts.items()})
+                self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, nt)
                 lt = {}
                 for b in (8, 16, 32, 64, 128):
                     self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, b)
@@ -129,9 +138,10 @@ class Autotuner:
                 self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, list_blocks)
         finally:
             lat.close()
-        rec = {"fuse": fuse, "rows": rows, "list_blocks": list_blocks, "ms": times,
+        rec = {"fuse": fuse, "rows": rows, "list_blocks": list_blocks, "nt": nt, "ms": times,
                "set": [[N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse], [N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows],
-                       [N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, list_blocks]]}
+                       [N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, list_blocks],
+                       [N.FB_TUNE_LBM_NT, nx, ny, prec, 1, nt]]}
         self.choices[key] = rec
         return rec
 

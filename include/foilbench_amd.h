@@ -435,13 +435,16 @@ int fb_pic_sample_velocity(fb_pic* pic, int64_t n, const double* points, void* o
  * FB_TUNE_LBM_ROWS: interior rows per thread of the lattice step, 1 or 2; unset, 2 from 2 Mi
  * cells (grids = 1);
  * FB_TUNE_DST_PPT: points per thread of the DST-I kernel's FFT passes, 16 or 8; unset, 8 for
- * FFT lengths 512 to 4096 (grids as for FB_TUNE_DST_ROWS). */
+ * FFT lengths 512 to 4096 (grids as for FB_TUNE_DST_ROWS);
+ * FB_TUNE_LBM_NT: 1 streams the interior populations of the lattice step with nontemporal loads and
+ * stores, 0 not; unset, on once the two population copies exceed 192 MiB (grids = 1). */
 enum fb_tune_param {
   FB_TUNE_DST_ROWS = 0,
   FB_TUNE_LBM_LIST_BLOCKS = 1,
   FB_TUNE_LBM_FUSE = 2,
   FB_TUNE_LBM_ROWS = 3,
-  FB_TUNE_DST_PPT = 4
+  FB_TUNE_DST_PPT = 4,
+  FB_TUNE_LBM_NT = 5
 };
 int fb_tune_set(int param, int nx, int ny, int precision, int grids, int value);
 int fb_tune_get(int param, int nx, int ny, int precision, int grids, int* value);

@@ -235,3 +235,46 @@ def test_lbm_fused_pairs_equal_single_steps(form, nx, ny, channel):
     np.testing.assert_array_equal(o1, o0)
     assert s1 == s0
 
+
+
+def _lattice_run_nt(form, nx, ny, channel, rows, nt, chunks):
+    """A harness lattice in single steps with `rows` interior rows per thread and nontemporal population
+    traffic on or off (tuning table); returns the state and the statistics of each advance."""
+    from foilbench_amd import _native as N
+    from foilbench_amd.benchmark.lbm_harness import Lattice, LatticeCase
+    from foilbench_amd.tuning import _PREC
+
+    ctx = N.Context.default()
+    prec = _PREC[form]
+    keys = ((N.FB_TUNE_LBM_FUSE, 0), (N.FB_TUNE_LBM_ROWS, rows), (N.FB_TUNE_LBM_NT, nt))
+    for k, v in keys:
+        N.check(ctx.lib.fb_tune_set(k, nx, ny, prec, 1, v))
+    lat = Lattice(LatticeCase(nx=nx, ny=ny, channel=channel), prec, ctx)
+    try:
+        stats = []
+        for n in chunks:
+            st = lat.advance(n)
+            stats.append((st.min_population, st.max_speed, st.density_excursion, st.nonfinite_populations,
+                          st.bad_fluid_density, st.nonfinite_velocity))
+        f, outlet = lat.state()
+    finally:
+        lat.close()
+        for k, _ in keys:
+            N.check(ctx.lib.fb_tune_set(k, nx, ny, prec, 1, -1))
+    return f, outlet, stats
+
+
+@pytest.mark.parametrize("form", ["f64", "f32", "f32-mixed", "f16-mixed"])
+@pytest.mark.parametrize("rows", [1, 2])
+def test_lbm_streaming_population_traffic_changes_no_bit(form, rows):
+    """The single-step kernel with nontemporal population loads and stores (the form chosen for lattices
+    larger than the caches) against the cached form: the same populations, outlet history and
+    postcondition statistics, with one and two interior rows per thread, on a walled box and a channel."""
+    for nx, ny, channel in ((200, 100, 0), (192, 98, 1)):
+        chunks = (5, 3)
+        f1, o1, s1 = _lattice_run_nt(form, nx, ny, channel, rows, 1, chunks)
+        f0, o0, s0 = _lattice_run_nt(form, nx, ny, channel, rows, 0, chunks)
+        assert np.isfinite(f1).all()
+        np.testing.assert_array_equal(f1, f0)
+        np.testing.assert_array_equal(o1, o0)
+        assert s1 == s0
