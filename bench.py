@@ -1,6 +1,7 @@
 """Headline benchmark: cell-steps/s of the D2Q9 TRT LBM lattice step on one MI355X.
 
-Workload (one "step" = one lattice step of the whole lattice, inputs resident in HBM):
+Workload (one "step" = one advance of the whole lattice by LATTICE_STEPS lattice steps, as the solver's advance
+runs them: the last writes the pre-collision state with the postcondition reductions; inputs resident in HBM):
 NACA 2412 at AoA 12 degrees (angle_degrees = -12), 1024 x 512 lattice, chord 160 cells,
 equilibrium inflow u = 0.05 lattice units, Re = 1000, TRT magic parameter 3/16,
 interpolated bounce-back on the foil, FP32 population storage.
@@ -35,6 +36,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 ROOFLINE_LAUNCHES = 4000  # launches in the event-timed sample of the step kernel
+# lattice steps per bench step: the reference runner times whole advances (fb/benchmark/runner.py:240-275), and an
+# advance of 100 steps keeps the host's submission and synchronisation (~80 us per timed window) out of the rate
+LATTICE_STEPS = 100
 
 
 FORMS = (("f32", "FB_F32"), ("f64", "FB_F64"), ("f32-mixed", "FB_F32_MIXED"), ("f16-mixed", "FB_F16_MIXED"))
@@ -46,8 +50,9 @@ LARGE_LAUNCHES = 1000
 
 
 def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
-    """Throughput over `steps` back-to-back lattice steps and the step kernel's event-timed average over
-    `launches` launches, with the autotuner's launch shape for this lattice and storage form."""
+    """Throughput over `steps` back-to-back advances of LATTICE_STEPS lattice steps and the step kernel's
+    event-timed average over `launches` launches, with the autotuner's launch shape for this lattice and storage
+    form."""
     from foilbench_amd import _native as N
     from foilbench_amd.benchmark.lbm_harness import BYTES_PER_CELL, Lattice
 
@@ -58,10 +63,12 @@ def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
     tuned = {k: rec[k] for k in ("fuse", "rows", "list_blocks", "nt")}
     lat = Lattice(case, prec, ctx)
     try:
-        lat.run(warmup)
+        for _ in range(warmup):
+            lat.run(LATTICE_STEPS)
         ctx.synchronize()
         t0 = time.perf_counter()
-        lat.run(steps)
+        for _ in range(steps):
+            lat.run(LATTICE_STEPS)
         ctx.synchronize()
         wall = time.perf_counter() - t0
         # average launch duration: one HIP event pair around `launches` back-to-back launches on the kernel's
@@ -72,14 +79,15 @@ def _time_form(tuner, case, name, pname, ctx, steps, warmup, launches):
     bytes_launch = BYTES_PER_CELL[prec] * lat.cells
     per_launch = 2 if tuned["fuse"] else 1  # lattice steps per launch of the step kernel
     return {
-        "cell_steps_per_s": lat.cells * steps / wall,
+        "cell_steps_per_s": lat.cells * LATTICE_STEPS * steps / wall,
         "ms_per_step": 1e3 * wall / steps,
+        "lattice_steps_per_step": LATTICE_STEPS,
         "kernel_ms": avg_ms,
         "kernel_gbs": bytes_launch / (avg_ms * 1e-3) / 1e9,
         "bytes_per_cell": BYTES_PER_CELL[prec],
         "tuned_step_shape": tuned,
         "kernel": (f"lbm_pair_kernel<Prec<{pname}>, 16>" if tuned["fuse"]
-                   else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}, {'true' if tuned['nt'] == 1 else 'false'}>"),
+                   else f"lbm_step_kernel<Prec<{pname}>, {tuned['rows']}, {'true' if tuned['nt'] == 1 else 'false'}, true>"),
         "steps_per_launch": per_launch,
         "launch_ms": avg_ms * per_launch,
         "event_launches": launches,
@@ -177,7 +185,7 @@ def _large_lattice(tuner) -> dict:
     for name, pname in FORMS:
         if name == "f64":
             continue
-        r = _time_form(tuner, case, name, pname, ctx, 200, 20, LARGE_LAUNCHES)
+        r = _time_form(tuner, case, name, pname, ctx, 2, 1, LARGE_LAUNCHES)
         r["frac"] = r["kernel_gbs"] / HBM_PEAK_GBS
         out[name] = r
         out["working_set_mb"][name] = r["bytes_per_cell"] * case.nx * case.ny / 1e6  # two population buffers
@@ -285,8 +293,9 @@ def _traffic(kernel: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200, help="advances of LATTICE_STEPS lattice steps (200: the "
+                    "LBM configuration's 20,000-step horizon)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-scale", type=float, default=1.0,
                     help="scale of the CPU baselines' step counts (0: no CPU baseline)")
     ap.add_argument("--no-legs", action="store_true", help="measure only the LBM headline")
@@ -325,6 +334,7 @@ def main() -> int:
             "workload": case.name + " fp32 storage, fp64 kernel arithmetic, interpolated bounce-back",
             "lattice": [case.nx, case.ny],
             "cells": case.nx * case.ny,
+            "lattice_steps_per_step": LATTICE_STEPS,
             "parallelism": "single-gpu",
         },
         "roofline": {

@@ -150,6 +150,9 @@ struct LbmArgs {
   const double* __restrict__ links;  // 8 Bouzidi fractions per list entry (0: straight link)
   const int* __restrict__ lidx;      // list position of each listed cell (other cells: unset)
   unsigned long long* red;  // [min f key, max speed key, max excursion key, nonfinite f, bad rho, nonfinite u]
+  // per-block slots of the statistics ([6][blocks]), folded by lbm_stats_fold_kernel; null: each wave merges
+  // into `red` with atomics (batched launches)
+  unsigned long long* spart;
 };
 
 // Where a lattice step reads the previous post-collision state f*, the collision density of the
@@ -813,11 +816,94 @@ __device__ __forceinline__ void put_step_stats(unsigned long long* red, double m
   }
 }
 
+// Block reduction of the postcondition statistics into the block's slots of `spart` (order-free maxima,
+// minima and counts), or the per-wave atomics when the launch has no slots. Every thread of the block
+// calls it.
+__device__ __forceinline__ void put_block_stats(unsigned long long* spart, unsigned long long* red, unsigned blk,
+                                                unsigned nblk, double mn, double vmax, double exc, long long nf,
+                                                long long bad, long long nfu) {
+  if (!spart) {
+    put_step_stats(red, mn, vmax, exc, nf, bad, nfu);
+    return;
+  }
+  __shared__ unsigned long long ws[4][6];
+  mn = wave_min(mn);
+  vmax = wave_max(vmax);
+  exc = wave_max(exc);
+  nf = wave_sum_ll(nf);
+  bad = wave_sum_ll(bad);
+  nfu = wave_sum_ll(nfu);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ws[w][0] = dkey(mn);
+    ws[w][1] = dkey(vmax);
+    ws[w][2] = dkey(exc);
+    ws[w][3] = (unsigned long long)nf;
+    ws[w][4] = (unsigned long long)bad;
+    ws[w][5] = (unsigned long long)nfu;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    const int nw = (int)(blockDim.x >> 6);
+    unsigned long long v = ws[0][q];
+    for (int k = 1; k < nw; ++k) {
+      const unsigned long long u = ws[k][q];
+      v = q == 0 ? (u < v ? u : v) : q < 3 ? (u > v ? u : v) : v + u;
+    }
+    spart[(size_t)q * nblk + blk] = v;
+  }
+}
+
+// Folds the per-block statistics of one step (lbm_step_kernel with `spart`) into `red`.
+__global__ void __launch_bounds__(256) lbm_stats_fold_kernel(const unsigned long long* __restrict__ spart, int nblk,
+                                                             unsigned long long* red) {
+  __shared__ unsigned long long ws[4][6];
+  unsigned long long v[6] = {~0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+  for (int b = (int)threadIdx.x; b < nblk; b += 256) {
+    const unsigned long long k0 = spart[b], k1 = spart[(size_t)nblk + b], k2 = spart[2 * (size_t)nblk + b];
+    v[0] = k0 < v[0] ? k0 : v[0];
+    v[1] = k1 > v[1] ? k1 : v[1];
+    v[2] = k2 > v[2] ? k2 : v[2];
+    v[3] += spart[3 * (size_t)nblk + b];
+    v[4] += spart[4 * (size_t)nblk + b];
+    v[5] += spart[5 * (size_t)nblk + b];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const unsigned long long u = __shfl_xor(v[q], o, 64);
+      v[q] = q == 0 ? (u < v[q] ? u : v[q]) : q < 3 ? (u > v[q] ? u : v[q]) : v[q] + u;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 6; ++q) ws[w][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      v[0] = ws[k][0] < v[0] ? ws[k][0] : v[0];
+      v[1] = ws[k][1] > v[1] ? ws[k][1] : v[1];
+      v[2] = ws[k][2] > v[2] ? ws[k][2] : v[2];
+      v[3] += ws[k][3];
+      v[4] += ws[k][4];
+      v[5] += ws[k][5];
+    }
+    atomicMin(&red[0], v[0]);
+    atomicMax(&red[1], v[1]);
+    atomicMax(&red[2], v[2]);
+    atomicAdd(&red[3], v[3]);
+    atomicAdd(&red[4], v[4]);
+    atomicAdd(&red[5], v[5]);
+  }
+}
+
 // ROWS interior cells of column x from row y0 (rows past ny-2 and columns 0, nx-1 idle): the
 // straight pull of every cell, its flags, next-step flags and sponge weights are loaded before any
 // cell is computed, so the cells share one round of loads; flagged cells are left to the list
 // blocks. step_cell's arithmetic for straight cells (whose own flags are clear, so no reset).
-template <class P, int ROWS, bool NT>
+template <class P, int ROWS, bool NT, bool PLAIN>
 __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, int x, int y0, double& mn,
                                                double& vmax, double& exc, long long& nf, long long& bad,
                                                long long& nfu) {
@@ -852,7 +938,7 @@ __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, 
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = ld<S, C>(&v[r][d]);
     apply_sponge<P>(a, spy[r], spx, g);
-    if (a.write_pre) {
+    if (!PLAIN && a.write_pre) {
 #pragma unroll
       for (int d = 0; d < 9; ++d) st<S, C>(a.fpre + (size_t)d * N + i, g[d]);
       cell_stats<P>(a, g, !(fnext[r] & FL_SOLID), mn, vmax, exc, nf, bad, nfu);
@@ -871,17 +957,22 @@ __device__ __forceinline__ void interior_cells(const LbmArgs<typename P::S>& a, 
 // run in the first blocks of the grid, so their longer dependent chains start early and do not
 // diverge the interior waves; interior blocks take the straight pull.
 // Block `blk` of the `nblk` blocks that cover lattice `a` for one step.
-template <class P, int ROWS, bool NT>
+// PLAIN: a step without priming, uncovered-cell reset or pre-collision write (every step of an
+// advance but its last, when the geometry holds still); those paths then cost no registers.
+template <class P, int ROWS, bool NT, bool PLAIN = false>
 __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsigned blk, unsigned nblk) {
   double mn = INFINITY, vmax = 0.0, exc = 0.0;
   long long nf = 0, bad = 0, nfu = 0;
+  const bool prime = PLAIN ? false : (a.prime != 0);
+  const bool reset = PLAIN ? false : (a.do_reset != 0);
+  const bool pre = PLAIN ? false : (a.write_pre != 0);
   const unsigned special = (unsigned)(a.edge_blocks + a.list_blocks);
   if (blk < (unsigned)a.edge_blocks) {
     const int e = (int)(blk * blockDim.x + threadIdx.x);
     if (e < a.nedge) {
       int x, y;
       edge_cell(e, a.nx, a.ny, x, y);
-      step_cell<P>(a, plane_src(a), plane_sink(a), x, y, false, nullptr, false, a.prime, a.do_reset, a.write_pre, mn,
+      step_cell<P>(a, plane_src(a), plane_sink(a), x, y, false, nullptr, false, prime, reset, pre, mn,
                    vmax, exc, nf, bad, nfu);
     }
   } else if (blk < special) {
@@ -891,33 +982,33 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
       const int entry = a.list[k];
       const int c = entry & 0x7fffffff;
       step_cell<P>(a, plane_src(a), plane_sink(a), c % a.nx, c / a.nx, false, a.links + (size_t)k * 8, entry < 0,
-                   a.prime, a.do_reset, a.write_pre, mn, vmax, exc, nf, bad, nfu);
+                   prime, reset, pre, mn, vmax, exc, nf, bad, nfu);
     }
   } else {
     const unsigned bi = xcd_remap(blk - special, nblk - special);
     const unsigned gx = (unsigned)a.int_gx;
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
     const int y0 = 1 + ROWS * (int)(bi / gx);
-    if ((ROWS == 1 && !NT) || a.prime) {
+    if ((ROWS == 1 && !NT) || prime) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         const int y = y0 + r;
         if (x > 0 && x < a.nx - 1 && y <= a.ny - 2 && a.flags[(size_t)y * a.nx + x] == 0)
-          step_cell<P>(a, plane_src(a), plane_sink(a), x, y, true, nullptr, false, a.prime, a.do_reset, a.write_pre,
+          step_cell<P>(a, plane_src(a), plane_sink(a), x, y, true, nullptr, false, prime, reset, pre,
                        mn, vmax, exc, nf, bad, nfu);
       }
     } else {
-      interior_cells<P, ROWS, NT>(a, x, y0, mn, vmax, exc, nf, bad, nfu);
+      interior_cells<P, ROWS, NT, PLAIN>(a, x, y0, mn, vmax, exc, nf, bad, nfu);
     }
   }
-  if (a.write_pre) put_step_stats(a.red, mn, vmax, exc, nf, bad, nfu);
+  if (pre) put_block_stats(a.spart, a.red, blk, nblk, mn, vmax, exc, nf, bad, nfu);
 }
 
 // One lattice step with ROWS interior rows per block (a kernel of its own per ROWS, so that each
 // gets its own register allocation); NT streams the interior populations past the caches.
-template <class P, int ROWS, bool NT>
+template <class P, int ROWS, bool NT, bool PLAIN>
 __global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
-  lbm_block<P, ROWS, NT>(a, blockIdx.x, gridDim.x);
+  lbm_block<P, ROWS, NT, PLAIN>(a, blockIdx.x, gridDim.x);
 }
 
 // Blocks of one lattice step.
@@ -1533,6 +1624,8 @@ struct Lattice {
   double* sponge_x = nullptr;
   unsigned long long* red = nullptr;
   unsigned long long* red_host = nullptr;
+  unsigned long long* spart = nullptr;  // per-block statistics slots of a step that writes the pre-collision state
+  int spart_blocks = 0;                  // blocks the slots hold
   void* host_stage = nullptr;
   // current indices
   int f = 0, o = 0, pre = 0, g = 0;
@@ -1626,6 +1719,7 @@ static LbmArgs<S> step_args(Lattice& L, const fb_lbm_params& p, int fin_idx, boo
   a.links = L.G[g_cur].links;
   a.lidx = L.G[g_cur].lidx;
   a.skip = 0;
+  a.spart = nullptr;
   return a;
 }
 
@@ -1633,16 +1727,26 @@ template <class P>
 static int launch_step(Lattice& L, const fb_lbm_params& p, int fin_idx, bool prime, int fout_idx, int o_in, int o_out,
                        int g_cur, int g_next, bool write_pre, int pre_out, bool do_reset) {
   using S = typename P::S;
-  const LbmArgs<S> a =
+  LbmArgs<S> a =
       step_args<S>(L, p, fin_idx, prime, fout_idx, o_in, o_out, g_cur, g_next, write_pre, pre_out, do_reset);
   const dim3 grid(lbm_step_blocks(a));
-  if (stream_pops(L)) {
-    if (a.int_rows == 2) hipLaunchKernelGGL((lbm_step_kernel<P, 2, true>), grid, dim3(256), 0, L.ctx->stream, a);
-    else hipLaunchKernelGGL((lbm_step_kernel<P, 1, true>), grid, dim3(256), 0, L.ctx->stream, a);
+  // the statistics of a step that writes the pre-collision state go through per-block slots and one fold
+  // (per-wave atomics on the six keys serialise on large grids)
+  const bool slots = write_pre && !prime && (int)grid.x <= L.spart_blocks;
+  if (slots) a.spart = L.spart;
+  const bool plain = !prime && !write_pre && !do_reset;
+  const bool nt = stream_pops(L);
+  auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), 0, L.ctx->stream, a); };
+  if (plain) {
+    if (a.int_rows == 2) nt ? go(lbm_step_kernel<P, 2, true, true>) : go(lbm_step_kernel<P, 2, false, true>);
+    else nt ? go(lbm_step_kernel<P, 1, true, true>) : go(lbm_step_kernel<P, 1, false, true>);
   } else {
-    if (a.int_rows == 2) hipLaunchKernelGGL((lbm_step_kernel<P, 2, false>), grid, dim3(256), 0, L.ctx->stream, a);
-    else hipLaunchKernelGGL((lbm_step_kernel<P, 1, false>), grid, dim3(256), 0, L.ctx->stream, a);
+    if (a.int_rows == 2) nt ? go(lbm_step_kernel<P, 2, true, false>) : go(lbm_step_kernel<P, 2, false, false>);
+    else nt ? go(lbm_step_kernel<P, 1, true, false>) : go(lbm_step_kernel<P, 1, false, false>);
   }
+  if (slots)
+    hipLaunchKernelGGL(lbm_stats_fold_kernel, dim3(1), dim3(256), 0, L.ctx->stream, (const unsigned long long*)L.spart,
+                       (int)grid.x, L.red);
   FB_HIP(hipGetLastError());
   return FB_OK;
 }
@@ -2055,6 +2159,7 @@ static void free_lattice(Lattice* L) {
   if (L->sponge_y) (void)dev_free(L->sponge_y);
   if (L->red) (void)dev_free(L->red);
   if (L->red_host) (void)host_free(L->red_host);
+  if (L->spart) (void)dev_free(L->spart);
   delete L;
 }
 
@@ -2447,6 +2552,12 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   if ((e = dev_alloc((void**)&L->sponge_y, dom->ny * sizeof(double)))) return fail(e);
   if ((e = dev_alloc((void**)&L->sponge_x, dom->nx * sizeof(double)))) return fail(e);
   if ((e = dev_alloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
+  {
+    // statistics slots for the largest single-step grid: one interior row per block, the most list blocks
+    const int nedge = 2 * dom->nx + 2 * (dom->ny - 2);
+    L->spart_blocks = (nedge + 255) / 256 + 1024 + fb::interior_blocks(dom->nx, dom->ny, 1);
+    if ((e = dev_alloc((void**)&L->spart, 6 * (size_t)L->spart_blocks * sizeof(unsigned long long)))) return fail(e);
+  }
   if ((e = host_alloc((void**)&L->red_host, 8 * sizeof(unsigned long long)))) return fail(e);
   // stream-ordered copies: a legacy-stream copy would wait on every other thread's stream
   if ((e = hipMemcpyAsync(L->sponge_y, sponge_y, dom->ny * sizeof(double), hipMemcpyHostToDevice, c->stream)))

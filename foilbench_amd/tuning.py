@@ -6,8 +6,8 @@ Tuned here, per grid shape and precision, and stored in the library's tuning tab
   x-direction transform), timed as whole preconditioner applications on stacked grids
   (`fb_poisson_time_precond`);
 * for a lattice, whether two steps run per launch (`lbm_pair_kernel`), the interior rows per
-  thread of the single-step kernel and the blocks striding over its flagged cells, timed as
-  lattice steps (`fb_lbm_time_fixed`).
+  thread and the nontemporal population traffic of the single-step kernel (jointly) and the blocks
+  striding over its flagged cells, timed as lattice steps (`fb_lbm_time_fixed`).
 For the request loop it also picks the lattice-batch chunk (steps per graph) by timing
 batched steps of the mix's lattice shapes (`fb_lbm_run_batch`).
 
@@ -92,10 +92,11 @@ class Autotuner:
         return rec
 
     # ------------------------------------------------------------------ lattice step shape
-    def lbm(self, nx: int, ny: int, form: str = "f32-mixed", steps: int = 50) -> dict:
-        """Fused pairs on or off and interior rows per thread, then (single steps) nontemporal population
-        traffic on or off and the blocks striding over the flagged cells, for one lattice shape and storage
-        form."""
+    def lbm(self, nx: int, ny: int, form: str = "f32-mixed", steps: int = 100) -> dict:
+        """The single-step shapes (interior rows per thread x nontemporal population traffic) and the fused pair,
+        timed jointly (rows and streaming interact: two rows per thread pay with cached loads, one row with
+        streaming ones), then the blocks striding over the flagged cells of the winner, for one lattice shape and
+        storage form."""
         from .benchmark.lbm_harness import Lattice, LatticeCase
 
         prec = _PREC[form]
@@ -106,29 +107,23 @@ class Autotuner:
 
         def timed() -> float:
             lat.time(4)
-            return min(lat.time(steps) / steps for _ in range(2))
+            return min(lat.time(steps) / steps for _ in range(3))
 
         times = {}
         try:
-            for fuse in (0, 1):
-                for rows in (1, 2):
-                    self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
-                    self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
-                    times[f"fuse{fuse}:rows{rows}"] = timed()
+            shapes = [(0, rows, nt) for rows in (1, 2) for nt in (0, 1)] + [(1, 1, -1)]
+            for fuse, rows, nt in shapes:
+                self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
+                self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
+                self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, nt)
+                times[f"fuse{fuse}:rows{rows}:nt{nt}"] = timed()
             best = min(times, key=times.get)
-            fuse, rows = int(best[4]), int(best[-1])
+            fuse, rows, nt = (int(t[len(n):]) for t, n in zip(best.split(":"), ("fuse", "rows", "nt")))
             self._set(N.FB_TUNE_LBM_FUSE, nx, ny, prec, 1, fuse)
             self._set(N.FB_TUNE_LBM_ROWS, nx, ny, prec, 1, rows)
+            self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, nt)
             list_blocks = -1
-            nt = -1
             if not fuse:
-                nts = {}
-                for v in (0, 1):
-                    self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, v)
-                    nts[v] = timed()
-                nt = min(nts, key=nts.get)
-                times.update({f"nt{v}": t for v, t in nts.items()})
-                self._set(N.FB_TUNE_LBM_NT, nx, ny, prec, 1, nt)
                 lt = {}
                 for b in (8, 16, 32, 64, 128):
                     self._set(N.FB_TUNE_LBM_LIST_BLOCKS, nx, ny, prec, 1, b)
