@@ -139,11 +139,11 @@ struct LbmArgs {
   double feq_u[9];
   int periodic_x, periodic_y, channel;
   int write_pre, do_reset, prime;
-  // launch shape: blocks [0, edge_blocks) cover the nedge edge cells, the next list_blocks
+  // launch shape: blocks [0, edge_blocks) cover the edge cells (edge_index), the next list_blocks
   // stride over the flagged (solid or near-wall) cells of rows 1..ny-2 listed in `list`;
   // the rest cover rows 1..ny-2, int_rows rows per block (1 or 2) and int_gx blocks across,
   // whose edge and flagged lanes idle
-  int int_gx, int_rows, edge_blocks, nedge, list_blocks;
+  int int_gx, int_rows, edge_blocks, list_blocks;
   int skip;  // batched launches: this lattice sits out the step (its blocks return at once)
   const int* __restrict__ list;
   const int* __restrict__ list_count;
@@ -703,6 +703,30 @@ __device__ __forceinline__ void edge_cell(int e, int nx, int ny, int& x, int& y)
   }
 }
 
+// The edge blocks: the bottom and top rows 256 cells to a block, then the left and right columns
+// kEdgeColumnLanes cells to a wave (lanes 0..kEdgeColumnLanes-1). A column cell's loads and stores
+// each touch a cache line of their own (the lanes sit a row apart), so a block full of them is bound
+// by its CU's address and cache-line throughput; spread a few to a wave over many CUs, the column
+// cells finish within the interior's time instead of setting the length of the step.
+constexpr int kEdgeColumnLanes = 16;
+__host__ __device__ inline int edge_row_blocks(int nx) { return (2 * nx + 255) / 256; }
+__host__ __device__ inline int edge_blocks_for(int nx, int ny) {
+  const int per_block = 4 * kEdgeColumnLanes;
+  return edge_row_blocks(nx) + (2 * (ny - 2) + per_block - 1) / per_block;
+}
+// Edge cell of thread `t` in edge block `blk` (-1: the thread has none).
+__device__ __forceinline__ int edge_index(int blk, int t, int nx, int ny) {
+  const int rb = edge_row_blocks(nx);
+  if (blk < rb) {
+    const int e = blk * 256 + t;
+    return e < 2 * nx ? e : -1;
+  }
+  const int lane = t & 63;
+  if (lane >= kEdgeColumnLanes) return -1;
+  const int k = ((blk - rb) * 4 + (t >> 6)) * kEdgeColumnLanes + lane;
+  return k < 2 * (ny - 2) ? 2 * nx + k : -1;
+}
+
 // Where a lattice step writes f* and the collision density of the near-wall cells: the global
 // planes, or the LDS tile of the first step of a fused pair.
 template <class S>
@@ -968,8 +992,8 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
   const bool pre = PLAIN ? false : (a.write_pre != 0);
   const unsigned special = (unsigned)(a.edge_blocks + a.list_blocks);
   if (blk < (unsigned)a.edge_blocks) {
-    const int e = (int)(blk * blockDim.x + threadIdx.x);
-    if (e < a.nedge) {
+    const int e = edge_index((int)blk, (int)threadIdx.x, a.nx, a.ny);
+    if (e >= 0) {
       int x, y;
       edge_cell(e, a.nx, a.ny, x, y);
       step_cell<P>(a, plane_src(a), plane_sink(a), x, y, false, nullptr, false, prime, reset, pre, mn,
@@ -989,7 +1013,7 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
     const unsigned gx = (unsigned)a.int_gx;
     const int x = (int)((bi % gx) * blockDim.x + threadIdx.x);
     const int y0 = 1 + ROWS * (int)(bi / gx);
-    if ((ROWS == 1 && !NT) || prime) {
+    if (prime) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         const int y = y0 + r;
@@ -1711,8 +1735,7 @@ static LbmArgs<S> step_args(Lattice& L, const fb_lbm_params& p, int fin_idx, boo
   a.red = L.red;
   a.int_gx = (L.dom.nx + 255) / 256;
   a.int_rows = int_rows(L);
-  a.nedge = 2 * L.dom.nx + 2 * (L.dom.ny - 2);
-  a.edge_blocks = (a.nedge + 255) / 256;
+  a.edge_blocks = edge_blocks_for(L.dom.nx, L.dom.ny);
   a.list_blocks = list_blocks(L);
   a.list = L.G[g_cur].list;
   a.list_count = L.G[g_cur].list_count;
@@ -2301,8 +2324,7 @@ static int batch_chunk(LbmBatch& B, std::vector<LbmDone>& done) {
     for (int i = 0; i < n; ++i) {
       B.h_start[i] = off;
       const Lattice& L = *B.m[i].L;
-      const int nb = interior_blocks(L.dom.nx, L.dom.ny, 1) + B.m[i].list_blocks +
-                     (2 * L.dom.nx + 2 * (L.dom.ny - 2) + 255) / 256;
+      const int nb = interior_blocks(L.dom.nx, L.dom.ny, 1) + B.m[i].list_blocks + edge_blocks_for(L.dom.nx, L.dom.ny);
       off += (nb + 7) / 8 * 8;
     }
     B.h_start[n] = off;
@@ -2554,8 +2576,7 @@ int fb_lbm_create(fb_ctx* ctx, const fb_domain* dom, const fb_foil* foil, int pr
   if ((e = dev_alloc((void**)&L->red, 8 * sizeof(unsigned long long)))) return fail(e);
   {
     // statistics slots for the largest single-step grid: one interior row per block, the most list blocks
-    const int nedge = 2 * dom->nx + 2 * (dom->ny - 2);
-    L->spart_blocks = (nedge + 255) / 256 + 1024 + fb::interior_blocks(dom->nx, dom->ny, 1);
+    L->spart_blocks = fb::edge_blocks_for(dom->nx, dom->ny) + 1024 + fb::interior_blocks(dom->nx, dom->ny, 1);
     if ((e = dev_alloc((void**)&L->spart, 6 * (size_t)L->spart_blocks * sizeof(unsigned long long)))) return fail(e);
   }
   if ((e = host_alloc((void**)&L->red_host, 8 * sizeof(unsigned long long)))) return fail(e);
