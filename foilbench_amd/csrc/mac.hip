@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256) skew_stage_kernel(MacGeom g, const T* __r
   }
 }
 
-// skew_face_advection_rate (fb/core/grid.py:434-443): per-wave maxima of the face rates, in T,
+// skew_face_advection_rate (fb/core/grid.py:434-443): per-block maxima of the face rates, in T,
 // into one order-preserving key.
 template <class T>
 __global__ void __launch_bounds__(256) skew_rate_kernel(MacGeom g, const T* __restrict__ u, const T* __restrict__ v,
@@ -510,9 +510,9 @@ __global__ void __launch_bounds__(256) skew_rate_kernel(MacGeom g, const T* __re
     const T a = (T)fabs((double)u_on_v(g, u, j, i)), b = (T)fabs((double)v[(size_t)j * g.nx + i]);
     r = fmax(r, (double)(a / rdx + b / rdy));
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
-  if ((threadIdx.x & 63) == 0) key_max(key, dkey(r));
+  __shared__ unsigned long long lmax[4];
+  const unsigned long long k = block_max_u64(dkey(r), lmax);
+  if (threadIdx.x == 0) key_max(key, k);
 }
 
 template <class T>

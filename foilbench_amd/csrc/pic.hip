@@ -125,35 +125,52 @@ __device__ __forceinline__ int ifloor(double v) {
   return (int)f;
 }
 
-// Gather of one component at (x, y) (kern/pic.py:164-205): FP64 weights and sums in the
-// fixed 3 x 3 order. With `q` and `p` given, also gathers the dtype difference q - p with the
-// same weights (the FLIP increment of fb/solvers/pic_flip.py:822-826).
+// Grid coordinates of a point before the component's face offset: ((x - x0) / dx, (y - y0) / dy),
+// the part of kern/pic.py:187-188 that the u and v gathers at one point share (the same FP64
+// quotients the reference computes once per component).
+__device__ __forceinline__ void grid_coords(const PicGeom& g, double x, double y, double& tx, double& ty) {
+  tx = (x - g.x0) / g.dx;
+  ty = (y - g.y0) / g.dy;
+}
+
+// Gather of one component at the point of grid coordinates (tx, ty) (kern/pic.py:164-205): FP64
+// weights and sums in the fixed 3 x 3 order. The three weights per axis are evaluated once (the
+// reference evaluates the x weights again for every row; the values are the same). With `q` and
+// `p` given, also gathers the dtype difference q - p with the same weights (the FLIP increment of
+// fb/solvers/pic_flip.py:822-826).
 template <class T, bool DIFF>
 __device__ __forceinline__ void gather(const PicGeom& g, const Comp& c, const T* __restrict__ q,
-                                       const T* __restrict__ p, double x, double y, double& out_q,
+                                       const T* __restrict__ p, double tx, double ty, double& out_q,
                                        double& out_d) {
-  double gx = (x - g.x0) / g.dx + c.gxo;
-  double gy = (y - g.y0) / g.dy + c.gyo;
-  int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+  const double gx = tx + c.gxo;
+  const double gy = ty + c.gyo;
+  const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+  double wxs[3], wys[3];
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    wxs[o] = bspline(gx - (double)(bx + o));
+    wys[o] = bspline(gy - (double)(by + o));
+  }
   double vq = 0.0, vd = 0.0, ws = 0.0;
+#pragma unroll
   for (int oy = 0; oy < 3; ++oy) {
-    int sy = by + oy;
-    int ty = g.py ? pmod(sy, c.uh) : sy;
-    if (ty < 0 || ty >= c.H) continue;
-    double wy = bspline(gy - (double)sy);
+    const int sy = by + oy;
+    const int ty_ = g.py ? pmod(sy, c.uh) : sy;
+    if (ty_ < 0 || ty_ >= c.H) continue;
+#pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
-      int sx = bx + ox;
-      int tx = g.px ? pmod(sx, c.uw) : sx;
-      if (tx < 0 || tx >= c.W) continue;
-      double w = wy * bspline(gx - (double)sx);
-      size_t k = (size_t)ty * c.W + tx;
-      T qv = q[k];
+      const int sx = bx + ox;
+      const int tx_ = g.px ? pmod(sx, c.uw) : sx;
+      if (tx_ < 0 || tx_ >= c.W) continue;
+      const double w = wys[oy] * wxs[ox];
+      const size_t k = (size_t)ty_ * c.W + tx_;
+      const T qv = q[k];
       vq += w * (double)qv;
       if constexpr (DIFF) vd += w * (double)(T)(qv - p[k]);
       ws += w;
     }
   }
-  double den = fmax(ws, 1.0e-12);
+  const double den = fmax(ws, 1.0e-12);
   out_q = vq / den;
   if constexpr (DIFF) out_d = vd / den;
 }
@@ -469,29 +486,45 @@ __global__ void __launch_bounds__(256) p2g_kernel(P2GArgs<T> a) {
 
 // The same gather for a non-periodic box, staged through LDS. A block owns a tile of
 // kTileW x kTileH faces and loads once the particles of the (kTileW + 3) x (kTileH + 3) cells that
-// can reach it (each cell row's slice of the sorted arrays is contiguous), with their grid
-// coordinates computed once per particle. Four threads share a face, one per row of its 4 x 4
-// candidate cells; each sums its row's contributions in FP64 in cell and particle order, and the
-// four partial sums are added in row order, so the result is deterministic. A face's weight and
-// momentum are rounded to the storage type once (the reference rounds after every particle, in
-// particle order). A tile whose cells hold more particles than the LDS capacity gathers from
-// global memory instead.
+// can reach it (each cell row's slice of the sorted arrays is contiguous). Staging evaluates each
+// particle's stencil base and its three B-spline weights per axis once, in FP64 (kern/pic.py:
+// 127-147 evaluates them per particle too), and keeps them in the storage type (FP64 weights are
+// therefore exact; FP32 weights are rounded once, 2^-24 relative, which keeps two tiles' staging in
+// one CU's LDS); a face then takes a candidate's weight as the FP64 product of the two weights its
+// offset selects, in FP64 the same product of the same values as evaluating both B-splines for that
+// face. Four threads share a face, one per row of its 4 x 4 candidate cells;
+// each sums its row's contributions in FP64 in cell and particle order, and the four partial sums
+// are added in row order, so the result is deterministic. A face's weight and momentum are rounded
+// to the storage type once (the reference rounds after every particle, in particle order). A tile
+// whose cells hold more particles than the LDS capacity gathers from global memory instead.
 constexpr int kTileW = 32, kTileH = 4, kTileRows = 4;
 constexpr int kTileThreads = kTileW * kTileH * kTileRows;
 template <class T>
 constexpr int tile_cap() {
-  return sizeof(T) == 4 ? 3800 : 3200;  // staged particles per tile (20 B or 24 B each, ~76 KB)
+  // staged particles per tile: six weights and the value in the storage type, the packed base
+  // (30 B each in FP32, ~64 KB, two tiles per CU; 58 B in FP64, ~114 KB)
+  return sizeof(T) == 4 ? 2150 : 2000;
+}
+// Stencil base of a staged particle relative to the tile's first face, packed in 16 bits (8 each,
+// biased by 8); a base that no face of the tile can use is stored as kNoBase.
+constexpr int kBaseBias = 8, kNoBase = 0xffff;
+__device__ __forceinline__ int pack_base(int rx, int ry) {
+  if (rx < -2 || rx > kTileW + 1 || ry < -2 || ry > kTileH + 1) return kNoBase;
+  return (rx + kBaseBias) | ((ry + kBaseBias) << 8);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
   constexpr int CAP = tile_cap<T>();
-  __shared__ double sgx[CAP], sgy[CAP];
+  __shared__ T swx[3][CAP], swy[3][CAP];
   __shared__ T sv[CAP];
+  __shared__ unsigned short sb[CAP];
+  // the four rows' partial sums reuse swx once the gather is done
+  double* part = reinterpret_cast<double*>(&swx[0][0]);
   __shared__ int cell_off[kTileH + 3][kTileW + 4];
   __shared__ int row_base[kTileH + 4];
   __shared__ int row_first[kTileH + 3];
-  static_assert(2 * kTileThreads <= CAP, "the partial sums reuse the staging arrays");
+  static_assert(2 * kTileThreads * sizeof(double) <= 3 * CAP * sizeof(T), "the partial sums reuse swx");
   const Comp& c = a.c;
   const PicGeom& g = a.g;
   const int face = threadIdx.y * kTileW + threadIdx.x;
@@ -541,8 +574,15 @@ __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
         ld2(a.spos, p, x, y);
         ld2(a.sval, p, vx, vy);
         const int o = row_base[r] + k;
-        sgx[o] = ((double)x - g.x0) / g.dx + c.gxo;
-        sgy[o] = ((double)y - g.y0) / g.dy + c.gyo;
+        const double gx = ((double)x - g.x0) / g.dx + c.gxo;
+        const double gy = ((double)y - g.y0) / g.dy + c.gyo;
+        const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          swx[q][o] = (T)bspline(gx - (double)(bx + q));
+          swy[q][o] = (T)bspline(gy - (double)(by + q));
+        }
+        sb[o] = (unsigned short)pack_base(bx - X0, by - Y0);
         sv[o] = a.comp ? vy : vx;
       }
     }
@@ -558,53 +598,48 @@ __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
       const int cx = fx + wx;
       if (cx < 0 || cx >= g.nx) continue;
       const int lc = (int)threadIdx.x + wx + 2;
-      int k0, k1;
       if (staged) {
-        k0 = cell_off[r][lc];
-        k1 = cell_off[r][lc + 1];
+        const int k0 = cell_off[r][lc], k1 = cell_off[r][lc + 1];
+        const int lx = (int)threadIdx.x + kBaseBias, ly = (int)threadIdx.y + kBaseBias;
+        for (int k = k0; k < k1; ++k) {
+          const int b = sb[k];
+          const unsigned ox = (unsigned)(lx - (b & 0xff)), oy = (unsigned)(ly - (b >> 8));
+          if (ox > 2u || oy > 2u) continue;
+          const double w = (double)swy[oy][k] * (double)swx[ox][k];
+          wsum = wsum + w;
+          msum = msum + w * (double)sv[k];
+        }
       } else {
-        k0 = a.start[cy * g.nx + cx];
-        k1 = a.start[cy * g.nx + cx + 1];
-      }
-      for (int k = k0; k < k1; ++k) {
-        double gx, gy, val;
-        if (staged) {
-          gx = sgx[k];
-          gy = sgy[k];
-        } else {
+        const int k0 = a.start[cy * g.nx + cx], k1 = a.start[cy * g.nx + cx + 1];
+        for (int k = k0; k < k1; ++k) {
           T x, y;
           ld2(a.spos, k, x, y);
-          gx = ((double)x - g.x0) / g.dx + c.gxo;
-          gy = ((double)y - g.y0) / g.dy + c.gyo;
-        }
-        const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
-        if (fx < bx || fx > bx + 2) continue;
-        if (fy < by || fy > by + 2) continue;
-        const double w = bspline(gy - (double)fy) * bspline(gx - (double)fx);
-        if (staged) {
-          val = (double)sv[k];
-        } else {
+          const double gx = ((double)x - g.x0) / g.dx + c.gxo;
+          const double gy = ((double)y - g.y0) / g.dy + c.gyo;
+          const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
+          if (fx < bx || fx > bx + 2) continue;
+          if (fy < by || fy > by + 2) continue;
+          const double w = bspline(gy - (double)fy) * bspline(gx - (double)fx);
           T vx, vy;
           ld2(a.sval, k, vx, vy);
-          val = (double)(a.comp ? vy : vx);
+          wsum = wsum + w;
+          msum = msum + w * (double)(a.comp ? vy : vx);
         }
-        wsum = wsum + w;
-        msum = msum + w * val;
       }
     }
   }
   // the four rows' partial sums, added in row order (the staging arrays are free now)
   __syncthreads();
-  sgx[threadIdx.z * (kTileW * kTileH) + face] = wsum;
-  sgy[threadIdx.z * (kTileW * kTileH) + face] = msum;
+  part[threadIdx.z * (kTileW * kTileH) + face] = wsum;
+  part[kTileThreads + threadIdx.z * (kTileW * kTileH) + face] = msum;
   __syncthreads();
   long long uns = 0;
   if (threadIdx.z == 0 && fx < c.W && fy < c.H) {
-    double w = sgx[face], m = sgy[face];
+    double w = part[face], m = part[kTileThreads + face];
 #pragma unroll
     for (int q = 1; q < kTileRows; ++q) {
-      w = w + sgx[q * (kTileW * kTileH) + face];
-      m = m + sgy[q * (kTileW * kTileH) + face];
+      w = w + part[q * (kTileW * kTileH) + face];
+      m = m + part[kTileThreads + q * (kTileW * kTileH) + face];
     }
     const T wt = (T)w, mt = (T)m;
     T outv;
@@ -630,8 +665,10 @@ __global__ void __launch_bounds__(256) g2p_kernel(PicGeom g, int n, const int* _
   T x, y;
   ld2(pos, i, x, y);
   double a, b, dummy;
-  gather<T, false>(g, comp_u(g), u, nullptr, (double)x, (double)y, a, dummy);
-  gather<T, false>(g, comp_v(g), v, nullptr, (double)x, (double)y, b, dummy);
+  double tx, ty;
+  grid_coords(g, (double)x, (double)y, tx, ty);
+  gather<T, false>(g, comp_u(g), u, nullptr, tx, ty, a, dummy);
+  gather<T, false>(g, comp_v(g), v, nullptr, tx, ty, b, dummy);
   st2(vel, i, (T)a, (T)b);
 }
 
@@ -838,9 +875,10 @@ __global__ void __launch_bounds__(256) flip_advect_kernel(FlipArgs a, int n, T* 
     T x, y, vx, vy;
     ld2(pos, i, x, y);
     ld2(vel, i, vx, vy);
-    double qa, da, qb, db;
-    gather<T, true>(g, cu, qu, pu, (double)x, (double)y, qa, da);
-    gather<T, true>(g, cv, qv, pv, (double)x, (double)y, qb, db);
+    double qa, da, qb, db, tx, ty;
+    grid_coords(g, (double)x, (double)y, tx, ty);
+    gather<T, true>(g, cu, qu, pu, tx, ty, qa, da);
+    gather<T, true>(g, cv, qv, pv, tx, ty, qb, db);
     const T picx = (T)qa, picy = (T)qb, dlx = (T)da, dly = (T)db;
     const T omb = (T)(1.0 - a.blend), b = (T)a.blend;
     vx = omb * picx + b * (vx + dlx);
@@ -850,8 +888,9 @@ __global__ void __launch_bounds__(256) flip_advect_kernel(FlipArgs a, int n, T* 
     const T h = (T)(0.5 * a.dt), dt = (T)a.dt;
     T mx = x + h * picx, my = y + h * picy;
     double ma, mb, dummy;
-    gather<T, false>(g, cu, qu, nullptr, (double)mx, (double)my, ma, dummy);
-    gather<T, false>(g, cv, qv, nullptr, (double)mx, (double)my, mb, dummy);
+    grid_coords(g, (double)mx, (double)my, tx, ty);
+    gather<T, false>(g, cu, qu, nullptr, tx, ty, ma, dummy);
+    gather<T, false>(g, cv, qv, nullptr, tx, ty, mb, dummy);
     T nx = x + dt * (T)ma, ny = y + dt * (T)mb;
     st2(start, i, x, y);
     st2(pos, i, nx, ny);

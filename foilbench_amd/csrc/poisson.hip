@@ -1048,10 +1048,11 @@ __global__ void __launch_bounds__(256) rhs_prep_kernel(size_t N, const double* _
     m = nf ? 0.0 : fabs(x);
   }
   if (__any(nf) && (threadIdx.x & 63) == 0) atomicAdd(&stat[0], 1ull);
-  double mw = m;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
-  if ((threadIdx.x & 63) == 0) key_max(&stat[1], (unsigned long long)__double_as_longlong(mw));
+  // the largest |b| of the block (the bits of a non-negative double order like the value), then one
+  // maximum per block into the grid's slot: every wave's requests on that one address serialise
+  __shared__ unsigned long long lmax[4];
+  const unsigned long long mk = block_max_u64((unsigned long long)__double_as_longlong(m), lmax);
+  if (threadIdx.x == 0) key_max(&stat[1], mk);
   double sv = block_sum<256>(v, lds);
   double sc = block_sum<256>(c, lds);
   if (threadIdx.x == 0) {
