@@ -3,8 +3,8 @@
 Three runs over the same mix (PCG32 seed 42, stream 1001; step counts scaled by `step_scale` so that the
 leg finishes in about a minute):
 * capacity: every request queued at t = 0, served with continuous batching (Stable Fluids in FP64 storage
-  with FP64 CG and the FP32 DST-I preconditioner, the reference's zero start, same-shape pressure solves
-  batched; lattices in shifted FP32 storage with FP64 collision, batched launches) -> aggregate cell-steps/s;
+  with the drop-in's FP64 CG, the reference's zero start, same-shape pressure solves batched; lattices in
+  shifted FP32 storage with FP64 collision, batched launches) -> aggregate cell-steps/s;
 * plain: the drop-in FP64 form, one request after another -> aggregate cell-steps/s;
 * latency: Poisson arrivals at the mix's rate on the wall clock -> p50/p99 completion latency, and the
   aggregate rate under those arrivals.
@@ -95,3 +95,57 @@ def serving_leg(n: int = 256, step_scale: float = 0.05, rate: float = 50.0, sf_w
         "latency_run": strip(lat),
         "plain_run": strip(plain),
     }
+
+
+def full_mix_leg(n: int = 256, rate: float = 50.0, sf_workers: int = 16, chunk: int = 16) -> dict:
+    """The mix at its own horizons (200-5000 steps per request, step scale 1): the capacity run of the
+    serving form, then the plain form over the same requests (in chunks, with a line printed after each),
+    and the relative field error of every served request against its plain FP64 result."""
+    import time
+
+    mix = serving_mix(n, rate=rate, step_scale=1.0)
+    ids = {r.id for r in mix}
+    srv = Server.tuned(mix, sf_workers=sf_workers)
+    try:
+        cap = srv.run(mix, keep_fields=ids, realtime=False)
+    finally:
+        srv.close()
+    print(f"capacity: {cap['cell_steps_per_s']:.4g} cell-steps/s over {cap['makespan_s']:.1f} s", flush=True)
+    served = {o.id: o for o in cap["outcomes"]}
+    order = sorted(mix, key=lambda q: (q.arrival_s, q.id))
+    cells = seconds = 0.0
+    errors = {}
+    for k in range(0, len(order), chunk):
+        part = order[k:k + chunk]
+        t0 = time.perf_counter()
+        plain = run_plain(part, keep_fields=True)
+        seconds += time.perf_counter() - t0
+        cells += float(sum(o.cells * o.steps for o in plain["outcomes"] if o.status == 0 and o.sf_status == 0))
+        errors.update(_field_errors(served, plain["outcomes"]))
+        print(f"plain: {k + len(part)}/{len(order)} requests, {cells / seconds:.4g} cell-steps/s", flush=True)
+    by_solver = {}
+    for key, v in errors.items():
+        s = key.split(":")[0]
+        by_solver[s] = max(by_solver.get(s, 0.0), v)
+    strip = lambda d: {k: v for k, v in d.items() if k != "outcomes"}  # noqa: E731
+    return {
+        "workload": f"serving mix: {n} requests, grids 128x64..1024x512, SF/LBM 50/50, steps 200..5000 (full)",
+        "value": cap["cell_steps_per_s"],
+        "plain_value": cells / seconds,
+        "ratio_vs_plain": cap["cell_steps_per_s"] / (cells / seconds),
+        "max_rel_field_error": max(errors.values()) if errors else None,
+        "max_rel_field_error_by_solver": by_solver,
+        "fields_checked": len(errors),
+        "capacity_run": strip(cap),
+        "plain_seconds": seconds,
+    }
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    if len(sys.argv) > 1 and sys.argv[1] == "full":
+        print(json.dumps(full_mix_leg(), default=str), flush=True)
+    else:
+        print(json.dumps(serving_leg(), default=str), flush=True)

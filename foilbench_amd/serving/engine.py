@@ -9,11 +9,14 @@ polled as they finish, their fields read back if asked for, and their memory fre
 the next waiting requests.
 
 Forms:
-* ``serving``: Stable Fluids in FP64 storage, pressure by FP64 CG (the reference's zero start and
-  stopping rule) with the FP32 DST-I preconditioner, pressure solves of same-shape requests batched;
-  lattices in FP32 storage shifted by the lattice weights with FP64 collision arithmetic, batched.
-  (FP32 faces, or a warm-started CG, put the fields of long requests beyond 1e-5 of the plain form:
-  1.5e-3 and 1.1e-5 after 3693 steps of a 128 x 64 request at AoA 15.7.)
+* ``serving``: Stable Fluids in FP64 storage with the drop-in's FP64 pressure CG (the reference's zero
+  start and stopping rule), pressure solves of same-shape requests batched, so every served Stable Fluids
+  request equals its plain result bit for bit; lattices in FP32 storage shifted by the lattice weights with
+  FP64 collision arithmetic, batched. (FP32 faces, a warm-started CG, or the FP32 DST-I preconditioner put
+  the fields of long requests beyond 1e-5 of the plain form: 1.5e-3 and 1.1e-5 after 3693 steps of a
+  128 x 64 request at AoA 15.7 for the first two; 2.0e-5 after 1811 steps of a 256 x 128 request at
+  Re 6588 and AoA 5.9 for the preconditioner, whose solves differ within the CG tolerance and whose
+  unsteady wakes amplify the difference.)
 * ``plain``: the drop-in form, FP64, one scenario at a time on the default stream with no server
   (substep calls for Stable Fluids, one launch per lattice step).
 """
@@ -44,6 +47,10 @@ _FIXED_BYTES = 8 << 20
 SF_WARM_PRESSURE = 0
 # Storage precision of the serving form's Stable Fluids faces.
 SF_PRECISION = "float64"
+# Pressure solve of the serving form's Stable Fluids requests (a StableFluidsSolver form): "plain", the
+# drop-in's FP64 CG; "precond32", FP64 CG with the DST-I preconditioner sweeps in FP32 (measured over the
+# full-horizon mix: 14% more Stable Fluids throughput, but two of 124 requests 1.1e-5 and 2.0e-5 from FP64).
+SF_SOLVER_FORM = "plain"
 
 
 def estimate_bytes(req: Request, form: str) -> int:
@@ -78,12 +85,11 @@ def _build(req: Request, form: str, ctx: N.Context):
     """Device state of a request through the solver lifecycle; returns (solver, handle, params, pose)."""
     if req.solver == SF:
         sc = req.scenario("float64" if form == "plain" else SF_PRECISION)
-        # serving form: FP64 CG with the DST-I preconditioner sweeps in FP32 (fewer, cheaper
-        # iterations than FP32 CG with FP64 refinement restarts, and a smaller error)
+        # serving form: SF_SOLVER_FORM's pressure solve, batched across same-shape requests by the server
         if form == "plain":
             s = StableFluidsSolver("plain", ctx=ctx)
         else:
-            s = StableFluidsSolver("precond32", ctx=ctx, warm_pressure=SF_WARM_PRESSURE)
+            s = StableFluidsSolver(SF_SOLVER_FORM, ctx=ctx, warm_pressure=SF_WARM_PRESSURE)
         s.initialize(sc, NacaFoil(sc.foil), 0)
         params = s._params(sc.output_dt, sc.control_at(0.0))
         return s, s._handle, params, pose_struct(-req.aoa_degrees)

@@ -169,7 +169,11 @@ class Autotuner:
     # ------------------------------------------------------------------ a request mix
     def for_mix(self, requests, sf_batch: int) -> dict:
         """Tunes every grid shape of a serving mix; returns the request loop's lattice chunk."""
+        from .serving import engine
         from .serving.mix import LBM, SF
+
+        # the precision of the preconditioner the serving form's Stable Fluids requests apply
+        sf_prec = N.FB_F32 if engine.SF_SOLVER_FORM == "precond32" else N.FB_F64
 
         sf_shapes: dict[tuple[int, int], int] = {}
         lbm_shapes: set[tuple[int, int]] = set()
@@ -182,7 +186,7 @@ class Autotuner:
         for (nx, ny), count in sorted(sf_shapes.items()):
             dx = 4.0 / nx
             g = max(1, min(sf_batch, count, N_MAX_BATCH))
-            out["dst_rows"][f"{nx}x{ny}"] = self.poisson(nx, ny, dx, dx, N.FB_F32, grids=g, key_grids=0)["value"]
+            out["dst_rows"][f"{nx}x{ny}"] = self.poisson(nx, ny, dx, dx, sf_prec, grids=g, key_grids=0)["value"]
         for nx, ny in sorted(lbm_shapes):
             r = self.lbm(nx, ny)
             out["lbm"][f"{nx}x{ny}"] = {k: r[k] for k in ("fuse", "rows", "list_blocks")}
