@@ -634,17 +634,167 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
 // ---------------------------------------------------------------- tridiagonal solves along y
 
 // Partition solve of (T_y/dy^2 + ex[i] I) z = w for every x-mode column i, in place. The ny rows
-// are cut into P chunks of L rows (the last has Llast). Three stages:
-//  1. tri_chunk_kernel: one thread per (mode, chunk) eliminates its chunk with the chunk's LU
-//     factors and zero boundary values, keeping the chunk's first and last value;
-//  2. tri_interface_kernel: one thread per mode solves the 2P interface values (a recurrence
-//     over the chunks with precomputed coefficients);
-//  3. tri_spike_kernel: one thread per (mode, chunk) adds its two spikes scaled by the
-//     neighbouring interface values.
-// Tables are [chunk type][quantity][row][mode] so that a wave's 64 modes read 64 neighbours.
+// are cut into P chunks of L rows (the last has Llast), and one launch runs three stages over a
+// block of 16 modes:
+//  1. each chunk of each mode is eliminated with the chunk's LU factors and zero boundary values,
+//     keeping the chunk's first and last value (one thread per chunk, blockDim.y chunks at a time);
+//  2. the 2P interface values of each mode: two affine recurrences over the chunks, each a
+//     segmented scan over blockDim.y segments of ceil(P/blockDim.y) chunks;
+//  3. each chunk adds its two spikes scaled by the neighbouring interface values.
+// blockDim.y (16 or 32) depends on P alone (tri_threads), so every grid and batch size gives the
+// same bits.
+// Tables are [chunk type][quantity][row][mode] so that 16 neighbouring modes read 16 neighbours.
 constexpr int kTriL = 8;  // rows per chunk
-constexpr int kTriFusedMaxP = 16;  // chunks per column up to which the three stages run as one launch
+constexpr int kTriFusedMaxP = 32;  // chunks per column up to which the three stages run as one launch
+// Chunk threads (and scan segments) per mode of the one-launch form: one per chunk, at least 16.
+inline int tri_threads(int P) { return P <= 16 ? 16 : 32; }
 
+template <class T>
+__global__ void __launch_bounds__(512) tri_solve_kernel(int nx, int ny, T* __restrict__ w, const T* __restrict__ fac,
+                                                         int Llast, int P, T a, const T* __restrict__ itf,
+                                                         T* __restrict__ yF, T* __restrict__ yE, const double* scal) {
+  w += (size_t)blockIdx.z * nx * ny;
+  yF += (size_t)blockIdx.z * P * nx;
+  yE += (size_t)blockIdx.z * P * nx;
+  if (scal) scal += (size_t)blockIdx.z * kScalStride;
+  if (scal && CG_DONE(scal)) return;
+  __shared__ T sA[32][16], sB[32][16], sS[32][16];
+  const int m = threadIdx.x, ct = threadIdx.y;
+  const int NS = (int)blockDim.y;
+  const int i = blockIdx.x * 16 + m;
+  const bool on = i < nx;
+  // 1. chunk elimination, forward then back substitution within the chunk
+  if (on)
+    for (int c = ct; c < P; c += (int)blockDim.y) {
+      const int last = c == P - 1;
+      const int Lc = last ? Llast : kTriL;
+      const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
+      const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
+      T* col = w + (size_t)c * kTriL * nx + i;
+      T v[kTriL];
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) v[j] = col[(size_t)j * nx];
+      T y = (T)0;
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) {
+          y = (v[j] - a * y) * invm[(size_t)j * nx];
+          v[j] = y;
+        }
+      T yn = y;
+#pragma unroll
+      for (int j = kTriL - 2; j >= 0; --j)
+        if (j < Lc - 1) {
+          yn = v[j] - cp[(size_t)j * nx] * yn;
+          v[j] = yn;
+        }
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) col[(size_t)j * nx] = v[j];
+      yF[(size_t)c * nx + i] = v[0];
+      yE[(size_t)c * nx + i] = y;
+    }
+  __syncthreads();
+  // 2. interface recurrences. Forward: r_k = g yF_k + h pp_{k-1}, pp_k = A pp_{k-1} + (yE_k + mg yF_k);
+  // back: F_k = sv F_{k+1} + r_k, E_k = pp_k + qq F_{k+1} (F_P = 0). Each is an affine recurrence:
+  // every thread composes its segment's maps, one thread per mode chains the NS segment maps, then
+  // each thread replays its segment from its start value. itf = [g, h, A, mg, sv, qq][chunk][mode].
+  const int gseg = ct;
+  const int S = (P + NS - 1) / NS;
+  const int k0 = min(P, gseg * S), k1 = min(P, k0 + S);
+  const size_t PQ = (size_t)P * nx;
+  const T* cg = itf;
+  const T* ch = itf + PQ;
+  const T* cA = itf + 2 * PQ;
+  const T* cmg = itf + 3 * PQ;
+  const T* csv = itf + 4 * PQ;
+  const T* cqq = itf + 5 * PQ;
+  T Ac = (T)1, Bc = (T)0;
+  if (on)
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T A = cA[o];
+      Ac = A * Ac;
+      Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
+    }
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = 0; q < NS; ++q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
+    }
+  }
+  __syncthreads();
+  if (on) {
+    T pp = sS[gseg][m];
+    for (int k = k0; k < k1; ++k) {
+      const size_t o = (size_t)k * nx + i;
+      const T f = yF[o];
+      const T r = cg[o] * f + ch[o] * pp;
+      pp = cA[o] * pp + (yE[o] + cmg[o] * f);
+      yF[o] = r;
+      yE[o] = pp;
+    }
+  }
+  Ac = (T)1;
+  Bc = (T)0;
+  if (on)
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T sv = csv[o];
+      Ac = sv * Ac;
+      Bc = sv * Bc + yF[o];
+    }
+  __syncthreads();
+  sA[gseg][m] = Ac;
+  sB[gseg][m] = Bc;
+  __syncthreads();
+  if (gseg == 0) {
+    T cur = (T)0;
+    for (int q = NS - 1; q >= 0; --q) {
+      sS[q][m] = cur;
+      cur = sA[q][m] * cur + sB[q][m];
+    }
+  }
+  __syncthreads();
+  if (on) {
+    T Fn = sS[gseg][m];
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t o = (size_t)k * nx + i;
+      const T F = yF[o] + csv[o] * Fn;
+      const T E = yE[o] + cqq[o] * Fn;
+      yF[o] = F;
+      yE[o] = E;
+      Fn = F;
+    }
+  }
+  // the neighbouring segments' interface values, written by other threads of this block
+  __syncthreads();
+  // 3. spikes
+  if (on)
+    for (int c = ct; c < P; c += (int)blockDim.y) {
+      const int last = c == P - 1;
+      const int Lc = last ? Llast : kTriL;
+      const T* al = fac + (size_t)(last * 4 + 2) * kTriL * nx + i;
+      const T* be = fac + (size_t)(last * 4 + 3) * kTriL * nx + i;
+      const T El = c > 0 ? yE[(size_t)(c - 1) * nx + i] : (T)0;
+      const T Fr = c < P - 1 ? yF[(size_t)(c + 1) * nx + i] : (T)0;
+      T* col = w + (size_t)c * kTriL * nx + i;
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j)
+        if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
+    }
+}
+
+// The same three stages as three launches, for long columns (P > kTriFusedMaxP): a thread per mode
+// and chunk in the first and last (64 neighbouring modes to a wave), a 16 x 16 block of modes and
+// scan segments in the second. Measured faster than the one-launch form there (1024 x 512 FP64
+// preconditioner 50.5 against 52.1 us; 2048 x 1024 117.4 against 119.4 us), and slower below
+// (512 x 256: 37.6 against 35.8 us).
 template <class T>
 __global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __restrict__ w,
                                                         const T* __restrict__ fac, int Llast, int P, T a,
@@ -778,145 +928,6 @@ __global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const
       Fn = F;
     }
   }
-}
-
-// The three stages in one launch for short columns: 16 modes x 16 chunk segments per block, each
-// thread eliminating, then composing and replaying, then correcting its segment's chunks (the same
-// arithmetic per chunk and mode as the three kernels, so the same bits). Used when P <= 16, where
-// the launches cost more than the work.
-template <class T>
-__global__ void __launch_bounds__(256) tri_fused_kernel(int nx, int ny, T* __restrict__ w, const T* __restrict__ fac,
-                                                        int Llast, int P, T a, const T* __restrict__ itf,
-                                                        T* __restrict__ yF, T* __restrict__ yE, const double* scal) {
-  w += (size_t)blockIdx.z * nx * ny;
-  yF += (size_t)blockIdx.z * P * nx;
-  yE += (size_t)blockIdx.z * P * nx;
-  if (scal) scal += (size_t)blockIdx.z * kScalStride;
-  if (scal && CG_DONE(scal)) return;
-  __shared__ T sA[16][16], sB[16][16], sS[16][16];
-  const int m = threadIdx.x, gseg = threadIdx.y;
-  const int i = blockIdx.x * 16 + m;
-  const bool on = i < nx;
-  const int S = (P + 15) / 16;
-  const int k0 = min(P, gseg * S), k1 = min(P, k0 + S);
-  // 1. chunk elimination (tri_chunk_kernel's arithmetic)
-  if (on)
-    for (int c = k0; c < k1; ++c) {
-      const int last = c == P - 1;
-      const int Lc = last ? Llast : kTriL;
-      const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
-      const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
-      T* col = w + (size_t)c * kTriL * nx + i;
-      T v[kTriL];
-#pragma unroll
-      for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) v[j] = col[(size_t)j * nx];
-      T y = (T)0;
-#pragma unroll
-      for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) {
-          y = (v[j] - a * y) * invm[(size_t)j * nx];
-          v[j] = y;
-        }
-      T yn = y;
-#pragma unroll
-      for (int j = kTriL - 2; j >= 0; --j)
-        if (j < Lc - 1) {
-          yn = v[j] - cp[(size_t)j * nx] * yn;
-          v[j] = yn;
-        }
-#pragma unroll
-      for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) col[(size_t)j * nx] = v[j];
-      yF[(size_t)c * nx + i] = v[0];
-      yE[(size_t)c * nx + i] = y;
-    }
-  // 2. interface recurrences (tri_interface_kernel's arithmetic); this thread's own chunks
-  const size_t PQ = (size_t)P * nx;
-  const T* cg = itf;
-  const T* ch = itf + PQ;
-  const T* cA = itf + 2 * PQ;
-  const T* cmg = itf + 3 * PQ;
-  const T* csv = itf + 4 * PQ;
-  const T* cqq = itf + 5 * PQ;
-  T Ac = (T)1, Bc = (T)0;
-  if (on)
-    for (int k = k0; k < k1; ++k) {
-      const size_t o = (size_t)k * nx + i;
-      const T A = cA[o];
-      Ac = A * Ac;
-      Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
-    }
-  sA[gseg][m] = Ac;
-  sB[gseg][m] = Bc;
-  __syncthreads();
-  if (gseg == 0) {
-    T cur = (T)0;
-    for (int q = 0; q < 16; ++q) {
-      sS[q][m] = cur;
-      cur = sA[q][m] * cur + sB[q][m];
-    }
-  }
-  __syncthreads();
-  if (on) {
-    T pp = sS[gseg][m];
-    for (int k = k0; k < k1; ++k) {
-      const size_t o = (size_t)k * nx + i;
-      const T f = yF[o];
-      const T r = cg[o] * f + ch[o] * pp;
-      pp = cA[o] * pp + (yE[o] + cmg[o] * f);
-      yF[o] = r;
-      yE[o] = pp;
-    }
-  }
-  Ac = (T)1;
-  Bc = (T)0;
-  if (on)
-    for (int k = k1 - 1; k >= k0; --k) {
-      const size_t o = (size_t)k * nx + i;
-      const T sv = csv[o];
-      Ac = sv * Ac;
-      Bc = sv * Bc + yF[o];
-    }
-  __syncthreads();
-  sA[gseg][m] = Ac;
-  sB[gseg][m] = Bc;
-  __syncthreads();
-  if (gseg == 0) {
-    T cur = (T)0;
-    for (int q = 15; q >= 0; --q) {
-      sS[q][m] = cur;
-      cur = sA[q][m] * cur + sB[q][m];
-    }
-  }
-  __syncthreads();
-  if (on) {
-    T Fn = sS[gseg][m];
-    for (int k = k1 - 1; k >= k0; --k) {
-      const size_t o = (size_t)k * nx + i;
-      const T F = yF[o] + csv[o] * Fn;
-      const T E = yE[o] + cqq[o] * Fn;
-      yF[o] = F;
-      yE[o] = E;
-      Fn = F;
-    }
-  }
-  // the neighbouring segments' interface values, written by other threads of this block
-  __syncthreads();
-  // 3. spikes (tri_spike_kernel's arithmetic)
-  if (on)
-    for (int c = k0; c < k1; ++c) {
-      const int last = c == P - 1;
-      const int Lc = last ? Llast : kTriL;
-      const T* al = fac + (size_t)(last * 4 + 2) * kTriL * nx + i;
-      const T* be = fac + (size_t)(last * 4 + 3) * kTriL * nx + i;
-      const T El = c > 0 ? yE[(size_t)(c - 1) * nx + i] : (T)0;
-      const T Fr = c < P - 1 ? yF[(size_t)(c + 1) * nx + i] : (T)0;
-      T* col = w + (size_t)c * kTriL * nx + i;
-#pragma unroll
-      for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
-    }
 }
 
 template <class T>
@@ -1425,12 +1436,12 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
   L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,
     (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 0, partial, scal, ps);
   FB_HIP(L.status());
-  const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
   const T a = (T)(-(1.0 / (dy * dy)));
   if (v.tri_P <= kTriFusedMaxP) {
-    L(tri_fused_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, ny, v.work, (const T*)v.tri,
-      v.tri_Llast, v.tri_P, a, (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
+    L(tri_solve_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, tri_threads(v.tri_P)), 0, nx, ny, v.work,
+      (const T*)v.tri, v.tri_Llast, v.tri_P, a, (const T*)v.tri_itf, v.tri_yF, v.tri_yE, scal);
   } else {
+    const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
     L(tri_chunk_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a, v.tri_yF, v.tri_yE,
       scal);
     L(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf,
