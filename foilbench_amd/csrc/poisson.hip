@@ -255,18 +255,22 @@ __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restri
 // kFoldBlocks blocks, where the saved launch gap outweighs it.
 constexpr int kFoldBlocks = 256;
 __device__ __forceinline__ void fold_if_last(const double* __restrict__ partial, double* scal, int mode,
-                                             double* lds) {
+                                             double* lds, unsigned nblk) {
   __shared__ bool last;
   if (threadIdx.x == 0) {
     unsigned* ticket = reinterpret_cast<unsigned*>(scal + S_TICKET);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this block's partial before its ticket
-    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    last = atomicAdd(ticket, 1u) == nblk - 1;
     if (last) *ticket = 0u;
   }
   __syncthreads();
   if (!last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every block's partial, not a stale copy
-  finalize_body(partial, (int)gridDim.x, scal, mode, lds);
+  finalize_body(partial, (int)nblk, scal, mode, lds);
+}
+__device__ __forceinline__ void fold_if_last(const double* __restrict__ partial, double* scal, int mode,
+                                             double* lds) {
+  fold_if_last(partial, scal, mode, lds, gridDim.x);
 }
 
 template <class T>
@@ -308,19 +312,61 @@ __device__ __forceinline__ T apply_row(int x, int y, int nx, int ny, const uint8
   return r;
 }
 
+// The masked operator's diagonal at fluid cell (x, y) from the operator flags: setup_kernel's sums
+// in its order (fb/core/_scipy_adapter.py:31-54), so the same bits as the stored diagonal. `fl` is
+// the cell's flags, `fleft` those of (x-1, y) and `fdown` those of (x, y-1) (0 outside the grid).
+__device__ __forceinline__ double diag_from_flags(int x, int y, int nx, int ny, int px, int py, uint8_t fl,
+                                                  uint8_t fleft, uint8_t fdown, double ix2, double iy2) {
+  double d = 0.0;
+  if (fl & PF_RIGHT) d += ix2;
+  if (x > 0 && (fleft & PF_RIGHT)) d += ix2;
+  if (fl & PF_UP) d += iy2;
+  if (y > 0 && (fdown & PF_UP)) d += iy2;
+  if (px) {
+    if ((fl & PF_WRAPX) && x == nx - 1) d += ix2;
+    if ((fl & PF_WRAPX) && x == 0) d += ix2;
+  } else {
+    if (x == 0) d += ix2;
+    if (x == nx - 1) d += ix2;
+  }
+  if (py) {
+    if ((fl & PF_WRAPY) && y == ny - 1) d += iy2;
+    if ((fl & PF_WRAPY) && y == 0) d += iy2;
+  } else {
+    if (y == 0) d += iy2;
+    if (y == ny - 1) d += iy2;
+  }
+  return d;
+}
+
 // p_new = first ? z : p_old*beta + z (SciPy: p *= beta; p += z); q = A p_new; partial p.q.
-template <class T>
+// A block of 256 threads owns a tile of kPqW x TH cells (TH / 4 rows per thread): it forms p once
+// over the tile and a one-cell halo in LDS (with the flags of the tile and its left and lower
+// neighbours), then applies the operator from LDS with the diagonal recomputed from the flags
+// (apply_row's arithmetic and order). Only a periodic seam reads beyond the halo, from global
+// memory. Each thread sums its cells' p.q in row order, the block sums its threads, and the tiles'
+// partials are summed in tile order (pq_tiles of them per grid). Tall tiles (16 rows) re-read 12%
+// of p's inputs as halo instead of 50%; grids of fewer than 256 rows keep 4-row tiles for more blocks.
+constexpr int kPqW = 64;
+__host__ __device__ inline int pq_rows(int ny) { return ny >= 256 ? 16 : 4; }
+__host__ __device__ inline int pq_tiles(int nx, int ny) {
+  return ((nx + kPqW - 1) / kPqW) * ((ny + pq_rows(ny) - 1) / pq_rows(ny));
+}
+
+template <class T, int TH>
 __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_t* __restrict__ fl,
-                                                    const double* __restrict__ diag, const T* __restrict__ z,
-                                                    const T* __restrict__ pold, T* __restrict__ pnew,
-                                                    T* __restrict__ q, double* __restrict__ partial,
-                                                    double* scal, T ix2, T iy2, int ps, int fold) {
+                                                    const T* __restrict__ z, const T* __restrict__ pold,
+                                                    T* __restrict__ pnew, T* __restrict__ q,
+                                                    double* __restrict__ partial, double* scal, T ix2, T iy2,
+                                                    double ix2d, double iy2d, int px, int py, int ps, int fold) {
+  constexpr int RPT = TH / 4;  // rows per thread
+  __shared__ T sp[TH + 2][kPqW + 2];
+  __shared__ uint8_t sf[TH + 1][kPqW + 1];
   __shared__ double lds[4];
   const size_t N = (size_t)nx * ny;
   {
     const size_t go = (size_t)blockIdx.z * N;
     fl += go;
-    diag += go;
     z += go;
     pold += go;
     pnew += go;
@@ -331,23 +377,61 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
   if (CG_DONE(scal)) return;
   const bool first = scal[S_ITER] == 0.0;
   const T beta = (T)scal[S_BETA];
-  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  auto pv = [&](int xx, int yy) -> T {
+    const size_t k = (size_t)yy * nx + xx;
+    return first ? z[k] : pold[k] * beta + z[k];
+  };
+  const int x0 = (int)blockIdx.x * kPqW, y0 = (int)blockIdx.y * TH;
+  // p over rows y0-1 .. y0+TH and columns x0-1 .. x0+kPqW
+  for (int k = (int)threadIdx.x; k < (TH + 2) * (kPqW + 2); k += 256) {
+    const int ly = k / (kPqW + 2), lx = k - ly * (kPqW + 2);
+    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+    sp[ly][lx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? pv(gx, gy) : (T)0;
+  }
+  // flags over rows y0-1 .. y0+TH-1 and columns x0-1 .. x0+kPqW-1
+  for (int k = (int)threadIdx.x; k < (TH + 1) * (kPqW + 1); k += 256) {
+    const int ly = k / (kPqW + 1), lx = k - ly * (kPqW + 1);
+    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+    sf[ly][lx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? fl[(size_t)gy * nx + gx] : (uint8_t)0;
+  }
+  __syncthreads();
+  const int tx = (int)threadIdx.x % kPqW;
+  const int x = x0 + tx;
   double v = 0.0;
-  if (i < N) {
-    int x = (int)(i % nx), y = (int)(i / nx);
-    auto pv = [&](int xx, int yy) -> T {
-      size_t k = (size_t)yy * nx + xx;
-      return first ? z[k] : pold[k] * beta + z[k];
-    };
-    T pc = pv(x, y);
-    T qv = apply_row<T>(x, y, nx, ny, fl, diag, ix2, iy2, pv);
-    pnew[i] = pc;
-    q[i] = qv;
-    v = (sizeof(T) == 4) ? (double)(pc * qv) : (double)pc * (double)qv;
+#pragma unroll
+  for (int rr = 0; rr < RPT; ++rr) {
+    const int ty = ((int)threadIdx.x / kPqW) * RPT + rr;
+    const int y = y0 + ty;
+    if (x < nx && y < ny) {
+      const size_t i = (size_t)y * nx + x;
+      const uint8_t f = sf[ty + 1][tx + 1];
+      const T pc = sp[ty + 1][tx + 1];
+      T qv = pc;
+      if (!(f & PF_SOLID)) {
+        const double d = diag_from_flags(x, y, nx, ny, px, py, f, sf[ty + 1][tx], sf[ty][tx + 1], ix2d, iy2d);
+        T r = (T)d * pc;
+        if (f & PF_RIGHT) r = r - sp[ty + 1][tx + 2] * ix2;
+        if (x > 0 && (sf[ty + 1][tx] & PF_RIGHT)) r = r - sp[ty + 1][tx] * ix2;
+        if (f & PF_UP) r = r - sp[ty + 2][tx + 1] * iy2;
+        if (y > 0 && (sf[ty][tx + 1] & PF_UP)) r = r - sp[ty][tx + 1] * iy2;
+        if (f & PF_WRAPX) {  // fluid pair across the periodic x seam
+          if (x == nx - 1) r = r - pv(0, y) * ix2;
+          if (x == 0) r = r - pv(nx - 1, y) * ix2;
+        }
+        if (f & PF_WRAPY) {
+          if (y == ny - 1) r = r - pv(x, 0) * iy2;
+          if (y == 0) r = r - pv(x, ny - 1) * iy2;
+        }
+        qv = r;
+      }
+      pnew[i] = pc;
+      q[i] = qv;
+      v += (sizeof(T) == 4) ? (double)(pc * qv) : (double)pc * (double)qv;
+    }
   }
   v = block_sum<256>(v, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v;
-  if (fold) fold_if_last(partial, scal, 2, lds);  // pq and alpha
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+  if (fold) fold_if_last(partial, scal, 2, lds, gridDim.x * gridDim.y);  // pq and alpha
 }
 
 // x += alpha p; r -= alpha q; partial ||r||^2; non-finite iterate flag (the reference's callback).
@@ -1577,15 +1661,21 @@ int Poisson::enqueue_iterations(Launcher& L, CgVecs<T>& v, T* x, int n, int pari
   T* pa = parity ? v.p + pn : v.p;
   T* pb = parity ? v.p : v.p + pn;
   const dim3 grid(nblocks, 1, nb_);
+  const dim3 pqgrid((unsigned)((nx + kPqW - 1) / kPqW), (unsigned)((ny + pq_rows(ny) - 1) / pq_rows(ny)), nb_);
+  FB_REQUIRE(pq_tiles(nx, ny) <= ps, FB_ERR_ARG, "operator tiles exceed the partial-sum slots");
   int rc;
   for (int c = 0; c < n; ++c) {
     if ((rc = precond(L, v, v.r, v.z))) return rc;
     // on small grids each of these folds its partials in its last block (pq and alpha; ||r||^2
     // and the check); otherwise a fold launch follows each
     const int fold = nblocks <= kFoldBlocks ? 1 : 0;
-    L(cg_pq_kernel<T>, grid, dim3(256), 0, nx, ny, flags, diag, v.z, pa, pb, v.q, partial, scal, ix2, iy2, ps, fold);
+    const int ntiles = pq_tiles(nx, ny);
+    const int pfold = ntiles <= kFoldBlocks ? 1 : 0;
+    L(pq_rows(ny) == 16 ? cg_pq_kernel<T, 16> : cg_pq_kernel<T, 4>, pqgrid, dim3(256), 0, nx, ny, flags, v.z, pa, pb,
+      v.q, partial, scal, ix2, iy2, 1.0 / (dx * dx), 1.0 / (dy * dy), periodic_x ? 1 : 0, periodic_y ? 1 : 0, ps,
+      pfold);
     FB_HIP(L.status());
-    if (!fold && (rc = reduce(L, partial, nblocks, 2))) return rc;
+    if (!pfold && (rc = reduce(L, partial, ntiles, 2))) return rc;
     L(cg_update_kernel<T>, grid, dim3(256), 0, N, x, v.r, pb, v.q, partial, scal, ps, fold);
     FB_HIP(L.status());
     if (!fold && (rc = reduce(L, partial, nblocks, 3))) return rc;
