@@ -251,8 +251,10 @@ __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restri
 // finishes last folds all gridDim.x partials with finalize_body: the same sums in the same order
 // as a cg_finalize_kernel launch, one launch fewer. The grid's ticket (zeroed by cg_init_kernel)
 // counts the finished blocks and is reset by the folding block. Every block takes the ticket with
-// one atomic on one address, which serialises; it is used only for grids of at most
-// kFoldBlocks blocks, where the saved launch gap outweighs it.
+// one atomic on one address, which serialises, and its release fence writes back the XCD's dirty
+// L2 lines; it is used only for grids of at most kFoldBlocks blocks, where the saved launch gap
+// outweighs both (with two levels of tickets on every size, 2048 x 1024 FP64 cg_update took 70 us
+// against 16 us and a fold launch).
 constexpr int kFoldBlocks = 256;
 __device__ __forceinline__ void fold_if_last(const double* __restrict__ partial, double* scal, int mode,
                                              double* lds, unsigned nblk) {
@@ -346,11 +348,14 @@ __device__ __forceinline__ double diag_from_flags(int x, int y, int nx, int ny, 
 // (apply_row's arithmetic and order). Only a periodic seam reads beyond the halo, from global
 // memory. Each thread sums its cells' p.q in row order, the block sums its threads, and the tiles'
 // partials are summed in tile order (pq_tiles of them per grid). Tall tiles (16 rows) re-read 12%
-// of p's inputs as halo instead of 50%; grids of fewer than 256 rows keep 4-row tiles for more blocks.
+// of p's inputs as halo instead of 50%; grids with fewer than 1024 such tiles keep 4-row tiles for
+// more blocks (measured: 2048 x 1024 FP64 24.3 -> 18.0 us with tall tiles, 1024 x 512 8.7 -> 8.9 us).
 constexpr int kPqW = 64;
-__host__ __device__ inline int pq_rows(int ny) { return ny >= 256 ? 16 : 4; }
+__host__ __device__ inline int pq_rows(int nx, int ny) {
+  return (nx + kPqW - 1) / kPqW * ((ny + 15) / 16) >= 1024 ? 16 : 4;
+}
 __host__ __device__ inline int pq_tiles(int nx, int ny) {
-  return ((nx + kPqW - 1) / kPqW) * ((ny + pq_rows(ny) - 1) / pq_rows(ny));
+  return ((nx + kPqW - 1) / kPqW) * ((ny + pq_rows(nx, ny) - 1) / pq_rows(nx, ny));
 }
 
 template <class T, int TH>
@@ -1661,7 +1666,7 @@ int Poisson::enqueue_iterations(Launcher& L, CgVecs<T>& v, T* x, int n, int pari
   T* pa = parity ? v.p + pn : v.p;
   T* pb = parity ? v.p : v.p + pn;
   const dim3 grid(nblocks, 1, nb_);
-  const dim3 pqgrid((unsigned)((nx + kPqW - 1) / kPqW), (unsigned)((ny + pq_rows(ny) - 1) / pq_rows(ny)), nb_);
+  const dim3 pqgrid((unsigned)((nx + kPqW - 1) / kPqW), (unsigned)((ny + pq_rows(nx, ny) - 1) / pq_rows(nx, ny)), nb_);
   FB_REQUIRE(pq_tiles(nx, ny) <= ps, FB_ERR_ARG, "operator tiles exceed the partial-sum slots");
   int rc;
   for (int c = 0; c < n; ++c) {
@@ -1671,7 +1676,7 @@ int Poisson::enqueue_iterations(Launcher& L, CgVecs<T>& v, T* x, int n, int pari
     const int fold = nblocks <= kFoldBlocks ? 1 : 0;
     const int ntiles = pq_tiles(nx, ny);
     const int pfold = ntiles <= kFoldBlocks ? 1 : 0;
-    L(pq_rows(ny) == 16 ? cg_pq_kernel<T, 16> : cg_pq_kernel<T, 4>, pqgrid, dim3(256), 0, nx, ny, flags, v.z, pa, pb,
+    L(pq_rows(nx, ny) == 16 ? cg_pq_kernel<T, 16> : cg_pq_kernel<T, 4>, pqgrid, dim3(256), 0, nx, ny, flags, v.z, pa, pb,
       v.q, partial, scal, ix2, iy2, 1.0 / (dx * dx), 1.0 / (dy * dy), periodic_x ? 1 : 0, periodic_y ? 1 : 0, ps,
       pfold);
     FB_HIP(L.status());
