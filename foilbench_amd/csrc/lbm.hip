@@ -13,6 +13,7 @@
 // (the reference's solver state) is written only on the last substep of an advance, where
 // the postcondition reductions are fused in.
 #include "lbm_batch.h"
+#include <type_traits>
 #include <vector>
 
 #include "diag.h"
@@ -1028,10 +1029,21 @@ __device__ __forceinline__ void lbm_block(const LbmArgs<typename P::S>& a, unsig
   if (pre) put_block_stats(a.spart, a.red, blk, nblk, mn, vmax, exc, nf, bad, nfu);
 }
 
+// Waves per SIMD the step kernel is compiled for. The edge and Bouzidi paths need 88 registers in
+// FP32 and 98 in shifted FP16 (five waves), the interior path 52; held to 80 (six waves) those
+// paths spill a few values, which pays for FP32 and shifted FP16 (measured at 1024 x 512: 9.31 ->
+// 8.96 and 7.60 -> 7.11 us per step) but not for FP64 (13.8 -> 15.0) or shifted FP32 (8.83 -> 8.96),
+// which keep the compiler's allocation. Registers change no result bit.
+template <class P>
+struct StepWaves {
+  static constexpr int min = (std::is_same<P, Prec<FB_F32>>::value || std::is_same<P, Prec<FB_F16_MIXED>>::value) ? 6 : 1;
+};
+
 // One lattice step with ROWS interior rows per block (a kernel of its own per ROWS, so that each
 // gets its own register allocation); NT streams the interior populations past the caches.
 template <class P, int ROWS, bool NT, bool PLAIN>
-__global__ void __launch_bounds__(256) lbm_step_kernel(LbmArgs<typename P::S> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<P>::min, 8)))
+lbm_step_kernel(LbmArgs<typename P::S> a) {
   lbm_block<P, ROWS, NT, PLAIN>(a, blockIdx.x, gridDim.x);
 }
 
