@@ -485,16 +485,18 @@ __global__ void __launch_bounds__(256) p2g_kernel(P2GArgs<T> a) {
 }
 
 // The same gather for a non-periodic box, staged through LDS. A block owns a tile of
-// kTileW x kTileH faces and loads once the particles of the (kTileW + 3) x (kTileH + 3) cells that
-// can reach it (each cell row's slice of the sorted arrays is contiguous). Staging evaluates each
+// kTileW x kTileH faces and loads once the particles of the cells that can reach it, (kTileW + 3) x
+// (kTileH + 2) for u faces and (kTileW + 2) x (kTileH + 3) for v faces (each cell row's slice of the
+// sorted arrays is contiguous), with one round of loads. Staging evaluates each
 // particle's stencil base and its three B-spline weights per axis once, in FP64 (kern/pic.py:
 // 127-147 evaluates them per particle too), and keeps them in the storage type (FP64 weights are
 // therefore exact; FP32 weights are rounded once, 2^-24 relative, which keeps two tiles' staging in
 // one CU's LDS); a face then takes a candidate's weight as the FP64 product of the two weights its
 // offset selects, in FP64 the same product of the same values as evaluating both B-splines for that
-// face. Four threads share a face, one per row of its 4 x 4 candidate cells;
-// each sums its row's contributions in FP64 in cell and particle order, and the four partial sums
-// are added in row order, so the result is deterministic. A face's weight and momentum are rounded
+// face. A face's candidates are 4 x 3 cells (u) or 3 x 4 (v); four threads share the face, one per
+// column (u) or row (v) of them, each sums its three cells' contributions in FP64 in cell and
+// particle order, and the four partial sums are added in thread order, so the result is
+// deterministic. A face's weight and momentum are rounded
 // to the storage type once (the reference rounds after every particle, in particle order). A tile
 // whose cells hold more particles than the LDS capacity gathers from global memory instead.
 constexpr int kTileW = 32, kTileH = 4, kTileRows = 4;
@@ -521,61 +523,81 @@ __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
   __shared__ unsigned short sb[CAP];
   // the four rows' partial sums reuse swx once the gather is done
   double* part = reinterpret_cast<double*>(&swx[0][0]);
-  __shared__ int cell_off[kTileH + 3][kTileW + 4];
-  __shared__ int row_base[kTileH + 4];
-  __shared__ int row_first[kTileH + 3];
+  constexpr int NR = kTileH + 3, NC = kTileW + 3;
+  // cell_raw[r][lc]: the sorted arrays' start of cell (Y0 - 2 + r, X0 - 2 + lc), clamped into the
+  // row's slice of cells X0-2 .. X0+kTileW (clipped to the box); lc = 0 is the slice's first
+  // particle, lc = NC one past its last. cell_off: the same positions in the tile's staging order.
+  __shared__ int cell_raw[NR][NC + 1];
+  __shared__ int cell_off[NR][NC + 1];
+  static_assert(NR * (NC + 1) <= kTileThreads, "one thread per cell boundary");
   static_assert(2 * kTileThreads * sizeof(double) <= 3 * CAP * sizeof(T), "the partial sums reuse swx");
   const Comp& c = a.c;
   const PicGeom& g = a.g;
   const int face = threadIdx.y * kTileW + threadIdx.x;
   const int tid = threadIdx.z * (kTileW * kTileH) + face;
   const int X0 = blockIdx.x * kTileW, Y0 = blockIdx.y * kTileH;
-  constexpr int NR = kTileH + 3, NC = kTileW + 3;
-  // per cell row: the particle slice of cells X0-2 .. X0+kTileW (clipped to the box)
-  if (tid < NR) {
-    const int cy = Y0 - 2 + tid;
-    int first = 0, cnt = 0;
-    if (cy >= 0 && cy < g.ny) {
-      const int cx0 = max(X0 - 2, 0), cx1 = min(X0 + kTileW, g.nx - 1);
-      if (cx0 <= cx1) {
-        first = a.start[cy * g.nx + cx0];
-        cnt = a.start[cy * g.nx + cx1 + 1] - first;
-      }
-    }
-    row_first[tid] = first;
-    row_base[tid + 1] = cnt;
+  // The cells whose particles can reach the tile's faces. Along the axis where a face sits half a
+  // cell off the cell centres (y for u faces, x for v faces) a particle's stencil base is its home
+  // cell minus one, (q - 0.5) - 0.5 being exactly q - 1 in FP64, so face f needs cells f-1 .. f+1
+  // there; along the other axis the base is the home cell or the one before it: cells f-2 .. f+1.
+  // A particle beyond the box, binned to an edge cell, reaches only that edge's faces, which those
+  // ranges hold. Staged rows: Y0-1 .. Y0+kTileH (u) or Y0-2 .. (v); columns X0-2 .. X0+kTileW (u)
+  // or X0-1 .. (v).
+  const bool vc = a.comp != 0;
+  const int RY0 = Y0 - (vc ? 2 : 1), RX0 = X0 - (vc ? 1 : 2);
+  const int nr = vc ? NR : NR - 1, nc = vc ? NC - 1 : NC;
+  const int cx0 = max(RX0, 0), cx1 = min(X0 + kTileW, g.nx - 1);
+  // every cell boundary of the tile in one round of loads
+  if (tid < NR * (NC + 1)) {
+    const int r = tid / (NC + 1), lc = tid - r * (NC + 1);
+    const int cy = RY0 + r;
+    int v = 0;
+    if (r < nr && cy >= 0 && cy < g.ny && cx0 <= cx1) v = a.start[cy * g.nx + min(max(RX0 + lc, cx0), cx1 + 1)];
+    cell_raw[r][lc] = v;
   }
   __syncthreads();
-  if (tid == 0) {
-    row_base[0] = 0;
-    for (int r = 0; r < NR; ++r) row_base[r + 1] += row_base[r];
+  // row r's particles take staging slots row_base[r] ..; every thread forms the NR + 1 bases
+  int row_base[NR + 1];
+  row_base[0] = 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) row_base[r + 1] = row_base[r] + (cell_raw[r][nc] - cell_raw[r][0]);
+  if (tid < NR * (NC + 1)) {
+    const int r = tid / (NC + 1), lc = tid - r * (NC + 1);
+    int rb = 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) rb = q == r ? row_base[q] : rb;
+    cell_off[r][lc] = rb + cell_raw[r][lc] - cell_raw[r][0];
   }
-  __syncthreads();
   const int total = row_base[NR];
   const bool staged = total <= CAP;
-  // local cell offsets: cell (r, lc) spans [cell_off[r][lc], cell_off[r][lc + 1])
-  for (int k = tid; k < NR * (NC + 1); k += kTileThreads) {
-    const int r = k / (NC + 1), lc = k % (NC + 1);
-    const int cy = Y0 - 2 + r;
-    int off = row_base[r];
-    if (cy >= 0 && cy < g.ny) {
-      const int cx0 = max(X0 - 2, 0), cx1 = min(X0 + kTileW, g.nx - 1);
-      const int cx = min(max(X0 - 2 + lc, cx0), cx1 + 1);
-      if (cx0 <= cx1) off += a.start[cy * g.nx + cx] - row_first[r];
-    }
-    cell_off[r][lc] = off;
-  }
   if (staged) {
-    for (int r = 0; r < NR; ++r) {
-      const int n = row_base[r + 1] - row_base[r];
-      for (int k = tid; k < n; k += kTileThreads) {
-        const size_t p = (size_t)row_first[r] + k;
-        T x, y, vx, vy;
-        ld2(a.spos, p, x, y);
-        ld2(a.sval, p, vx, vy);
-        const int o = row_base[r] + k;
-        const double gx = ((double)x - g.x0) / g.dx + c.gxo;
-        const double gy = ((double)y - g.y0) / g.dy + c.gyo;
+    // each thread's slots o = tid + i * kTileThreads, their loads issued together
+    constexpr int NIT = (CAP + kTileThreads - 1) / kTileThreads;
+    T px[NIT], py[NIT], pv[NIT];
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int o = tid + i * kTileThreads;
+      if (o < total) {
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < NR; ++q) r += o >= row_base[q] ? 1 : 0;
+        int rb = 0, first = 0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          rb = q == r ? row_base[q] : rb;
+          first = q == r ? cell_raw[q][0] : first;
+        }
+        const size_t p = (size_t)first + (o - rb);
+        ld2(a.spos, p, px[i], py[i]);
+        pv[i] = a.sval[2 * p + (a.comp ? 1 : 0)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int o = tid + i * kTileThreads;
+      if (o < total) {
+        const double gx = ((double)px[i] - g.x0) / g.dx + c.gxo;
+        const double gy = ((double)py[i] - g.y0) / g.dy + c.gyo;
         const int bx = ifloor(gx - 0.5), by = ifloor(gy - 0.5);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -583,21 +605,21 @@ __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
           swy[q][o] = (T)bspline(gy - (double)(by + q));
         }
         sb[o] = (unsigned short)pack_base(bx - X0, by - Y0);
-        sv[o] = a.comp ? vy : vx;
+        sv[o] = pv[i];
       }
     }
   }
   __syncthreads();
+  // Thread z of a face takes the face's z-th column of candidate cells (u) or row (v), three
+  // cells each.
   const int fx = X0 + (int)threadIdx.x, fy = Y0 + (int)threadIdx.y;
-  const int wy = (int)threadIdx.z - 2;
+  const int z = (int)threadIdx.z;
   double wsum = 0.0, msum = 0.0;
-  const int cy = fy + wy;
-  if (fx < c.W && fy < c.H && cy >= 0 && cy < g.ny) {
-    const int r = (int)threadIdx.y + wy + 2;
-    for (int wx = -2; wx <= 1; ++wx) {
-      const int cx = fx + wx;
-      if (cx < 0 || cx >= g.nx) continue;
-      const int lc = (int)threadIdx.x + wx + 2;
+  if (fx < c.W && fy < c.H) {
+    for (int j = 0; j < 3; ++j) {
+      const int r = (int)threadIdx.y + (vc ? z : j), lc = (int)threadIdx.x + (vc ? j : z);
+      const int cx = RX0 + lc, cy = RY0 + r;
+      if (cx < 0 || cx >= g.nx || cy < 0 || cy >= g.ny) continue;
       if (staged) {
         const int k0 = cell_off[r][lc], k1 = cell_off[r][lc + 1];
         const int lx = (int)threadIdx.x + kBaseBias, ly = (int)threadIdx.y + kBaseBias;
@@ -628,7 +650,7 @@ __global__ void __launch_bounds__(kTileThreads) p2g_tile_kernel(P2GArgs<T> a) {
       }
     }
   }
-  // the four rows' partial sums, added in row order (the staging arrays are free now)
+  // the four threads' partial sums, added in z order (the staging arrays are free now)
   __syncthreads();
   part[threadIdx.z * (kTileW * kTileH) + face] = wsum;
   part[kTileThreads + threadIdx.z * (kTileW * kTileH) + face] = msum;
