@@ -170,23 +170,8 @@ constexpr int kScalStride = S_COUNT + 8;
 // mode 2: pq = sum; alpha = rho / pq
 // mode 3: rn2 = sum; rho_prev = rho; iteration += 1; then the top-of-loop check
 // mode 4: rn2 = sum (initial) and the top-of-loop check
-__device__ __forceinline__ void finalize_body(const double* __restrict__ partial, int n, double* scal, int mode,
-                                              double* lds) {
-  if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
-  // eight independent strided sums per thread (loads in flight together), combined pairwise: an
-  // order fixed by n alone
-  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  int i = threadIdx.x;
-  for (; i + 7 * 256 < n; i += 8 * 256) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] += partial[i + u * 256];
-  }
-#pragma unroll
-  for (int u = 0; u < 7; ++u)
-    if (i + u * 256 < n) a[u] += partial[i + u * 256];
-  double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-  s = block_sum<256>(s, lds);
-  if (threadIdx.x != 0) return;
+// What the folded sum `s` sets in the grid's scalars, by `mode` (run by one thread).
+__device__ __forceinline__ void finalize_tail(double s, double* scal, int mode) {
   if (mode == 0) {
     scal[S_DOT] = s;
     return;
@@ -240,6 +225,27 @@ __device__ __forceinline__ void finalize_body(const double* __restrict__ partial
     scal[S_INFO] = 0.0;
   }
 }
+
+__device__ __forceinline__ void finalize_body(const double* __restrict__ partial, int n, double* scal, int mode,
+                                              double* lds) {
+  if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
+  // eight independent strided sums per thread (loads in flight together), combined pairwise: an
+  // order fixed by n alone
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < n; i += 8 * 256) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += partial[i + u * 256];
+  }
+#pragma unroll
+  for (int u = 0; u < 7; ++u)
+    if (i + u * 256 < n) a[u] += partial[i + u * 256];
+  double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  s = block_sum<256>(s, lds);
+  if (threadIdx.x != 0) return;
+  finalize_tail(s, scal, mode);
+}
+
 
 __global__ void __launch_bounds__(256) cg_finalize_kernel(const double* __restrict__ partial, int n, double* scal,
                                                           int mode, int ps) {
@@ -379,25 +385,56 @@ __global__ void __launch_bounds__(256) cg_pq_kernel(int nx, int ny, const uint8_
     partial += (size_t)blockIdx.z * ps;
     scal += (size_t)blockIdx.z * kScalStride;
   }
-  if (CG_DONE(scal)) return;
-  const bool first = scal[S_ITER] == 0.0;
-  const T beta = (T)scal[S_BETA];
+  // the scalars are read together, ahead of the done test
+  const double s_done = scal[S_DONE], s_iter = scal[S_ITER], s_beta = scal[S_BETA];
+  if (s_done != 0.0) return;
+  const bool first = s_iter == 0.0;
+  const T beta = (T)s_beta;
   auto pv = [&](int xx, int yy) -> T {
     const size_t k = (size_t)yy * nx + xx;
     return first ? z[k] : pold[k] * beta + z[k];
   };
   const int x0 = (int)blockIdx.x * kPqW, y0 = (int)blockIdx.y * TH;
-  // p over rows y0-1 .. y0+TH and columns x0-1 .. x0+kPqW
-  for (int k = (int)threadIdx.x; k < (TH + 2) * (kPqW + 2); k += 256) {
+  // p over rows y0-1 .. y0+TH and columns x0-1 .. x0+kPqW, flags over rows y0-1 .. y0+TH-1 and
+  // columns x0-1 .. x0+kPqW-1. Every load of the thread is issued before its first LDS store, at
+  // a clamped index (cell 0) where the cell lies outside the grid, and selected away there.
+  constexpr int NPV = (TH + 2) * (kPqW + 2), NFL = (TH + 1) * (kPqW + 1);
+  constexpr int UPV = (NPV + 255) / 256, UFL = (NFL + 255) / 256;
+  T zv[UPV], ov[UPV];
+  uint8_t fv[UFL];
+#pragma unroll
+  for (int u = 0; u < UPV; ++u) {
+    const int k = (int)threadIdx.x + u * 256;
     const int ly = k / (kPqW + 2), lx = k - ly * (kPqW + 2);
     const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-    sp[ly][lx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? pv(gx, gy) : (T)0;
+    const bool in = k < NPV && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    const size_t g = in ? (size_t)gy * nx + gx : 0;
+    zv[u] = z[g];
+    ov[u] = pold[g];
   }
-  // flags over rows y0-1 .. y0+TH-1 and columns x0-1 .. x0+kPqW-1
-  for (int k = (int)threadIdx.x; k < (TH + 1) * (kPqW + 1); k += 256) {
+#pragma unroll
+  for (int u = 0; u < UFL; ++u) {
+    const int k = (int)threadIdx.x + u * 256;
     const int ly = k / (kPqW + 1), lx = k - ly * (kPqW + 1);
     const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-    sf[ly][lx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? fl[(size_t)gy * nx + gx] : (uint8_t)0;
+    const bool in = k < NFL && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    fv[u] = fl[in ? (size_t)gy * nx + gx : 0];
+  }
+#pragma unroll
+  for (int u = 0; u < UPV; ++u) {
+    const int k = (int)threadIdx.x + u * 256;
+    const int ly = k / (kPqW + 2), lx = k - ly * (kPqW + 2);
+    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+    const bool in = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    if (k < NPV) sp[ly][lx] = in ? (first ? zv[u] : ov[u] * beta + zv[u]) : (T)0;
+  }
+#pragma unroll
+  for (int u = 0; u < UFL; ++u) {
+    const int k = (int)threadIdx.x + u * 256;
+    const int ly = k / (kPqW + 1), lx = k - ly * (kPqW + 1);
+    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+    const bool in = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    if (k < NFL) sf[ly][lx] = in ? fv[u] : (uint8_t)0;
   }
   __syncthreads();
   const int tx = (int)threadIdx.x % kPqW;
@@ -454,8 +491,9 @@ __global__ void __launch_bounds__(256) cg_update_kernel(size_t N, T* __restrict_
     partial += (size_t)blockIdx.z * ps;
     scal += (size_t)blockIdx.z * kScalStride;
   }
-  if (CG_DONE(scal)) return;
-  const T alpha = (T)scal[S_ALPHA];
+  const double s_done = scal[S_DONE], s_alpha = scal[S_ALPHA];  // read together
+  if (s_done != 0.0) return;
+  const T alpha = (T)s_alpha;
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   double v = 0.0;
   bool bad = false;
@@ -582,7 +620,7 @@ __device__ __forceinline__ void stockham_pass(Ld&& ld, St&& st, const cplx<T>* _
     dft_r<T, R>(v[u]);
     const int base = (b - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) st(base + q * L, v[u][q]);
+    for (int q = 0; q < R; ++q) st(base + q * L, v[u][q], u * R + q);
   }
   __syncthreads();
 }
@@ -595,7 +633,7 @@ template <class T, int PPT, bool SYNC_FIRST, class Ld, class St>
 __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict__ roots, int M, int logM, int t,
                                          int nthr, Ld&& first_ld, St&& last_st) {
   auto lds_ld = [buf](int i) { return buf[lds_slot(i)]; };
-  auto lds_st = [buf](int i, cplx<T> v) { buf[lds_slot(i)] = v; };
+  auto lds_st = [buf](int i, cplx<T> v, int) { buf[lds_slot(i)] = v; };
   const int n8 = logM / 3, rem = logM % 3;
   const int last8 = rem ? -1 : n8 - 1;  // the last radix-8 pass is the transform's last pass
   int L = 1;
@@ -616,18 +654,21 @@ __device__ __forceinline__ void fft_rows(cplx<T>* buf, const cplx<T>* __restrict
 // FFT's last pass stores. FUSE bit 1: the chirp product feeds the first FFT pass straight from
 // global memory; bit 2: the second FFT's last pass hands its points to the output. Without them
 // (fewer registers) the input and output go through the LDS row. The arithmetic is the same.
-// mode 0: input r masked to fluid cells, output to `out`.
-// mode 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
+// MODE 0: input r masked to fluid cells, output to `out`.
+// MODE 1: output z on fluid cells, z = r on solid cells, partial r.z per row, summed in an order
 // fixed by M alone: M/16 strided sums, then a pairwise tree.
-template <class T, class TI, class TO, int PPT, int FUSE>
+// MODE is a template parameter so that neither stage branches per point: every point's loads
+// (at a clamped index where the point lies past the row) are issued together, and only the
+// stores are predicated.
+template <class T, class TI, class TO, int PPT, int FUSE, int MODE>
 __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI* __restrict__ in,
                                                        const uint8_t* __restrict__ fl, const TO* __restrict__ rsrc,
                                                        TO* __restrict__ out, const cplx<T>* __restrict__ chirp,
                                                        const cplx<T>* __restrict__ fbt,
                                                        const cplx<T>* __restrict__ roots, int M, int logM, int rows,
-                                                       T scale, int mode, double* __restrict__ partial,
+                                                       T scale, double* __restrict__ partial,
                                                        const double* scal, int ps) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   {
     const size_t go = (size_t)blockIdx.z * nx * ny;
     in += go;
@@ -650,14 +691,19 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
   const size_t base = (size_t)(live ? row : 0) * nx;
   // first FFT: the row times the chirp (zero past nx), then the convolution kernel's spectrum,
   // conjugated for the inverse FFT taken as conj(FFT(conj(.)))
+  // The loads take a clamped index (base is a valid row, column 0 stays inside it) and are
+  // issued whatever the point's fate, so that the points of a pass have their loads in flight
+  // together rather than one branch and one memory round trip after another.
   auto chirped = [&](int j) -> cplx<T> {
-    if (!(live && j < nx)) return {(T)0, (T)0};
-    T v = (T)in[base + j];
-    if (mode == 0 && (fl[base + j] & PF_SOLID)) v = (T)0;
-    const cplx<T> c = chirp[j];
-    return {v * c.x, v * c.y};
+    const bool ok = live && j < nx;
+    const size_t jj = ok ? (size_t)j : 0;
+    T v = (T)in[base + jj];
+    const uint8_t f = fl[base + jj];
+    const cplx<T> c = chirp[jj];
+    if (MODE == 0 && (f & PF_SOLID)) v = (T)0;
+    return ok ? cplx<T>{v * c.x, v * c.y} : cplx<T>{(T)0, (T)0};
   };
-  auto spectrum = [&](int k, cplx<T> w) {
+  auto spectrum = [&](int k, cplx<T> w, int) {
     const cplx<T> v = cmul(w, fbt[k]);
     buf[lds_slot(k)] = {v.x, -v.y};
   };
@@ -669,26 +715,31 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
     __syncthreads();
     fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, spectrum);
   }
-  // second FFT: its outputs k < nx times the chirp, scaled, are the transform
+  // second FFT: its outputs k < nx times the chirp, scaled, are the transform (loads at a
+  // clamped index, only the stores predicated)
   const T invM = (T)1 / (T)M;
-  auto output = [&](int k, cplx<T> w) {
-    if (!(live && k < nx)) return;
+  auto output = [&](int k, cplx<T> w, int) {
+    const bool ok = live && k < nx;
+    const size_t kk = ok ? (size_t)k : 0;
+    const cplx<T> ck = chirp[kk];
     const cplx<T> conv{w.x * invM, -w.y * invM};
-    const cplx<T> y = cmul(chirp[k], conv);
+    const cplx<T> y = cmul(ck, conv);
     const T dv = -y.y * scale;
-    if (mode == 0) {
-      out[base + k] = (TO)dv;
-      return;
+    if constexpr (MODE == 0) {
+      if (ok) out[base + k] = (TO)dv;
+    } else {
+      const TO rv = rsrc[base + kk];
+      const TO zv = (fl[base + kk] & PF_SOLID) ? rv : (TO)dv;
+      if (ok) {
+        out[base + k] = zv;
+        prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
+      }
     }
-    const TO rv = rsrc[base + k];
-    const TO zv = (fl[base + k] & PF_SOLID) ? rv : (TO)dv;
-    out[base + k] = zv;
-    prod[k] = (sizeof(TO) == 4) ? (double)(rv * zv) : (double)rv * (double)zv;
   };
   if constexpr ((FUSE & 2) != 0) {
     fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, output);
   } else {
-    auto lds_st = [buf](int i, cplx<T> v) { buf[lds_slot(i)] = v; };
+    auto lds_st = [buf](int i, cplx<T> v, int) { buf[lds_slot(i)] = v; };
     fft_rows<T, PPT, true>(buf, roots, M, logM, t, nthr, lds_ld, lds_st);
     // this thread's outputs k = t + j nthr < nx (nx <= M / 2, so at most PPT / 2 of them), read
     // before any product overwrites the row
@@ -701,10 +752,10 @@ __global__ void __launch_bounds__(512) dst_rows_kernel(int nx, int ny, const TI*
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < KMAX; ++j) output(t + j * nthr, w[j]);
+    for (int j = 0; j < KMAX; ++j) output(t + j * nthr, w[j], j);
     __syncthreads();
   }
-  if (mode == 0) return;
+  if constexpr (MODE == 0) return;
   // the products are summed in the fixed order (the last writes are behind a barrier)
   for (int v = t; v < nred; v += nthr) {
     double acc = 0.0;
@@ -760,22 +811,27 @@ __global__ void __launch_bounds__(512) tri_solve_kernel(int nx, int ny, T* __res
       const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
       const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
       T* col = w + (size_t)c * kTriL * nx + i;
-      T v[kTriL];
+      // every load first (rows past a short last chunk read its last row; the factor tables hold
+      // kTriL rows for either chunk type), so that they are in flight together
+      T v[kTriL], fi[kTriL], fc[kTriL];
 #pragma unroll
-      for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) v[j] = col[(size_t)j * nx];
+      for (int j = 0; j < kTriL; ++j) {
+        v[j] = col[(size_t)(j < Lc ? j : Lc - 1) * nx];
+        fi[j] = invm[(size_t)j * nx];
+        fc[j] = cp[(size_t)j * nx];
+      }
       T y = (T)0;
 #pragma unroll
       for (int j = 0; j < kTriL; ++j)
         if (j < Lc) {
-          y = (v[j] - a * y) * invm[(size_t)j * nx];
+          y = (v[j] - a * y) * fi[j];
           v[j] = y;
         }
       T yn = y;
 #pragma unroll
       for (int j = kTriL - 2; j >= 0; --j)
         if (j < Lc - 1) {
-          yn = v[j] - cp[(size_t)j * nx] * yn;
+          yn = v[j] - fc[j] * yn;
           v[j] = yn;
         }
 #pragma unroll
@@ -873,9 +929,16 @@ __global__ void __launch_bounds__(512) tri_solve_kernel(int nx, int ny, T* __res
       const T El = c > 0 ? yE[(size_t)(c - 1) * nx + i] : (T)0;
       const T Fr = c < P - 1 ? yF[(size_t)(c + 1) * nx + i] : (T)0;
       T* col = w + (size_t)c * kTriL * nx + i;
+      T cv[kTriL], av[kTriL], bv[kTriL];
+#pragma unroll
+      for (int j = 0; j < kTriL; ++j) {
+        cv[j] = col[(size_t)(j < Lc ? j : Lc - 1) * nx];
+        av[j] = al[(size_t)j * nx];
+        bv[j] = be[(size_t)j * nx];
+      }
 #pragma unroll
       for (int j = 0; j < kTriL; ++j)
-        if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
+        if (j < Lc) col[(size_t)j * nx] = cv[j] + av[j] * El + bv[j] * Fr;
     }
 }
 
@@ -901,23 +964,27 @@ __global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __res
   const T* invm = fac + (size_t)(last * 4 + 0) * kTriL * nx + i;
   const T* cp = fac + (size_t)(last * 4 + 1) * kTriL * nx + i;
   T* col = w + (size_t)c * kTriL * nx + i;
-  T v[kTriL];
+  // every load first, as in tri_solve_kernel
+  T v[kTriL], fi[kTriL], fc[kTriL];
 #pragma unroll
-  for (int j = 0; j < kTriL; ++j)
-    if (j < Lc) v[j] = col[(size_t)j * nx];
+  for (int j = 0; j < kTriL; ++j) {
+    v[j] = col[(size_t)(j < Lc ? j : Lc - 1) * nx];
+    fi[j] = invm[(size_t)j * nx];
+    fc[j] = cp[(size_t)j * nx];
+  }
   // forward elimination, then back substitution, within the chunk
   T y = (T)0;
 #pragma unroll
   for (int j = 0; j < kTriL; ++j)
     if (j < Lc) {
-      y = (v[j] - a * y) * invm[(size_t)j * nx];
+      y = (v[j] - a * y) * fi[j];
       v[j] = y;
     }
   T yn = y;
 #pragma unroll
   for (int j = kTriL - 2; j >= 0; --j)
     if (j < Lc - 1) {
-      yn = v[j] - cp[(size_t)j * nx] * yn;
+      yn = v[j] - fc[j] * yn;
       v[j] = yn;
     }
 #pragma unroll
@@ -1038,9 +1105,16 @@ __global__ void __launch_bounds__(256) tri_spike_kernel(int nx, int ny, T* __res
   const T El = c > 0 ? E[(size_t)(c - 1) * nx + i] : (T)0;
   const T Fr = c < P - 1 ? F[(size_t)(c + 1) * nx + i] : (T)0;
   T* col = w + (size_t)c * kTriL * nx + i;
+  T cv[kTriL], av[kTriL], bv[kTriL];
+#pragma unroll
+  for (int j = 0; j < kTriL; ++j) {
+    cv[j] = col[(size_t)(j < Lc ? j : Lc - 1) * nx];
+    av[j] = al[(size_t)j * nx];
+    bv[j] = be[(size_t)j * nx];
+  }
 #pragma unroll
   for (int j = 0; j < kTriL; ++j)
-    if (j < Lc) col[(size_t)j * nx] = col[(size_t)j * nx] + al[(size_t)j * nx] * El + be[(size_t)j * nx] * Fr;
+    if (j < Lc) col[(size_t)j * nx] = cv[j] + av[j] * El + bv[j] * Fr;
 }
 
 template <class T>
@@ -1520,10 +1594,10 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
   // the FP64 transform gains from the fused input and output passes; the FP32 one from the fused
   // output only (a fused input takes it from 59 to 94 registers, and was measured slower)
   constexpr int fuse = sizeof(T) == 8 ? 3 : 2;
-  auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8, fuse> : dst_rows_kernel<T, TIO, T, 16, fuse>;
-  auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8, fuse> : dst_rows_kernel<T, T, TIO, 16, fuse>;
+  auto* kern_in = ppt == 8 ? dst_rows_kernel<T, TIO, T, 8, fuse, 0> : dst_rows_kernel<T, TIO, T, 16, fuse, 0>;
+  auto* kern_out = ppt == 8 ? dst_rows_kernel<T, T, TIO, 8, fuse, 1> : dst_rows_kernel<T, T, TIO, 16, fuse, 1>;
   L(kern_in, dg, dim3(rows * nthr), lds, nx, ny, rin, flags, (const T*)v.work, v.work, (const cplx<T>*)P.chirp,
-    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 0, partial, scal, ps);
+    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, partial, scal, ps);
   FB_HIP(L.status());
   const T a = (T)(-(1.0 / (dy * dy)));
   if (v.tri_P <= kTriFusedMaxP) {
@@ -1539,8 +1613,11 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
       (const T*)v.tri_yE, scal);
   }
   FB_HIP(L.status());
+  // r.z is folded by a launch of its own: folding it in the transform's last block was measured
+  // slower (256 x 128 FP64: 12.8 us against 7.3 + 3.9), the release fence of every block writing
+  // back its XCD's L2 lines of z
   L(kern_out, dg, dim3(rows * nthr), lds, nx, ny, (const T*)v.work, flags, rin, zout, (const cplx<T>*)P.chirp,
-    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, 1, partial, scal, ps);
+    (const cplx<T>*)P.fb, (const cplx<T>*)P.tw, P.M, P.logM, rows, (T)P.scale, partial, scal, ps);
   FB_HIP(L.status());
   return reduce(L, partial, ny, 1);
 }
