@@ -229,17 +229,32 @@ __device__ __forceinline__ void finalize_tail(double s, double* scal, int mode) 
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partial, int n, double* scal, int mode,
                                               double* lds) {
   if (mode != 0 && mode != 4 && CG_DONE(scal)) return;
-  // eight independent strided sums per thread (loads in flight together), combined pairwise: an
-  // order fixed by n alone
+  // eight independent strided sums per thread, combined pairwise: an order fixed by n alone.
+  // Four rounds of eight loads are issued together where n allows, and the tail's loads are
+  // unconditional (clamped index, the sum taking them only below n); the additions keep their order.
   double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int i = threadIdx.x;
+  for (; i + 31 * 256 < n; i += 32 * 256) {
+    double w[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) w[u] = partial[i + u * 256];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] += w[c * 8 + u];
+  }
   for (; i + 7 * 256 < n; i += 8 * 256) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) a[u] += partial[i + u * 256];
   }
+  if (n > 0) {
+    double w[7];
 #pragma unroll
-  for (int u = 0; u < 7; ++u)
-    if (i + u * 256 < n) a[u] += partial[i + u * 256];
+    for (int u = 0; u < 7; ++u) w[u] = partial[min(i + u * 256, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+      if (i + u * 256 < n) a[u] += w[u];
+  }
   double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   s = block_sum<256>(s, lds);
   if (threadIdx.x != 0) return;
@@ -999,7 +1014,7 @@ __global__ void __launch_bounds__(256) tri_chunk_kernel(int nx, int ny, T* __res
 // F_k = sv F_{k+1} + r_k, E_k = pp_k + qq F_{k+1} (F_P = 0). Each is an affine recurrence: every
 // thread composes its segment's maps, one thread per mode chains the 16 segment maps, then each
 // thread replays its segment from its start value. itf = [g, h, A, mg, sv, qq][chunk][mode].
-template <class T>
+template <class T, int SMAX>
 __global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const T* __restrict__ itf,
                                                             T* __restrict__ yF, T* __restrict__ yE,
                                                             const double* scal) {
@@ -1020,69 +1035,156 @@ __global__ void __launch_bounds__(256) tri_interface_kernel(int nx, int P, const
   const T* cmg = itf + 3 * PQ;
   const T* csv = itf + 4 * PQ;
   const T* cqq = itf + 5 * PQ;
-  // forward: compose pp_{k1-1} = Ac pp_{k0-1} + Bc
-  T Ac = (T)1, Bc = (T)0;
-  if (on)
-    for (int k = k0; k < k1; ++k) {
-      const size_t o = (size_t)k * nx + i;
-      const T A = cA[o];
-      Ac = A * Ac;
-      Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
+  if constexpr (SMAX > 0) {
+    // the segment (S <= SMAX chunks) is loaded once into registers, every load in flight
+    // together, and the four passes below run on it in the order of the loop form
+    T vF[SMAX], vE[SMAX], vg[SMAX], vh[SMAX], vA[SMAX], vmg[SMAX], vsv[SMAX], vqq[SMAX];
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j) {
+      const bool ok = on && k0 + j < k1;
+      const size_t o = ok ? (size_t)(k0 + j) * nx + i : 0;
+      vF[j] = yF[o];
+      vE[j] = yE[o];
+      vg[j] = cg[o];
+      vh[j] = ch[o];
+      vA[j] = cA[o];
+      vmg[j] = cmg[o];
+      vsv[j] = csv[o];
+      vqq[j] = cqq[o];
     }
-  sA[gseg][m] = Ac;
-  sB[gseg][m] = Bc;
-  __syncthreads();
-  if (gseg == 0) {
-    T cur = (T)0;
-    for (int q = 0; q < 16; ++q) {
-      sS[q][m] = cur;
-      cur = sA[q][m] * cur + sB[q][m];
+    T Ac = (T)1, Bc = (T)0;
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j)
+      if (k0 + j < k1) {
+        Ac = vA[j] * Ac;
+        Bc = vA[j] * Bc + (vE[j] + vmg[j] * vF[j]);
+      }
+    sA[gseg][m] = Ac;
+    sB[gseg][m] = Bc;
+    __syncthreads();
+    if (gseg == 0) {
+      T cur = (T)0;
+      for (int q = 0; q < 16; ++q) {
+        sS[q][m] = cur;
+        cur = sA[q][m] * cur + sB[q][m];
+      }
     }
-  }
-  __syncthreads();
-  if (on) {
+    __syncthreads();
     T pp = sS[gseg][m];
-    for (int k = k0; k < k1; ++k) {
-      const size_t o = (size_t)k * nx + i;
-      const T f = yF[o];
-      const T r = cg[o] * f + ch[o] * pp;
-      pp = cA[o] * pp + (yE[o] + cmg[o] * f);
-      yF[o] = r;
-      yE[o] = pp;
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j)
+      if (k0 + j < k1) {
+        const T f = vF[j];
+        const T r = vg[j] * f + vh[j] * pp;
+        pp = vA[j] * pp + (vE[j] + vmg[j] * f);
+        vF[j] = r;
+        vE[j] = pp;
+      }
+    Ac = (T)1;
+    Bc = (T)0;
+#pragma unroll
+    for (int j = SMAX - 1; j >= 0; --j)
+      if (k0 + j < k1) {
+        Ac = vsv[j] * Ac;
+        Bc = vsv[j] * Bc + vF[j];
+      }
+    __syncthreads();
+    sA[gseg][m] = Ac;
+    sB[gseg][m] = Bc;
+    __syncthreads();
+    if (gseg == 0) {
+      T cur = (T)0;
+      for (int q = 15; q >= 0; --q) {
+        sS[q][m] = cur;
+        cur = sA[q][m] * cur + sB[q][m];
+      }
     }
-  }
-  // back: compose F_{k0} = Ac F_{k1} + Bc
-  Ac = (T)1;
-  Bc = (T)0;
-  if (on)
-    for (int k = k1 - 1; k >= k0; --k) {
-      const size_t o = (size_t)k * nx + i;
-      const T sv = csv[o];
-      Ac = sv * Ac;
-      Bc = sv * Bc + yF[o];
-    }
-  __syncthreads();
-  sA[gseg][m] = Ac;
-  sB[gseg][m] = Bc;
-  __syncthreads();
-  if (gseg == 0) {
-    T cur = (T)0;
-    for (int q = 15; q >= 0; --q) {
-      sS[q][m] = cur;
-      cur = sA[q][m] * cur + sB[q][m];
-    }
-  }
-  __syncthreads();
-  if (on) {
+    __syncthreads();
     T Fn = sS[gseg][m];
-    for (int k = k1 - 1; k >= k0; --k) {
-      const size_t o = (size_t)k * nx + i;
-      const T F = yF[o] + csv[o] * Fn;
-      const T E = yE[o] + cqq[o] * Fn;
-      yF[o] = F;
-      yE[o] = E;
-      Fn = F;
+#pragma unroll
+    for (int j = SMAX - 1; j >= 0; --j)
+      if (k0 + j < k1) {
+        const T F = vF[j] + vsv[j] * Fn;
+        const T E = vE[j] + vqq[j] * Fn;
+        vF[j] = F;
+        vE[j] = E;
+        Fn = F;
+      }
+    if (on)
+#pragma unroll
+      for (int j = 0; j < SMAX; ++j)
+        if (k0 + j < k1) {
+          const size_t o = (size_t)(k0 + j) * nx + i;
+          yF[o] = vF[j];
+          yE[o] = vE[j];
+        }
+    return;
+  } else {
+    // forward: compose pp_{k1-1} = Ac pp_{k0-1} + Bc
+    T Ac = (T)1, Bc = (T)0;
+    if (on)
+      for (int k = k0; k < k1; ++k) {
+        const size_t o = (size_t)k * nx + i;
+        const T A = cA[o];
+        Ac = A * Ac;
+        Bc = A * Bc + (yE[o] + cmg[o] * yF[o]);
+      }
+    sA[gseg][m] = Ac;
+    sB[gseg][m] = Bc;
+    __syncthreads();
+    if (gseg == 0) {
+      T cur = (T)0;
+      for (int q = 0; q < 16; ++q) {
+        sS[q][m] = cur;
+        cur = sA[q][m] * cur + sB[q][m];
+      }
     }
+    __syncthreads();
+    if (on) {
+      T pp = sS[gseg][m];
+      for (int k = k0; k < k1; ++k) {
+        const size_t o = (size_t)k * nx + i;
+        const T f = yF[o];
+        const T r = cg[o] * f + ch[o] * pp;
+        pp = cA[o] * pp + (yE[o] + cmg[o] * f);
+        yF[o] = r;
+        yE[o] = pp;
+      }
+    }
+    // back: compose F_{k0} = Ac F_{k1} + Bc
+    Ac = (T)1;
+    Bc = (T)0;
+    if (on)
+      for (int k = k1 - 1; k >= k0; --k) {
+        const size_t o = (size_t)k * nx + i;
+        const T sv = csv[o];
+        Ac = sv * Ac;
+        Bc = sv * Bc + yF[o];
+      }
+    __syncthreads();
+    sA[gseg][m] = Ac;
+    sB[gseg][m] = Bc;
+    __syncthreads();
+    if (gseg == 0) {
+      T cur = (T)0;
+      for (int q = 15; q >= 0; --q) {
+        sS[q][m] = cur;
+        cur = sA[q][m] * cur + sB[q][m];
+      }
+    }
+    __syncthreads();
+    if (on) {
+      T Fn = sS[gseg][m];
+      for (int k = k1 - 1; k >= k0; --k) {
+        const size_t o = (size_t)k * nx + i;
+        const T F = yF[o] + csv[o] * Fn;
+        const T E = yE[o] + cqq[o] * Fn;
+        yF[o] = F;
+        yE[o] = E;
+        Fn = F;
+      }
+    }
+
   }
 }
 
@@ -1607,8 +1709,10 @@ int Poisson::dst_precond(Launcher& L, CgVecs<T>& v, const TIO* rin, TIO* zout) {
     const dim3 tb(64, 4), tg((nx + 63) / 64, (v.tri_P + 3) / 4, nb_);
     L(tri_chunk_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, a, v.tri_yF, v.tri_yE,
       scal);
-    L(tri_interface_kernel<T>, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf,
-      v.tri_yF, v.tri_yE, scal);
+    // segments of up to 8 chunks run from registers (columns of up to 128 chunks)
+    auto* itf_kern = (v.tri_P + 15) / 16 <= 8 ? tri_interface_kernel<T, 8> : tri_interface_kernel<T, 0>;
+    L(itf_kern, dim3((nx + 15) / 16, 1, nb_), dim3(16, 16), 0, nx, v.tri_P, (const T*)v.tri_itf, v.tri_yF, v.tri_yE,
+      scal);
     L(tri_spike_kernel<T>, tg, tb, 0, nx, ny, v.work, (const T*)v.tri, v.tri_Llast, v.tri_P, (const T*)v.tri_yF,
       (const T*)v.tri_yE, scal);
   }
