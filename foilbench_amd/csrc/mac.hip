@@ -224,12 +224,16 @@ __global__ void __launch_bounds__(256) jacobi_sweep_kernel(JacobiArgs<T> a) {
   unsigned long long m = 0;
   const size_t n = (size_t)W * H;
   for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256) {
-    const int x = (int)(idx % W), y = (int)(idx / W);
+    // 32-bit index arithmetic (a grid has far fewer than 2^32 cells; a 64-bit division is a long
+    // software sequence on the GPU)
+    const unsigned ui = (unsigned)idx;
+    const int y = (int)(ui / (unsigned)W), x = (int)(ui - (unsigned)y * (unsigned)W);
     auto at = [&](int xx, int yy) { return r[(size_t)yy * P + xx]; };
-    int xl = a.px ? (x - 1 + W) % W : max(x - 1, 0);
-    int xr = a.px ? (x + 1) % W : min(x + 1, W - 1);
-    int yd = a.py ? (y - 1 + H) % H : max(y - 1, 0);
-    int yu = a.py ? (y + 1) % H : min(y + 1, H - 1);
+    // neighbours: wrapped when periodic, else the edge replicated (selects rather than a modulo)
+    const int xl = x > 0 ? x - 1 : (a.px ? W - 1 : 0);
+    const int xr = x < W - 1 ? x + 1 : (a.px ? 0 : W - 1);
+    const int yd = y > 0 ? y - 1 : (a.py ? H - 1 : 0);
+    const int yu = y < H - 1 ? y + 1 : (a.py ? 0 : H - 1);
     T o = orig[(size_t)y * P + x];
     T upd = (o + ax * (at(xl, y) + at(xr, y)) + ay * (at(x, yd) + at(x, yu))) / den;
     out[(size_t)y * P + x] = upd;
@@ -592,7 +596,8 @@ __global__ void __launch_bounds__(256) mac_stats_kernel(StatArgs s, const T* __r
   const size_t PU = g.nx + 1;
   const size_t total = (size_t)(g.nx + 1) * (g.ny + 1);
   for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
-    const int x = (int)(idx % (g.nx + 1)), y = (int)(idx / (g.nx + 1));
+    const unsigned ui = (unsigned)idx, pu = (unsigned)(g.nx + 1);  // 32-bit index arithmetic
+    const int y = (int)(ui / pu), x = (int)(ui - (unsigned)y * pu);
     if (y < g.ny) {
       T a = u[(size_t)y * PU + x];
       if (!isfinite((double)a)) nf++;
@@ -657,7 +662,8 @@ __global__ void __launch_bounds__(256) mac_force_kernel(MacGeom g, double nu, do
   const size_t PU = g.nx + 1;
   const size_t total = (size_t)g.nx * g.ny;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-    const int x = (int)(i % g.nx), y = (int)(i / g.nx);
+    const unsigned ui = (unsigned)i, unx = (unsigned)g.nx;  // 32-bit index arithmetic
+    const int y = (int)(ui / unx), x = (int)(ui - (unsigned)y * unx);
     const bool s0 = solid[i] != 0;
     if (x + 1 < g.nx && s0 != (solid[i + 1] != 0)) {
       const size_t f = s0 ? i + 1 : i;
